@@ -1,0 +1,301 @@
+"""femtech_amd -- MI355X explicit C3D8 engine with FemTech's call surface.
+
+The compute path is the HIP library built by ``femtech_amd.build`` (C ABI in
+include/femtech_amd.h).  This module binds it with ctypes; there is no CPU
+fallback: without the built library every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+FT_FP64 = 0
+FT_FP32 = 1
+MOTION_STRIDE = 16
+MOTION_FIXED = 0
+MOTION_LINEAR = 1
+MOTION_RAMP_HOLD = 2
+MOTION_ROT_HALFSINE = 3
+STATUS_NONFINITE = 1
+STATUS_INVERTED = 2
+STATUS_DT_COLLAPSE = 4
+
+# FemTech's termination codes (InitFinalizeFemTech.cpp:82-86 callers) and the
+# engine's own.
+ERR_MATERIAL, ERR_INPUT, ERR_ALLOC, ERR_DT = 1, 3, 12, 19
+ERR_HIP, ERR_ARG, ERR_STATE, ERR_NUMERIC = 50, 51, 52, 53
+
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+
+# name -> argtypes (all return int unless listed in _RET)
+_SIGS = {
+    "ft_last_error": [],
+    "ft_version": [],
+    "ft_device_count": [_I],
+    "ft_engine_create": [ctypes.c_int, ctypes.c_int, _D, _I, _I, ctypes.c_int, _I, _D,
+                         ctypes.c_int, ctypes.POINTER(_VP)],
+    "ft_engine_destroy": [_VP],
+    "ft_engine_info": [_VP, _I, _I, _I],
+    "ft_set_boundary": [_VP, _I, ctypes.c_int, _D],
+    "ft_set_state": [_VP, _D, _D],
+    "ft_assemble_lumped_mass": [_VP, _D],
+    "ft_stable_time_step": [_VP, _D, _I, ctypes.c_double, _D],
+    "ft_get_force": [_VP, _D, ctypes.c_double, _D],
+    "ft_element_forces": [_VP, _D, ctypes.c_double, ctypes.c_int, _D],
+    "ft_run_init": [_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+    "ft_run_steps": [_VP, ctypes.c_int, _I],
+    "ft_get_time": [_VP, _D, _D, _I],
+    "ft_get_state": [_VP, _D, _D, _D, _D],
+    "ft_get_energy": [_VP, _D],
+    "ft_set_history": [_VP, ctypes.c_int],
+    "ft_get_history": [_VP, _D, _D, _D, _I],
+    "ft_get_status": [_VP, _I],
+    "ft_get_visco_history": [_VP, _D],
+    "ft_set_timing": [_VP, ctypes.c_int],
+    "ft_get_timing": [_VP, _D, _I],
+    "ft_traffic_model": [_VP, _D],
+    "ft_mesh_block": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                      ctypes.c_ulonglong, _D, _I],
+    "ft_write_inp": [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, _I, _I],
+    "ft_read_mesh": [ctypes.c_char_p, _I, _I, _D, _I, _I, _I],
+    "ft_read_materials": [ctypes.c_char_p, ctypes.c_int, _I, _D],
+}
+_RET = {"ft_last_error": ctypes.c_char_p}
+
+
+class FemTechError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("error %d: %s" % (code, msg))
+        self.code = code
+
+
+def library():
+    """Loads the built HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("femtech_amd: %s is missing; run femtech_amd.build.build() "
+                               "(the engine has no CPU fallback)" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RET.get(name, ctypes.c_int)
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FemTechError(rc, library().ft_last_error().decode())
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(_I)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    _check(library().ft_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def mesh_block(nx, ny, nz, h=(1e-3, 1e-3, 1e-3), jitter=0.0, seed=0):
+    """Structured hex block (nodes x-fastest), interior nodes jittered by
+    +-jitter*h from a seeded 64-bit Mersenne twister."""
+    if np.isscalar(h):
+        h = (h, h, h)
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    ne = nx * ny * nz
+    X = np.empty(3 * nn, np.float64)
+    conn = np.empty(8 * ne, np.int32)
+    _check(library().ft_mesh_block(nx, ny, nz, h[0], h[1], h[2], jitter, seed, _dp(X), _ip(conn)))
+    return X, conn
+
+
+def write_inp(path, X, conn, pid=None):
+    X, conn = _f64(X), _i32(conn)
+    ne = conn.size // 8
+    pid = None if pid is None else _i32(pid)
+    _check(library().ft_write_inp(path.encode(), X.size // 3, ne, _dp(X), _ip(conn), _ip(pid)))
+
+
+def read_mesh(path):
+    nn, ne = ctypes.c_int(0), ctypes.c_int(0)
+    lib = library()
+    _check(lib.ft_read_mesh(path.encode(), ctypes.byref(nn), ctypes.byref(ne), None, None, None, None))
+    X = np.empty(3 * nn.value, np.float64)
+    conn = np.empty(8 * ne.value, np.int32)
+    pid = np.empty(ne.value, np.int32)
+    gid = np.empty(nn.value, np.int32)
+    _check(lib.ft_read_mesh(path.encode(), ctypes.byref(nn), ctypes.byref(ne), _dp(X), _ip(conn),
+                            _ip(pid), _ip(gid)))
+    return X, conn, pid, gid
+
+
+def read_materials(path, n_parts):
+    mat = np.zeros(n_parts, np.int32)
+    props = np.zeros(9 * n_parts, np.float64)
+    _check(library().ft_read_materials(path.encode(), n_parts, _ip(mat), _dp(props)))
+    return mat, props
+
+
+def motion(kind, *params):
+    m = np.zeros(MOTION_STRIDE, np.float64)
+    m[0] = kind
+    m[1:1 + len(params)] = params
+    return m
+
+
+class Engine:
+    """One mesh resident on the current HIP device (see include/femtech_amd.h)."""
+
+    def __init__(self, X, conn, pid, material_id, props, precision=FT_FP64):
+        self._lib = library()
+        self.X = _f64(X)
+        self.conn = _i32(conn)
+        self.pid = _i32(pid)
+        self.material_id = _i32(material_id)
+        self.props = _f64(props)
+        self.nn = self.X.size // 3
+        self.ne = self.conn.size // 8
+        self.precision = precision
+        h = _VP()
+        _check(self._lib.ft_engine_create(self.nn, self.ne, _dp(self.X), _ip(self.conn), _ip(self.pid),
+                                          self.material_id.size, _ip(self.material_id), _dp(self.props),
+                                          precision, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ft_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- set-up -----------------------------------------------------------
+    def set_boundary(self, dof_motion, motions):
+        dm = _i32(dof_motion)
+        mt = _f64(np.asarray(motions, np.float64).reshape(-1))
+        nm = mt.size // MOTION_STRIDE
+        _check(self._lib.ft_set_boundary(self._h, _ip(dm), nm, _dp(mt) if nm else None))
+
+    def set_state(self, u=None, v=None):
+        u = None if u is None else _f64(u)
+        v = None if v is None else _f64(v)
+        _check(self._lib.ft_set_state(self._h, _dp(u), _dp(v)))
+
+    def lumped_mass(self):
+        m = np.empty(3 * self.nn, np.float64)
+        _check(self._lib.ft_assemble_lumped_mass(self._h, _dp(m)))
+        return m
+
+    # --- per-routine ------------------------------------------------------
+    def stable_time_step(self, u=None, boundary=None, failure_dt=0.0):
+        out = ctypes.c_double(0)
+        u = None if u is None else _f64(u)
+        b = None if boundary is None else _i32(boundary)
+        _check(self._lib.ft_stable_time_step(self._h, _dp(u), _ip(b), failure_dt, ctypes.byref(out)))
+        return out.value
+
+    def get_force(self, u=None, dt=0.0):
+        fi = np.empty(3 * self.nn, np.float64)
+        u = None if u is None else _f64(u)
+        _check(self._lib.ft_get_force(self._h, _dp(u), dt, _dp(fi)))
+        return fi
+
+    def element_forces(self, u=None, dt=0.0, update_history=False):
+        fe = np.empty(24 * self.ne, np.float64)
+        u = None if u is None else _f64(u)
+        _check(self._lib.ft_element_forces(self._h, _dp(u), dt, int(update_history), _dp(fe)))
+        return fe
+
+    # --- resident loop ----------------------------------------------------
+    def run_init(self, tmax=float("inf"), dt_factor=0.8, failure_dt=1e-11):
+        _check(self._lib.ft_run_init(self._h, tmax, dt_factor, failure_dt))
+
+    def run_steps(self, n):
+        done = ctypes.c_int(0)
+        _check(self._lib.ft_run_steps(self._h, int(n), ctypes.byref(done)))
+        return done.value
+
+    def time(self):
+        t, dt, k = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+        _check(self._lib.ft_get_time(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(k)))
+        return t.value, dt.value, k.value
+
+    def state(self):
+        u, v, a, fi = (np.empty(3 * self.nn, np.float64) for _ in range(4))
+        _check(self._lib.ft_get_state(self._h, _dp(u), _dp(v), _dp(a), _dp(fi)))
+        return u, v, a, fi
+
+    def energy(self):
+        out = np.empty(4, np.float64)
+        _check(self._lib.ft_get_energy(self._h, _dp(out)))
+        return out
+
+    def set_history(self, capacity):
+        _check(self._lib.ft_set_history(self._h, int(capacity)))
+        self._hist_cap = int(capacity)
+
+    def history(self):
+        cap = getattr(self, "_hist_cap", 0)
+        t, dt = np.empty(max(cap, 1)), np.empty(max(cap, 1))
+        en = np.empty(4 * max(cap, 1))
+        n = ctypes.c_int(0)
+        _check(self._lib.ft_get_history(self._h, _dp(t), _dp(dt), _dp(en), ctypes.byref(n)))
+        k = n.value
+        return t[:k], dt[:k], en[:4 * k].reshape(k, 4)
+
+    def status(self):
+        s = ctypes.c_int(0)
+        _check(self._lib.ft_get_status(self._h, ctypes.byref(s)))
+        return s.value
+
+    def visco_history(self):
+        h = np.empty(144 * self.ne, np.float64)
+        _check(self._lib.ft_get_visco_history(self._h, _dp(h)))
+        return h
+
+    def set_timing(self, enable):
+        _check(self._lib.ft_set_timing(self._h, int(enable)))
+
+    def timing(self):
+        ms = np.zeros(3)
+        n = np.zeros(3, np.int32)
+        _check(self._lib.ft_get_timing(self._h, _dp(ms), _ip(n)))
+        return ms, n
+
+    def traffic_model(self):
+        b = np.zeros(2)
+        _check(self._lib.ft_traffic_model(self._h, _dp(b)))
+        return b

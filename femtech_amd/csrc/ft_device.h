@@ -1,0 +1,362 @@
+// ft_device.h -- device-side element mechanics shared by the element kernels.
+//
+// One thread evaluates one C3D8 element: the trilinear position and
+// displacement fields are taken to their Walsh-Hadamard (hourglass) basis, the
+// 2x2x2 Gauss-point gradients come from that basis, the material law runs in
+// registers, and the nodal forces come back through the transposed transform.
+// Reference semantics: ShapeFunction_C3D8.cpp:4-128 (dN/dxi, J0),
+// CalculateDeformationGradient.cpp:11-25 (F), StressUpdate.cpp:5-29 and the
+// material files (S), InternalForceUpdate.cpp:4-28 with
+// StrainDisplacementMatrix.cpp:27-76 (f_a = sum_gp w detJ0 B_a^T S),
+// CalculateCharacteristicLength_C3D8.cpp:3-30 with Geometry.cpp:3-64 (Le).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace ft {
+
+// Gauss abscissa as written in GaussQuadrature3D.cpp:19-49 and the face
+// quadrature abscissa of Geometry.cpp:50 (sqrt(3)/3 evaluated in FP64).
+constexpr double kGp = 0.577350269189626;
+constexpr double kFaceQ = 0.57735026918962573;  // sqrt(3.0)/3.0 rounded to FP64
+
+struct PartDev {
+  int mat;     // material id (StressUpdate.cpp:7-27)
+  int visco;   // 1 if the part carries Prony history
+  double rho, mu, lambda, k1, k2, g1, t1, g2, t2;
+  double ce;   // dilatational wave speed, CalculateTimeStep.cpp:13-19
+};
+
+template <typename R> struct Num;
+template <> struct Num<float> {
+  __device__ static float log1p_(float x) { return log1pf(x); }
+  __device__ static float expm1_(float x) { return expm1f(x); }
+  __device__ static float exp_(float x) { return expf(x); }
+  __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+};
+template <> struct Num<double> {
+  __device__ static double log1p_(double x) { return log1p(x); }
+  __device__ static double expm1_(double x) { return expm1(x); }
+  __device__ static double exp_(double x) { return exp(x); }
+  __device__ static double sqrt_(double x) { return sqrt(x); }
+};
+
+// Position p = i + 2j + 4k of the 2x2x2 node cube (i: xi, j: eta, k: zeta
+// sign bit) -> C3D8 local node number.
+__host__ __device__ constexpr int p2n(int p) {
+  return p == 0 ? 0 : p == 1 ? 1 : p == 2 ? 3 : p == 3 ? 2 : p == 4 ? 4 : p == 5 ? 5 : p == 6 ? 7 : 6;
+}
+
+// Unnormalised Walsh-Hadamard transform over the three sign bits.
+template <typename R>
+__device__ inline void wht8(R v[8]) {
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      if (p & b) continue;
+      R lo = v[p], hi = v[p | b];
+      v[p] = hi + lo;
+      v[p | b] = hi - lo;
+    }
+  }
+}
+
+// Inverse direction used for forces: coefficients -> nodal values with sign
+// convention (bit set -> +1, clear -> -1).
+template <typename R>
+__device__ inline void iwht8(R c[8]) {
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (q & b) continue;
+      R lo = c[q], hi = c[q | b];
+      c[q] = lo - hi;
+      c[q | b] = lo + hi;
+    }
+  }
+}
+
+// Symmetric 3x3 stored as (00, 11, 22, 12, 02, 01) -- Voigt order
+// (11, 22, 33, 23, 13, 12) of CompressibleNeoHookean.cpp:45-55.
+template <typename R> struct Sym { R v[6]; };
+
+template <typename R>
+__device__ inline R sym_at(const Sym<R> &s, int i, int j) {
+  if (i == j) return s.v[i];
+  int k = 3 - i - j;  // (1,2)->0 ... map: pair {1,2}->3, {0,2}->4, {0,1}->5
+  return s.v[3 + k];
+}
+
+// det(I + H) - 1 without forming I + H (no cancellation at small strain).
+template <typename R>
+__device__ inline R det_ip1_m1(const R H[3][3]) {
+  R tr = H[0][0] + H[1][1] + H[2][2];
+  R m2 = (H[0][0] * H[1][1] - H[0][1] * H[1][0]) + (H[0][0] * H[2][2] - H[0][2] * H[2][0]) +
+         (H[1][1] * H[2][2] - H[1][2] * H[2][1]);
+  R d3 = H[0][0] * (H[1][1] * H[2][2] - H[1][2] * H[2][1]) -
+         H[0][1] * (H[1][0] * H[2][2] - H[1][2] * H[2][0]) +
+         H[0][2] * (H[1][0] * H[2][1] - H[1][1] * H[2][0]);
+  return tr + m2 + d3;
+}
+
+// Green-Lagrange E = (H + H^T + H^T H) / 2.
+template <typename R>
+__device__ inline Sym<R> green_lagrange(const R H[3][3]) {
+  Sym<R> E;
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int i = I[k], j = J[k];
+    R hth = H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j];
+    E.v[k] = R(0.5) * (H[i][j] + H[j][i] + hth);
+  }
+  return E;
+}
+
+// adj(C) for C = I + 2E (symmetric).
+template <typename R>
+__device__ inline Sym<R> adj_c(const Sym<R> &E) {
+  R c00 = R(1) + R(2) * E.v[0], c11 = R(1) + R(2) * E.v[1], c22 = R(1) + R(2) * E.v[2];
+  R c12 = R(2) * E.v[3], c02 = R(2) * E.v[4], c01 = R(2) * E.v[5];
+  Sym<R> A;
+  A.v[0] = c11 * c22 - c12 * c12;
+  A.v[1] = c00 * c22 - c02 * c02;
+  A.v[2] = c00 * c11 - c01 * c01;
+  A.v[3] = c01 * c02 - c00 * c12;
+  A.v[4] = c01 * c12 - c02 * c11;
+  A.v[5] = c02 * c12 - c01 * c22;
+  return A;
+}
+
+// S = (A T) * s for symmetric A, T whose product is symmetric (they commute:
+// both are functions of E).
+template <typename R>
+__device__ inline Sym<R> sym_mul(const Sym<R> &A, const Sym<R> &T, R s) {
+  Sym<R> S;
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int i = I[k], j = J[k];
+    R acc = sym_at(A, i, 0) * sym_at(T, 0, j) + sym_at(A, i, 1) * sym_at(T, 1, j) +
+            sym_at(A, i, 2) * sym_at(T, 2, j);
+    S.v[k] = acc * s;
+  }
+  return S;
+}
+
+struct ViscoCoef { double c11, c12, c21, c22; };
+
+// PK2 stress of the reference's material routines, restated in terms of the
+// displacement gradient H = F - I.  hist: 18 values (H1, H2, S0 as Voigt
+// 6-vectors) for material 5 or nullptr.  Returns false for an unknown id.
+template <typename R>
+__device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R *hist,
+                                  const ViscoCoef &vc, Sym<R> &S, R &Jm1) {
+  Jm1 = det_ip1_m1(H);
+  const int mat = pp.mat;
+  if (mat == 0) {  // rigid: StressUpdate.cpp:8-9
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.v[k] = R(0);
+    return true;
+  }
+  const R mu = R(pp.mu), lambda = R(pp.lambda);
+  if (mat == 3) {
+    // LinearElastic.cpp:14-70: P = lambda tr(eps) I + 2 mu eps, S = F^-1 P,
+    // unsymmetrised, upper triangle kept.
+    R trE = H[0][0] + H[1][1] + H[2][2];
+    R P[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) P[i][j] = mu * (H[i][j] + H[j][i]);
+    P[0][0] += lambda * trE;
+    P[1][1] += lambda * trE;
+    P[2][2] += lambda * trE;
+    R F[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? R(1) : R(0));
+    R A[3][3];  // adj(F)
+    A[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    A[0][1] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    A[0][2] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    A[1][0] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    A[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    A[1][2] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    A[2][0] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    A[2][1] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    A[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    R iJ = R(1) / (R(1) + Jm1);
+    const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      int i = I[k], j = J[k];
+      S.v[k] = (A[i][0] * P[0][j] + A[i][1] * P[1][j] + A[i][2] * P[2][j]) * iJ;
+    }
+    return true;
+  }
+  Sym<R> E = green_lagrange(H);
+  R trE = E.v[0] + E.v[1] + E.v[2];
+  if (mat == 2) {  // StVenantKirchhoff.cpp:24-41
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.v[k] = R(2) * mu * E.v[k];
+    S.v[0] += lambda * trE;
+    S.v[1] += lambda * trE;
+    S.v[2] += lambda * trE;
+    return true;
+  }
+  const R lnJ = Num<R>::log1p_(Jm1);
+  const R J = R(1) + Jm1;
+  const R iJ2 = R(1) / (J * J);
+  Sym<R> Ac = adj_c(E);  // C^-1 = Ac / J^2
+  if (mat == 1) {
+    // CompressibleNeoHookean.cpp:26-55: S = mu (I - C^-1) + lambda ln J C^-1
+    //                                     = C^-1 (2 mu E + lambda ln J I)
+    Sym<R> T;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) T.v[k] = R(2) * mu * E.v[k];
+    T.v[0] += lambda * lnJ;
+    T.v[1] += lambda * lnJ;
+    T.v[2] += lambda * lnJ;
+    S = sym_mul(Ac, T, iJ2);
+    return true;
+  }
+  if (mat == 4 || mat == 5) {
+    // HGOIsotropic.cpp:37-90 with kappa = 1/3:
+    //   S = J F^-1 sigma F^-T = C^-1 [J^-2/3 (mu + fib) dev(2E) + K/2 (J^2-1) I]
+    const R kappa = R(1.0 / 3.0);
+    const R K = lambda + R(2) * mu / R(3);
+    const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * lnJ);
+    const R Jm23 = R(1) + Jm23m1;
+    const R Ea = kappa * (R(3) * Jm23m1 + R(2) * Jm23 * trE);
+    R fib = R(0);
+    if (Ea > R(0)) fib = R(2) * R(pp.k1) * Num<R>::exp_(R(pp.k2) * Ea * Ea) * Ea * kappa;
+    const R a = Jm23 * (mu + fib);
+    const R tr3 = R(2) * trE / R(3);
+    Sym<R> Tdev;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Tdev.v[k] = a * R(2) * E.v[k];
+    Tdev.v[0] -= a * tr3;
+    Tdev.v[1] -= a * tr3;
+    Tdev.v[2] -= a * tr3;
+    Sym<R> Sdev = sym_mul(Ac, Tdev, iJ2);
+    const R pvol = R(0.5) * K * Jm1 * (Jm1 + R(2)) * iJ2;
+    S.v[0] = Sdev.v[0] + pvol * Ac.v[0];
+    S.v[1] = Sdev.v[1] + pvol * Ac.v[1];
+    S.v[2] = Sdev.v[2] + pvol * Ac.v[2];
+    S.v[3] = Sdev.v[3] + pvol * Ac.v[3];
+    S.v[4] = Sdev.v[4] + pvol * Ac.v[4];
+    S.v[5] = Sdev.v[5] + pvol * Ac.v[5];
+    if (mat == 5 && hist) {
+      // HGOIsotropicViscoelastic.cpp:94-159: Sdev = S - (C:S)/3 C^-1 is the
+      // isochoric part above; H_j <- c1j H_j + c2j (Sdev - S0); S += H1 + H2.
+      const R c11 = R(vc.c11), c12 = R(vc.c12), c21 = R(vc.c21), c22 = R(vc.c22);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        R d = Sdev.v[k] - hist[12 + k];
+        R h1 = c11 * hist[k] + c21 * d;
+        R h2 = c12 * hist[6 + k] + c22 * d;
+        hist[k] = h1;
+        hist[6 + k] = h2;
+        hist[12 + k] = Sdev.v[k];
+        S.v[k] = S.v[k] + h1 + h2;
+      }
+    }
+    return true;
+  }
+  return false;
+}
+
+template <typename R>
+__device__ inline R norm_cross(const R a[3], const R b[3]) {
+  // normOfCrossProduct, math.cpp:30-43
+  R z = a[0] * b[1] - a[1] * b[0];
+  R x = a[1] * b[2] - a[2] * b[1];
+  R y = -a[0] * b[2] + a[2] * b[0];
+  return Num<R>::sqrt_(x * x + y * y + z * z);
+}
+
+template <typename R>
+__device__ inline R triple(const R s[3], const R a[3], const R b[3]) {
+  // tripleProduct, math.cpp:52-56
+  return s[2] * (a[0] * b[1] - a[1] * b[0]) + s[0] * (a[1] * b[2] - a[2] * b[1]) -
+         s[1] * (a[0] * b[2] - a[2] * b[0]);
+}
+
+// areaHexahedronFace (Geometry.cpp:29-64) from a face's twist cD and edge
+// vectors c1, c2 (each the quarter-sum of the reference's corner formula).
+template <typename R>
+__device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3]) {
+  const R tol = R(1e-6);
+  if ((cD[0] < tol) && (cD[1] < tol) && (cD[2] < tol)) return R(4) * norm_cross(c1, c2);
+  const R t = R(kFaceQ);
+  const R q[2] = {-t, t};
+  R area = R(0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      R v1[3], v2[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        v1[k] = q[j] * cD[k] + c1[k];
+        v2[k] = q[i] * cD[k] + c2[k];
+      }
+      area += norm_cross(v1, v2);
+    }
+  return area;
+}
+
+// Characteristic length V / max face area (CalculateCharacteristicLength_
+// C3D8.cpp:3-30) from the normalised Walsh-Hadamard coefficients of the
+// deformed position field: d[m][comp] for m = xi, eta, xi.eta, zeta,
+// xi.zeta, eta.zeta, xi.eta.zeta.  The reference's q0..q5 of
+// volumeHexahedron are 8 x (xi.eta, xi.zeta, xi, eta.zeta, eta, zeta).
+template <typename R>
+__device__ inline R char_length(const R d[7][3]) {
+  const R *c1 = d[0], *c2 = d[1], *c3 = d[2], *c4 = d[3], *c5 = d[4], *c6 = d[5], *c7 = d[6];
+  R q0[3], q1[3], q2[3], q3[3], q4[3], q5[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    q0[k] = R(8) * c3[k];
+    q1[k] = R(8) * c5[k];
+    q2[k] = R(8) * c1[k];
+    q3[k] = R(8) * c6[k];
+    q4[k] = R(8) * c2[k];
+    q5[k] = R(8) * c4[k];
+  }
+  R vol = (triple(q0, q4, q3) + triple(q2, q0, q1) + triple(q1, q3, q5)) / R(192) +
+          triple(q2, q4, q5) / R(64);
+  // Faces in the reference's order (CalculateCharacteristicLength_C3D8.cpp:12-13):
+  // zeta-, zeta+, xi-, xi+, eta-, eta+.
+  R amax = R(0);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    R cD[3], e1[3], e2[3];
+    const R s = (f & 1) ? R(1) : R(-1);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (f < 2) {
+        cD[k] = c3[k] + s * c7[k];
+        e1[k] = c1[k] + s * c5[k];
+        e2[k] = c2[k] + s * c6[k];
+      } else if (f < 4) {
+        cD[k] = c6[k] + s * c7[k];
+        e1[k] = c2[k] + s * c3[k];
+        e2[k] = c4[k] + s * c5[k];
+      } else {
+        cD[k] = c5[k] + s * c7[k];
+        e1[k] = c1[k] + s * c3[k];
+        e2[k] = c4[k] + s * c6[k];
+      }
+    }
+    R fa = face_area(cD, e1, e2);
+    if (fa > amax) amax = fa;
+  }
+  return vol / amax;
+}
+
+}  // namespace ft

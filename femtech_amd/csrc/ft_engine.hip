@@ -1,0 +1,1403 @@
+// ft_engine.hip -- MI355X kernels and the C ABI of the explicit C3D8 engine.
+//
+// Kernels (one GPU holds the whole mesh, so the reference's ParMETIS
+// partition and halo exchange collapse to a single partition):
+//   k_element  fused C3D8 element kernel: gradients, F, PK2 stress, 24 nodal
+//              forces written once per element, and the element's stable time
+//              step folded into a block minimum + one integer atomicMin
+//              (GetForce_3D.cpp:15-46, StableTimeStep.cpp:11-33).
+//   k_nodal    node-to-element CSR gather of the element forces in element
+//              order (deterministic, no float atomics), a = f/m, velocity and
+//              displacement updates, boundary motions, and the FP64 energy
+//              balance (ex9.cpp:116-152, CalculateAcclerations.cpp:7-11,
+//              CheckEnergy.cpp:15-66).
+//   k_mass     one-off lumped mass (Mass3D.cpp:127-157) via the same gather.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/femtech_amd.h"
+#include "ft_device.h"
+
+using namespace ft;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+}  // namespace
+
+// Shared with the host-side mesh tools (ft_mesh.cpp).
+extern "C" __attribute__((visibility("hidden"))) void ft_internal_set_error(const char *msg) {
+  g_err = msg;
+}
+
+namespace {
+
+#define HIPCHK(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(FT_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                             \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr uint64_t kInfBits = 0x7FF0000000000000ull;
+constexpr int kSkipBit = 1 << 30;
+constexpr int kPidMask = (1 << 20) - 1;
+constexpr int kMaxMotions = 64;
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks one contiguous range of elements or nodes (its own L2 then sees the
+// shared nodes of neighbouring elements).  gridDim.x is a multiple of 8.
+__device__ inline int xcd_block() {
+  int per = gridDim.x >> 3;
+  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+int grid_for(long n) {
+  long nb = (n + kBlock - 1) / kBlock;
+  nb = (nb + 7) & ~7L;
+  return (int)std::max(nb, 8L);
+}
+
+struct StepSlot {
+  double t;        // t_k
+  double t_prev;   // t_{k-1}
+  double dt;       // dt_k, the size of step k
+  unsigned long long dtmin_bits;  // min_e Le/c at u_k (order-preserving bits)
+  double t_done;   // time of the last completed step
+  double dt_done;  // dt computed at its end (the next step's size)
+  int step;        // k: step in progress (done == 0) or last completed (done == 1)
+  int done;        // 1 when the run has stopped at the end of step k
+  int fresh;       // 1 if the last nodal launch completed a step
+  int pad;
+};
+
+struct Ctl {
+  StepSlot slot[2];
+  int status;
+  int hist_count;
+  int pad[2];
+};
+
+__device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+template <typename T>
+__device__ inline T block_min(T v, T *sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  T r = sh[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = fmin(r, sh[i]);
+  return r;
+}
+
+// Fixed-order block sum (same tree every launch -> bitwise reproducible).
+__device__ inline double block_sum(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary motions (the reference drivers' ApplyBoundaryConditions, as data)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void motion_eval(const double *mo, const double *X, double t,
+                                            double u[3], double v[3], double a[3]) {
+  int type = (int)mo[0];
+  for (int d = 0; d < 3; ++d) u[d] = v[d] = a[d] = 0.0;
+  if (type == FT_MOTION_LINEAR) {  // ex9.cpp:191-243: AppliedDisp = Time*dMax/tMax
+    for (int d = 0; d < 3; ++d) {
+      u[d] = t * mo[1];
+      v[d] = mo[1];
+    }
+  } else if (type == FT_MOTION_RAMP_HOLD) {
+    double amp = mo[1], T = mo[2];
+    double s = t < T ? t / T : 1.0;
+    for (int d = 0; d < 3; ++d) {
+      u[d] = amp * s;
+      v[d] = t < T ? amp / T : 0.0;
+    }
+  } else if (type == FT_MOTION_ROT_HALFSINE) {
+    const double n0 = mo[1], n1 = mo[2], n2 = mo[3];
+    const double A = mo[7], T = mo[8];
+    const double pi = 3.14159265358979323846;
+    double alpha, omega, theta;
+    if (t <= T) {
+      alpha = A * sin(pi * t / T);
+      omega = A * T / pi * (1.0 - cos(pi * t / T));
+      theta = A * T / pi * (t - T / pi * sin(pi * t / T));
+    } else {
+      alpha = 0.0;
+      omega = 2.0 * A * T / pi;
+      theta = A * T * T / pi + omega * (t - T);
+    }
+    double r0[3] = {X[0] - mo[4], X[1] - mo[5], X[2] - mo[6]};
+    double ct = cos(theta), st = sin(theta);
+    double nxr0[3] = {n1 * r0[2] - n2 * r0[1], n2 * r0[0] - n0 * r0[2], n0 * r0[1] - n1 * r0[0]};
+    double ndr0 = n0 * r0[0] + n1 * r0[1] + n2 * r0[2];
+    const double n[3] = {n0, n1, n2};
+    double r[3];
+    for (int d = 0; d < 3; ++d) r[d] = r0[d] * ct + nxr0[d] * st + n[d] * ndr0 * (1.0 - ct);
+    double nxr[3] = {n1 * r[2] - n2 * r[1], n2 * r[0] - n0 * r[2], n0 * r[1] - n1 * r[0]};
+    double nxnxr[3] = {n1 * nxr[2] - n2 * nxr[1], n2 * nxr[0] - n0 * nxr[2],
+                       n0 * nxr[1] - n1 * nxr[0]};
+    for (int d = 0; d < 3; ++d) {
+      u[d] = r[d] - r0[d];
+      v[d] = omega * nxr[d];
+      a[d] = alpha * nxr[d] + omega * omega * nxnxr[d];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element kernel
+// ---------------------------------------------------------------------------
+template <typename R>
+struct ElemArgs {
+  int ne;
+  const int *conn;       // 8 per element
+  const int *einfo;      // part id | skip-dt flag
+  const double *X;       // 3 per node
+  const double *u;       // 3 per node
+  const PartDev *parts;
+  R *fe;                 // 24 per element (nullable when only dt is wanted)
+  R *hist;               // 144 per element, component-major [c * ne + e]
+  const StepSlot *slot;  // nullable: step slot giving dt_k and done
+  double dt_fixed;       // used when slot == nullptr
+  unsigned long long *dtmin_bits;  // nullable
+  int *status;
+  int want_force;
+  int want_dt;
+  int update_hist;
+};
+
+template <typename R, bool FORCE, bool DT>
+__global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
+  __shared__ double sh_min[kBlock / 64];
+  const int L = xcd_block();
+  const int e = L * kBlock + threadIdx.x;
+  double dt_step = a.dt_fixed;
+  if (a.slot) {
+    if (a.slot->done) return;  // uniform over the grid
+    dt_step = a.slot->dt;
+  }
+  double my_dt = 1e300;
+  int flags = 0;
+  if (e < a.ne) {
+    const int4 c0 = reinterpret_cast<const int4 *>(a.conn)[2 * e];
+    const int4 c1 = reinterpret_cast<const int4 *>(a.conn)[2 * e + 1];
+    const int nd[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    const int info = a.einfo[e];
+    const PartDev pp = a.parts[info & kPidMask];
+    // Node-0-relative coordinates and displacements, subtracted in FP64 and
+    // then rounded to R (keeps the relative precision of the element's own
+    // geometry, not of its absolute position).
+    double X0[3], U0[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      X0[d] = a.X[3 * nd[0] + d];
+      U0[d] = a.u[3 * nd[0] + d];
+    }
+    R xv[3][8], uv[3][8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int n = nd[p2n(p)];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        xv[d][p] = p == 0 ? R(0) : R(a.X[3 * n + d] - X0[d]);
+        uv[d][p] = p == 0 ? R(0) : R(a.u[3 * n + d] - U0[d]);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      wht8(xv[d]);
+      wht8(uv[d]);
+    }
+    // Normalised coefficients c[m][d], m = xi, eta, xi.eta, zeta, xi.zeta,
+    // eta.zeta, xi.eta.zeta (WHT index m + 1).
+    R cx[7][3], cu[7][3];
+#pragma unroll
+    for (int m = 0; m < 7; ++m)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cx[m][d] = xv[d][m + 1] * R(0.125);
+        cu[m][d] = uv[d][m + 1] * R(0.125);
+      }
+    if (FORCE) {
+      ViscoCoef vc = {0, 0, 0, 0};
+      if (pp.mat == 5) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
+        const double rt1 = dt_step / pp.t1, rt2 = dt_step / pp.t2;
+        vc.c11 = exp(-rt1);
+        vc.c12 = exp(-rt2);
+        vc.c21 = pp.g1 * (1 - vc.c11) / rt1;
+        vc.c22 = pp.g2 * (1 - vc.c12) / rt2;
+      }
+      const R g = R(kGp), g2 = R(kGp * kGp);
+      // Force accumulators in the same basis: h[d][m].
+      R h[3][7];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < 7; ++m) h[d][m] = R(0);
+      // Gauss points in the reference's order (GaussQuadrature3D.cpp:19-49).
+#pragma unroll
+      for (int gp = 0; gp < 8; ++gp) {
+        const R sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? R(1) : R(-1);
+        const R sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? R(1) : R(-1);
+        const R sz = gp < 4 ? R(1) : R(-1);
+        // J[d][k] = d(field_d)/d(xi_k) at this Gauss point.
+        R J0[3][3], Ju[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          J0[d][0] = cx[0][d] + (sy * g) * cx[2][d] + (sz * g) * cx[4][d] + (sy * sz * g2) * cx[6][d];
+          J0[d][1] = cx[1][d] + (sx * g) * cx[2][d] + (sz * g) * cx[5][d] + (sx * sz * g2) * cx[6][d];
+          J0[d][2] = cx[3][d] + (sx * g) * cx[4][d] + (sy * g) * cx[5][d] + (sx * sy * g2) * cx[6][d];
+          Ju[d][0] = cu[0][d] + (sy * g) * cu[2][d] + (sz * g) * cu[4][d] + (sy * sz * g2) * cu[6][d];
+          Ju[d][1] = cu[1][d] + (sx * g) * cu[2][d] + (sz * g) * cu[5][d] + (sx * sz * g2) * cu[6][d];
+          Ju[d][2] = cu[3][d] + (sx * g) * cu[4][d] + (sy * g) * cu[5][d] + (sx * sy * g2) * cu[6][d];
+        }
+        // adj(J0): J0^-1 = A / det0 (ShapeFunction_C3D8.cpp:95-123)
+        R A[3][3];
+        A[0][0] = J0[1][1] * J0[2][2] - J0[1][2] * J0[2][1];
+        A[0][1] = J0[0][2] * J0[2][1] - J0[0][1] * J0[2][2];
+        A[0][2] = J0[0][1] * J0[1][2] - J0[0][2] * J0[1][1];
+        A[1][0] = J0[1][2] * J0[2][0] - J0[1][0] * J0[2][2];
+        A[1][1] = J0[0][0] * J0[2][2] - J0[0][2] * J0[2][0];
+        A[1][2] = J0[0][2] * J0[1][0] - J0[0][0] * J0[1][2];
+        A[2][0] = J0[1][0] * J0[2][1] - J0[1][1] * J0[2][0];
+        A[2][1] = J0[0][1] * J0[2][0] - J0[0][0] * J0[2][1];
+        A[2][2] = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
+        const R det0 = J0[0][0] * A[0][0] + J0[0][1] * A[1][0] + J0[0][2] * A[2][0];
+        if (!(det0 > R(0))) flags |= FT_STATUS_INVERTED;
+        const R idet0 = R(1) / det0;
+        // H = Ju J0^-1 = grad_X u  (F = I + H, CalculateDeformationGradient.cpp:15-25)
+        R H[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
+        R hloc[18];
+        R *hp = nullptr;
+        if (pp.mat == 5 && a.hist) {
+#pragma unroll
+          for (int k = 0; k < 18; ++k) hloc[k] = a.hist[(size_t)(gp * 18 + k) * a.ne + e];
+          hp = hloc;
+        }
+        Sym<R> S;
+        R Jm1;
+        if (!pk2_stress(pp, H, hp, vc, S, Jm1)) flags |= 8;
+        if (!(Jm1 > R(-1))) flags |= FT_STATUS_INVERTED;
+        if (hp && a.update_hist) {
+#pragma unroll
+          for (int k = 0; k < 18; ++k) a.hist[(size_t)(gp * 18 + k) * a.ne + e] = hloc[k];
+        }
+        // P = F S (first Piola), Q = w detJ0 P J0^-T = P A^T  (w = 1)
+        R P[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            P[i][j] = sym_at(S, i, j) + H[i][0] * sym_at(S, 0, j) + H[i][1] * sym_at(S, 1, j) +
+                      H[i][2] * sym_at(S, 2, j);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const R q0 = P[i][0] * A[0][0] + P[i][1] * A[0][1] + P[i][2] * A[0][2];
+          const R q1 = P[i][0] * A[1][0] + P[i][1] * A[1][1] + P[i][2] * A[1][2];
+          const R q2 = P[i][0] * A[2][0] + P[i][1] * A[2][1] + P[i][2] * A[2][2];
+          // f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k in the monomial basis
+          h[i][0] += q0;
+          h[i][1] += q1;
+          h[i][3] += q2;
+          h[i][2] += sy * q0 + sx * q1;
+          h[i][4] += sz * q0 + sx * q2;
+          h[i][5] += sz * q1 + sy * q2;
+          h[i][6] += (sy * sz) * q0 + (sx * sz) * q1 + (sx * sy) * q2;
+        }
+      }
+      // Back to the nodes: f_p = sum_q c_q prod(signs), c = h scaled by
+      // 1/8 (linear), g/8 (bilinear), g^2/8 (trilinear).
+      R fo[3][8];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        R c[8];
+        c[0] = R(0);
+        c[1] = h[d][0] * R(0.125);
+        c[2] = h[d][1] * R(0.125);
+        c[4] = h[d][3] * R(0.125);
+        c[3] = h[d][2] * (g * R(0.125));
+        c[5] = h[d][4] * (g * R(0.125));
+        c[6] = h[d][5] * (g * R(0.125));
+        c[7] = h[d][6] * (g2 * R(0.125));
+        iwht8(c);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) fo[d][p] = c[p];
+      }
+      R outv[24];
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        const int nl = p2n(p);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) outv[3 * nl + d] = fo[d][p];
+      }
+      bool finite = true;
+#pragma unroll
+      for (int k = 0; k < 24; ++k) finite = finite && isfinite(outv[k]);
+      if (!finite) flags |= FT_STATUS_NONFINITE;
+      if (sizeof(R) == 4) {
+        float4 *dst = reinterpret_cast<float4 *>(a.fe + (size_t)24 * e);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          dst[k] = make_float4(outv[4 * k], outv[4 * k + 1], outv[4 * k + 2], outv[4 * k + 3]);
+      } else {
+        double2 *dst = reinterpret_cast<double2 *>(a.fe + (size_t)24 * e);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) dst[k] = make_double2(outv[2 * k], outv[2 * k + 1]);
+      }
+    }
+    if (DT && !(info & kSkipBit)) {
+      // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
+      R dcf[7][3];
+#pragma unroll
+      for (int m = 0; m < 7; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
+      const R le = char_length(dcf);
+      const double dte = (double)le / pp.ce;
+      if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
+    }
+  }
+  if (DT) {
+    double bmin = block_min(my_dt, sh_min);
+    if (threadIdx.x == 0 && bmin < 1e300)
+      atomicMin(a.dtmin_bits, (unsigned long long)__double_as_longlong(bmin));
+  }
+  if (flags) atomicOr(a.status, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Node kernels
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ inline void gather3(const int *ptr, const int *idx, const R *fe, int n, double f[3]) {
+  f[0] = f[1] = f[2] = 0.0;
+  const int b = ptr[n], end = ptr[n + 1];
+  for (int k = b; k < end; ++k) {
+    const size_t o = (size_t)idx[k] * 3;
+    f[0] += (double)fe[o];
+    f[1] += (double)fe[o + 1];
+    f[2] += (double)fe[o + 2];
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gather(int nn, const int *ptr, const int *idx,
+                                                   const R *fe, double *fi) {
+  const int n = xcd_block() * kBlock + threadIdx.x;
+  if (n >= nn) return;
+  double f[3];
+  gather3(ptr, idx, fe, n, f);
+  fi[3 * n] = f[0];
+  fi[3 * n + 1] = f[1];
+  fi[3 * n + 2] = f[2];
+}
+
+enum NodalMode { kInitial = 0, kStep = 1, kResume = 2 };
+
+template <typename R>
+struct NodalArgs {
+  int nn;
+  const int *ptr, *idx;
+  const R *fe;
+  const double *X;
+  double *u, *vs, *fi;
+  const double *mass;  // 1 per node
+  const signed char *dof_motion;
+  const double *motions;
+  Ctl *ctl;
+  int parity;
+  int mode;
+  int energy;
+  double *eacc;  // 4 per logical block: W_int, W_ext, KE, unused
+  double tmax, factor, failure_dt;
+  int max_steps;
+};
+
+// End of step k (gather, a = f/m, v = v_half + dt/2 a, energy) fused with the
+// start of step k+1 (v_half, u += dt v_half, boundary motions at t_{k+1}),
+// ex9.cpp:107-175.  vs holds v_half(k) for free DOFs and the prescribed v
+// for constrained ones while a run is in progress, v(k) once it stops.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
+  __shared__ double sh[kBlock / 64];
+  const StepSlot cur = a.ctl->slot[a.parity];
+  if (cur.done && a.mode != kResume) {
+    // keep both ping-pong slots stopped so later launches stay no-ops
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      StepSlot nx = cur;
+      nx.fresh = 0;
+      a.ctl->slot[a.parity ^ 1] = nx;
+    }
+    return;
+  }
+  const int L = xcd_block();
+  const int n = L * kBlock + threadIdx.x;
+  const double t_k = cur.t, t_km1 = cur.t_prev, dt_k = cur.dt;
+  const double dtmin = bits_to_double(cur.dtmin_bits);
+  const double dt_next = a.factor * dtmin;
+  const bool collapse = !(dtmin >= a.failure_dt);
+  const bool go = (t_k < a.tmax) && (cur.step < a.max_steps) && !collapse;
+  double sWint = 0.0, sWext = 0.0, KE = 0.0;
+  if (n < a.nn) {
+    double f[3];
+    if (a.mode == kResume) {
+      f[0] = a.fi[3 * n];
+      f[1] = a.fi[3 * n + 1];
+      f[2] = a.fi[3 * n + 2];
+    } else {
+      gather3(a.ptr, a.idx, a.fe, n, f);
+    }
+    const double m = a.mass[n];
+    const double *Xn = a.X + 3 * (size_t)n;  // read only for constrained DOFs
+    const double t_khalf = 0.5 * (t_k + t_km1);
+    const double t_k1 = t_k + dt_next;
+    const double t_k1half = 0.5 * (t_k1 + t_k);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int mi = a.dof_motion[3 * n + d];
+      const double uo = a.u[3 * n + d];
+      const double vso = a.vs[3 * n + d];
+      double acc, v;
+      double ub[3], vb[3], ab[3];
+      if (mi < 0) {
+        acc = (0.0 - f[d]) / m;
+        v = (a.mode == kStep) ? vso + (t_k - t_khalf) * acc : vso;
+      } else {
+        motion_eval(a.motions + FT_MOTION_STRIDE * mi, Xn, t_k, ub, vb, ab);
+        acc = ab[d];
+        v = vb[d];
+      }
+      if (a.mode == kStep && a.energy) {
+        // CheckEnergy.cpp:34-46 with Delta u = dt v_half for free DOFs
+        const double fp = a.fi[3 * n + d];
+        double dd;
+        if (mi < 0) {
+          dd = dt_k * vso;
+        } else {
+          double up[3], vp[3], ap[3];
+          motion_eval(a.motions + FT_MOTION_STRIDE * mi, Xn, t_km1, up, vp, ap);
+          dd = ub[d] - up[d];
+          sWext += dd * (fp + f[d] + m * (acc + ap[d]));
+        }
+        KE += m * v * v;
+        sWint += dd * (fp + f[d]);
+      }
+      if (a.mode != kResume) a.fi[3 * n + d] = f[d];
+      if (go) {
+        if (mi < 0) {
+          const double vh = v + (t_k1half - t_k) * acc;
+          a.vs[3 * n + d] = vh;
+          a.u[3 * n + d] = uo + dt_next * vh;
+        } else {
+          motion_eval(a.motions + FT_MOTION_STRIDE * mi, Xn, t_k1, ub, vb, ab);
+          a.u[3 * n + d] = ub[d];
+          a.vs[3 * n + d] = vb[d];
+        }
+      } else {
+        a.vs[3 * n + d] = v;
+      }
+    }
+  }
+  if (a.mode == kStep && a.energy) {
+    const double w = block_sum(sWint, sh);
+    const double x = block_sum(sWext, sh);
+    const double k = block_sum(KE, sh);
+    if (threadIdx.x == 0) {
+      a.eacc[4 * L + 0] += 0.5 * w;
+      a.eacc[4 * L + 1] += 0.5 * x;
+      a.eacc[4 * L + 2] = 0.5 * k;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    StepSlot nx;
+    if (go) {
+      nx.t = t_k + dt_next;
+      nx.t_prev = t_k;
+      nx.dt = dt_next;
+      nx.dtmin_bits = kInfBits;
+      nx.step = cur.step + 1;
+      nx.done = 0;
+    } else {
+      nx = cur;
+      nx.done = 1;
+      if (collapse && cur.step < a.max_steps && t_k < a.tmax) atomicOr(&a.ctl->status, FT_STATUS_DT_COLLAPSE);
+    }
+    nx.t_done = t_k;
+    nx.dt_done = dt_next;
+    nx.fresh = a.mode == kStep;
+    nx.pad = 0;
+    a.ctl->slot[a.parity ^ 1] = nx;
+  }
+}
+
+// Per-step history row (time, dt, W_int, W_ext, KE, |KE + W_int - W_ext|):
+// sums the per-block energy accumulators in block order.
+__global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, double *rows,
+                          int cap) {
+  __shared__ double sh[kBlock / 64];
+  double w = 0, x = 0, k = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    w += eacc[4 * b];
+    x += eacc[4 * b + 1];
+    k += eacc[4 * b + 2];
+  }
+  w = block_sum(w, sh);
+  x = block_sum(x, sh);
+  k = block_sum(k, sh);
+  const StepSlot s = ctl->slot[parity];
+  if (threadIdx.x == 0 && s.fresh) {
+    const int i = ctl->hist_count;
+    if (i < cap) {
+      double *row = rows + 6 * (size_t)i;
+      row[0] = s.t_done;
+      row[1] = s.dt_done;
+      row[2] = w;
+      row[3] = x;
+      row[4] = k;
+      row[5] = fabs(k + w - x);
+    }
+    ctl->hist_count = i + 1;
+  }
+}
+
+// Lumped mass: per-element row sums (Mass3D.cpp:5-67,134-144) ...
+__global__ __launch_bounds__(kBlock) void k_mass_elem(int ne, const int *conn, const int *einfo,
+                                                      const double *X, const PartDev *parts,
+                                                      double *me) {
+  const int e = xcd_block() * kBlock + threadIdx.x;
+  if (e >= ne) return;
+  int nd[8];
+  for (int k = 0; k < 8; ++k) nd[k] = conn[8 * e + k];
+  const double rho = parts[einfo[e] & kPidMask].rho;
+  double xv[3][8];
+  for (int p = 0; p < 8; ++p)
+    for (int d = 0; d < 3; ++d) xv[d][p] = X[3 * nd[p2n(p)] + d] - X[3 * nd[0] + d];
+  for (int d = 0; d < 3; ++d) wht8(xv[d]);
+  double c[7][3];
+  for (int m = 0; m < 7; ++m)
+    for (int d = 0; d < 3; ++d) c[m][d] = xv[d][m + 1] * 0.125;
+  double m8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double g = kGp, g2 = kGp * kGp;
+  for (int gp = 0; gp < 8; ++gp) {
+    const double sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? 1.0 : -1.0;
+    const double sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? 1.0 : -1.0;
+    const double sz = gp < 4 ? 1.0 : -1.0;
+    double J[3][3];
+    for (int d = 0; d < 3; ++d) {
+      J[d][0] = c[0][d] + (sy * g) * c[2][d] + (sz * g) * c[4][d] + (sy * sz * g2) * c[6][d];
+      J[d][1] = c[1][d] + (sx * g) * c[2][d] + (sz * g) * c[5][d] + (sx * sz * g2) * c[6][d];
+      J[d][2] = c[3][d] + (sx * g) * c[4][d] + (sy * g) * c[5][d] + (sx * sy * g2) * c[6][d];
+    }
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    // row sum of N_a N_b over b is N_a (partition of unity)
+    for (int a = 0; a < 8; ++a) {
+      const double sa = (a == 1 || a == 2 || a == 5 || a == 6) ? 1.0 : -1.0;
+      const double ta = (a == 2 || a == 3 || a == 6 || a == 7) ? 1.0 : -1.0;
+      const double ua = a >= 4 ? 1.0 : -1.0;
+      const double Na = (1 + sa * sx * g) * (1 + ta * sy * g) * (1 + ua * sz * g) / 8;
+      m8[a] += Na * det;
+    }
+  }
+  for (int a = 0; a < 8; ++a) me[8 * e + a] = rho * m8[a];
+}
+
+// ... gathered per node in element order.
+__global__ __launch_bounds__(kBlock) void k_mass_gather(int nn, const int *ptr, const int *idx,
+                                                        const double *me, double *mass) {
+  const int n = xcd_block() * kBlock + threadIdx.x;
+  if (n >= nn) return;
+  double s = 0.0;
+  for (int k = ptr[n]; k < ptr[n + 1]; ++k) s += me[idx[k]];
+  mass[n] = s;
+}
+
+// Per-element skip flag of StableTimeStep.cpp:12-20 from per-node masks.
+__global__ void k_elem_flags(int ne, const int *conn, const unsigned char *node_mask, int *einfo) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  bool all = true;
+  for (int k = 0; k < 8; ++k) all = all && (node_mask[conn[8 * e + k]] == 7);
+  int v = einfo[e] & ~kSkipBit;
+  einfo[e] = all ? (v | kSkipBit) : v;
+}
+
+// Velocity / acceleration read-out for ft_get_state.
+__global__ void k_state_out(int nn, const double *fi, const double *vs, const double *mass,
+                            const double *X, const signed char *dof_motion, const double *motions,
+                            double t, double *a_out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  for (int d = 0; d < 3; ++d) {
+    const int mi = dof_motion[3 * n + d];
+    if (mi < 0) {
+      a_out[3 * n + d] = (0.0 - fi[3 * n + d]) / mass[n];
+    } else {
+      double ub[3], vb[3], ab[3];
+      const double Xn[3] = {X[3 * n], X[3 * n + 1], X[3 * n + 2]};
+      motion_eval(motions + FT_MOTION_STRIDE * mi, Xn, t, ub, vb, ab);
+      a_out[3 * n + d] = ab[d];
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+struct ft_engine {
+  int nn = 0, ne = 0, nparts = 0, precision = FT_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // mesh
+  double *X = nullptr;
+  int *conn = nullptr, *einfo = nullptr, *ptr = nullptr, *idx = nullptr;
+  PartDev *parts = nullptr;
+  std::vector<PartDev> h_parts;
+  bool has_visco = false;
+  // state
+  double *u = nullptr, *vs = nullptr, *fi = nullptr, *mass = nullptr, *tmp3 = nullptr;
+  void *fe = nullptr;
+  void *hist = nullptr;
+  signed char *dof_motion = nullptr;
+  unsigned char *node_mask = nullptr;
+  double *motions = nullptr;
+  int n_motions = 0;
+  std::vector<int> h_dof_motion;
+  std::vector<double> h_motions;
+  Ctl *ctl = nullptr;
+  unsigned long long *dtbits = nullptr;
+  int *status = nullptr;
+  double *eacc = nullptr;
+  int nodal_grid = 0;
+  bool mass_ready = false;
+  // run
+  bool run_active = false;
+  int parity = 0;
+  double tmax = 0, factor = 0.8, failure_dt = 1e-11;
+  // history
+  int hist_cap = 0, hist_n = 0;
+  double *hist_rows = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  std::vector<hipEvent_t> ev_pool;
+  double ms[3] = {0, 0, 0};
+  int launches[3] = {0, 0, 0};
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(FT_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                hipGetErrorString(e));
+  return 0;
+}
+
+hipEvent_t take_event(ft_engine *g) {
+  if (!g->ev_pool.empty()) {
+    hipEvent_t e = g->ev_pool.back();
+    g->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Timed {
+  ft_engine *g;
+  int cls;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(ft_engine *g_, int c) : g(g_), cls(c) {
+    if (g->timing) {
+      a = take_event(g);
+      b = take_event(g);
+      if (a) (void)hipEventRecord(a, g->stream);
+    }
+  }
+  ~Timed() {
+    if (g->timing && a && b) {
+      (void)hipEventRecord(b, g->stream);
+      g->ev[cls].push_back({a, b});
+    }
+  }
+};
+
+int collect_timing(ft_engine *g) {
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (int c = 0; c < 3; ++c) {
+    for (auto &pr : g->ev[c]) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+      g->ms[c] += ms;
+      g->launches[c] += 1;
+      g->ev_pool.push_back(pr.first);
+      g->ev_pool.push_back(pr.second);
+    }
+    g->ev[c].clear();
+  }
+  return 0;
+}
+
+template <typename R, bool FORCE, bool DT>
+int launch_element(ft_engine *g, const StepSlot *slot, double dt_fixed, unsigned long long *dtbits,
+                   int update_hist) {
+  ElemArgs<R> a;
+  a.ne = g->ne;
+  a.conn = g->conn;
+  a.einfo = g->einfo;
+  a.X = g->X;
+  a.u = g->u;
+  a.parts = g->parts;
+  a.fe = reinterpret_cast<R *>(g->fe);
+  a.hist = reinterpret_cast<R *>(g->hist);
+  a.slot = slot;
+  a.dt_fixed = dt_fixed;
+  a.dtmin_bits = dtbits;
+  a.status = g->status;
+  a.want_force = FORCE;
+  a.want_dt = DT;
+  a.update_hist = update_hist;
+  Timed t(g, 0);
+  hipLaunchKernelGGL((k_element<R, FORCE, DT>), dim3(grid_for(g->ne)), dim3(kBlock), 0, g->stream, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int element_pass(ft_engine *g, bool force, bool dt, const StepSlot *slot, double dt_fixed,
+                 unsigned long long *dtbits, int update_hist) {
+  if (g->precision == FT_FP32) {
+    if (force && dt) return launch_element<float, true, true>(g, slot, dt_fixed, dtbits, update_hist);
+    if (force) return launch_element<float, true, false>(g, slot, dt_fixed, dtbits, update_hist);
+    return launch_element<float, false, true>(g, slot, dt_fixed, dtbits, update_hist);
+  }
+  if (force && dt) return launch_element<double, true, true>(g, slot, dt_fixed, dtbits, update_hist);
+  if (force) return launch_element<double, true, false>(g, slot, dt_fixed, dtbits, update_hist);
+  return launch_element<double, false, true>(g, slot, dt_fixed, dtbits, update_hist);
+}
+
+template <typename R>
+int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
+  NodalArgs<R> a;
+  a.nn = g->nn;
+  a.ptr = g->ptr;
+  a.idx = g->idx;
+  a.fe = reinterpret_cast<const R *>(g->fe);
+  a.X = g->X;
+  a.u = g->u;
+  a.vs = g->vs;
+  a.fi = g->fi;
+  a.mass = g->mass;
+  a.dof_motion = g->dof_motion;
+  a.motions = g->motions;
+  a.ctl = g->ctl;
+  a.parity = g->parity;
+  a.mode = mode;
+  a.energy = 1;
+  a.eacc = g->eacc;
+  a.tmax = g->tmax;
+  a.factor = g->factor;
+  a.failure_dt = g->failure_dt;
+  a.max_steps = max_steps;
+  Timed t(g, 1);
+  hipLaunchKernelGGL((k_nodal<R>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int launch_nodal(ft_engine *g, int mode, int max_steps) {
+  int rc = g->precision == FT_FP32 ? launch_nodal_t<float>(g, mode, max_steps)
+                                   : launch_nodal_t<double>(g, mode, max_steps);
+  if (rc) return rc;
+  g->parity ^= 1;
+  return 0;
+}
+
+int upload_u(ft_engine *g, const double *u) {
+  if (u) HIPCHK(hipMemcpyAsync(g->u, u, 3 * (size_t)g->nn * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+int check_status(ft_engine *g) {
+  int st = 0;
+  HIPCHK(hipMemcpyAsync(&st, g->status, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (st & 8) return fail(FT_ERR_MATERIAL, "unknown material type");
+  return 0;
+}
+
+int read_slot(ft_engine *g, int parity, StepSlot *s) {
+  HIPCHK(hipMemcpyAsync(s, &g->ctl->slot[parity], sizeof(StepSlot), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ft_last_error(void) { return g_err.c_str(); }
+int ft_version(void) { return 1; }
+
+int ft_device_count(int *count) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  if (count) *count = c;
+  return 0;
+}
+
+int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *conn,
+                     const int *pid, int n_parts, const int *material_id, const double *props,
+                     int precision, ft_engine **out) {
+  if (!out) return fail(FT_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (n_nodes <= 0 || n_elems <= 0 || n_parts <= 0 || !coords || !conn || !pid || !material_id ||
+      !props)
+    return fail(FT_ERR_ARG, "invalid mesh arguments");
+  if (precision != FT_FP64 && precision != FT_FP32) return fail(FT_ERR_ARG, "bad precision");
+  if ((long)n_elems * 24 > (long)INT32_MAX / 1 || n_parts > kPidMask)
+    return fail(FT_ERR_ARG, "mesh too large");
+  for (long i = 0; i < 8L * n_elems; ++i)
+    if (conn[i] < 0 || conn[i] >= n_nodes) return fail(FT_ERR_INPUT, "connectivity index out of range at %ld", i);
+  for (int e = 0; e < n_elems; ++e)
+    if (pid[e] < 0 || pid[e] >= n_parts) return fail(FT_ERR_INPUT, "part id out of range at element %d", e);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(FT_ERR_HIP, "no HIP device available");
+  ft_engine *g = new ft_engine;
+  g->nn = n_nodes;
+  g->ne = n_elems;
+  g->nparts = n_parts;
+  g->precision = precision;
+  HIPCHK(hipGetDevice(&g->device));
+  int rc = 0;
+  auto bail = [&](int code) {
+    ft_engine_destroy(g);
+    return code;
+  };
+  if (hipStreamCreate(&g->stream) != hipSuccess) return bail(fail(FT_ERR_HIP, "stream creation failed"));
+  // parts (CalculateTimeStep.cpp:13-19 wave speed kept in FP64)
+  g->h_parts.resize(n_parts);
+  for (int p = 0; p < n_parts; ++p) {
+    PartDev &q = g->h_parts[p];
+    memset(&q, 0, sizeof(q));
+    const double *pr = props + 9 * (size_t)p;
+    q.mat = material_id[p];
+    if (q.mat < 0 || q.mat > 5) return bail(fail(FT_ERR_MATERIAL, "Unknown material type %d for part %d", q.mat, p));
+    q.visco = q.mat == 5;
+    q.rho = pr[0];
+    q.mu = pr[1];
+    q.lambda = pr[2];
+    q.k1 = pr[3];
+    q.k2 = pr[4];
+    q.g1 = pr[5];
+    q.t1 = pr[6];
+    q.g2 = pr[7];
+    q.t2 = pr[8];
+    double nu = 0.5 * q.lambda / (q.lambda + q.mu);
+    q.ce = sqrt(q.lambda * (1.0 / nu - 1.0) / q.rho);
+    if (q.visco) g->has_visco = true;
+  }
+  const size_t nd = 3 * (size_t)n_nodes;
+  if ((rc = dalloc(&g->X, nd)) || (rc = dalloc(&g->conn, 8 * (size_t)n_elems)) ||
+      (rc = dalloc(&g->einfo, (size_t)n_elems)) || (rc = dalloc(&g->parts, (size_t)n_parts)) ||
+      (rc = dalloc(&g->u, nd)) || (rc = dalloc(&g->vs, nd)) || (rc = dalloc(&g->fi, nd)) ||
+      (rc = dalloc(&g->tmp3, nd)) || (rc = dalloc(&g->mass, (size_t)n_nodes)) ||
+      (rc = dalloc(&g->dof_motion, nd)) || (rc = dalloc(&g->node_mask, (size_t)n_nodes)) ||
+      (rc = dalloc(&g->motions, (size_t)FT_MOTION_STRIDE * kMaxMotions)) ||
+      (rc = dalloc(&g->ctl, 1)) || (rc = dalloc(&g->dtbits, 1)) || (rc = dalloc(&g->status, 1)) ||
+      (rc = dalloc(&g->ptr, (size_t)n_nodes + 1)) || (rc = dalloc(&g->idx, 8 * (size_t)n_elems)))
+    return bail(rc);
+  const size_t esz = precision == FT_FP32 ? 4 : 8;
+  if (hipMalloc(&g->fe, 24 * (size_t)n_elems * esz) != hipSuccess)
+    return bail(fail(FT_ERR_ALLOC, "element force buffer allocation failed"));
+  if (g->has_visco && hipMalloc(&g->hist, 144 * (size_t)n_elems * esz) != hipSuccess)
+    return bail(fail(FT_ERR_ALLOC, "history allocation failed"));
+  g->nodal_grid = grid_for(n_nodes);
+  if ((rc = dalloc(&g->eacc, 4 * (size_t)g->nodal_grid))) return bail(rc);
+  // node -> (element, slot) CSR sorted by element (counting sort: O(N))
+  std::vector<int> hptr(n_nodes + 1, 0), hidx(8 * (size_t)n_elems);
+  for (long i = 0; i < 8L * n_elems; ++i) hptr[conn[i] + 1]++;
+  for (int n = 0; n < n_nodes; ++n) hptr[n + 1] += hptr[n];
+  {
+    std::vector<int> fill(hptr.begin(), hptr.end() - 1);
+    for (int e = 0; e < n_elems; ++e)
+      for (int k = 0; k < 8; ++k) hidx[fill[conn[8 * e + k]]++] = 8 * e + k;
+  }
+  std::vector<int> hinfo(pid, pid + n_elems);
+  HIPCHK(hipMemcpy(g->X, coords, nd * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->conn, conn, 8 * (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->einfo, hinfo.data(), (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->parts, g->h_parts.data(), n_parts * sizeof(PartDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->ptr, hptr.data(), hptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->idx, hidx.data(), hidx.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(g->u, 0, nd * sizeof(double)));
+  HIPCHK(hipMemset(g->vs, 0, nd * sizeof(double)));
+  HIPCHK(hipMemset(g->fi, 0, nd * sizeof(double)));
+  HIPCHK(hipMemset(g->dof_motion, 0xff, nd));
+  HIPCHK(hipMemset(g->node_mask, 0, (size_t)n_nodes));
+  HIPCHK(hipMemset(g->status, 0, sizeof(int)));
+  HIPCHK(hipMemset(g->ctl, 0, sizeof(Ctl)));
+  if (g->hist) HIPCHK(hipMemset(g->hist, 0, 144 * (size_t)n_elems * esz));
+  g->h_dof_motion.assign(nd, -1);
+  HIPCHK(hipDeviceSynchronize());
+  *out = g;
+  return 0;
+}
+
+int ft_engine_destroy(ft_engine *g) {
+  if (!g) return 0;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
+                  g->dof_motion, g->node_mask, g->motions, g->ctl, g->dtbits, g->status,
+                  g->ptr, g->idx, g->fe, g->hist, g->eacc, g->hist_rows};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int c = 0; c < 3; ++c)
+    for (auto &pr : g->ev[c]) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (auto e : g->ev_pool) (void)hipEventDestroy(e);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return 0;
+}
+
+int ft_engine_info(ft_engine *g, int *n_nodes, int *n_elems, int *precision) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (n_nodes) *n_nodes = g->nn;
+  if (n_elems) *n_elems = g->ne;
+  if (precision) *precision = g->precision;
+  return 0;
+}
+
+int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const double *motions) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (n_motions < 0 || n_motions > kMaxMotions) return fail(FT_ERR_ARG, "at most %d motions", kMaxMotions);
+  if (n_motions > 0 && !motions) return fail(FT_ERR_ARG, "motions is null");
+  const size_t nd = 3 * (size_t)g->nn;
+  std::vector<signed char> dm(nd, -1);
+  std::vector<unsigned char> mask(g->nn, 0);
+  for (size_t i = 0; i < nd; ++i) {
+    int k = dof_motion ? dof_motion[i] : -1;
+    if (k >= n_motions || k < -1) return fail(FT_ERR_ARG, "dof %zu: motion index %d out of range", i, k);
+    dm[i] = (signed char)k;
+    if (k >= 0) mask[i / 3] |= (unsigned char)(1 << (i % 3));
+  }
+  for (int k = 0; k < n_motions; ++k) {
+    int type = (int)motions[FT_MOTION_STRIDE * k];
+    if (type < 0 || type > 3) return fail(FT_ERR_ARG, "motion %d: unknown type %d", k, type);
+  }
+  g->h_dof_motion.assign(dm.begin(), dm.end());
+  g->h_motions.assign(motions, motions + FT_MOTION_STRIDE * (size_t)n_motions);
+  g->n_motions = n_motions;
+  HIPCHK(hipMemcpyAsync(g->dof_motion, dm.data(), nd, hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(g->node_mask, mask.data(), g->nn, hipMemcpyHostToDevice, g->stream));
+  if (n_motions)
+    HIPCHK(hipMemcpyAsync(g->motions, motions, FT_MOTION_STRIDE * (size_t)n_motions * sizeof(double),
+                          hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(k_elem_flags, dim3((g->ne + 255) / 256), dim3(256), 0, g->stream, g->ne, g->conn,
+                     g->node_mask, g->einfo);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int ft_set_state(ft_engine *g, const double *u, const double *v) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  const size_t nb = 3 * (size_t)g->nn * sizeof(double);
+  if (u) HIPCHK(hipMemcpyAsync(g->u, u, nb, hipMemcpyHostToDevice, g->stream));
+  else HIPCHK(hipMemsetAsync(g->u, 0, nb, g->stream));
+  if (v) HIPCHK(hipMemcpyAsync(g->vs, v, nb, hipMemcpyHostToDevice, g->stream));
+  else HIPCHK(hipMemsetAsync(g->vs, 0, nb, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  g->run_active = false;
+  return 0;
+}
+
+int ft_assemble_lumped_mass(ft_engine *g, double *mass_out) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  double *me = nullptr;
+  int rc = dalloc(&me, 8 * (size_t)g->ne);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mass_elem, dim3(grid_for(g->ne)), dim3(kBlock), 0, g->stream, g->ne, g->conn,
+                     g->einfo, g->X, g->parts, me);
+  hipLaunchKernelGGL(k_mass_gather, dim3(grid_for(g->nn)), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
+                     g->idx, me, g->mass);
+  hipError_t e = hipGetLastError();
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(me);
+  if (e != hipSuccess) return fail(FT_ERR_HIP, "mass kernels: %s", hipGetErrorString(e));
+  g->mass_ready = true;
+  if (mass_out) {
+    std::vector<double> m(g->nn);
+    HIPCHK(hipMemcpy(m.data(), g->mass, g->nn * sizeof(double), hipMemcpyDeviceToHost));
+    for (int n = 0; n < g->nn; ++n) mass_out[3 * n] = mass_out[3 * n + 1] = mass_out[3 * n + 2] = m[n];
+  }
+  return 0;
+}
+
+int ft_stable_time_step(ft_engine *g, const double *u, const int *boundary, double failure_dt,
+                        double *dt_out) {
+  if (!g || !dt_out) return fail(FT_ERR_ARG, "null argument");
+  int rc = upload_u(g, u);
+  if (rc) return rc;
+  if (boundary) {
+    std::vector<unsigned char> mask(g->nn, 0);
+    for (int n = 0; n < g->nn; ++n)
+      for (int d = 0; d < 3; ++d)
+        if (boundary[3 * n + d]) mask[n] |= (unsigned char)(1 << d);
+    HIPCHK(hipMemcpyAsync(g->node_mask, mask.data(), g->nn, hipMemcpyHostToDevice, g->stream));
+    hipLaunchKernelGGL(k_elem_flags, dim3((g->ne + 255) / 256), dim3(256), 0, g->stream, g->ne, g->conn,
+                       g->node_mask, g->einfo);
+    HIPCHK(hipGetLastError());
+  }
+  const unsigned long long inf = kInfBits;
+  HIPCHK(hipMemcpyAsync(g->dtbits, &inf, sizeof(inf), hipMemcpyHostToDevice, g->stream));
+  if ((rc = element_pass(g, false, true, nullptr, 0.0, g->dtbits, 0))) return rc;
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, g->dtbits, sizeof(bits), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  double dt;
+  memcpy(&dt, &bits, sizeof(dt));
+  if (bits == kInfBits) dt = 1e20;  // every element skipped: GlobalVariables.h huge
+  *dt_out = dt;
+  if (dt < failure_dt) return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", dt);
+  return 0;
+}
+
+int ft_element_forces(ft_engine *g, const double *u, double dt, int update_history, double *fe_out) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  int rc = upload_u(g, u);
+  if (rc) return rc;
+  if ((rc = element_pass(g, true, false, nullptr, dt, nullptr, update_history))) return rc;
+  if ((rc = check_status(g))) return rc;
+  if (fe_out) {
+    const size_t n = 24 * (size_t)g->ne;
+    if (g->precision == FT_FP32) {
+      std::vector<float> tmp(n);
+      HIPCHK(hipMemcpy(tmp.data(), g->fe, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) fe_out[i] = tmp[i];
+    } else {
+      HIPCHK(hipMemcpy(fe_out, g->fe, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  return 0;
+}
+
+int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  int rc = upload_u(g, u);
+  if (rc) return rc;
+  if ((rc = element_pass(g, true, false, nullptr, dt, nullptr, 1))) return rc;
+  {
+    Timed t(g, 1);
+    if (g->precision == FT_FP32)
+      hipLaunchKernelGGL(k_gather<float>, dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
+                         g->idx, (const float *)g->fe, g->fi);
+    else
+      hipLaunchKernelGGL(k_gather<double>, dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
+                         g->idx, (const double *)g->fe, g->fi);
+    HIPCHK(hipGetLastError());
+  }
+  if ((rc = check_status(g))) return rc;
+  if (fi_out)
+    HIPCHK(hipMemcpy(fi_out, g->fi, 3 * (size_t)g->nn * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (!g->mass_ready) return fail(FT_ERR_STATE, "ft_assemble_lumped_mass must run before ft_run_init");
+  if (!(dt_factor > 0)) return fail(FT_ERR_ARG, "dt_factor must be positive");
+  g->tmax = tmax;
+  g->factor = dt_factor;
+  g->failure_dt = failure_dt;
+  g->parity = 0;
+  g->run_active = false;
+  // Boundary motions at t = 0 (ex9.cpp:84) on the constrained DOFs.
+  const size_t nd = 3 * (size_t)g->nn;
+  if (g->n_motions > 0) {
+    std::vector<double> X(nd), u(nd), v(nd);
+    HIPCHK(hipMemcpy(X.data(), g->X, nd * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(u.data(), g->u, nd * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(v.data(), g->vs, nd * sizeof(double), hipMemcpyDeviceToHost));
+    for (int n = 0; n < g->nn; ++n)
+      for (int d = 0; d < 3; ++d) {
+        int mi = g->h_dof_motion[3 * n + d];
+        if (mi < 0) continue;
+        double ub[3], vb[3], ab[3];
+        motion_eval(&g->h_motions[FT_MOTION_STRIDE * mi], &X[3 * n], 0.0, ub, vb, ab);
+        u[3 * n + d] = ub[d];
+        v[3 * n + d] = vb[d];
+      }
+    HIPCHK(hipMemcpy(g->u, u.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g->vs, v.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemset(g->eacc, 0, 4 * (size_t)g->nodal_grid * sizeof(double)));
+  HIPCHK(hipMemset(g->status, 0, sizeof(int)));
+  Ctl c;
+  memset(&c, 0, sizeof(c));
+  c.slot[0].dtmin_bits = kInfBits;
+  c.slot[1].dtmin_bits = kInfBits;
+  HIPCHK(hipMemcpy(g->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+  // dt_1 = factor * StableTimeStep() at u_0 (ex9.cpp:87)
+  int rc;
+  if ((rc = element_pass(g, false, true, nullptr, 0.0, &g->ctl->slot[0].dtmin_bits, 0))) return rc;
+  StepSlot s0;
+  if ((rc = read_slot(g, 0, &s0))) return rc;
+  double dtmin = 1e20;
+  if (s0.dtmin_bits != kInfBits) memcpy(&dtmin, &s0.dtmin_bits, sizeof(dtmin));
+  if (dtmin < failure_dt) return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", dtmin);
+  s0.t = 0.0;
+  s0.t_prev = 0.0;
+  s0.dt = dt_factor * dtmin;
+  memcpy(&s0.dtmin_bits, &dtmin, sizeof(dtmin));
+  s0.step = 0;
+  s0.done = 0;
+  s0.fresh = 0;
+  HIPCHK(hipMemcpy(&g->ctl->slot[0], &s0, sizeof(s0), hipMemcpyHostToDevice));
+  // Initial GetForce + CalculateAccelerations (ex9.cpp:89-92); the Prony
+  // update of this pass uses dt_1 like the reference's global dt.
+  if ((rc = element_pass(g, true, false, &g->ctl->slot[0], 0.0, nullptr, 1))) return rc;
+  if ((rc = launch_nodal(g, kInitial, 0))) return rc;  // stops: state of step 0
+  if ((rc = check_status(g))) return rc;
+  g->run_active = true;
+  return 0;
+}
+
+int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (steps_done) *steps_done = 0;
+  if (!g->run_active) return fail(FT_ERR_STATE, "ft_run_init must run first");
+  if (max_steps < 0) return fail(FT_ERR_ARG, "max_steps < 0");
+  StepSlot s;
+  int rc;
+  if ((rc = read_slot(g, g->parity, &s))) return rc;
+  const int before = s.step;  // the slot is always stopped between calls
+  if (max_steps == 0 || !(s.t < g->tmax)) return 0;
+  const long target_l = (long)before + max_steps;
+  const int target = (int)std::min<long>(target_l, INT32_MAX - 1);
+  if ((rc = launch_nodal(g, kResume, target))) return rc;  // start step before+1
+  for (int k = before + 1; k <= target; ++k) {
+    if ((rc = element_pass(g, true, true, &g->ctl->slot[g->parity], 0.0,
+                           &g->ctl->slot[g->parity].dtmin_bits, 1)))
+      return rc;
+    if ((rc = launch_nodal(g, kStep, target))) return rc;
+    if (g->hist_cap > 0) {
+      Timed t(g, 2);
+      hipLaunchKernelGGL(k_history, dim3(1), dim3(kBlock), 0, g->stream, g->eacc, g->nodal_grid,
+                         g->ctl, g->parity, g->hist_rows, g->hist_cap);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if ((rc = read_slot(g, g->parity, &s))) return rc;
+  if (steps_done) *steps_done = s.step - before;
+  int st = 0;
+  HIPCHK(hipMemcpy(&st, g->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (st & 8) return fail(FT_ERR_MATERIAL, "Unknown material type");
+  if (st & FT_STATUS_DT_COLLAPSE) return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", s.dt_done / g->factor);
+  return 0;
+}
+
+int ft_get_time(ft_engine *g, double *time, double *dt, int *step) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  StepSlot s;
+  int rc = read_slot(g, g->parity, &s);
+  if (rc) return rc;
+  if (time) *time = s.t;
+  if (dt) *dt = g->run_active ? s.dt_done : 0.0;
+  if (step) *step = s.step;
+  return 0;
+}
+
+int ft_get_state(ft_engine *g, double *u, double *v, double *a, double *fi) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  const size_t nb = 3 * (size_t)g->nn * sizeof(double);
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (u) HIPCHK(hipMemcpy(u, g->u, nb, hipMemcpyDeviceToHost));
+  if (v) HIPCHK(hipMemcpy(v, g->vs, nb, hipMemcpyDeviceToHost));
+  if (fi) HIPCHK(hipMemcpy(fi, g->fi, nb, hipMemcpyDeviceToHost));
+  if (a) {
+    StepSlot s;
+    int rc = read_slot(g, g->parity, &s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_state_out, dim3((g->nn + 255) / 256), dim3(256), 0, g->stream, g->nn, g->fi,
+                       g->vs, g->mass, g->X, g->dof_motion, g->motions, s.t, g->tmp3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipMemcpy(a, g->tmp3, nb, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int ft_get_energy(ft_engine *g, double out4[4]) {
+  if (!g || !out4) return fail(FT_ERR_ARG, "null argument");
+  std::vector<double> e(4 * (size_t)g->nodal_grid);
+  HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipMemcpy(e.data(), g->eacc, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double w = 0, x = 0, k = 0;
+  for (int b = 0; b < g->nodal_grid; ++b) {
+    w += e[4 * b];
+    x += e[4 * b + 1];
+    k += e[4 * b + 2];
+  }
+  out4[0] = w;
+  out4[1] = x;
+  out4[2] = k;
+  out4[3] = fabs(k + w - x);
+  return 0;
+}
+
+int ft_set_history(ft_engine *g, int capacity) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (capacity < 0) return fail(FT_ERR_ARG, "capacity < 0");
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (g->hist_rows) HIPCHK(hipFree(g->hist_rows));
+  g->hist_rows = nullptr;
+  g->hist_cap = 0;
+  if (capacity > 0) {
+    int rc = dalloc(&g->hist_rows, 6 * (size_t)capacity);
+    if (rc) return rc;
+    g->hist_cap = capacity;
+  }
+  int zero = 0;
+  HIPCHK(hipMemcpy(&g->ctl->hist_count, &zero, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ft_get_history(ft_engine *g, double *time, double *dt, double *energy, int *n) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  HIPCHK(hipStreamSynchronize(g->stream));
+  int cnt = 0;
+  HIPCHK(hipMemcpy(&cnt, &g->ctl->hist_count, sizeof(int), hipMemcpyDeviceToHost));
+  cnt = std::min(cnt, g->hist_cap);
+  if (n) *n = cnt;
+  if (cnt == 0) return 0;
+  std::vector<double> rows(6 * (size_t)cnt);
+  HIPCHK(hipMemcpy(rows.data(), g->hist_rows, rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < cnt; ++i) {
+    if (time) time[i] = rows[6 * i];
+    if (dt) dt[i] = rows[6 * i + 1];
+    if (energy)
+      for (int k = 0; k < 4; ++k) energy[4 * i + k] = rows[6 * i + 2 + k];
+  }
+  return 0;
+}
+
+int ft_get_status(ft_engine *g, int *status) {
+  if (!g || !status) return fail(FT_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipMemcpy(status, g->status, sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int ft_get_visco_history(ft_engine *g, double *hist) {
+  if (!g || !hist) return fail(FT_ERR_ARG, "null argument");
+  const size_t n = 144 * (size_t)g->ne;
+  if (!g->hist) {
+    memset(hist, 0, n * sizeof(double));
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
+  std::vector<double> cm(n);
+  if (g->precision == FT_FP32) {
+    std::vector<float> t(n);
+    HIPCHK(hipMemcpy(t.data(), g->hist, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) cm[i] = t[i];
+  } else {
+    HIPCHK(hipMemcpy(cm.data(), g->hist, n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  // component-major [c * ne + e] -> element-major [e * 144 + c]
+  for (size_t c = 0; c < 144; ++c)
+    for (size_t e = 0; e < (size_t)g->ne; ++e) hist[e * 144 + c] = cm[c * g->ne + e];
+  return 0;
+}
+
+int ft_set_timing(ft_engine *g, int enable) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  int rc = collect_timing(g);
+  if (rc) return rc;
+  g->timing = enable != 0;
+  for (int c = 0; c < 3; ++c) {
+    g->ms[c] = 0;
+    g->launches[c] = 0;
+  }
+  return 0;
+}
+
+int ft_get_timing(ft_engine *g, double ms[3], int launches[3]) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  int rc = collect_timing(g);
+  if (rc) return rc;
+  for (int c = 0; c < 3; ++c) {
+    if (ms) ms[c] = g->ms[c];
+    if (launches) launches[c] = g->launches[c];
+  }
+  return 0;
+}
+
+int ft_traffic_model(ft_engine *g, double bytes[2]) {
+  if (!g || !bytes) return fail(FT_ERR_ARG, "null argument");
+  const double es = g->precision == FT_FP32 ? 4.0 : 8.0;
+  const double ne = g->ne, nn = g->nn;
+  // element kernel: connectivity 32 + einfo 4 + force write 24 es per element;
+  // X (3 doubles) + u (3 doubles) per node; Prony history read + write.
+  double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0);
+  if (g->has_visco) elem += ne * 2.0 * 144.0 * es;
+  // nodal kernel: force read 24 es + CSR entries 32 per element; per node CSR
+  // row pointer 4, mass 8, motion codes 3, X 24 (boundary nodes only: not
+  // counted), u, v_half and fi read + write 3 x 48.
+  double nodal = ne * (24.0 * es + 32.0) + nn * (4.0 + 8.0 + 3.0 + 3.0 * 48.0);
+  bytes[0] = elem;
+  bytes[1] = nodal;
+  return 0;
+}
+
+}  // extern "C"

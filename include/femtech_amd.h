@@ -1,0 +1,170 @@
+/*
+ * femtech_amd.h -- C ABI of the MI355X explicit C3D8 engine.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a
+ * nonzero code; FemTech's own termination codes are kept (1 unknown material,
+ * 3 input error, 12 allocation failure, 19 time step below the failure time
+ * step) and the engine adds FT_ERR_* codes >= 50.  ft_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Each entry point names the reference routine it replaces (paths relative to
+ * the FemTech source tree).  Arrays follow the reference layout: nodal arrays
+ * are 3 doubles per node (x[3 n + d]), connectivity is 8 zero-based node
+ * indices per element in C3D8 order (ShapeFunction_C3D8.cpp:22-29), material
+ * properties are 9 doubles per part (ReadMaterials.cpp:43-122 order: rho, mu,
+ * lambda, k1, k2, g1, tau1, g2, tau2).
+ */
+#ifndef FEMTECH_AMD_H
+#define FEMTECH_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FT_OK 0
+#define FT_ERR_MATERIAL 1   /* StressUpdate.cpp:25-26 */
+#define FT_ERR_INPUT 3      /* ReadMaterials.cpp:13 and siblings */
+#define FT_ERR_ALLOC 12     /* AllocateArrays.cpp:34 */
+#define FT_ERR_DT 19        /* StableTimeStep.cpp:35-38 */
+#define FT_ERR_HIP 50       /* a HIP runtime call failed */
+#define FT_ERR_ARG 51       /* invalid argument (sizes, null pointers, ids) */
+#define FT_ERR_STATE 52     /* call out of order (e.g. run before mass) */
+#define FT_ERR_NUMERIC 53   /* non-finite force or inverted element seen */
+
+/* Element arithmetic precision (nodal state is always FP64). */
+#define FT_FP64 0
+#define FT_FP32 1
+
+/* Motion table entry stride and types (boundary conditions as data; the
+ * reference drivers encode them in code, e.g. ex9.cpp:187-248). */
+#define FT_MOTION_STRIDE 16
+#define FT_MOTION_FIXED 0        /* u = v = a = 0                          */
+#define FT_MOTION_LINEAR 1       /* u = p1 t, v = p1, a = 0 (ex9.cpp:191)   */
+#define FT_MOTION_RAMP_HOLD 2    /* u = p1 min(t/p2, 1), v = p1/p2 (t<p2)   */
+#define FT_MOTION_ROT_HALFSINE 3 /* rigid rotation, axis p1..p3, centre
+                                    p4..p6, alpha = p7 sin(pi t/p8), t<=p8 */
+
+/* Status bits reported by ft_get_status. */
+#define FT_STATUS_NONFINITE 1
+#define FT_STATUS_INVERTED 2
+#define FT_STATUS_DT_COLLAPSE 4
+
+typedef struct ft_engine ft_engine;
+
+const char *ft_last_error(void);
+int ft_version(void);
+/* Number of visible HIP devices (0 when none). */
+int ft_device_count(int *count);
+
+/* ---- engine lifecycle ---------------------------------------------------
+ * Uploads the mesh once; the device then holds the whole mesh as a single
+ * partition (replaces ReadInputFile -> PartitionMesh -> AllocateArrays ->
+ * ShapeFunctions: ReadInputFile.cpp:26, PartitionMesh.cpp:20,
+ * AllocateArrays.cpp:29, ShapeFunctions.cpp:32).  Builds the node-to-element
+ * CSR used for the force gather.  material_id per part follows
+ * StressUpdate.cpp:7-27 (0 rigid, 1 neo-Hookean, 2 St Venant-Kirchhoff,
+ * 3 linear elastic, 4 HGO isotropic, 5 HGO isotropic + Prony). */
+int ft_engine_create(int n_nodes, int n_elems, const double *coords,
+                     const int *conn, const int *pid, int n_parts,
+                     const int *material_id, const double *props,
+                     int precision, ft_engine **out);
+int ft_engine_destroy(ft_engine *eng);
+int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
+
+/* ---- set-up --------------------------------------------------------------
+ * Boundary conditions as data: dof_motion[3 n + d] = -1 for a free DOF or the
+ * index of a motion in `motions` (FT_MOTION_STRIDE doubles each).  Marks the
+ * DOFs the way the reference's boundary[] array does (AllocateArrays.cpp:38,
+ * ex9.cpp:199-243) and sets the per-element "all nodes constrained" flag of
+ * StableTimeStep.cpp:12-20. */
+int ft_set_boundary(ft_engine *eng, const int *dof_motion, int n_motions,
+                    const double *motions);
+/* Displacement and velocity (3 per node); either may be NULL (zero). */
+int ft_set_state(ft_engine *eng, const double *u, const double *v);
+/* Lumped mass: row-sum of the consistent element mass, gathered per node
+ * (AssembleLumpedMass, Mass3D.cpp:127-157).  mass_out: 3 per node or NULL. */
+int ft_assemble_lumped_mass(ft_engine *eng, double *mass_out);
+
+/* ---- per-routine entry points (host arrays authoritative) ---------------
+ * StableTimeStep (StableTimeStep.cpp:4-40): min over elements not fully
+ * constrained of Le / c on the deformed geometry.  u: 3 per node or NULL to
+ * use the device displacement.  boundary: 3 ints per node or NULL to use the
+ * flags from ft_set_boundary.  Returns FT_ERR_DT below failure_dt. */
+int ft_stable_time_step(ft_engine *eng, const double *u, const int *boundary,
+                        double failure_dt, double *dt_out);
+/* GetForce_3D (GetForce_3D.cpp:5-53): internal force fi (3 per node) at
+ * displacement u (NULL = device displacement).  dt is the step size used by
+ * the Prony update of material 5 (the reference reads the global dt). */
+int ft_get_force(ft_engine *eng, const double *u, double dt, double *fi_out);
+/* Per-element forces (24 per element) at u; history is advanced only when
+ * update_history != 0.  Exposes InternalForceUpdate.cpp:4-28 summed over the
+ * Gauss points of each element. */
+int ft_element_forces(ft_engine *eng, const double *u, double dt,
+                      int update_history, double *fe_out);
+
+/* ---- resident explicit loop ---------------------------------------------
+ * The driver loop of ex9.cpp:79-175 run on the device: initial BCs, dt,
+ * GetForce and accelerations, then central-difference steps while
+ * time < tmax and steps < max_steps, with dt = dt_factor * StableTimeStep()
+ * recomputed every step and CheckEnergy's balance accumulated in FP64.
+ * ft_run_init starts a run from the state given by ft_set_state;
+ * ft_run_steps advances it (it may be called repeatedly). */
+int ft_run_init(ft_engine *eng, double tmax, double dt_factor,
+                double failure_dt);
+int ft_run_steps(ft_engine *eng, int max_steps, int *steps_done);
+/* Time, current dt (the next step's), step count. */
+int ft_get_time(ft_engine *eng, double *time, double *dt, int *step);
+/* u, v, a, fi (3 per node each; any may be NULL) at the end of the last
+ * completed step (CheckEnergy-time values). */
+int ft_get_state(ft_engine *eng, double *u, double *v, double *a, double *fi);
+/* Energy balance (CheckEnergy.cpp:3-85): W_int, W_ext, KE, |KE+W_int-W_ext|
+ * after the last completed step. */
+int ft_get_energy(ft_engine *eng, double out4[4]);
+/* Per-step history (time, dt, energy[4]) of the steps run after this call;
+ * capacity 0 disables.  Costs one small reduction kernel per step. */
+int ft_set_history(ft_engine *eng, int capacity);
+int ft_get_history(ft_engine *eng, double *time, double *dt, double *energy,
+                   int *n);
+int ft_get_status(ft_engine *eng, int *status);
+/* Prony history of material-5 elements (8 Gauss points x 3 tensors x 6
+ * Voigt components per element, element order), as FP64. */
+int ft_get_visco_history(ft_engine *eng, double *hist);
+
+/* ---- instrumentation -----------------------------------------------------
+ * When enabled, every kernel launch of the loop is bracketed by HIP events on
+ * the engine's stream.  Times are summed in milliseconds per kernel class
+ * (0 element, 1 nodal/gather, 2 other) together with launch counts. */
+int ft_set_timing(ft_engine *eng, int enable);
+int ft_get_timing(ft_engine *eng, double ms[3], int launches[3]);
+/* Algorithmic bytes moved per launch of the element kernel and of the nodal
+ * kernel, as defined in the design notes (compulsory traffic, each array
+ * touched once). */
+int ft_traffic_model(ft_engine *eng, double bytes[2]);
+
+/* ---- host-side mesh tools (no GPU needed) -------------------------------
+ * Structured hex block nx x ny x nz with spacings h*, nodes numbered
+ * x-fastest; interior nodes jittered uniformly by +-jitter*h per coordinate
+ * from a 64-bit Mersenne twister seeded with `seed` (draw order node-major,
+ * x, y, z).  coords: 3*(nx+1)(ny+1)(nz+1), conn: 8*nx*ny*nz. */
+int ft_mesh_block(int nx, int ny, int nz, double hx, double hy, double hz,
+                  double jitter, unsigned long long seed, double *coords,
+                  int *conn);
+/* Abaqus .inp writer readable by ReadAbaqus.cpp:26-277 (one ELSET per part,
+ * node ids 1..n in array order, %.17g coordinates). */
+int ft_write_inp(const char *path, int n_nodes, int n_elems, const double *coords,
+                 const int *conn, const int *pid);
+/* Mesh reader for .inp (ReadAbaqus.cpp) and .k (ReadLsDyna.cpp), C3D8 only,
+ * with the one-rank node renumbering of PartitionMesh.cpp:485-536 (local
+ * node i = i-th smallest global node id).  Two calls: first with NULL arrays
+ * to get the sizes, then with arrays of those sizes. */
+int ft_read_mesh(const char *path, int *n_nodes, int *n_elems, double *coords,
+                 int *conn, int *pid, int *global_node_id);
+/* materials.dat reader (ReadMaterials.cpp:8-138): fills material_id[n_parts]
+ * and props[9 n_parts]. */
+int ft_read_materials(const char *path, int n_parts, int *material_id,
+                      double *props);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

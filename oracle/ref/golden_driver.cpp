@@ -1,0 +1,373 @@
+/*
+ * golden_driver.cpp -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A driver of our own, linked against FemTech's library translation units
+ * compiled unmodified from the reference source tree (oracle/ref/Makefile).
+ * It writes golden vectors that pin the oracle.  It never ships and runs only
+ * where the reference source tree is present.
+ *
+ * Modes
+ *   run <dir>     ex9-style explicit run (ex9.cpp:19-175) on <dir>/mesh.inp and
+ *                 <dir>/materials.dat with boundary motions given as data
+ *                 (<dir>/bc.bin, <dir>/params.txt, optional v0.bin / u0.bin).
+ *   stress <dir>  StressUpdate (StressUpdate.cpp:5-29) on given F sequences.
+ *   geom <dir>    volumeHexahedron / areaHexahedronFace /
+ *                 CalculateCharacteristicLength_C3D8 on given element corners.
+ *
+ * The two library units that cannot be compiled here are replaced by what a
+ * one-rank run of them amounts to:
+ *   InitFinalizeFemTech.cpp (needs a CMake-generated header): MPI_Init,
+ *     world_rank = 0, world_size = 1, uid, and TerminateFemTech -> exit(code).
+ *   PartitionMesh() (needs the ParMETIS library): on one rank it keeps every
+ *     element and calls updateConnectivityGlobalToLocal(), which this driver
+ *     calls directly (the unit is compiled; its ParMETIS calls are never
+ *     reached), then sets nDOF = 3 nNodes (PartitionMesh.cpp:433-435).
+ */
+#include "FemTech.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <string>
+#include <vector>
+
+/* Driver-owned globals (ex9.cpp:9-17). */
+double Time, dt;
+double ExplicitTimeStepReduction = 0.8;
+double FailureTimeStep = 1e-11;
+bool ImplicitStatic = false;
+bool ImplicitDynamic = false;
+bool ExplicitDynamic = true;
+
+/* One-rank stand-ins for InitFinalizeFemTech.cpp's definitions. */
+int world_rank;
+int world_size;
+std::string uid;
+void TerminateFemTech(int errorCode) {
+  fprintf(stderr, "TerminateFemTech(%d)\n", errorCode);
+  finaliseLog();
+  MPI_Finalize();
+  exit(100 + errorCode);
+}
+
+void updateConnectivityGlobalToLocal(void);  // PartitionMesh.cpp:485
+
+namespace {
+
+template <typename T>
+std::vector<T> read_bin(const std::string &path, bool required = true) {
+  std::vector<T> v;
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) {
+    if (required) {
+      fprintf(stderr, "cannot open %s\n", path.c_str());
+      exit(2);
+    }
+    return v;
+  }
+  fseek(f, 0, SEEK_END);
+  long n = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  v.resize(n / sizeof(T));
+  if (fread(v.data(), sizeof(T), v.size(), f) != v.size()) exit(2);
+  fclose(f);
+  return v;
+}
+
+template <typename T>
+void write_bin(const std::string &path, const T *p, size_t n) {
+  FILE *f = fopen(path.c_str(), "wb");
+  fwrite(p, sizeof(T), n, f);
+  fclose(f);
+}
+
+void one_rank_partition() {
+  sendProcessCount = 0;
+  sendNeighbourCountCum = (int *)calloc(1, sizeof(int));
+  sendNeighbourCount = (int *)calloc(1, sizeof(int));
+  sendProcessID = (int *)calloc(1, sizeof(int));
+  sendNodeIndex = (int *)calloc(1, sizeof(int));
+  sendNodeDisplacement = (double *)calloc(1, sizeof(double));
+  recvNodeDisplacement = (double *)calloc(1, sizeof(double));
+  nNodes = eptr[nelements];
+  updateConnectivityGlobalToLocal();
+  nDOF = nNodes * ndim;
+}
+
+/* Boundary motions as data: same table layout as the oracle. */
+struct Motions {
+  std::vector<int> dof_motion;
+  std::vector<double> table;
+};
+
+void motion_eval(const double *mo, const double *X, double t, double *u, double *v,
+                 double *a) {
+  int type = (int)mo[0];
+  for (int d = 0; d < 3; ++d) u[d] = v[d] = a[d] = 0.0;
+  if (type == 1) {
+    for (int d = 0; d < 3; ++d) {
+      u[d] = t * mo[1];
+      v[d] = mo[1];
+    }
+  } else if (type == 2) {
+    double amp = mo[1], T = mo[2];
+    double s = t < T ? t / T : 1.0;
+    for (int d = 0; d < 3; ++d) {
+      u[d] = amp * s;
+      v[d] = t < T ? amp / T : 0.0;
+    }
+  } else if (type != 0) {
+    fprintf(stderr, "motion type %d not supported by the golden driver\n", type);
+    exit(2);
+  }
+}
+
+void apply_bc(const Motions &m) {
+  for (int n = 0; n < nNodes; ++n)
+    for (int d = 0; d < 3; ++d) {
+      int k = m.dof_motion[3 * n + d];
+      if (k < 0) continue;
+      double u[3], v[3], a[3];
+      motion_eval(&m.table[16 * k], &coordinates[3 * n], Time, u, v, a);
+      boundary[3 * n + d] = 1;
+      displacements[3 * n + d] = u[d];
+      velocities[3 * n + d] = v[d];
+      accelerations[3 * n + d] = a[d];
+    }
+}
+
+int mode_run(const std::string &dir) {
+  if (chdir(dir.c_str()) != 0) return 2;
+  ReadInputFile(access("mesh.k", F_OK) == 0 ? "mesh.k" : "mesh.inp");
+  ReadMaterials();
+  one_rank_partition();
+  AllocateArrays();
+  Time = 0.0;
+
+  Motions mot;
+  {
+    std::vector<int> bc = read_bin<int>("bc.bin");
+    int nd = 3 * nNodes;
+    mot.dof_motion.assign(bc.begin(), bc.begin() + nd);
+    int nm = bc[nd];
+    std::vector<double> tab = read_bin<double>("motions.bin");
+    if ((int)tab.size() != 16 * nm) return 3;
+    mot.table = tab;
+  }
+  int max_steps = 0, nsnap = 0;
+  double tmax = 0, factor = 0;
+  std::vector<int> snaps;
+  {
+    FILE *f = fopen("params.txt", "r");
+    if (fscanf(f, "%d %lf %lf %lf %d", &max_steps, &tmax, &factor, &FailureTimeStep,
+               &nsnap) != 5)
+      return 4;
+    snaps.resize(nsnap);
+    for (int i = 0; i < nsnap; ++i)
+      if (fscanf(f, "%d", &snaps[i]) != 1) return 4;
+    fclose(f);
+  }
+  ExplicitTimeStepReduction = factor;
+  std::vector<double> u0 = read_bin<double>("u0.bin", false);
+  std::vector<double> v0 = read_bin<double>("v0.bin", false);
+  if (!u0.empty()) memcpy(displacements, u0.data(), nDOF * sizeof(double));
+  if (!v0.empty()) memcpy(velocities, v0.data(), nDOF * sizeof(double));
+
+  write_bin("out_coords.f64", coordinates, nDOF);
+  write_bin("out_conn.i32", connectivity, eptr[nelements]);
+
+  ShapeFunctions();
+  write_bin("out_dshp.f64", dshp, 192 * (size_t)nelements);
+  write_bin("out_detJ.f64", detJacobian, 8 * (size_t)nelements);
+  AssembleLumpedMass();
+  write_bin("out_mass.f64", mass, nDOF);
+
+  apply_bc(mot);
+  dt = ExplicitTimeStepReduction * StableTimeStep();
+  GetForce();
+  CalculateAccelerations();
+  write_bin("out_fi0.f64", fi, nDOF);
+  write_bin("out_a0.f64", accelerations, nDOF);
+
+  std::vector<double> hist_time(1, Time), hist_dt(1, dt), hist_energy(4, 0.0);
+  std::vector<double> snap_u;
+  auto snap = [&](int step) {
+    for (int s = 0; s < nsnap; ++s)
+      if (snaps[s] == step) snap_u.insert(snap_u.end(), displacements, displacements + nDOF);
+  };
+  snap(0);
+  double Wint = 0.0, Wext = 0.0;
+  int step = 0;
+  // ex9.cpp:107-175
+  while (Time < tmax && step < max_steps) {
+    double t_n = Time;
+    double t_np1 = Time + dt;
+    Time = t_np1;
+    double dt_nphalf = dt;
+    double t_nphalf = 0.5 * (t_np1 + t_n);
+    for (int i = 0; i < nDOF; i++) {
+      if (boundary[i])
+        velocities_half[i] = velocities[i];
+      else
+        velocities_half[i] = velocities[i] + (t_nphalf - t_n) * accelerations[i];
+    }
+    memcpy(displacements_prev, displacements, nDOF * sizeof(double));
+    memcpy(accelerations_prev, accelerations, nDOF * sizeof(double));
+    memcpy(fi_prev, fi, nDOF * sizeof(double));
+    memcpy(fe_prev, fe, nDOF * sizeof(double));
+    for (int i = 0; i < nDOF; i++)
+      if (!boundary[i]) displacements[i] = displacements[i] + dt_nphalf * velocities_half[i];
+    apply_bc(mot);
+    GetForce();
+    CalculateAccelerations();
+    for (int i = 0; i < nDOF; i++)
+      if (!boundary[i])
+        velocities[i] = velocities_half[i] + (t_np1 - t_nphalf) * accelerations[i];
+    CheckEnergy(Time, 0);  // writes its own rows (%12.6e) to energy_<uid>.dat
+    // Full-precision copy of the same sums (CheckEnergy.cpp:15-66, one rank),
+    // computed from the library's arrays.
+    double sWint = 0.0, sWext = 0.0, WKE = 0.0;
+    for (int i = 0; i < nDOF; ++i) {
+      double dd = displacements[i] - displacements_prev[i];
+      WKE += mass[i] * velocities[i] * velocities[i];
+      if (boundary[i]) {
+        double reaction = fi_prev[i] + fi[i] + mass[i] * (accelerations[i] + accelerations_prev[i]);
+        sWext += dd * reaction;
+      }
+      sWint += dd * (fi_prev[i] + fi[i]);
+      sWext += dd * (fe_prev[i] + fe[i]);
+    }
+    WKE *= 0.5;
+    sWint *= 0.5;
+    sWext *= 0.5;
+    Wint += sWint;
+    Wext += sWext;
+    ++step;
+    dt = ExplicitTimeStepReduction * StableTimeStep();
+    hist_time.push_back(Time);
+    hist_dt.push_back(dt);
+    hist_energy.push_back(Wint);
+    hist_energy.push_back(Wext);
+    hist_energy.push_back(WKE);
+    hist_energy.push_back(fabs(WKE + Wint - Wext));
+    snap(step);
+  }
+  write_bin("out_u.f64", displacements, nDOF);
+  write_bin("out_v.f64", velocities, nDOF);
+  write_bin("out_a.f64", accelerations, nDOF);
+  write_bin("out_fi.f64", fi, nDOF);
+  write_bin("out_time.f64", hist_time.data(), hist_time.size());
+  write_bin("out_dt.f64", hist_dt.data(), hist_dt.size());
+  write_bin("out_energy.f64", hist_energy.data(), hist_energy.size());
+  write_bin("out_snap_u.f64", snap_u.data(), snap_u.size());
+  fclose(energyFile);
+  printf("steps %d time %.17g\n", step, Time);
+  return 0;
+}
+
+/* Sets up n one-Gauss-point "elements" so the reference's material routines
+ * can be called on given deformation gradients. */
+int mode_stress(const std::string &dir) {
+  if (chdir(dir.c_str()) != 0) return 2;
+  std::vector<double> in = read_bin<double>("stress_in.f64");
+  // record: mat, props[9], dt, T, F[T][9]  (T fixed for the file: first record)
+  size_t pos = 0;
+  std::vector<double> out_pk2, out_hist;
+  ndim = 3;
+  nelements = 1;
+  nPIDglobal = 1;
+  fptr = (int *)calloc(2, sizeof(int));
+  detFptr = (int *)calloc(2, sizeof(int));
+  gpPtr = (int *)calloc(2, sizeof(int));
+  pk2ptr = (int *)calloc(2, sizeof(int));
+  pid = (int *)calloc(1, sizeof(int));
+  materialID = (int *)calloc(1, sizeof(int));
+  properties = (double *)calloc(9, sizeof(double));
+  F = (double *)calloc(9, sizeof(double));
+  detF = (double *)calloc(1, sizeof(double));
+  pk2 = (double *)calloc(6, sizeof(double));
+  Hn_1 = (double *)calloc(9, sizeof(double));
+  Hn_2 = (double *)calloc(9, sizeof(double));
+  S0n = (double *)calloc(9, sizeof(double));
+  mat1 = (double *)calloc(9, sizeof(double));
+  mat2 = (double *)calloc(9, sizeof(double));
+  mat3 = (double *)calloc(9, sizeof(double));
+  mat4 = (double *)calloc(9, sizeof(double));
+  while (pos < in.size()) {
+    int mat = (int)in[pos++];
+    materialID[0] = mat;
+    for (int k = 0; k < 9; ++k) properties[k] = in[pos++];
+    dt = in[pos++];
+    int T = (int)in[pos++];
+    memset(Hn_1, 0, 9 * sizeof(double));
+    memset(Hn_2, 0, 9 * sizeof(double));
+    memset(S0n, 0, 9 * sizeof(double));
+    for (int t = 0; t < T; ++t) {
+      for (int k = 0; k < 9; ++k) F[k] = in[pos++];
+      memset(pk2, 0, 6 * sizeof(double));
+      DeterminateF(0, 0);
+      StressUpdate(0, 0);
+      out_pk2.insert(out_pk2.end(), pk2, pk2 + 6);
+      out_pk2.push_back(detF[0]);
+    }
+    out_hist.insert(out_hist.end(), Hn_1, Hn_1 + 9);
+    out_hist.insert(out_hist.end(), Hn_2, Hn_2 + 9);
+    out_hist.insert(out_hist.end(), S0n, S0n + 9);
+  }
+  write_bin("stress_out.f64", out_pk2.data(), out_pk2.size());
+  write_bin("stress_hist.f64", out_hist.data(), out_hist.size());
+  return 0;
+}
+
+int mode_geom(const std::string &dir) {
+  if (chdir(dir.c_str()) != 0) return 2;
+  std::vector<double> in = read_bin<double>("geom_in.f64");
+  int n = (int)(in.size() / 24);
+  ndim = 3;
+  nelements = n;
+  nNodes = 8 * n;
+  coordinates = in.data();
+  displacements = (double *)calloc(24 * (size_t)n, sizeof(double));
+  connectivity = (int *)malloc(8 * (size_t)n * sizeof(int));
+  eptr = (int *)malloc((n + 1) * sizeof(int));
+  for (int i = 0; i < 8 * n; ++i) connectivity[i] = i;
+  for (int e = 0; e <= n; ++e) eptr[e] = 8 * e;
+  static const int idx[24] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 3, 7, 4,
+                              1, 2, 6, 5, 0, 1, 5, 4, 3, 2, 6, 7};
+  std::vector<double> out;
+  for (int e = 0; e < n; ++e) {
+    double *xe = &in[24 * (size_t)e];
+    out.push_back(volumeHexahedron(xe));
+    for (int f = 0; f < 6; ++f) out.push_back(areaHexahedronFace(xe, &idx[4 * f]));
+    out.push_back(CalculateCharacteristicLength_C3D8(e));
+  }
+  write_bin("geom_out.f64", out.data(), out.size());
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 3) {
+    fprintf(stderr, "usage: %s run|stress|geom <dir>\n", argv[0]);
+    return 2;
+  }
+  MPI_Init(NULL, NULL);
+  MPI_Comm_size(MPI_COMM_WORLD, &world_size);
+  MPI_Comm_rank(MPI_COMM_WORLD, &world_rank);
+  uid = "golden";
+  std::string dir = argv[2];
+  std::string logFile = dir + "/femtech_golden.log";
+  initLog(logFile.c_str());
+  int rc = 2;
+  std::string mode = argv[1];
+  if (mode == "run") rc = mode_run(dir);
+  if (mode == "stress") rc = mode_stress(dir);
+  if (mode == "geom") rc = mode_geom(dir);
+  finaliseLog();
+  MPI_Finalize();
+  return rc;
+}

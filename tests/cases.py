@@ -1,0 +1,134 @@
+"""Canonical test problems shared by the golden-vector generator and tests.
+
+Each builder returns a dict of plain arrays (mesh, materials, boundary motions
+as data, run parameters).  Materials follow ReadMaterials.cpp:43-122
+(9 properties per part: rho, mu, lambda, k1, k2, g1, tau1, g2, tau2).
+"""
+import numpy as np
+
+import femtech_amd as ft
+
+TOL = 1e-5  # coordinate test tolerance of the reference drivers (ex9.cpp:188)
+
+
+def props_row(rho, mu=0.0, lam=0.0, k1=0.0, k2=0.0, g1=0.0, t1=0.0, g2=0.0, t2=0.0):
+    return np.array([rho, mu, lam, k1, k2, g1, t1, g2, t2], np.float64)
+
+
+def plane_rules(X, rules, nn):
+    """rules: (axis, value, dof, motion index) applied in order (later wins),
+    like the coordinate tests of ex9.cpp:199-243."""
+    X = X.reshape(-1, 3)
+    dm = -np.ones(3 * nn, np.int32)
+    for axis, value, dof, mi in rules:
+        nodes = np.nonzero(np.abs(X[:, axis] - value) < TOL)[0]
+        dm[3 * nodes + dof] = mi
+    return dm
+
+
+def case_ex9_like(X, conn, pid, material_id, props, L=0.005, dmax=0.007, tmax=1.0, steps=100000):
+    """ex9's loading (ex9.cpp:187-248): symmetry planes x=y=z=0 and a
+    prescribed y-displacement Time*dMax/tMax on the face y = L."""
+    nn = X.size // 3
+    motions = np.stack([ft.motion(ft.MOTION_FIXED), ft.motion(ft.MOTION_LINEAR, dmax / tmax)])
+    dm = plane_rules(X, [(0, 0.0, 0, 0), (1, 0.0, 1, 0), (2, 0.0, 2, 0), (1, L, 1, 1)], nn)
+    return dict(X=X, conn=conn, pid=pid, material_id=material_id, props=props, dof_motion=dm,
+                motions=motions, u0=None, v0=None, max_steps=steps, tmax=tmax, dt_factor=0.8,
+                failure_dt=1e-11, snaps=[0, 1, 10, 50])
+
+
+def case_c1(steps=2000):
+    """10x10x10 block, h = 1 mm; neo-Hookean G = 1 MPa, K = 10 MPa, rho = 1000;
+    bottom face u_z = 0 with one corner pinned; top face u_z ramps to 10 %
+    stretch over 1 ms; dt = 0.9 x stable, 2000 steps."""
+    X, conn = ft.mesh_block(10, 10, 10, 1e-3)
+    nn = X.size // 3
+    G, K = 1.0e6, 10.0e6
+    props = props_row(1000.0, G, K - 2.0 * G / 3.0)[None, :]
+    motions = np.stack([ft.motion(ft.MOTION_FIXED), ft.motion(ft.MOTION_RAMP_HOLD, 1e-3, 1e-3)])
+    Xr = X.reshape(-1, 3)
+    dm = plane_rules(X, [(2, 0.0, 2, 0), (2, 0.01, 2, 1)], nn)
+    corner = np.nonzero(np.all(np.abs(Xr) < TOL, axis=1))[0]
+    dm[3 * corner + 0] = 0
+    dm[3 * corner + 1] = 0
+    dm[3 * corner + 2] = 0
+    return dict(X=X, conn=conn, pid=np.zeros(conn.size // 8, np.int32),
+                material_id=np.array([1], np.int32), props=props, dof_motion=dm, motions=motions,
+                u0=None, v0=None, max_steps=steps, tmax=1e9, dt_factor=0.9, failure_dt=1e-11,
+                snaps=[0, 1, 10, 100, 500, 1000, 2000])
+
+
+def case_impact(nx=6, ny=5, nz=4, jitter=0.15, seed=20261202, steps=300, slab=1):
+    """Jittered block (h = 1 mm) with the impact loading of the 1M-hex
+    workload: base face fixed, the top `slab` element layers start at
+    v_z = -5 m/s; neo-Hookean G = 1 MPa, K = 10 MPa."""
+    X, conn = ft.mesh_block(nx, ny, nz, 1e-3, jitter=jitter, seed=seed)
+    nn = X.size // 3
+    G, K = 1.0e6, 10.0e6
+    props = props_row(1000.0, G, K - 2.0 * G / 3.0)[None, :]
+    motions = np.stack([ft.motion(ft.MOTION_FIXED)])
+    dm = plane_rules(X, [(2, 0.0, 0, 0), (2, 0.0, 1, 0), (2, 0.0, 2, 0)], nn)
+    v0 = np.zeros(3 * nn)
+    Xr = X.reshape(-1, 3)
+    top = np.nonzero(Xr[:, 2] >= (nz - slab) * 1e-3 - 0.2e-3)[0]
+    v0[3 * top + 2] = -5.0
+    return dict(X=X, conn=conn, pid=np.zeros(conn.size // 8, np.int32),
+                material_id=np.array([1], np.int32), props=props, dof_motion=dm, motions=motions,
+                u0=None, v0=v0, max_steps=steps, tmax=1e9, dt_factor=0.9, failure_dt=1e-11,
+                snaps=[0, 1, 10, 100, steps])
+
+
+def case_materials(steps=250):
+    """4x4x6 jittered block in six parts (one per element layer): neo-Hookean,
+    St Venant-Kirchhoff, linear elastic, HGO isotropic, HGO + Prony and rigid;
+    stretched 10 % along z by a prescribed top motion with symmetry planes,
+    plus a lateral initial velocity."""
+    X, conn = ft.mesh_block(4, 4, 6, 1e-3, jitter=0.1, seed=7)
+    nn, ne = X.size // 3, conn.size // 8
+    # part = element layer (z index), parts first appear in increasing order
+    pid = (np.arange(ne) // 16).astype(np.int32)
+    mu, lam = 2.0e4, 2.0e5
+    props = np.stack([
+        props_row(1000.0, mu, lam),                                    # 1 NH
+        props_row(1000.0, mu, lam),                                    # 2 SVK
+        props_row(1000.0, mu, lam),                                    # 3 LE
+        props_row(1000.0, mu, lam, 3.0e4, 8.0),                        # 4 HGO
+        props_row(1000.0, mu, lam, 3.0e4, 8.0, 0.4, 2e-3, 0.3, 2e-2),  # 5 HGO + Prony
+        props_row(1000.0),                                             # 0 rigid
+    ])
+    material_id = np.array([1, 2, 3, 4, 5, 0], np.int32)
+    motions = np.stack([ft.motion(ft.MOTION_FIXED), ft.motion(ft.MOTION_RAMP_HOLD, 0.6e-3, 4e-3)])
+    dm = plane_rules(X, [(0, 0.0, 0, 0), (1, 0.0, 1, 0), (2, 0.0, 2, 0), (2, 6e-3, 2, 1)], nn)
+    v0 = np.zeros(3 * nn)
+    Xr = X.reshape(-1, 3)
+    v0[0::3] = 0.5 * Xr[:, 2] / 6e-3
+    return dict(X=X, conn=conn, pid=pid, material_id=material_id, props=props.reshape(-1),
+                dof_motion=dm, motions=motions, u0=None, v0=v0, max_steps=steps, tmax=1e9,
+                dt_factor=0.8, failure_dt=1e-11, snaps=[0, 1, 10, 100, steps])
+
+
+def random_F(rng, n, scale):
+    F = np.eye(3)[None, :, :] + scale * rng.uniform(-1, 1, (n, 3, 3))
+    # keep det F > 0
+    for i in range(n):
+        if np.linalg.det(F[i]) <= 0.2:
+            F[i] = np.eye(3) + 0.1 * scale * rng.uniform(-1, 1, (3, 3))
+    return F
+
+
+def random_hexes(rng, n):
+    """Distorted hexes around the unit-millimetre cube; a third get large
+    negative twists so the parallelogram shortcut of Geometry.cpp:46 (no fabs)
+    is taken on non-parallelogram faces."""
+    base = 1e-3 * np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                            [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float64)
+    out = []
+    for i in range(n):
+        x = base + 1e-3 * rng.uniform(-0.2, 0.2, (8, 3)) * (i % 3 != 0)
+        if i % 3 == 1:
+            x[2] -= 1e-3 * np.array([0.3, 0.3, 0.3])
+            x[6] -= 1e-3 * np.array([0.2, 0.25, 0.3])
+        if i % 7 == 0:
+            x = x @ np.array([[0.8, 0.6, 0.0], [-0.6, 0.8, 0.0], [0.0, 0.0, 1.0]])
+        out.append(x.reshape(24))
+    return np.array(out)
