@@ -29,16 +29,16 @@ struct PartDev {
 
 template <typename R> struct Num;
 template <> struct Num<float> {
-  __device__ static float log1p_(float x) { return log1pf(x); }
-  __device__ static float expm1_(float x) { return expm1f(x); }
-  __device__ static float exp_(float x) { return expf(x); }
-  __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+  __host__ __device__ static float log1p_(float x) { return log1pf(x); }
+  __host__ __device__ static float expm1_(float x) { return expm1f(x); }
+  __host__ __device__ static float exp_(float x) { return expf(x); }
+  __host__ __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
 };
 template <> struct Num<double> {
-  __device__ static double log1p_(double x) { return log1p(x); }
-  __device__ static double expm1_(double x) { return expm1(x); }
-  __device__ static double exp_(double x) { return exp(x); }
-  __device__ static double sqrt_(double x) { return sqrt(x); }
+  __host__ __device__ static double log1p_(double x) { return log1p(x); }
+  __host__ __device__ static double expm1_(double x) { return expm1(x); }
+  __host__ __device__ static double exp_(double x) { return exp(x); }
+  __host__ __device__ static double sqrt_(double x) { return sqrt(x); }
 };
 
 // Position p = i + 2j + 4k of the 2x2x2 node cube (i: xi, j: eta, k: zeta
@@ -49,7 +49,7 @@ __host__ __device__ constexpr int p2n(int p) {
 
 // Unnormalised Walsh-Hadamard transform over the three sign bits.
 template <typename R>
-__device__ inline void wht8(R v[8]) {
+__host__ __device__ inline void wht8(R v[8]) {
 #pragma unroll
   for (int b = 1; b < 8; b <<= 1) {
 #pragma unroll
@@ -65,7 +65,7 @@ __device__ inline void wht8(R v[8]) {
 // Inverse direction used for forces: coefficients -> nodal values with sign
 // convention (bit set -> +1, clear -> -1).
 template <typename R>
-__device__ inline void iwht8(R c[8]) {
+__host__ __device__ inline void iwht8(R c[8]) {
 #pragma unroll
   for (int b = 1; b < 8; b <<= 1) {
 #pragma unroll
@@ -83,7 +83,7 @@ __device__ inline void iwht8(R c[8]) {
 template <typename R> struct Sym { R v[6]; };
 
 template <typename R>
-__device__ inline R sym_at(const Sym<R> &s, int i, int j) {
+__host__ __device__ inline R sym_at(const Sym<R> &s, int i, int j) {
   if (i == j) return s.v[i];
   int k = 3 - i - j;  // (1,2)->0 ... map: pair {1,2}->3, {0,2}->4, {0,1}->5
   return s.v[3 + k];
@@ -91,7 +91,7 @@ __device__ inline R sym_at(const Sym<R> &s, int i, int j) {
 
 // det(I + H) - 1 without forming I + H (no cancellation at small strain).
 template <typename R>
-__device__ inline R det_ip1_m1(const R H[3][3]) {
+__host__ __device__ inline R det_ip1_m1(const R H[3][3]) {
   R tr = H[0][0] + H[1][1] + H[2][2];
   R m2 = (H[0][0] * H[1][1] - H[0][1] * H[1][0]) + (H[0][0] * H[2][2] - H[0][2] * H[2][0]) +
          (H[1][1] * H[2][2] - H[1][2] * H[2][1]);
@@ -103,7 +103,7 @@ __device__ inline R det_ip1_m1(const R H[3][3]) {
 
 // Green-Lagrange E = (H + H^T + H^T H) / 2.
 template <typename R>
-__device__ inline Sym<R> green_lagrange(const R H[3][3]) {
+__host__ __device__ inline Sym<R> green_lagrange(const R H[3][3]) {
   Sym<R> E;
   const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
@@ -117,7 +117,7 @@ __device__ inline Sym<R> green_lagrange(const R H[3][3]) {
 
 // adj(C) for C = I + 2E (symmetric).
 template <typename R>
-__device__ inline Sym<R> adj_c(const Sym<R> &E) {
+__host__ __device__ inline Sym<R> adj_c(const Sym<R> &E) {
   R c00 = R(1) + R(2) * E.v[0], c11 = R(1) + R(2) * E.v[1], c22 = R(1) + R(2) * E.v[2];
   R c12 = R(2) * E.v[3], c02 = R(2) * E.v[4], c01 = R(2) * E.v[5];
   Sym<R> A;
@@ -133,7 +133,7 @@ __device__ inline Sym<R> adj_c(const Sym<R> &E) {
 // S = (A T) * s for symmetric A, T whose product is symmetric (they commute:
 // both are functions of E).
 template <typename R>
-__device__ inline Sym<R> sym_mul(const Sym<R> &A, const Sym<R> &T, R s) {
+__host__ __device__ inline Sym<R> sym_mul(const Sym<R> &A, const Sym<R> &T, R s) {
   Sym<R> S;
   const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
@@ -152,7 +152,7 @@ struct ViscoCoef { double c11, c12, c21, c22; };
 // displacement gradient H = F - I.  hist: 18 values (H1, H2, S0 as Voigt
 // 6-vectors) for material 5 or nullptr.  Returns false for an unknown id.
 template <typename R>
-__device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R *hist,
+__host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R *hist,
                                   const ViscoCoef &vc, Sym<R> &S, R &Jm1) {
   Jm1 = det_ip1_m1(H);
   const int mat = pp.mat;
@@ -271,7 +271,7 @@ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R *hist,
 }
 
 template <typename R>
-__device__ inline R norm_cross(const R a[3], const R b[3]) {
+__host__ __device__ inline R norm_cross(const R a[3], const R b[3]) {
   // normOfCrossProduct, math.cpp:30-43
   R z = a[0] * b[1] - a[1] * b[0];
   R x = a[1] * b[2] - a[2] * b[1];
@@ -280,7 +280,7 @@ __device__ inline R norm_cross(const R a[3], const R b[3]) {
 }
 
 template <typename R>
-__device__ inline R triple(const R s[3], const R a[3], const R b[3]) {
+__host__ __device__ inline R triple(const R s[3], const R a[3], const R b[3]) {
   // tripleProduct, math.cpp:52-56
   return s[2] * (a[0] * b[1] - a[1] * b[0]) + s[0] * (a[1] * b[2] - a[2] * b[1]) -
          s[1] * (a[0] * b[2] - a[2] * b[0]);
@@ -289,7 +289,7 @@ __device__ inline R triple(const R s[3], const R a[3], const R b[3]) {
 // areaHexahedronFace (Geometry.cpp:29-64) from a face's twist cD and edge
 // vectors c1, c2 (each the quarter-sum of the reference's corner formula).
 template <typename R>
-__device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3]) {
+__host__ __device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3]) {
   const R tol = R(1e-6);
   if ((cD[0] < tol) && (cD[1] < tol) && (cD[2] < tol)) return R(4) * norm_cross(c1, c2);
   const R t = R(kFaceQ);
@@ -316,7 +316,7 @@ __device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3]) {
 // xi.zeta, eta.zeta, xi.eta.zeta.  The reference's q0..q5 of
 // volumeHexahedron are 8 x (xi.eta, xi.zeta, xi, eta.zeta, eta, zeta).
 template <typename R>
-__device__ inline R char_length(const R d[7][3]) {
+__host__ __device__ inline R char_length(const R d[7][3]) {
   const R *c1 = d[0], *c2 = d[1], *c3 = d[2], *c4 = d[3], *c5 = d[4], *c6 = d[5], *c7 = d[6];
   R q0[3], q1[3], q2[3], q3[3], q4[3], q5[3];
 #pragma unroll
@@ -357,6 +357,164 @@ __device__ inline R char_length(const R d[7][3]) {
     if (fa > amax) amax = fa;
   }
   return vol / amax;
+}
+
+// Status bits of element_eval (FT_STATUS_* of the C ABI plus an internal
+// "unknown material" bit).
+constexpr int kFlagNonFinite = 1;
+constexpr int kFlagInverted = 2;
+constexpr int kFlagMaterial = 8;
+
+// One element: xv/uv hold the reference coordinates and displacements of
+// the 8 nodes relative to node 0, in cube-position order p (see p2n).  Writes
+// the 24 internal-force components in C3D8 node order (node-major, xyz) and
+// lowers dt_e when DT is set and the element is not skipped.  `hist` is an
+// accessor with load(gp, k) / store(gp, k, v) for the 18 Prony values of a
+// Gauss point (material 5 only).
+template <typename R, bool FORCE, bool DT, class Hist>
+__host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartDev &pp,
+                                             bool skip_dt, double dt_step, Hist &hist,
+                                             bool update_hist, R outv[24], double &my_dt,
+                                             int &flags) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    wht8(xv[d]);
+    wht8(uv[d]);
+  }
+  // Normalised coefficients c[m][d], m = xi, eta, xi.eta, zeta, xi.zeta,
+  // eta.zeta, xi.eta.zeta (Walsh-Hadamard index m + 1).
+  R cx[7][3], cu[7][3];
+#pragma unroll
+  for (int m = 0; m < 7; ++m)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      cx[m][d] = xv[d][m + 1] * R(0.125);
+      cu[m][d] = uv[d][m + 1] * R(0.125);
+    }
+  if (FORCE) {
+    ViscoCoef vc = {0, 0, 0, 0};
+    if (pp.mat == 5) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
+      const double rt1 = dt_step / pp.t1, rt2 = dt_step / pp.t2;
+      vc.c11 = exp(-rt1);
+      vc.c12 = exp(-rt2);
+      vc.c21 = pp.g1 * (1 - vc.c11) / rt1;
+      vc.c22 = pp.g2 * (1 - vc.c12) / rt2;
+    }
+    const R g = R(kGp), g2 = R(kGp * kGp);
+    R h[3][7];  // force accumulators in the monomial basis
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int m = 0; m < 7; ++m) h[d][m] = R(0);
+    // Gauss points in the reference's order (GaussQuadrature3D.cpp:19-49).
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) {
+      const R sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? R(1) : R(-1);
+      const R sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? R(1) : R(-1);
+      const R sz = gp < 4 ? R(1) : R(-1);
+      // J[d][k] = d(field_d)/d(xi_k) at this Gauss point.
+      R J0[3][3], Ju[3][3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        J0[d][0] = cx[0][d] + (sy * g) * cx[2][d] + (sz * g) * cx[4][d] + (sy * sz * g2) * cx[6][d];
+        J0[d][1] = cx[1][d] + (sx * g) * cx[2][d] + (sz * g) * cx[5][d] + (sx * sz * g2) * cx[6][d];
+        J0[d][2] = cx[3][d] + (sx * g) * cx[4][d] + (sy * g) * cx[5][d] + (sx * sy * g2) * cx[6][d];
+        Ju[d][0] = cu[0][d] + (sy * g) * cu[2][d] + (sz * g) * cu[4][d] + (sy * sz * g2) * cu[6][d];
+        Ju[d][1] = cu[1][d] + (sx * g) * cu[2][d] + (sz * g) * cu[5][d] + (sx * sz * g2) * cu[6][d];
+        Ju[d][2] = cu[3][d] + (sx * g) * cu[4][d] + (sy * g) * cu[5][d] + (sx * sy * g2) * cu[6][d];
+      }
+      // adj(J0): J0^-1 = A / det0 (ShapeFunction_C3D8.cpp:95-123)
+      R A[3][3];
+      A[0][0] = J0[1][1] * J0[2][2] - J0[1][2] * J0[2][1];
+      A[0][1] = J0[0][2] * J0[2][1] - J0[0][1] * J0[2][2];
+      A[0][2] = J0[0][1] * J0[1][2] - J0[0][2] * J0[1][1];
+      A[1][0] = J0[1][2] * J0[2][0] - J0[1][0] * J0[2][2];
+      A[1][1] = J0[0][0] * J0[2][2] - J0[0][2] * J0[2][0];
+      A[1][2] = J0[0][2] * J0[1][0] - J0[0][0] * J0[1][2];
+      A[2][0] = J0[1][0] * J0[2][1] - J0[1][1] * J0[2][0];
+      A[2][1] = J0[0][1] * J0[2][0] - J0[0][0] * J0[2][1];
+      A[2][2] = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
+      const R det0 = J0[0][0] * A[0][0] + J0[0][1] * A[1][0] + J0[0][2] * A[2][0];
+      if (!(det0 > R(0))) flags |= kFlagInverted;
+      const R idet0 = R(1) / det0;
+      // H = Ju J0^-1 = grad_X u  (F = I + H, CalculateDeformationGradient.cpp:15-25)
+      R H[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
+      R hloc[18];
+      R *hp = nullptr;
+      if (pp.mat == 5 && hist.valid()) {
+#pragma unroll
+        for (int k = 0; k < 18; ++k) hloc[k] = hist.load(gp, k);
+        hp = hloc;
+      }
+      Sym<R> S;
+      R Jm1;
+      if (!pk2_stress(pp, H, hp, vc, S, Jm1)) flags |= kFlagMaterial;
+      if (!(Jm1 > R(-1))) flags |= kFlagInverted;
+      if (hp && update_hist) {
+#pragma unroll
+        for (int k = 0; k < 18; ++k) hist.store(gp, k, hloc[k]);
+      }
+      // P = F S (first Piola), Q = w detJ0 P J0^-T = P A^T  (w = 1)
+      R P[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          P[i][j] = sym_at(S, i, j) + H[i][0] * sym_at(S, 0, j) + H[i][1] * sym_at(S, 1, j) +
+                    H[i][2] * sym_at(S, 2, j);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const R q0 = P[i][0] * A[0][0] + P[i][1] * A[0][1] + P[i][2] * A[0][2];
+        const R q1 = P[i][0] * A[1][0] + P[i][1] * A[1][1] + P[i][2] * A[1][2];
+        const R q2 = P[i][0] * A[2][0] + P[i][1] * A[2][1] + P[i][2] * A[2][2];
+        // f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k in the monomial basis
+        h[i][0] += q0;
+        h[i][1] += q1;
+        h[i][3] += q2;
+        h[i][2] += sy * q0 + sx * q1;
+        h[i][4] += sz * q0 + sx * q2;
+        h[i][5] += sz * q1 + sy * q2;
+        h[i][6] += (sy * sz) * q0 + (sx * sz) * q1 + (sx * sy) * q2;
+      }
+    }
+    // Back to the nodes: f_p = sum_q c_q prod(signs), c = h scaled by
+    // 1/8 (linear), g/8 (bilinear), g^2/8 (trilinear).
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R c[8];
+      c[0] = R(0);
+      c[1] = h[d][0] * R(0.125);
+      c[2] = h[d][1] * R(0.125);
+      c[4] = h[d][3] * R(0.125);
+      c[3] = h[d][2] * (g * R(0.125));
+      c[5] = h[d][4] * (g * R(0.125));
+      c[6] = h[d][5] * (g * R(0.125));
+      c[7] = h[d][6] * (g2 * R(0.125));
+      iwht8(c);
+#pragma unroll
+      for (int p = 0; p < 8; ++p) outv[3 * p2n(p) + d] = c[p];
+    }
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < 24; ++k) finite = finite && isfinite(outv[k]);
+    if (!finite) flags |= kFlagNonFinite;
+  }
+  if (DT && !skip_dt) {
+    // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
+    R dcf[7][3];
+#pragma unroll
+    for (int m = 0; m < 7; ++m)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
+    const R le = char_length(dcf);
+    const double dte = (double)le / pp.ce;
+    if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
+  }
 }
 
 }  // namespace ft

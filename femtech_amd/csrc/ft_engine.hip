@@ -180,6 +180,17 @@ __host__ __device__ inline void motion_eval(const double *mo, const double *X, d
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
+// Prony history, component-major in HBM: value k of Gauss point gp of
+// element e at [(gp * 18 + k) * ne + e] (coalesced across a wavefront).
+template <typename R>
+struct HistSoA {
+  R *base;
+  int ne, e;
+  __device__ bool valid() const { return base != nullptr; }
+  __device__ R load(int gp, int k) const { return base[(size_t)(gp * 18 + k) * ne + e]; }
+  __device__ void store(int gp, int k, R v) { base[(size_t)(gp * 18 + k) * ne + e] = v; }
+};
+
 template <typename R>
 struct ElemArgs {
   int ne;
@@ -236,142 +247,11 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
         uv[d][p] = p == 0 ? R(0) : R(a.u[3 * n + d] - U0[d]);
       }
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      wht8(xv[d]);
-      wht8(uv[d]);
-    }
-    // Normalised coefficients c[m][d], m = xi, eta, xi.eta, zeta, xi.zeta,
-    // eta.zeta, xi.eta.zeta (WHT index m + 1).
-    R cx[7][3], cu[7][3];
-#pragma unroll
-    for (int m = 0; m < 7; ++m)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        cx[m][d] = xv[d][m + 1] * R(0.125);
-        cu[m][d] = uv[d][m + 1] * R(0.125);
-      }
+    HistSoA<R> hist{a.hist, a.ne, e};
+    R outv[24];
+    element_eval<R, FORCE, DT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
+                               a.update_hist != 0, outv, my_dt, flags);
     if (FORCE) {
-      ViscoCoef vc = {0, 0, 0, 0};
-      if (pp.mat == 5) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
-        const double rt1 = dt_step / pp.t1, rt2 = dt_step / pp.t2;
-        vc.c11 = exp(-rt1);
-        vc.c12 = exp(-rt2);
-        vc.c21 = pp.g1 * (1 - vc.c11) / rt1;
-        vc.c22 = pp.g2 * (1 - vc.c12) / rt2;
-      }
-      const R g = R(kGp), g2 = R(kGp * kGp);
-      // Force accumulators in the same basis: h[d][m].
-      R h[3][7];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int m = 0; m < 7; ++m) h[d][m] = R(0);
-      // Gauss points in the reference's order (GaussQuadrature3D.cpp:19-49).
-#pragma unroll
-      for (int gp = 0; gp < 8; ++gp) {
-        const R sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? R(1) : R(-1);
-        const R sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? R(1) : R(-1);
-        const R sz = gp < 4 ? R(1) : R(-1);
-        // J[d][k] = d(field_d)/d(xi_k) at this Gauss point.
-        R J0[3][3], Ju[3][3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          J0[d][0] = cx[0][d] + (sy * g) * cx[2][d] + (sz * g) * cx[4][d] + (sy * sz * g2) * cx[6][d];
-          J0[d][1] = cx[1][d] + (sx * g) * cx[2][d] + (sz * g) * cx[5][d] + (sx * sz * g2) * cx[6][d];
-          J0[d][2] = cx[3][d] + (sx * g) * cx[4][d] + (sy * g) * cx[5][d] + (sx * sy * g2) * cx[6][d];
-          Ju[d][0] = cu[0][d] + (sy * g) * cu[2][d] + (sz * g) * cu[4][d] + (sy * sz * g2) * cu[6][d];
-          Ju[d][1] = cu[1][d] + (sx * g) * cu[2][d] + (sz * g) * cu[5][d] + (sx * sz * g2) * cu[6][d];
-          Ju[d][2] = cu[3][d] + (sx * g) * cu[4][d] + (sy * g) * cu[5][d] + (sx * sy * g2) * cu[6][d];
-        }
-        // adj(J0): J0^-1 = A / det0 (ShapeFunction_C3D8.cpp:95-123)
-        R A[3][3];
-        A[0][0] = J0[1][1] * J0[2][2] - J0[1][2] * J0[2][1];
-        A[0][1] = J0[0][2] * J0[2][1] - J0[0][1] * J0[2][2];
-        A[0][2] = J0[0][1] * J0[1][2] - J0[0][2] * J0[1][1];
-        A[1][0] = J0[1][2] * J0[2][0] - J0[1][0] * J0[2][2];
-        A[1][1] = J0[0][0] * J0[2][2] - J0[0][2] * J0[2][0];
-        A[1][2] = J0[0][2] * J0[1][0] - J0[0][0] * J0[1][2];
-        A[2][0] = J0[1][0] * J0[2][1] - J0[1][1] * J0[2][0];
-        A[2][1] = J0[0][1] * J0[2][0] - J0[0][0] * J0[2][1];
-        A[2][2] = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
-        const R det0 = J0[0][0] * A[0][0] + J0[0][1] * A[1][0] + J0[0][2] * A[2][0];
-        if (!(det0 > R(0))) flags |= FT_STATUS_INVERTED;
-        const R idet0 = R(1) / det0;
-        // H = Ju J0^-1 = grad_X u  (F = I + H, CalculateDeformationGradient.cpp:15-25)
-        R H[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
-        R hloc[18];
-        R *hp = nullptr;
-        if (pp.mat == 5 && a.hist) {
-#pragma unroll
-          for (int k = 0; k < 18; ++k) hloc[k] = a.hist[(size_t)(gp * 18 + k) * a.ne + e];
-          hp = hloc;
-        }
-        Sym<R> S;
-        R Jm1;
-        if (!pk2_stress(pp, H, hp, vc, S, Jm1)) flags |= 8;
-        if (!(Jm1 > R(-1))) flags |= FT_STATUS_INVERTED;
-        if (hp && a.update_hist) {
-#pragma unroll
-          for (int k = 0; k < 18; ++k) a.hist[(size_t)(gp * 18 + k) * a.ne + e] = hloc[k];
-        }
-        // P = F S (first Piola), Q = w detJ0 P J0^-T = P A^T  (w = 1)
-        R P[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            P[i][j] = sym_at(S, i, j) + H[i][0] * sym_at(S, 0, j) + H[i][1] * sym_at(S, 1, j) +
-                      H[i][2] * sym_at(S, 2, j);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const R q0 = P[i][0] * A[0][0] + P[i][1] * A[0][1] + P[i][2] * A[0][2];
-          const R q1 = P[i][0] * A[1][0] + P[i][1] * A[1][1] + P[i][2] * A[1][2];
-          const R q2 = P[i][0] * A[2][0] + P[i][1] * A[2][1] + P[i][2] * A[2][2];
-          // f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k in the monomial basis
-          h[i][0] += q0;
-          h[i][1] += q1;
-          h[i][3] += q2;
-          h[i][2] += sy * q0 + sx * q1;
-          h[i][4] += sz * q0 + sx * q2;
-          h[i][5] += sz * q1 + sy * q2;
-          h[i][6] += (sy * sz) * q0 + (sx * sz) * q1 + (sx * sy) * q2;
-        }
-      }
-      // Back to the nodes: f_p = sum_q c_q prod(signs), c = h scaled by
-      // 1/8 (linear), g/8 (bilinear), g^2/8 (trilinear).
-      R fo[3][8];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        R c[8];
-        c[0] = R(0);
-        c[1] = h[d][0] * R(0.125);
-        c[2] = h[d][1] * R(0.125);
-        c[4] = h[d][3] * R(0.125);
-        c[3] = h[d][2] * (g * R(0.125));
-        c[5] = h[d][4] * (g * R(0.125));
-        c[6] = h[d][5] * (g * R(0.125));
-        c[7] = h[d][6] * (g2 * R(0.125));
-        iwht8(c);
-#pragma unroll
-        for (int p = 0; p < 8; ++p) fo[d][p] = c[p];
-      }
-      R outv[24];
-#pragma unroll
-      for (int p = 0; p < 8; ++p) {
-        const int nl = p2n(p);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) outv[3 * nl + d] = fo[d][p];
-      }
-      bool finite = true;
-#pragma unroll
-      for (int k = 0; k < 24; ++k) finite = finite && isfinite(outv[k]);
-      if (!finite) flags |= FT_STATUS_NONFINITE;
       if (sizeof(R) == 4) {
         float4 *dst = reinterpret_cast<float4 *>(a.fe + (size_t)24 * e);
 #pragma unroll
@@ -382,17 +262,6 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) dst[k] = make_double2(outv[2 * k], outv[2 * k + 1]);
       }
-    }
-    if (DT && !(info & kSkipBit)) {
-      // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
-      R dcf[7][3];
-#pragma unroll
-      for (int m = 0; m < 7; ++m)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
-      const R le = char_length(dcf);
-      const double dte = (double)le / pp.ce;
-      if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
     }
   }
   if (DT) {

@@ -1,0 +1,121 @@
+"""CPU-side checks: the C ABI library loads and exports every declared entry
+point, and the host mesh tools (generator, .inp/.k readers, materials.dat)
+follow the reference's conventions."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import femtech_amd as ft
+from conftest import ROOT, golden
+from oracle import oracle as O
+
+
+def declared_symbols():
+    names = set()
+    for hdr in os.listdir(os.path.join(ROOT, "include")):
+        if not hdr.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(ft_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(ft.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_device_count_without_gpu_is_harmless():
+    assert ft.device_count() >= 0
+
+
+def test_engine_create_without_device_fails_loudly():
+    if ft.device_count() > 0:
+        pytest.skip("a GPU is present")
+    X, conn = ft.mesh_block(1, 1, 1, 1e-3)
+    with pytest.raises(ft.FemTechError) as ei:
+        ft.Engine(X, conn, np.zeros(1, np.int32), np.array([1], np.int32), np.zeros(9))
+    assert ei.value.code == ft.ERR_HIP
+
+
+def test_block_generator_geometry():
+    X, conn = ft.mesh_block(4, 3, 2, (1e-3, 2e-3, 3e-3))
+    X = X.reshape(-1, 3)
+    assert X.shape == (5 * 4 * 3, 3)
+    assert conn.size == 8 * 24
+    # every element is a positive-volume box of the requested size
+    for e in range(24):
+        xe = X[conn[8 * e:8 * e + 8]]
+        assert O.hex_volume(xe) == pytest.approx(6e-9, rel=1e-12)
+    # node numbering x-fastest
+    assert np.allclose(X[1], [1e-3, 0, 0]) and np.allclose(X[5], [0, 2e-3, 0])
+
+
+def test_block_jitter_is_seeded_interior_only():
+    X1, _ = ft.mesh_block(5, 5, 5, 1e-3, jitter=0.15, seed=42)
+    X2, _ = ft.mesh_block(5, 5, 5, 1e-3, jitter=0.15, seed=42)
+    X3, _ = ft.mesh_block(5, 5, 5, 1e-3, jitter=0.15, seed=43)
+    X0, _ = ft.mesh_block(5, 5, 5, 1e-3)
+    assert np.array_equal(X1, X2) and not np.array_equal(X1, X3)
+    d = (X1 - X0).reshape(-1, 3)
+    assert np.abs(d).max() <= 0.15e-3 + 1e-18
+    Xr = X0.reshape(-1, 3)
+    on_face = np.any((Xr < 1e-9) | (Xr > 5e-3 - 1e-9), axis=1)
+    assert np.all(d[on_face] == 0) and np.all(np.abs(d[~on_face]).max(axis=1) > 0)
+
+
+def test_inp_round_trip(tmp_path):
+    X, conn = ft.mesh_block(3, 2, 2, 1e-3, jitter=0.1, seed=3)
+    pid = np.array([0] * 4 + [1] * 4 + [2] * 4, np.int32)
+    p = str(tmp_path / "m.inp")
+    ft.write_inp(p, X, conn, pid)
+    X2, conn2, pid2, gid = ft.read_mesh(p)
+    assert np.array_equal(X, X2) and np.array_equal(conn, conn2) and np.array_equal(pid, pid2)
+    assert np.array_equal(gid, np.arange(1, X.size // 3 + 1))
+
+
+def test_reader_renumbers_like_partition_mesh(tmp_path):
+    """Node ids need not be contiguous: local node i is the i-th smallest id
+    used by the connectivity (PartitionMesh.cpp:489-530)."""
+    p = tmp_path / "gap.inp"
+    ids = [10, 3, 7, 40, 12, 5, 9, 1, 99]  # 99 is unused
+    coords = {i: (0.001 * k, 0.002 * k, 0.003 * k) for k, i in enumerate(ids)}
+    lines = ["*Node"] + ["%d, %r, %r, %r" % ((i,) + coords[i]) for i in ids]
+    lines += ["*Element, type=C3D8, elset=A", "1, 10, 3, 7, 40, 12, 5, 9, 1"]
+    p.write_text("\n".join(lines) + "\n")
+    X, conn, pid, gid = ft.read_mesh(str(p))
+    used = sorted([10, 3, 7, 40, 12, 5, 9, 1])
+    assert list(gid) == used
+    assert list(conn) == [used.index(i) for i in [10, 3, 7, 40, 12, 5, 9, 1]]
+    assert np.allclose(X.reshape(-1, 3)[used.index(40)], coords[40])
+
+
+def test_reader_matches_reference_on_its_meshes():
+    for name in ("ex9", "bench10"):
+        z = golden("run_" + name)
+        assert np.array_equal(z["X"], z["ref_coords"])
+        assert np.array_equal(z["conn"], z["ref_conn"])
+
+
+def test_read_materials(tmp_path):
+    p = tmp_path / "materials.dat"
+    p.write_text("1 5 1000 1 2 3 4 5 6 7 8\n0 1 1040 100.0 100.00\n\n# comment\n")
+    mat, props = ft.read_materials(str(p), 2)
+    assert list(mat) == [1, 5]
+    assert np.allclose(props.reshape(2, 9)[0, :3], [1040, 100, 100])
+    assert np.allclose(props.reshape(2, 9)[1], [1000, 1, 2, 3, 4, 5, 6, 7, 8])
+    p.write_text("0 9 1 2 3\n")
+    with pytest.raises(ft.FemTechError) as ei:
+        ft.read_materials(str(p), 1)
+    assert ei.value.code == ft.ERR_INPUT
+    p.write_text("0 1 1 2 3\n")
+    with pytest.raises(ft.FemTechError):
+        ft.read_materials(str(p), 2)  # part 1 missing

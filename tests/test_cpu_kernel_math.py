@@ -1,0 +1,113 @@
+"""The element kernel's math, built for the host from the same header the
+HIP kernel compiles (femtech_amd/csrc/ft_device.h: element_eval), against the
+oracle on the golden meshes: forces for every material, the stable-step
+length, and multi-step Prony history."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import HERE, golden
+from oracle import oracle as O
+
+SRC = os.path.join(HERE, "native", "kernel_math_host.cpp")
+OUT = os.path.join(HERE, "native", "_build", "libkm.so")
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def km():
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-fPIC", "-shared", "-x", "c++",
+                    "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", SRC, "-o", OUT], check=True)
+    lib = ctypes.CDLL(OUT)
+    lib.km_element.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int, _D, ctypes.c_double, _D,
+                               ctypes.c_int, _D, _D]
+    return lib
+
+
+def dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def elem(km, fp32, x24, u24, mat, p9, dt=1e-5, hist=None, update=1):
+    f = np.zeros(24)
+    dte = np.zeros(1)
+    flags = km.km_element(int(fp32), dp(np.ascontiguousarray(x24)), dp(np.ascontiguousarray(u24)),
+                          int(mat), dp(np.ascontiguousarray(p9)), dt, dp(hist), update, dp(f), dp(dte))
+    return f, dte[0], flags
+
+
+def field(X, scale, seed):
+    rng = np.random.default_rng(seed)
+    return (scale * 1e-3 * (np.sin(X / 3e-3) + 0.3 * rng.uniform(-1, 1, X.shape))).reshape(-1)
+
+
+@pytest.mark.parametrize("name", ["bench10", "impact", "materials"])
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_element_forces_and_dt(km, name, fp32):
+    z = golden("run_" + name)
+    X = z["X"].reshape(-1, 3)
+    conn = z["conn"].reshape(-1, 8)
+    props = z["props"].reshape(-1, 9)
+    mats = z["material_id"][z["pid"]]
+    o = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    for scale, seed in ((1e-3, 1), (5e-2, 2)):
+        u = field(X, scale, seed)
+        fo = o.element_forces(u, 1e-5).reshape(-1, 24)
+        num = den = 0.0
+        worst = 0.0
+        for e in range(len(conn)):
+            x24 = X[conn[e]].reshape(24)
+            u24 = u.reshape(-1, 3)[conn[e]].reshape(24)
+            f, dte, flags = elem(km, fp32, x24, u24, mats[e], props[z["pid"][e]], hist=np.zeros(144))
+            assert flags == 0
+            num += np.sum((f - fo[e]) ** 2)
+            den += np.sum(fo[e] ** 2)
+            if mats[e] != 0:
+                p = props[z["pid"][e]]
+                nu = 0.5 * p[2] / (p[2] + p[1])
+                ref_dt = O.char_length(x24 + u24) / np.sqrt(p[2] * (1 / nu - 1) / p[0])
+                worst = max(worst, abs(dte - ref_dt) / ref_dt)
+        err = np.sqrt(num / den)
+        assert err < (1e-6 if fp32 else 1e-12), (name, scale, err)
+        assert worst < (1e-6 if fp32 else 1e-13), (name, scale, worst)
+
+
+def test_prony_history_multi_step(km):
+    """Material 5 history over a deformation path, against the oracle's
+    HGOIsotropicViscoelastic restatement (pinned by the stress KATs)."""
+    z = golden("run_materials")
+    X = z["X"].reshape(-1, 3)
+    conn = z["conn"].reshape(-1, 8)
+    e = int(np.nonzero(z["material_id"][z["pid"]] == 5)[0][0])
+    p = z["props"].reshape(-1, 9)[z["pid"][e]]
+    x24 = X[conn[e]].reshape(24)
+    o = O.Model(x24, np.arange(8, dtype=np.int32), np.zeros(1, np.int32), np.array([5], np.int32), p)
+    hist = np.zeros(144)
+    for k in range(8):
+        u24 = field(x24.reshape(8, 3), 2e-2 * (k + 1), 30 + k)
+        f, _, flags = elem(km, 0, x24, u24, 5, p, dt=3e-5, hist=hist)
+        fo = o.element_forces(u24, 3e-5, update_history=True)
+        assert np.linalg.norm(f - fo) <= 1e-12 * np.linalg.norm(fo)
+    ho = o.history().reshape(8, 3, 9)[..., [0, 4, 8, 7, 6, 3]]
+    assert np.abs(hist.reshape(8, 3, 6) - ho).max() <= 1e-12 * np.abs(ho).max()
+
+
+def test_rigid_rotation_gives_zero_force(km):
+    """Objectivity: a finite rigid rotation produces no internal force."""
+    z = golden("run_materials")
+    X = z["X"].reshape(-1, 3)
+    conn = z["conn"].reshape(-1, 8)
+    c, s = np.cos(0.7), np.sin(0.7)
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    for mat, row in ((1, 0), (2, 1), (4, 3), (5, 4)):
+        e = int(np.nonzero(z["pid"] == row)[0][0])
+        x = X[conn[e]]
+        u = (x @ R.T - x).reshape(24)
+        p = z["props"].reshape(-1, 9)[row]
+        f, _, _ = elem(km, 0, x.reshape(24), u, mat, p, hist=np.zeros(144))
+        f_ref, _, _ = elem(km, 0, x.reshape(24), 1e-3 * u, mat, p, hist=np.zeros(144))
+        assert np.abs(f).max() <= 1e-9 * max(np.abs(f_ref).max(), p[1] * 1e-6)

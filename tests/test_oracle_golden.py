@@ -1,0 +1,111 @@
+"""The oracle against the reference build's golden vectors (tests/golden)."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel
+from oracle import oracle as O
+
+RUNS = ["ex9", "bench10", "c1", "impact", "materials"]
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_setup_matches_reference(name):
+    z = golden("run_" + name)
+    m = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    dN, dJ = m.shape_functions()
+    # ShapeFunctions.cpp:207-228 and AssembleLumpedMass (Mass3D.cpp:127-157)
+    assert np.array_equal(z["X"], z["ref_coords"])
+    assert np.array_equal(z["conn"], z["ref_conn"])
+    assert rel(dN, z["ref_dshp"]) < 1e-14
+    assert rel(dJ, z["ref_detJ"]) < 1e-14
+    assert rel(m.lumped_mass(), z["ref_mass"]) < 1e-14
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_run_matches_reference(name):
+    z = golden("run_" + name)
+    m = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    m.set_bc(z["dof_motion"], z["motions"])
+    m.set_initial(None, z["v0"] if "v0" in z else None)
+    r = m.run(int(z["max_steps"]), float(z["tmax"]), float(z["dt_factor"]), float(z["failure_dt"]),
+              list(z["snaps"]))
+    assert r["steps"] == int(z["ref_steps"])
+    # FP64 restatement vs FP64 reference: round-off only
+    assert rel(r["u"], z["ref_u"]) < 1e-12
+    assert rel(r["snap_u"], z["ref_snap_u"]) < 1e-12
+    assert rel(r["v"], z["ref_v"]) < 1e-10
+    assert rel(r["a"], z["ref_a"]) < 1e-9
+    assert rel(r["fi"], z["ref_fi"]) < 1e-10
+    assert rel(r["time"], z["ref_time"]) < 1e-14
+    assert rel(r["dt"], z["ref_dt"]) < 1e-13
+    assert rel(r["energy"], z["ref_energy"]) < 1e-12
+    # CheckEnergy's own rows (printed with 7 significant digits)
+    ef = z["ref_energy_file"]
+    E = r["energy"][1:]
+    for col, k in ((1, 0), (2, 1), (3, 2)):
+        scale = max(np.abs(ef[:, col]).max(), 1e-300)
+        assert np.abs(E[:, k] - ef[:, col]).max() <= 1e-6 * scale + 1e-30
+
+
+def test_initial_force_matches_reference():
+    for name in RUNS:
+        z = golden("run_" + name)
+        m = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+        m.set_bc(z["dof_motion"], z["motions"])
+        m.set_initial(None, z["v0"] if "v0" in z else None)
+        r = m.run(0, float(z["tmax"]), float(z["dt_factor"]), float(z["failure_dt"]))
+        assert rel(r["fi"], z["ref_fi0"]) < 1e-12
+        assert rel(r["a"], z["ref_a0"]) < 1e-12
+
+
+def test_stress_kats():
+    z = golden("kat_stress")
+    for i in range(len(z["material"])):
+        hist = np.zeros(27)
+        for t in range(z["F"].shape[1]):
+            S, J = O.stress(int(z["material"][i]), z["props"][i], z["F"][i, t], float(z["dt"][i]), hist)
+            ref = z["ref_pk2"][i, t]
+            assert abs(J - z["ref_detF"][i, t]) <= 1e-15 * abs(J)
+            scale = max(np.abs(ref).max(), 1e-12 * float(z["props"][i][1]))
+            assert np.abs(S - ref).max() <= 1e-12 * scale, (i, t, S, ref)
+        assert np.abs(hist - z["ref_hist"][i]).max() <= 1e-12 * max(np.abs(z["ref_hist"][i]).max(), 1.0)
+
+
+def test_geometry_kats():
+    z = golden("kat_geom")
+    for i, xe in enumerate(z["xe"]):
+        assert O.hex_volume(xe) == pytest.approx(z["ref_volume"][i], rel=1e-14)
+        for f in range(6):
+            assert O.face_area(xe, f) == pytest.approx(z["ref_area"][i, f], rel=1e-14)
+        assert O.char_length(xe) == pytest.approx(z["ref_char_length"][i], rel=1e-14)
+
+
+def test_geometry_kats_cover_shortcut_branch():
+    """Some faces take the parallelogram shortcut while not being
+    parallelograms (negative twist, Geometry.cpp:46 has no fabs)."""
+    z = golden("kat_geom")
+    twisted = 0
+    for xe in z["xe"]:
+        x = xe.reshape(8, 3)
+        for f in range(6):
+            p = x[O.FACES[4 * f:4 * f + 4]]
+            cD = 0.25 * (p[0] - p[1] + p[2] - p[3])
+            if np.all(cD < 1e-6) and np.abs(cD).max() > 1e-6:
+                twisted += 1
+    assert twisted > 20
+
+
+def test_ex9_against_abaqus_report():
+    """The reference CI check (examples/ex9/compareResults.py:14-28): final
+    time within 1 %, displacement of the loaded corner within 5 %."""
+    z = golden("run_ex9")
+    rpt = z["abaqus_rpt"]
+    m = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    m.set_bc(z["dof_motion"], z["motions"])
+    r = m.run(int(z["max_steps"]), float(z["tmax"]), float(z["dt_factor"]))
+    X = z["X"].reshape(-1, 3)
+    node = int(np.nonzero(np.all(np.abs(X - 0.005) < 1e-5, axis=1))[0][0])
+    u = r["u"][3 * node:3 * node + 3]
+    t_ab, u_ab = rpt[-1, 0], rpt[-1, 9:12]  # columns X, A(4), RF(4), U mag, U1..U3
+    assert abs(r["time"][-1] - t_ab) / t_ab < 0.01
+    assert np.max(np.abs(u - rpt[-1, 10:13]) / np.abs(rpt[-1, 10:13])) < 0.05
