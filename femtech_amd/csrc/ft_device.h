@@ -33,12 +33,21 @@ template <> struct Num<float> {
   __host__ __device__ static float expm1_(float x) { return expm1f(x); }
   __host__ __device__ static float exp_(float x) { return expf(x); }
   __host__ __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+  // hardware reciprocal (1 ulp) in the FP32 element path
+  __host__ __device__ static float rcp_(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+  }
 };
 template <> struct Num<double> {
   __host__ __device__ static double log1p_(double x) { return log1p(x); }
   __host__ __device__ static double expm1_(double x) { return expm1(x); }
   __host__ __device__ static double exp_(double x) { return exp(x); }
   __host__ __device__ static double sqrt_(double x) { return sqrt(x); }
+  __host__ __device__ static double rcp_(double x) { return 1.0 / x; }
 };
 
 // Position p = i + 2j + 4k of the 2x2x2 node cube (i: xi, j: eta, k: zeta
@@ -150,12 +159,14 @@ struct ViscoCoef { double c11, c12, c21, c22; };
 
 // PK2 stress of the reference's material routines, restated in terms of the
 // displacement gradient H = F - I.  hist: 18 values (H1, H2, S0 as Voigt
-// 6-vectors) for material 5 or nullptr.  Returns false for an unknown id.
-template <typename R>
+// 6-vectors) for material 5 or nullptr.  MAT >= 0 fixes the material at
+// compile time; MAT < 0 dispatches on pp.mat.  Returns false for an unknown
+// id.
+template <typename R, int MAT = -1>
 __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R *hist,
                                   const ViscoCoef &vc, Sym<R> &S, R &Jm1) {
   Jm1 = det_ip1_m1(H);
-  const int mat = pp.mat;
+  const int mat = MAT >= 0 ? MAT : pp.mat;
   if (mat == 0) {  // rigid: StressUpdate.cpp:8-9
 #pragma unroll
     for (int k = 0; k < 6; ++k) S.v[k] = R(0);
@@ -189,7 +200,7 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
     A[2][0] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
     A[2][1] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
     A[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
-    R iJ = R(1) / (R(1) + Jm1);
+    R iJ = Num<R>::rcp_(R(1) + Jm1);
     const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -210,7 +221,7 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
   }
   const R lnJ = Num<R>::log1p_(Jm1);
   const R J = R(1) + Jm1;
-  const R iJ2 = R(1) / (J * J);
+  const R iJ2 = Num<R>::rcp_(J * J);
   Sym<R> Ac = adj_c(E);  // C^-1 = Ac / J^2
   if (mat == 1) {
     // CompressibleNeoHookean.cpp:26-55: S = mu (I - C^-1) + lambda ln J C^-1
@@ -286,27 +297,47 @@ __host__ __device__ inline R triple(const R s[3], const R a[3], const R b[3]) {
          s[1] * (a[0] * b[2] - a[2] * b[0]);
 }
 
+template <typename R>
+__host__ __device__ inline void cross3(const R a[3], const R b[3], R c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = -a[0] * b[2] + a[2] * b[0];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename R>
+__host__ __device__ inline R norm3(const R x[3]) {
+  return Num<R>::sqrt_(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+}
+
 // areaHexahedronFace (Geometry.cpp:29-64) from a face's twist cD and edge
 // vectors c1, c2 (each the quarter-sum of the reference's corner formula).
+// The 2x2 rule sums |v1 x v2| with v1 = q_j cD + c1, v2 = q_i cD + c2; since
+// cD x cD = 0 that is |c1 x c2 + q_i (c1 x cD) + q_j (cD x c2)|, so three cross
+// products serve all four points.
 template <typename R>
 __host__ __device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3]) {
   const R tol = R(1e-6);
-  if ((cD[0] < tol) && (cD[1] < tol) && (cD[2] < tol)) return R(4) * norm_cross(c1, c2);
+  R w0[3];
+  cross3(c1, c2, w0);
+  if ((cD[0] < tol) && (cD[1] < tol) && (cD[2] < tol)) return R(4) * norm3(w0);
+  R w1[3], w2[3];
+  cross3(c1, cD, w1);
+  cross3(cD, c2, w2);
   const R t = R(kFaceQ);
-  const R q[2] = {-t, t};
   R area = R(0);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
+    R b[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b[k] = w0[k] + (i ? t : -t) * w1[k];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      R v1[3], v2[3];
+      R x[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        v1[k] = q[j] * cD[k] + c1[k];
-        v2[k] = q[i] * cD[k] + c2[k];
-      }
-      area += norm_cross(v1, v2);
+      for (int k = 0; k < 3; ++k) x[k] = b[k] + (j ? t : -t) * w2[k];
+      area += norm3(x);
     }
+  }
   return area;
 }
 
@@ -371,7 +402,7 @@ constexpr int kFlagMaterial = 8;
 // lowers dt_e when DT is set and the element is not skipped.  `hist` is an
 // accessor with load(gp, k) / store(gp, k, v) for the 18 Prony values of a
 // Gauss point (material 5 only).
-template <typename R, bool FORCE, bool DT, class Hist>
+template <typename R, bool FORCE, bool DT, int MAT, class Hist>
 __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartDev &pp,
                                              bool skip_dt, double dt_step, Hist &hist,
                                              bool update_hist, R outv[24], double &my_dt,
@@ -391,9 +422,13 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
       cx[m][d] = xv[d][m + 1] * R(0.125);
       cu[m][d] = uv[d][m + 1] * R(0.125);
     }
-  if (FORCE) {
+  const int mat = MAT >= 0 ? MAT : pp.mat;
+  if (FORCE && mat == 0) {  // rigid: no stress, no internal force
+#pragma unroll
+    for (int k = 0; k < 24; ++k) outv[k] = R(0);
+  } else if (FORCE) {
     ViscoCoef vc = {0, 0, 0, 0};
-    if (pp.mat == 5) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
+    if (mat == 5) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
       const double rt1 = dt_step / pp.t1, rt2 = dt_step / pp.t2;
       vc.c11 = exp(-rt1);
       vc.c12 = exp(-rt2);
@@ -406,11 +441,13 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int m = 0; m < 7; ++m) h[d][m] = R(0);
-    // Gauss points in the reference's order (GaussQuadrature3D.cpp:19-49).
-#pragma unroll
+    // Gauss points in the reference's order (GaussQuadrature3D.cpp:19-49):
+    // (-,-,+), (+,-,+), (+,+,+), (-,+,+), then the same at zeta = -.  The loop
+    // is kept rolled so one Gauss point's temporaries are live at a time.
+#pragma unroll 1
     for (int gp = 0; gp < 8; ++gp) {
-      const R sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? R(1) : R(-1);
-      const R sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? R(1) : R(-1);
+      const R sx = ((gp & 3) == 1 || (gp & 3) == 2) ? R(1) : R(-1);
+      const R sy = (gp & 2) ? R(1) : R(-1);
       const R sz = gp < 4 ? R(1) : R(-1);
       // J[d][k] = d(field_d)/d(xi_k) at this Gauss point.
       R J0[3][3], Ju[3][3];
@@ -436,7 +473,7 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
       A[2][2] = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
       const R det0 = J0[0][0] * A[0][0] + J0[0][1] * A[1][0] + J0[0][2] * A[2][0];
       if (!(det0 > R(0))) flags |= kFlagInverted;
-      const R idet0 = R(1) / det0;
+      const R idet0 = Num<R>::rcp_(det0);
       // H = Ju J0^-1 = grad_X u  (F = I + H, CalculateDeformationGradient.cpp:15-25)
       R H[3][3];
 #pragma unroll
@@ -446,14 +483,14 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
           H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
       R hloc[18];
       R *hp = nullptr;
-      if (pp.mat == 5 && hist.valid()) {
+      if (mat == 5 && hist.valid()) {
 #pragma unroll
         for (int k = 0; k < 18; ++k) hloc[k] = hist.load(gp, k);
         hp = hloc;
       }
       Sym<R> S;
       R Jm1;
-      if (!pk2_stress(pp, H, hp, vc, S, Jm1)) flags |= kFlagMaterial;
+      if (!pk2_stress<R, MAT>(pp, H, hp, vc, S, Jm1)) flags |= kFlagMaterial;
       if (!(Jm1 > R(-1))) flags |= kFlagInverted;
       if (hp && update_hist) {
 #pragma unroll

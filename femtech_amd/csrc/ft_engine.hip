@@ -129,24 +129,24 @@ __device__ inline double block_sum(double v, double *sh) {
 // ---------------------------------------------------------------------------
 // Boundary motions (the reference drivers' ApplyBoundaryConditions, as data)
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void motion_eval(const double *mo, const double *X, double t,
-                                            double u[3], double v[3], double a[3]) {
-  int type = (int)mo[0];
-  for (int d = 0; d < 3; ++d) u[d] = v[d] = a[d] = 0.0;
+// A motion at one instant: the scalar part (time functions) is evaluated once
+// per block and per time into LDS; the per-node part is cheap and inline.
+struct MotionT {
+  double s0, s1, s2, s3;  // LINEAR/RAMP: u, v scalars; ROT: cos, sin, omega, alpha
+};
+
+__host__ __device__ inline void motion_time(const double *mo, double t, MotionT &m) {
+  const int type = (int)mo[0];
+  m.s0 = m.s1 = m.s2 = m.s3 = 0.0;
   if (type == FT_MOTION_LINEAR) {  // ex9.cpp:191-243: AppliedDisp = Time*dMax/tMax
-    for (int d = 0; d < 3; ++d) {
-      u[d] = t * mo[1];
-      v[d] = mo[1];
-    }
+    m.s0 = t * mo[1];
+    m.s1 = mo[1];
   } else if (type == FT_MOTION_RAMP_HOLD) {
-    double amp = mo[1], T = mo[2];
-    double s = t < T ? t / T : 1.0;
-    for (int d = 0; d < 3; ++d) {
-      u[d] = amp * s;
-      v[d] = t < T ? amp / T : 0.0;
-    }
+    const double amp = mo[1], T = mo[2];
+    const double s = t < T ? t / T : 1.0;
+    m.s0 = amp * s;
+    m.s1 = t < T ? amp / T : 0.0;
   } else if (type == FT_MOTION_ROT_HALFSINE) {
-    const double n0 = mo[1], n1 = mo[2], n2 = mo[3];
     const double A = mo[7], T = mo[8];
     const double pi = 3.14159265358979323846;
     double alpha, omega, theta;
@@ -159,22 +159,48 @@ __host__ __device__ inline void motion_eval(const double *mo, const double *X, d
       omega = 2.0 * A * T / pi;
       theta = A * T * T / pi + omega * (t - T);
     }
-    double r0[3] = {X[0] - mo[4], X[1] - mo[5], X[2] - mo[6]};
-    double ct = cos(theta), st = sin(theta);
-    double nxr0[3] = {n1 * r0[2] - n2 * r0[1], n2 * r0[0] - n0 * r0[2], n0 * r0[1] - n1 * r0[0]};
-    double ndr0 = n0 * r0[0] + n1 * r0[1] + n2 * r0[2];
+    m.s0 = cos(theta);
+    m.s1 = sin(theta);
+    m.s2 = omega;
+    m.s3 = alpha;
+  }
+}
+
+// Component d of (u, v, a) of motion `mo` at the node with reference
+// position X, given the motion's scalar state m.
+__host__ __device__ inline void motion_apply(const double *mo, const MotionT &m, const double *X,
+                                             int d, double &u, double &v, double &a) {
+  const int type = (int)mo[0];
+  u = v = a = 0.0;
+  if (type == FT_MOTION_LINEAR || type == FT_MOTION_RAMP_HOLD) {
+    u = m.s0;
+    v = m.s1;
+  } else if (type == FT_MOTION_ROT_HALFSINE) {
+    // Rodrigues: r = r0 cos + (n x r0) sin + n (n.r0)(1 - cos); v = w n x r;
+    // a = alpha n x r + w^2 n x (n x r)
+    const double n0 = mo[1], n1 = mo[2], n2 = mo[3];
+    const double r0[3] = {X[0] - mo[4], X[1] - mo[5], X[2] - mo[6]};
+    const double ct = m.s0, st = m.s1;
+    const double nxr0[3] = {n1 * r0[2] - n2 * r0[1], n2 * r0[0] - n0 * r0[2], n0 * r0[1] - n1 * r0[0]};
+    const double ndr0 = n0 * r0[0] + n1 * r0[1] + n2 * r0[2];
     const double n[3] = {n0, n1, n2};
     double r[3];
-    for (int d = 0; d < 3; ++d) r[d] = r0[d] * ct + nxr0[d] * st + n[d] * ndr0 * (1.0 - ct);
-    double nxr[3] = {n1 * r[2] - n2 * r[1], n2 * r[0] - n0 * r[2], n0 * r[1] - n1 * r[0]};
-    double nxnxr[3] = {n1 * nxr[2] - n2 * nxr[1], n2 * nxr[0] - n0 * nxr[2],
-                       n0 * nxr[1] - n1 * nxr[0]};
-    for (int d = 0; d < 3; ++d) {
-      u[d] = r[d] - r0[d];
-      v[d] = omega * nxr[d];
-      a[d] = alpha * nxr[d] + omega * omega * nxnxr[d];
-    }
+    for (int k = 0; k < 3; ++k) r[k] = r0[k] * ct + nxr0[k] * st + n[k] * ndr0 * (1.0 - ct);
+    const double nxr[3] = {n1 * r[2] - n2 * r[1], n2 * r[0] - n0 * r[2], n0 * r[1] - n1 * r[0]};
+    const double nxnxr[3] = {n1 * nxr[2] - n2 * nxr[1], n2 * nxr[0] - n0 * nxr[2],
+                             n0 * nxr[1] - n1 * nxr[0]};
+    u = r[d] - r0[d];
+    v = m.s2 * nxr[d];
+    a = m.s3 * nxr[d] + m.s2 * m.s2 * nxnxr[d];
   }
+}
+
+// Whole-vector evaluation (host set-up and read-out).
+__host__ __device__ inline void motion_eval(const double *mo, const double *X, double t, double u[3],
+                                            double v[3], double a[3]) {
+  MotionT m;
+  motion_time(mo, t, m);
+  for (int d = 0; d < 3; ++d) motion_apply(mo, m, X, d, u[d], v[d], a[d]);
 }
 
 // ---------------------------------------------------------------------------
@@ -193,11 +219,13 @@ struct HistSoA {
 
 template <typename R>
 struct ElemArgs {
-  int ne;
+  int nn;                // nodes (stride of the component-major nodal arrays)
+  int ne;                // all elements (history stride)
+  int begin, end;        // this launch's element range (one material)
   const int *conn;       // 8 per element
   const int *einfo;      // part id | skip-dt flag
-  const double *X;       // 3 per node
-  const double *u;       // 3 per node
+  const double *X;       // component-major [3][nn]
+  const double *u;       // component-major [3][nn]
   const PartDev *parts;
   R *fe;                 // 24 per element (nullable when only dt is wanted)
   R *hist;               // 144 per element, component-major [c * ne + e]
@@ -210,11 +238,11 @@ struct ElemArgs {
   int update_hist;
 };
 
-template <typename R, bool FORCE, bool DT>
+template <typename R, bool FORCE, bool DT, int MAT>
 __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
   __shared__ double sh_min[kBlock / 64];
   const int L = xcd_block();
-  const int e = L * kBlock + threadIdx.x;
+  const int e = a.begin + L * kBlock + threadIdx.x;
   double dt_step = a.dt_fixed;
   if (a.slot) {
     if (a.slot->done) return;  // uniform over the grid
@@ -222,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
   }
   double my_dt = 1e300;
   int flags = 0;
-  if (e < a.ne) {
+  if (e < a.end) {
     const int4 c0 = reinterpret_cast<const int4 *>(a.conn)[2 * e];
     const int4 c1 = reinterpret_cast<const int4 *>(a.conn)[2 * e + 1];
     const int nd[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
@@ -234,8 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
     double X0[3], U0[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      X0[d] = a.X[3 * nd[0] + d];
-      U0[d] = a.u[3 * nd[0] + d];
+      X0[d] = a.X[(size_t)d * a.nn + nd[0]];
+      U0[d] = a.u[(size_t)d * a.nn + nd[0]];
     }
     R xv[3][8], uv[3][8];
 #pragma unroll
@@ -243,24 +271,24 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
       const int n = nd[p2n(p)];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        xv[d][p] = p == 0 ? R(0) : R(a.X[3 * n + d] - X0[d]);
-        uv[d][p] = p == 0 ? R(0) : R(a.u[3 * n + d] - U0[d]);
+        xv[d][p] = p == 0 ? R(0) : R(a.X[(size_t)d * a.nn + n] - X0[d]);
+        uv[d][p] = p == 0 ? R(0) : R(a.u[(size_t)d * a.nn + n] - U0[d]);
       }
     }
     HistSoA<R> hist{a.hist, a.ne, e};
     R outv[24];
-    element_eval<R, FORCE, DT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
+    element_eval<R, FORCE, DT, MAT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
                                a.update_hist != 0, outv, my_dt, flags);
     if (FORCE) {
-      if (sizeof(R) == 4) {
-        float4 *dst = reinterpret_cast<float4 *>(a.fe + (size_t)24 * e);
+      // Slot-major layout fe[slot][element][xyz]: each of the 8 stores of a
+      // wavefront covers one contiguous run, and so does each of the 8 loads
+      // of the node gather.
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-          dst[k] = make_float4(outv[4 * k], outv[4 * k + 1], outv[4 * k + 2], outv[4 * k + 3]);
-      } else {
-        double2 *dst = reinterpret_cast<double2 *>(a.fe + (size_t)24 * e);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) dst[k] = make_double2(outv[2 * k], outv[2 * k + 1]);
+      for (int k = 0; k < 8; ++k) {
+        R *dst = a.fe + 3 * ((size_t)k * a.ne + e);
+        dst[0] = outv[3 * k];
+        dst[1] = outv[3 * k + 1];
+        dst[2] = outv[3 * k + 2];
       }
     }
   }
@@ -294,9 +322,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(int nn, const int *ptr, const
   if (n >= nn) return;
   double f[3];
   gather3(ptr, idx, fe, n, f);
-  fi[3 * n] = f[0];
-  fi[3 * n + 1] = f[1];
-  fi[3 * n + 2] = f[2];
+  fi[n] = f[0];
+  fi[(size_t)nn + n] = f[1];
+  fi[2 * (size_t)nn + n] = f[2];
 }
 
 enum NodalMode { kInitial = 0, kStep = 1, kResume = 2 };
@@ -309,8 +337,10 @@ struct NodalArgs {
   const double *X;
   double *u, *vs, *fi;
   const double *mass;  // 1 per node
-  const signed char *dof_motion;
+  const int *node_code;  // per node: byte d = motion index + 1 of DOF d (0 free)
   const double *motions;
+  int n_motions;
+  const MotionT *mstate;  // 3 x n_motions, from k_motion_states
   Ctl *ctl;
   int parity;
   int mode;
@@ -344,64 +374,77 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
   const double dt_next = a.factor * dtmin;
   const bool collapse = !(dtmin >= a.failure_dt);
   const bool go = (t_k < a.tmax) && (cur.step < a.max_steps) && !collapse;
+  // motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
+  __shared__ MotionT ms[3][kMaxMotions];
+  for (int i = threadIdx.x; i < 3 * a.n_motions; i += blockDim.x)
+    ms[i / a.n_motions][i % a.n_motions] = a.mstate[i];
+  __syncthreads();
   double sWint = 0.0, sWext = 0.0, KE = 0.0;
   if (n < a.nn) {
     double f[3];
     if (a.mode == kResume) {
-      f[0] = a.fi[3 * n];
-      f[1] = a.fi[3 * n + 1];
-      f[2] = a.fi[3 * n + 2];
+      f[0] = a.fi[n];
+      f[1] = a.fi[(size_t)a.nn + n];
+      f[2] = a.fi[2 * (size_t)a.nn + n];
     } else {
       gather3(a.ptr, a.idx, a.fe, n, f);
     }
     const double m = a.mass[n];
-    const double *Xn = a.X + 3 * (size_t)n;  // read only for constrained DOFs
+    const int code = a.node_code[n];  // byte d: 0 free, else motion index + 1
     const double t_khalf = 0.5 * (t_k + t_km1);
     const double t_k1 = t_k + dt_next;
     const double t_k1half = 0.5 * (t_k1 + t_k);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int mi = a.dof_motion[3 * n + d];
-      const double uo = a.u[3 * n + d];
-      const double vso = a.vs[3 * n + d];
+      const int mi = ((code >> (8 * d)) & 0xff) - 1;
+      const size_t id = (size_t)d * a.nn + n;
+      const double uo = a.u[id];
+      const double vso = a.vs[id];
       double acc, v;
-      double ub[3], vb[3], ab[3];
+      double ub = 0.0, vb = 0.0, ab = 0.0;
+      const double *mo = a.motions + FT_MOTION_STRIDE * (mi < 0 ? 0 : mi);
+      double Xn[3] = {0.0, 0.0, 0.0};
+      if (mi >= 0) {  // reference position, read for constrained DOFs only
+        Xn[0] = a.X[n];
+        Xn[1] = a.X[(size_t)a.nn + n];
+        Xn[2] = a.X[2 * (size_t)a.nn + n];
+      }
       if (mi < 0) {
         acc = (0.0 - f[d]) / m;
         v = (a.mode == kStep) ? vso + (t_k - t_khalf) * acc : vso;
       } else {
-        motion_eval(a.motions + FT_MOTION_STRIDE * mi, Xn, t_k, ub, vb, ab);
-        acc = ab[d];
-        v = vb[d];
+        motion_apply(mo, ms[1][mi], Xn, d, ub, vb, ab);
+        acc = ab;
+        v = vb;
       }
       if (a.mode == kStep && a.energy) {
         // CheckEnergy.cpp:34-46 with Delta u = dt v_half for free DOFs
-        const double fp = a.fi[3 * n + d];
+        const double fp = a.fi[id];
         double dd;
         if (mi < 0) {
           dd = dt_k * vso;
         } else {
-          double up[3], vp[3], ap[3];
-          motion_eval(a.motions + FT_MOTION_STRIDE * mi, Xn, t_km1, up, vp, ap);
-          dd = ub[d] - up[d];
-          sWext += dd * (fp + f[d] + m * (acc + ap[d]));
+          double up, vp, ap;
+          motion_apply(mo, ms[0][mi], Xn, d, up, vp, ap);
+          dd = ub - up;
+          sWext += dd * (fp + f[d] + m * (acc + ap));
         }
         KE += m * v * v;
         sWint += dd * (fp + f[d]);
       }
-      if (a.mode != kResume) a.fi[3 * n + d] = f[d];
+      if (a.mode != kResume) a.fi[id] = f[d];
       if (go) {
         if (mi < 0) {
           const double vh = v + (t_k1half - t_k) * acc;
-          a.vs[3 * n + d] = vh;
-          a.u[3 * n + d] = uo + dt_next * vh;
+          a.vs[id] = vh;
+          a.u[id] = uo + dt_next * vh;
         } else {
-          motion_eval(a.motions + FT_MOTION_STRIDE * mi, Xn, t_k1, ub, vb, ab);
-          a.u[3 * n + d] = ub[d];
-          a.vs[3 * n + d] = vb[d];
+          motion_apply(mo, ms[2][mi], Xn, d, ub, vb, ab);
+          a.u[id] = ub;
+          a.vs[id] = vb;
         }
       } else {
-        a.vs[3 * n + d] = v;
+        a.vs[id] = v;
       }
     }
   }
@@ -437,6 +480,20 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
   }
 }
 
+// Scalar state of every motion at t_{k-1}, t_k and t_{k+1} = t_k + dt_{k+1}
+// for the nodal kernel of step k (one small block; the trigonometry of the
+// rotation motions is kept out of the per-node kernel).
+__global__ void k_motion_states(const Ctl *ctl, int parity, double factor, const double *motions,
+                                int nm, MotionT *out) {
+  const StepSlot cur = ctl->slot[parity];
+  const double t_next = cur.t + factor * bits_to_double(cur.dtmin_bits);
+  for (int i = threadIdx.x; i < 3 * nm; i += blockDim.x) {
+    const int w = i / nm, k = i % nm;
+    const double tt = w == 0 ? cur.t_prev : (w == 1 ? cur.t : t_next);
+    motion_time(motions + FT_MOTION_STRIDE * k, tt, out[i]);
+  }
+}
+
 // Per-step history row (time, dt, W_int, W_ext, KE, |KE + W_int - W_ext|):
 // sums the per-block energy accumulators in block order.
 __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, double *rows,
@@ -468,7 +525,7 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
 }
 
 // Lumped mass: per-element row sums (Mass3D.cpp:5-67,134-144) ...
-__global__ __launch_bounds__(kBlock) void k_mass_elem(int ne, const int *conn, const int *einfo,
+__global__ __launch_bounds__(kBlock) void k_mass_elem(int nn, int ne, const int *conn, const int *einfo,
                                                       const double *X, const PartDev *parts,
                                                       double *me) {
   const int e = xcd_block() * kBlock + threadIdx.x;
@@ -478,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_elem(int ne, const int *conn, c
   const double rho = parts[einfo[e] & kPidMask].rho;
   double xv[3][8];
   for (int p = 0; p < 8; ++p)
-    for (int d = 0; d < 3; ++d) xv[d][p] = X[3 * nd[p2n(p)] + d] - X[3 * nd[0] + d];
+    for (int d = 0; d < 3; ++d) xv[d][p] = X[(size_t)d * nn + nd[p2n(p)]] - X[(size_t)d * nn + nd[0]];
   for (int d = 0; d < 3; ++d) wht8(xv[d]);
   double c[7][3];
   for (int m = 0; m < 7; ++m)
@@ -507,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_elem(int ne, const int *conn, c
       m8[a] += Na * det;
     }
   }
-  for (int a = 0; a < 8; ++a) me[8 * e + a] = rho * m8[a];
+  for (int a = 0; a < 8; ++a) me[(size_t)a * ne + e] = rho * m8[a];  // slot-major
 }
 
 // ... gathered per node in element order.
@@ -531,20 +588,21 @@ __global__ void k_elem_flags(int ne, const int *conn, const unsigned char *node_
 }
 
 // Velocity / acceleration read-out for ft_get_state.
-__global__ void k_state_out(int nn, const double *fi, const double *vs, const double *mass,
-                            const double *X, const signed char *dof_motion, const double *motions,
-                            double t, double *a_out) {
+__global__ void k_state_out(int nn, const double *fi, const double *mass, const double *X,
+                            const int *node_code, const double *motions, double t,
+                            double *a_out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
   for (int d = 0; d < 3; ++d) {
-    const int mi = dof_motion[3 * n + d];
+    const int mi = ((node_code[n] >> (8 * d)) & 0xff) - 1;
+    const size_t id = (size_t)d * nn + n;
     if (mi < 0) {
-      a_out[3 * n + d] = (0.0 - fi[3 * n + d]) / mass[n];
+      a_out[id] = (0.0 - fi[id]) / mass[n];
     } else {
       double ub[3], vb[3], ab[3];
-      const double Xn[3] = {X[3 * n], X[3 * n + 1], X[3 * n + 2]};
+      const double Xn[3] = {X[n], X[(size_t)nn + n], X[2 * (size_t)nn + n]};
       motion_eval(motions + FT_MOTION_STRIDE * mi, Xn, t, ub, vb, ab);
-      a_out[3 * n + d] = ab[d];
+      a_out[id] = ab[d];
     }
   }
 }
@@ -564,14 +622,22 @@ struct ft_engine {
   PartDev *parts = nullptr;
   std::vector<PartDev> h_parts;
   bool has_visco = false;
+  // Elements are stored grouped by material (stable within a group) so each
+  // material runs its own specialised element kernel over a contiguous range.
+  std::vector<int> perm;  // internal -> original element index
+  std::vector<int> inv;   // original -> internal
+  struct Seg { int mat, begin, end; };
+  std::vector<Seg> segs;
   // state
   double *u = nullptr, *vs = nullptr, *fi = nullptr, *mass = nullptr, *tmp3 = nullptr;
   void *fe = nullptr;
   void *hist = nullptr;
-  signed char *dof_motion = nullptr;
+  int *node_code = nullptr;
   unsigned char *node_mask = nullptr;
   double *motions = nullptr;
   int n_motions = 0;
+  bool timed_motions = false;  // any motion other than FIXED
+  MotionT *mstate = nullptr;
   std::vector<int> h_dof_motion;
   std::vector<double> h_motions;
   Ctl *ctl = nullptr;
@@ -653,11 +719,41 @@ int collect_timing(ft_engine *g) {
   return 0;
 }
 
-template <typename R, bool FORCE, bool DT>
-int launch_element(ft_engine *g, const StepSlot *slot, double dt_fixed, unsigned long long *dtbits,
-                   int update_hist) {
+// The device keeps nodal arrays component-major ([3][nn]); the C ABI uses the
+// reference's node-major layout (x[3 n + d]).
+std::vector<double> to_soa(const double *aos, size_t nn) {
+  std::vector<double> soa(3 * nn);
+  for (size_t n = 0; n < nn; ++n)
+    for (int d = 0; d < 3; ++d) soa[d * nn + n] = aos[3 * n + d];
+  return soa;
+}
+
+int h2d_nodal(ft_engine *g, double *dev, const double *aos) {
+  std::vector<double> soa = to_soa(aos, g->nn);
+  HIPCHK(hipMemcpyAsync(dev, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int d2h_nodal(ft_engine *g, double *aos, const double *dev) {
+  const size_t nn = g->nn;
+  std::vector<double> soa(3 * nn);
+  HIPCHK(hipMemcpyAsync(soa.data(), dev, soa.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (size_t n = 0; n < nn; ++n)
+    for (int d = 0; d < 3; ++d) aos[3 * n + d] = soa[d * nn + n];
+  return 0;
+}
+
+template <typename R, bool FORCE, bool DT, int MAT>
+int launch_element(ft_engine *g, int begin, int end, const StepSlot *slot, double dt_fixed,
+                   unsigned long long *dtbits, int update_hist) {
+  if (end <= begin) return 0;
   ElemArgs<R> a;
+  a.nn = g->nn;
   a.ne = g->ne;
+  a.begin = begin;
+  a.end = end;
   a.conn = g->conn;
   a.einfo = g->einfo;
   a.X = g->X;
@@ -673,21 +769,42 @@ int launch_element(ft_engine *g, const StepSlot *slot, double dt_fixed, unsigned
   a.want_dt = DT;
   a.update_hist = update_hist;
   Timed t(g, 0);
-  hipLaunchKernelGGL((k_element<R, FORCE, DT>), dim3(grid_for(g->ne)), dim3(kBlock), 0, g->stream, a);
+  hipLaunchKernelGGL((k_element<R, FORCE, DT, MAT>), dim3(grid_for(end - begin)), dim3(kBlock), 0,
+                     g->stream, a);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R, bool DT>
+int force_segment(ft_engine *g, const ft_engine::Seg &sg, const StepSlot *slot, double dt_fixed,
+                  unsigned long long *dtbits, int update_hist) {
+  switch (sg.mat) {
+    case 0: return launch_element<R, true, DT, 0>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 1: return launch_element<R, true, DT, 1>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 2: return launch_element<R, true, DT, 2>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 3: return launch_element<R, true, DT, 3>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 4: return launch_element<R, true, DT, 4>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 5: return launch_element<R, true, DT, 5>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+  }
+  return fail(FT_ERR_MATERIAL, "Unknown material type %d", sg.mat);
+}
+
+template <typename R>
+int element_pass_t(ft_engine *g, bool force, bool dt, const StepSlot *slot, double dt_fixed,
+                   unsigned long long *dtbits, int update_hist) {
+  if (!force) return launch_element<R, false, true, -1>(g, 0, g->ne, slot, dt_fixed, dtbits, 0);
+  for (const auto &sg : g->segs) {
+    int rc = dt ? force_segment<R, true>(g, sg, slot, dt_fixed, dtbits, update_hist)
+                : force_segment<R, false>(g, sg, slot, dt_fixed, dtbits, update_hist);
+    if (rc) return rc;
+  }
   return 0;
 }
 
 int element_pass(ft_engine *g, bool force, bool dt, const StepSlot *slot, double dt_fixed,
                  unsigned long long *dtbits, int update_hist) {
-  if (g->precision == FT_FP32) {
-    if (force && dt) return launch_element<float, true, true>(g, slot, dt_fixed, dtbits, update_hist);
-    if (force) return launch_element<float, true, false>(g, slot, dt_fixed, dtbits, update_hist);
-    return launch_element<float, false, true>(g, slot, dt_fixed, dtbits, update_hist);
-  }
-  if (force && dt) return launch_element<double, true, true>(g, slot, dt_fixed, dtbits, update_hist);
-  if (force) return launch_element<double, true, false>(g, slot, dt_fixed, dtbits, update_hist);
-  return launch_element<double, false, true>(g, slot, dt_fixed, dtbits, update_hist);
+  if (g->precision == FT_FP32) return element_pass_t<float>(g, force, dt, slot, dt_fixed, dtbits, update_hist);
+  return element_pass_t<double>(g, force, dt, slot, dt_fixed, dtbits, update_hist);
 }
 
 template <typename R>
@@ -702,8 +819,10 @@ int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
   a.vs = g->vs;
   a.fi = g->fi;
   a.mass = g->mass;
-  a.dof_motion = g->dof_motion;
+  a.node_code = g->node_code;
   a.motions = g->motions;
+  a.n_motions = g->n_motions;
+  a.mstate = g->mstate;
   a.ctl = g->ctl;
   a.parity = g->parity;
   a.mode = mode;
@@ -720,6 +839,12 @@ int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
 }
 
 int launch_nodal(ft_engine *g, int mode, int max_steps) {
+  if (g->timed_motions) {
+    Timed t(g, 2);
+    hipLaunchKernelGGL(k_motion_states, dim3(1), dim3(kBlock), 0, g->stream, g->ctl, g->parity,
+                       g->factor, g->motions, g->n_motions, g->mstate);
+    HIPCHK(hipGetLastError());
+  }
   int rc = g->precision == FT_FP32 ? launch_nodal_t<float>(g, mode, max_steps)
                                    : launch_nodal_t<double>(g, mode, max_steps);
   if (rc) return rc;
@@ -728,7 +853,7 @@ int launch_nodal(ft_engine *g, int mode, int max_steps) {
 }
 
 int upload_u(ft_engine *g, const double *u) {
-  if (u) HIPCHK(hipMemcpyAsync(g->u, u, 3 * (size_t)g->nn * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  if (u) return h2d_nodal(g, g->u, u);
   return 0;
 }
 
@@ -817,8 +942,9 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       (rc = dalloc(&g->einfo, (size_t)n_elems)) || (rc = dalloc(&g->parts, (size_t)n_parts)) ||
       (rc = dalloc(&g->u, nd)) || (rc = dalloc(&g->vs, nd)) || (rc = dalloc(&g->fi, nd)) ||
       (rc = dalloc(&g->tmp3, nd)) || (rc = dalloc(&g->mass, (size_t)n_nodes)) ||
-      (rc = dalloc(&g->dof_motion, nd)) || (rc = dalloc(&g->node_mask, (size_t)n_nodes)) ||
+      (rc = dalloc(&g->node_code, (size_t)n_nodes)) || (rc = dalloc(&g->node_mask, (size_t)n_nodes)) ||
       (rc = dalloc(&g->motions, (size_t)FT_MOTION_STRIDE * kMaxMotions)) ||
+      (rc = dalloc(&g->mstate, (size_t)3 * kMaxMotions)) ||
       (rc = dalloc(&g->ctl, 1)) || (rc = dalloc(&g->dtbits, 1)) || (rc = dalloc(&g->status, 1)) ||
       (rc = dalloc(&g->ptr, (size_t)n_nodes + 1)) || (rc = dalloc(&g->idx, 8 * (size_t)n_elems)))
     return bail(rc);
@@ -829,18 +955,45 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
     return bail(fail(FT_ERR_ALLOC, "history allocation failed"));
   g->nodal_grid = grid_for(n_nodes);
   if ((rc = dalloc(&g->eacc, 4 * (size_t)g->nodal_grid))) return bail(rc);
-  // node -> (element, slot) CSR sorted by element (counting sort: O(N))
+  // Group elements by material (stable counting sort on the material id).
+  g->perm.resize(n_elems);
+  g->inv.resize(n_elems);
+  {
+    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int e = 0; e < n_elems; ++e) cnt[g->h_parts[pid[e]].mat + 1]++;
+    for (int m = 0; m < 6; ++m) cnt[m + 1] += cnt[m];
+    for (int m = 0; m < 6; ++m)
+      if (cnt[m + 1] > cnt[m]) g->segs.push_back({m, cnt[m], cnt[m + 1]});
+    std::vector<int> fill(cnt, cnt + 6);
+    for (int e = 0; e < n_elems; ++e) {
+      int i = fill[g->h_parts[pid[e]].mat]++;
+      g->perm[i] = e;
+      g->inv[e] = i;
+    }
+  }
+  // node -> (slot, element) CSR with entries slot * ne + element (the index of
+  // the element-force triple in the slot-major buffer); each node's entries
+  // follow the original element order, i.e. the reference's scatter-add
+  // order (counting sort: O(N)).
   std::vector<int> hptr(n_nodes + 1, 0), hidx(8 * (size_t)n_elems);
   for (long i = 0; i < 8L * n_elems; ++i) hptr[conn[i] + 1]++;
   for (int n = 0; n < n_nodes; ++n) hptr[n + 1] += hptr[n];
   {
     std::vector<int> fill(hptr.begin(), hptr.end() - 1);
     for (int e = 0; e < n_elems; ++e)
-      for (int k = 0; k < 8; ++k) hidx[fill[conn[8 * e + k]]++] = 8 * e + k;
+      for (int k = 0; k < 8; ++k) hidx[fill[conn[8 * e + k]]++] = k * n_elems + g->inv[e];
   }
-  std::vector<int> hinfo(pid, pid + n_elems);
-  HIPCHK(hipMemcpy(g->X, coords, nd * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->conn, conn, 8 * (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> hinfo(n_elems), hconn(8 * (size_t)n_elems);
+  for (int i = 0; i < n_elems; ++i) {
+    const int e = g->perm[i];
+    hinfo[i] = pid[e];
+    for (int k = 0; k < 8; ++k) hconn[8 * (size_t)i + k] = conn[8 * (size_t)e + k];
+  }
+  {
+    std::vector<double> soa = to_soa(coords, n_nodes);
+    HIPCHK(hipMemcpy(g->X, soa.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(g->conn, hconn.data(), 8 * (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->einfo, hinfo.data(), (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->parts, g->h_parts.data(), n_parts * sizeof(PartDev), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->ptr, hptr.data(), hptr.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -848,10 +1001,11 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   HIPCHK(hipMemset(g->u, 0, nd * sizeof(double)));
   HIPCHK(hipMemset(g->vs, 0, nd * sizeof(double)));
   HIPCHK(hipMemset(g->fi, 0, nd * sizeof(double)));
-  HIPCHK(hipMemset(g->dof_motion, 0xff, nd));
+  HIPCHK(hipMemset(g->node_code, 0, (size_t)n_nodes * sizeof(int)));
   HIPCHK(hipMemset(g->node_mask, 0, (size_t)n_nodes));
   HIPCHK(hipMemset(g->status, 0, sizeof(int)));
   HIPCHK(hipMemset(g->ctl, 0, sizeof(Ctl)));
+  HIPCHK(hipMemset(g->mstate, 0, 3 * kMaxMotions * sizeof(MotionT)));
   if (g->hist) HIPCHK(hipMemset(g->hist, 0, 144 * (size_t)n_elems * esz));
   g->h_dof_motion.assign(nd, -1);
   HIPCHK(hipDeviceSynchronize());
@@ -863,7 +1017,7 @@ int ft_engine_destroy(ft_engine *g) {
   if (!g) return 0;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
-                  g->dof_motion, g->node_mask, g->motions, g->ctl, g->dtbits, g->status,
+                  g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ptr, g->idx, g->fe, g->hist, g->eacc, g->hist_rows};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -891,13 +1045,17 @@ int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const do
   if (n_motions < 0 || n_motions > kMaxMotions) return fail(FT_ERR_ARG, "at most %d motions", kMaxMotions);
   if (n_motions > 0 && !motions) return fail(FT_ERR_ARG, "motions is null");
   const size_t nd = 3 * (size_t)g->nn;
-  std::vector<signed char> dm(nd, -1);
+  std::vector<int> dm(nd, -1);
+  std::vector<int> code(g->nn, 0);
   std::vector<unsigned char> mask(g->nn, 0);
   for (size_t i = 0; i < nd; ++i) {
     int k = dof_motion ? dof_motion[i] : -1;
     if (k >= n_motions || k < -1) return fail(FT_ERR_ARG, "dof %zu: motion index %d out of range", i, k);
-    dm[i] = (signed char)k;
-    if (k >= 0) mask[i / 3] |= (unsigned char)(1 << (i % 3));
+    dm[i] = k;
+    if (k >= 0) {
+      mask[i / 3] |= (unsigned char)(1 << (i % 3));
+      code[i / 3] |= (k + 1) << (8 * (i % 3));
+    }
   }
   for (int k = 0; k < n_motions; ++k) {
     int type = (int)motions[FT_MOTION_STRIDE * k];
@@ -906,7 +1064,10 @@ int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const do
   g->h_dof_motion.assign(dm.begin(), dm.end());
   g->h_motions.assign(motions, motions + FT_MOTION_STRIDE * (size_t)n_motions);
   g->n_motions = n_motions;
-  HIPCHK(hipMemcpyAsync(g->dof_motion, dm.data(), nd, hipMemcpyHostToDevice, g->stream));
+  g->timed_motions = false;
+  for (int k = 0; k < n_motions; ++k)
+    if ((int)motions[FT_MOTION_STRIDE * k] != FT_MOTION_FIXED) g->timed_motions = true;
+  HIPCHK(hipMemcpyAsync(g->node_code, code.data(), g->nn * sizeof(int), hipMemcpyHostToDevice, g->stream));
   HIPCHK(hipMemcpyAsync(g->node_mask, mask.data(), g->nn, hipMemcpyHostToDevice, g->stream));
   if (n_motions)
     HIPCHK(hipMemcpyAsync(g->motions, motions, FT_MOTION_STRIDE * (size_t)n_motions * sizeof(double),
@@ -921,10 +1082,17 @@ int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const do
 int ft_set_state(ft_engine *g, const double *u, const double *v) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   const size_t nb = 3 * (size_t)g->nn * sizeof(double);
-  if (u) HIPCHK(hipMemcpyAsync(g->u, u, nb, hipMemcpyHostToDevice, g->stream));
-  else HIPCHK(hipMemsetAsync(g->u, 0, nb, g->stream));
-  if (v) HIPCHK(hipMemcpyAsync(g->vs, v, nb, hipMemcpyHostToDevice, g->stream));
-  else HIPCHK(hipMemsetAsync(g->vs, 0, nb, g->stream));
+  int rc;
+  if (u) {
+    if ((rc = h2d_nodal(g, g->u, u))) return rc;
+  } else {
+    HIPCHK(hipMemsetAsync(g->u, 0, nb, g->stream));
+  }
+  if (v) {
+    if ((rc = h2d_nodal(g, g->vs, v))) return rc;
+  } else {
+    HIPCHK(hipMemsetAsync(g->vs, 0, nb, g->stream));
+  }
   HIPCHK(hipStreamSynchronize(g->stream));
   g->run_active = false;
   return 0;
@@ -935,7 +1103,7 @@ int ft_assemble_lumped_mass(ft_engine *g, double *mass_out) {
   double *me = nullptr;
   int rc = dalloc(&me, 8 * (size_t)g->ne);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_mass_elem, dim3(grid_for(g->ne)), dim3(kBlock), 0, g->stream, g->ne, g->conn,
+  hipLaunchKernelGGL(k_mass_elem, dim3(grid_for(g->ne)), dim3(kBlock), 0, g->stream, g->nn, g->ne, g->conn,
                      g->einfo, g->X, g->parts, me);
   hipLaunchKernelGGL(k_mass_gather, dim3(grid_for(g->nn)), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
                      g->idx, me, g->mass);
@@ -989,13 +1157,18 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
   if ((rc = check_status(g))) return rc;
   if (fe_out) {
     const size_t n = 24 * (size_t)g->ne;
+    std::vector<double> dev(n);
     if (g->precision == FT_FP32) {
       std::vector<float> tmp(n);
       HIPCHK(hipMemcpy(tmp.data(), g->fe, n * sizeof(float), hipMemcpyDeviceToHost));
-      for (size_t i = 0; i < n; ++i) fe_out[i] = tmp[i];
+      for (size_t i = 0; i < n; ++i) dev[i] = tmp[i];
     } else {
-      HIPCHK(hipMemcpy(fe_out, g->fe, n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(dev.data(), g->fe, n * sizeof(double), hipMemcpyDeviceToHost));
     }
+    for (int e = 0; e < g->ne; ++e)
+      for (int k = 0; k < 8; ++k)
+        for (int d = 0; d < 3; ++d)
+          fe_out[24 * (size_t)e + 3 * k + d] = dev[3 * ((size_t)k * g->ne + g->inv[e]) + d];
   }
   return 0;
 }
@@ -1016,8 +1189,7 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
     HIPCHK(hipGetLastError());
   }
   if ((rc = check_status(g))) return rc;
-  if (fi_out)
-    HIPCHK(hipMemcpy(fi_out, g->fi, 3 * (size_t)g->nn * sizeof(double), hipMemcpyDeviceToHost));
+  if (fi_out) return d2h_nodal(g, fi_out, g->fi);
   return 0;
 }
 
@@ -1034,9 +1206,10 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
   const size_t nd = 3 * (size_t)g->nn;
   if (g->n_motions > 0) {
     std::vector<double> X(nd), u(nd), v(nd);
-    HIPCHK(hipMemcpy(X.data(), g->X, nd * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(u.data(), g->u, nd * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(v.data(), g->vs, nd * sizeof(double), hipMemcpyDeviceToHost));
+    int rc;
+    if ((rc = d2h_nodal(g, X.data(), g->X)) || (rc = d2h_nodal(g, u.data(), g->u)) ||
+        (rc = d2h_nodal(g, v.data(), g->vs)))
+      return rc;
     for (int n = 0; n < g->nn; ++n)
       for (int d = 0; d < 3; ++d) {
         int mi = g->h_dof_motion[3 * n + d];
@@ -1046,8 +1219,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
         u[3 * n + d] = ub[d];
         v[3 * n + d] = vb[d];
       }
-    HIPCHK(hipMemcpy(g->u, u.data(), nd * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(g->vs, v.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = h2d_nodal(g, g->u, u.data())) || (rc = h2d_nodal(g, g->vs, v.data()))) return rc;
   }
   HIPCHK(hipMemset(g->eacc, 0, 4 * (size_t)g->nodal_grid * sizeof(double)));
   HIPCHK(hipMemset(g->status, 0, sizeof(int)));
@@ -1130,18 +1302,19 @@ int ft_get_state(ft_engine *g, double *u, double *v, double *a, double *fi) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   const size_t nb = 3 * (size_t)g->nn * sizeof(double);
   HIPCHK(hipStreamSynchronize(g->stream));
-  if (u) HIPCHK(hipMemcpy(u, g->u, nb, hipMemcpyDeviceToHost));
-  if (v) HIPCHK(hipMemcpy(v, g->vs, nb, hipMemcpyDeviceToHost));
-  if (fi) HIPCHK(hipMemcpy(fi, g->fi, nb, hipMemcpyDeviceToHost));
+  (void)nb;
+  int rc0;
+  if (u && (rc0 = d2h_nodal(g, u, g->u))) return rc0;
+  if (v && (rc0 = d2h_nodal(g, v, g->vs))) return rc0;
+  if (fi && (rc0 = d2h_nodal(g, fi, g->fi))) return rc0;
   if (a) {
     StepSlot s;
     int rc = read_slot(g, g->parity, &s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_state_out, dim3((g->nn + 255) / 256), dim3(256), 0, g->stream, g->nn, g->fi,
-                       g->vs, g->mass, g->X, g->dof_motion, g->motions, s.t, g->tmp3);
+                       g->mass, g->X, g->node_code, g->motions, s.t, g->tmp3);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(g->stream));
-    HIPCHK(hipMemcpy(a, g->tmp3, nb, hipMemcpyDeviceToHost));
+    if ((rc = d2h_nodal(g, a, g->tmp3))) return rc;
   }
   return 0;
 }
@@ -1223,9 +1396,9 @@ int ft_get_visco_history(ft_engine *g, double *hist) {
   } else {
     HIPCHK(hipMemcpy(cm.data(), g->hist, n * sizeof(double), hipMemcpyDeviceToHost));
   }
-  // component-major [c * ne + e] -> element-major [e * 144 + c]
+  // component-major [c * ne + internal e] -> element-major original order
   for (size_t c = 0; c < 144; ++c)
-    for (size_t e = 0; e < (size_t)g->ne; ++e) hist[e * 144 + c] = cm[c * g->ne + e];
+    for (size_t e = 0; e < (size_t)g->ne; ++e) hist[e * 144 + c] = cm[c * g->ne + g->inv[e]];
   return 0;
 }
 

@@ -45,7 +45,7 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
   R out[24];
   double my = 1e300;
   int flags = 0;
-  ft::element_eval<R, true, true>(xv, uv, pp, false, dt, h, update != 0, out, my, flags);
+  ft::element_eval<R, true, true, -1>(xv, uv, pp, false, dt, h, update != 0, out, my, flags);
   for (int k = 0; k < 24; ++k) f24[k] = out[k];
   if (hist144)
     for (int k = 0; k < 144; ++k) hist144[k] = hl[k];
