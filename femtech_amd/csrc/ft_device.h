@@ -25,14 +25,36 @@ struct PartDev {
   int visco;   // 1 if the part carries Prony history
   double rho, mu, lambda, k1, k2, g1, t1, g2, t2;
   double ce;   // dilatational wave speed, CalculateTimeStep.cpp:13-19
+  double inv_ce;
 };
 
 template <typename R> struct Num;
 template <> struct Num<float> {
-  __host__ __device__ static float log1p_(float x) { return log1pf(x); }
+  // log(1 + x): 7-term series for |x| < 1/16 (truncation < 4e-11 relative),
+  // the library routine otherwise
+  __host__ __device__ static float log1p_(float x) {
+    if (fabsf(x) < 0.0625f) {
+      float p = 1.0f / 7.0f;
+      p = p * x - 1.0f / 6.0f;
+      p = p * x + 1.0f / 5.0f;
+      p = p * x - 1.0f / 4.0f;
+      p = p * x + 1.0f / 3.0f;
+      p = p * x - 0.5f;
+      p = p * x + 1.0f;
+      return p * x;
+    }
+    return log1pf(x);
+  }
   __host__ __device__ static float expm1_(float x) { return expm1f(x); }
   __host__ __device__ static float exp_(float x) { return expf(x); }
-  __host__ __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+  // hardware square root (1 ulp) in the FP32 element path
+  __host__ __device__ static float sqrt_(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+  }
   // hardware reciprocal (1 ulp) in the FP32 element path
   __host__ __device__ static float rcp_(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -390,6 +412,45 @@ __host__ __device__ inline R char_length(const R d[7][3]) {
   return vol / amax;
 }
 
+// One piece of the characteristic length of a deformed hex from its
+// normalised coefficients d[m][comp]: face area k (k < 6, faces in the order
+// of CalculateCharacteristicLength_C3D8.cpp:12-13) or the volume (k == 6).
+template <typename R>
+__host__ __device__ inline R hex_part(const R d[7][3], int k) {
+  if (k == 6) {
+    R q0[3], q1[3], q2[3], q3[3], q4[3], q5[3];
+    for (int j = 0; j < 3; ++j) {
+      q0[j] = R(8) * d[2][j];
+      q1[j] = R(8) * d[4][j];
+      q2[j] = R(8) * d[0][j];
+      q3[j] = R(8) * d[5][j];
+      q4[j] = R(8) * d[1][j];
+      q5[j] = R(8) * d[3][j];
+    }
+    return (triple(q0, q4, q3) + triple(q2, q0, q1) + triple(q1, q3, q5)) / R(192) +
+           triple(q2, q4, q5) / R(64);
+  }
+  const R s = (k & 1) ? R(1) : R(-1);
+  const int g = k >> 1;
+  R cD[3], e1[3], e2[3];
+  for (int j = 0; j < 3; ++j) {
+    if (g == 0) {
+      cD[j] = d[2][j] + s * d[6][j];
+      e1[j] = d[0][j] + s * d[4][j];
+      e2[j] = d[1][j] + s * d[5][j];
+    } else if (g == 1) {
+      cD[j] = d[5][j] + s * d[6][j];
+      e1[j] = d[1][j] + s * d[2][j];
+      e2[j] = d[3][j] + s * d[4][j];
+    } else {
+      cD[j] = d[4][j] + s * d[6][j];
+      e1[j] = d[0][j] + s * d[2][j];
+      e2[j] = d[3][j] + s * d[5][j];
+    }
+  }
+  return face_area(cD, e1, e2);
+}
+
 // Status bits of element_eval (FT_STATUS_* of the C ABI plus an internal
 // "unknown material" bit).
 constexpr int kFlagNonFinite = 1;
@@ -549,7 +610,8 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
 #pragma unroll
       for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
     const R le = char_length(dcf);
-    const double dte = (double)le / pp.ce;
+    // FP64 keeps the reference's division; FP32 multiplies by 1 / ce
+    const double dte = sizeof(R) == 8 ? (double)le / pp.ce : (double)(le * R(pp.inv_ce));
     if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
   }
 }

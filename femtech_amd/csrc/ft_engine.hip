@@ -88,6 +88,7 @@ struct StepSlot {
   unsigned long long dtmin_bits;  // min_e Le/c at u_k (order-preserving bits)
   double t_done;   // time of the last completed step
   double dt_done;  // dt computed at its end (the next step's size)
+  double dt_guide; // last min_e Le/c: elements near it get an FP64 dt (FP32 kernel)
   int step;        // k: step in progress (done == 0) or last completed (done == 1)
   int done;        // 1 when the run has stopped at the end of step k
   int fresh;       // 1 if the last nodal launch completed a step
@@ -100,6 +101,8 @@ struct Ctl {
   int hist_count;
   int pad[2];
 };
+
+constexpr int kStatusDtFromFp32 = FT_STATUS_DT_FP32;  // no element refined in FP64
 
 __device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 
@@ -230,6 +233,7 @@ struct ElemArgs {
   R *fe;                 // 24 per element (nullable when only dt is wanted)
   R *hist;               // 144 per element, component-major [c * ne + e]
   const StepSlot *slot;  // nullable: step slot giving dt_k and done
+  int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
   double dt_fixed;       // used when slot == nullptr
   unsigned long long *dtmin_bits;  // nullable
   int *status;
@@ -244,9 +248,11 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
   const int L = xcd_block();
   const int e = a.begin + L * kBlock + threadIdx.x;
   double dt_step = a.dt_fixed;
+  double dt_guide = 1e300;  // no previous minimum: refine every element
   if (a.slot) {
     if (a.slot->done) return;  // uniform over the grid
     dt_step = a.slot->dt;
+    dt_guide = a.slot->dt_guide;
   }
   double my_dt = 1e300;
   int flags = 0;
@@ -259,20 +265,24 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
     // Node-0-relative coordinates and displacements, subtracted in FP64 and
     // then rounded to R (keeps the relative precision of the element's own
     // geometry, not of its absolute position).
+    // uniform per-component bases + 32-bit node offsets (scalar-base
+    // addressing, no per-load 64-bit address arithmetic)
+    const double *Xd[3] = {a.X, a.X + a.nn, a.X + 2 * (size_t)a.nn};
+    const double *Ud[3] = {a.u, a.u + a.nn, a.u + 2 * (size_t)a.nn};
     double X0[3], U0[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      X0[d] = a.X[(size_t)d * a.nn + nd[0]];
-      U0[d] = a.u[(size_t)d * a.nn + nd[0]];
+      X0[d] = Xd[d][(unsigned)nd[0]];
+      U0[d] = Ud[d][(unsigned)nd[0]];
     }
     R xv[3][8], uv[3][8];
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      const int n = nd[p2n(p)];
+      const unsigned n = (unsigned)nd[p2n(p)];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        xv[d][p] = p == 0 ? R(0) : R(a.X[(size_t)d * a.nn + n] - X0[d]);
-        uv[d][p] = p == 0 ? R(0) : R(a.u[(size_t)d * a.nn + n] - U0[d]);
+        xv[d][p] = p == 0 ? R(0) : R(Xd[d][n] - X0[d]);
+        uv[d][p] = p == 0 ? R(0) : R(Ud[d][n] - U0[d]);
       }
     }
     HistSoA<R> hist{a.hist, a.ne, e};
@@ -285,17 +295,57 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
       // of the node gather.
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        R *dst = a.fe + 3 * ((size_t)k * a.ne + e);
-        dst[0] = outv[3 * k];
-        dst[1] = outv[3 * k + 1];
-        dst[2] = outv[3 * k + 2];
+        R *dst = a.fe + 3 * (size_t)k * a.ne;  // uniform
+        const unsigned o = 3u * (unsigned)e;
+        dst[o] = outv[3 * k];
+        dst[o + 1] = outv[3 * k + 1];
+        dst[o + 2] = outv[3 * k + 2];
       }
     }
   }
   if (DT) {
-    double bmin = block_min(my_dt, sh_min);
-    if (threadIdx.x == 0 && bmin < 1e300)
+    const double bmin = block_min(my_dt, sh_min);
+    if (sizeof(R) == 4 && a.refine && bmin <= dt_guide * 1.001) {
+      // The step size sets the time grid and is sensitive to round-off: the
+      // block's minimising element (smallest id on ties) recomputes Le / ce
+      // from FP64 geometry (StableTimeStep.cpp:22-27).  Rare (blocks near the
+      // last minimum only) and spread over a few threads through LDS, so it
+      // adds no registers to the kernel.
+      __shared__ int sh_arg;
+      __shared__ double sh_c[7][3];
+      __shared__ double sh_a[7];
+      if (threadIdx.x == 0) sh_arg = INT32_MAX;
+      __syncthreads();
+      if (e < a.end && my_dt == bmin) atomicMin(&sh_arg, e);
+      __syncthreads();
+      const int es = sh_arg;
+      if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        const double *Xc = a.X + (size_t)d * a.nn, *Uc = a.u + (size_t)d * a.nn;
+        const int n0 = a.conn[8 * (size_t)es];
+        double v[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          const int n = a.conn[8 * (size_t)es + p2n(p)];
+          v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
+        }
+        wht8(v);
+#pragma unroll
+        for (int m = 0; m < 7; ++m) sh_c[m][d] = v[m + 1] * 0.125;
+      }
+      __syncthreads();
+      if (threadIdx.x < 7) sh_a[threadIdx.x] = hex_part(sh_c, threadIdx.x);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double amax = 0.0;
+        for (int f = 0; f < 6; ++f)
+          if (sh_a[f] > amax) amax = sh_a[f];
+        const double dte = sh_a[6] / amax / a.parts[a.einfo[es] & kPidMask].ce;
+        atomicMin(a.dtmin_bits, (unsigned long long)__double_as_longlong(dte));
+      }
+    } else if (threadIdx.x == 0 && bmin < 1e300) {
       atomicMin(a.dtmin_bits, (unsigned long long)__double_as_longlong(bmin));
+    }
   }
   if (flags) atomicOr(a.status, flags);
 }
@@ -303,25 +353,53 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
 // ---------------------------------------------------------------------------
 // Node kernels
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ inline void gather3(const int *ptr, const int *idx, const R *fe, int n, double f[3]) {
+// Node-to-element incidences in ELL form: entry j of node n at ell[j * nn + n]
+// (slot * ne + element, or -1 past the node's valence), in the original
+// element order, so the sum below adds the element forces in the order of the
+// reference's scatter-add (GetForce_3D.cpp:39-44).  W is the ELL width when
+// it is the common 8 (fully unrolled: 8 independent loads in flight).
+template <typename R, int W>
+__device__ inline void gather3(const int *ell, int nn, int width, const R *fe, int n, double f[3]) {
   f[0] = f[1] = f[2] = 0.0;
-  const int b = ptr[n], end = ptr[n + 1];
-  for (int k = b; k < end; ++k) {
-    const size_t o = (size_t)idx[k] * 3;
-    f[0] += (double)fe[o];
-    f[1] += (double)fe[o + 1];
-    f[2] += (double)fe[o + 2];
+  if (W > 0) {
+    constexpr int WA = W > 0 ? W : 1;
+    int id[WA];
+#pragma unroll
+    for (int j = 0; j < W; ++j) id[j] = ell[(size_t)j * nn + n];
+    R v[WA][3];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const size_t o = 3 * (size_t)(id[j] < 0 ? 0 : id[j]);
+      v[j][0] = id[j] < 0 ? R(0) : fe[o];
+      v[j][1] = id[j] < 0 ? R(0) : fe[o + 1];
+      v[j][2] = id[j] < 0 ? R(0) : fe[o + 2];
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      if (id[j] < 0) continue;
+      f[0] += (double)v[j][0];
+      f[1] += (double)v[j][1];
+      f[2] += (double)v[j][2];
+    }
+  } else {
+    for (int j = 0; j < width; ++j) {
+      const int id = ell[(size_t)j * nn + n];
+      if (id < 0) break;
+      const size_t o = 3 * (size_t)id;
+      f[0] += (double)fe[o];
+      f[1] += (double)fe[o + 1];
+      f[2] += (double)fe[o + 2];
+    }
   }
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_gather(int nn, const int *ptr, const int *idx,
-                                                   const R *fe, double *fi) {
+template <typename R, int W>
+__global__ __launch_bounds__(kBlock) void k_gather(int nn, const int *ell, int width, const R *fe,
+                                                   double *fi) {
   const int n = xcd_block() * kBlock + threadIdx.x;
   if (n >= nn) return;
   double f[3];
-  gather3(ptr, idx, fe, n, f);
+  gather3<R, W>(ell, nn, width, fe, n, f);
   fi[n] = f[0];
   fi[(size_t)nn + n] = f[1];
   fi[2 * (size_t)nn + n] = f[2];
@@ -332,7 +410,8 @@ enum NodalMode { kInitial = 0, kStep = 1, kResume = 2 };
 template <typename R>
 struct NodalArgs {
   int nn;
-  const int *ptr, *idx;
+  const int *ell;
+  int width;
   const R *fe;
   const double *X;
   double *u, *vs, *fi;
@@ -354,7 +433,7 @@ struct NodalArgs {
 // start of step k+1 (v_half, u += dt v_half, boundary motions at t_{k+1}),
 // ex9.cpp:107-175.  vs holds v_half(k) for free DOFs and the prescribed v
 // for constrained ones while a run is in progress, v(k) once it stops.
-template <typename R>
+template <typename R, int W>
 __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
   __shared__ double sh[kBlock / 64];
   const StepSlot cur = a.ctl->slot[a.parity];
@@ -387,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
       f[1] = a.fi[(size_t)a.nn + n];
       f[2] = a.fi[2 * (size_t)a.nn + n];
     } else {
-      gather3(a.ptr, a.idx, a.fe, n, f);
+      gather3<R, W>(a.ell, a.nn, a.width, a.fe, n, f);
     }
     const double m = a.mass[n];
     const int code = a.node_code[n];  // byte d: 0 free, else motion index + 1
@@ -474,6 +553,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
     }
     nx.t_done = t_k;
     nx.dt_done = dt_next;
+    nx.dt_guide = dtmin;
+    if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(&a.ctl->status, kStatusDtFromFp32);
     nx.fresh = a.mode == kStep;
     nx.pad = 0;
     a.ctl->slot[a.parity ^ 1] = nx;
@@ -568,12 +649,16 @@ __global__ __launch_bounds__(kBlock) void k_mass_elem(int nn, int ne, const int 
 }
 
 // ... gathered per node in element order.
-__global__ __launch_bounds__(kBlock) void k_mass_gather(int nn, const int *ptr, const int *idx,
+__global__ __launch_bounds__(kBlock) void k_mass_gather(int nn, const int *ell, int width,
                                                         const double *me, double *mass) {
   const int n = xcd_block() * kBlock + threadIdx.x;
   if (n >= nn) return;
   double s = 0.0;
-  for (int k = ptr[n]; k < ptr[n + 1]; ++k) s += me[idx[k]];
+  for (int j = 0; j < width; ++j) {
+    const int id = ell[(size_t)j * nn + n];
+    if (id < 0) break;
+    s += me[id];
+  }
   mass[n] = s;
 }
 
@@ -618,7 +703,8 @@ struct ft_engine {
   hipStream_t stream = nullptr;
   // mesh
   double *X = nullptr;
-  int *conn = nullptr, *einfo = nullptr, *ptr = nullptr, *idx = nullptr;
+  int *conn = nullptr, *einfo = nullptr, *ell = nullptr;
+  int width = 8;  // ELL width: max node valence, at least 8
   PartDev *parts = nullptr;
   std::vector<PartDev> h_parts;
   bool has_visco = false;
@@ -762,6 +848,7 @@ int launch_element(ft_engine *g, int begin, int end, const StepSlot *slot, doubl
   a.fe = reinterpret_cast<R *>(g->fe);
   a.hist = reinterpret_cast<R *>(g->hist);
   a.slot = slot;
+  a.refine = (sizeof(R) == 4 && slot) ? 1 : 0;
   a.dt_fixed = dt_fixed;
   a.dtmin_bits = dtbits;
   a.status = g->status;
@@ -792,7 +879,8 @@ int force_segment(ft_engine *g, const ft_engine::Seg &sg, const StepSlot *slot, 
 template <typename R>
 int element_pass_t(ft_engine *g, bool force, bool dt, const StepSlot *slot, double dt_fixed,
                    unsigned long long *dtbits, int update_hist) {
-  if (!force) return launch_element<R, false, true, -1>(g, 0, g->ne, slot, dt_fixed, dtbits, 0);
+  // A stand-alone stable-time-step pass always runs in FP64 (exact minimum).
+  if (!force) return launch_element<double, false, true, -1>(g, 0, g->ne, slot, dt_fixed, dtbits, 0);
   for (const auto &sg : g->segs) {
     int rc = dt ? force_segment<R, true>(g, sg, slot, dt_fixed, dtbits, update_hist)
                 : force_segment<R, false>(g, sg, slot, dt_fixed, dtbits, update_hist);
@@ -811,8 +899,8 @@ template <typename R>
 int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
   NodalArgs<R> a;
   a.nn = g->nn;
-  a.ptr = g->ptr;
-  a.idx = g->idx;
+  a.ell = g->ell;
+  a.width = g->width;
   a.fe = reinterpret_cast<const R *>(g->fe);
   a.X = g->X;
   a.u = g->u;
@@ -833,7 +921,10 @@ int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
   a.failure_dt = g->failure_dt;
   a.max_steps = max_steps;
   Timed t(g, 1);
-  hipLaunchKernelGGL((k_nodal<R>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+  if (g->width == 8)
+    hipLaunchKernelGGL((k_nodal<R, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+  else
+    hipLaunchKernelGGL((k_nodal<R, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -935,6 +1026,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
     q.t2 = pr[8];
     double nu = 0.5 * q.lambda / (q.lambda + q.mu);
     q.ce = sqrt(q.lambda * (1.0 / nu - 1.0) / q.rho);
+    q.inv_ce = 1.0 / q.ce;
     if (q.visco) g->has_visco = true;
   }
   const size_t nd = 3 * (size_t)n_nodes;
@@ -946,7 +1038,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       (rc = dalloc(&g->motions, (size_t)FT_MOTION_STRIDE * kMaxMotions)) ||
       (rc = dalloc(&g->mstate, (size_t)3 * kMaxMotions)) ||
       (rc = dalloc(&g->ctl, 1)) || (rc = dalloc(&g->dtbits, 1)) || (rc = dalloc(&g->status, 1)) ||
-      (rc = dalloc(&g->ptr, (size_t)n_nodes + 1)) || (rc = dalloc(&g->idx, 8 * (size_t)n_elems)))
+      false)
     return bail(rc);
   const size_t esz = precision == FT_FP32 ? 4 : 8;
   if (hipMalloc(&g->fe, 24 * (size_t)n_elems * esz) != hipSuccess)
@@ -975,14 +1067,19 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   // the element-force triple in the slot-major buffer); each node's entries
   // follow the original element order, i.e. the reference's scatter-add
   // order (counting sort: O(N)).
-  std::vector<int> hptr(n_nodes + 1, 0), hidx(8 * (size_t)n_elems);
-  for (long i = 0; i < 8L * n_elems; ++i) hptr[conn[i] + 1]++;
-  for (int n = 0; n < n_nodes; ++n) hptr[n + 1] += hptr[n];
+  std::vector<int> val(n_nodes, 0);
+  for (long i = 0; i < 8L * n_elems; ++i) val[conn[i]]++;
+  g->width = std::max(8, *std::max_element(val.begin(), val.end()));
+  std::vector<int> hell((size_t)g->width * n_nodes, -1);
   {
-    std::vector<int> fill(hptr.begin(), hptr.end() - 1);
+    std::fill(val.begin(), val.end(), 0);
     for (int e = 0; e < n_elems; ++e)
-      for (int k = 0; k < 8; ++k) hidx[fill[conn[8 * e + k]]++] = k * n_elems + g->inv[e];
+      for (int k = 0; k < 8; ++k) {
+        const int n = conn[8 * e + k];
+        hell[(size_t)val[n]++ * n_nodes + n] = k * n_elems + g->inv[e];
+      }
   }
+  if ((rc = dalloc(&g->ell, hell.size()))) return bail(rc);
   std::vector<int> hinfo(n_elems), hconn(8 * (size_t)n_elems);
   for (int i = 0; i < n_elems; ++i) {
     const int e = g->perm[i];
@@ -996,8 +1093,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   HIPCHK(hipMemcpy(g->conn, hconn.data(), 8 * (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->einfo, hinfo.data(), (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->parts, g->h_parts.data(), n_parts * sizeof(PartDev), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->ptr, hptr.data(), hptr.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->idx, hidx.data(), hidx.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->ell, hell.data(), hell.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(g->u, 0, nd * sizeof(double)));
   HIPCHK(hipMemset(g->vs, 0, nd * sizeof(double)));
   HIPCHK(hipMemset(g->fi, 0, nd * sizeof(double)));
@@ -1018,7 +1114,7 @@ int ft_engine_destroy(ft_engine *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
-                  g->ptr, g->idx, g->fe, g->hist, g->eacc, g->hist_rows};
+                  g->ell, g->fe, g->hist, g->eacc, g->hist_rows};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int c = 0; c < 3; ++c)
@@ -1105,8 +1201,8 @@ int ft_assemble_lumped_mass(ft_engine *g, double *mass_out) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_mass_elem, dim3(grid_for(g->ne)), dim3(kBlock), 0, g->stream, g->nn, g->ne, g->conn,
                      g->einfo, g->X, g->parts, me);
-  hipLaunchKernelGGL(k_mass_gather, dim3(grid_for(g->nn)), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
-                     g->idx, me, g->mass);
+  hipLaunchKernelGGL(k_mass_gather, dim3(grid_for(g->nn)), dim3(kBlock), 0, g->stream, g->nn, g->ell,
+                     g->width, me, g->mass);
   hipError_t e = hipGetLastError();
   (void)hipStreamSynchronize(g->stream);
   (void)hipFree(me);
@@ -1180,12 +1276,22 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
   if ((rc = element_pass(g, true, false, nullptr, dt, nullptr, 1))) return rc;
   {
     Timed t(g, 1);
-    if (g->precision == FT_FP32)
-      hipLaunchKernelGGL(k_gather<float>, dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
-                         g->idx, (const float *)g->fe, g->fi);
-    else
-      hipLaunchKernelGGL(k_gather<double>, dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn, g->ptr,
-                         g->idx, (const double *)g->fe, g->fi);
+    const bool w8 = g->width == 8;
+    if (g->precision == FT_FP32) {
+      if (w8)
+        hipLaunchKernelGGL((k_gather<float, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const float *)g->fe, g->fi);
+      else
+        hipLaunchKernelGGL((k_gather<float, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const float *)g->fe, g->fi);
+    } else {
+      if (w8)
+        hipLaunchKernelGGL((k_gather<double, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const double *)g->fe, g->fi);
+      else
+        hipLaunchKernelGGL((k_gather<double, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const double *)g->fe, g->fi);
+    }
     HIPCHK(hipGetLastError());
   }
   if ((rc = check_status(g))) return rc;
@@ -1240,6 +1346,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
   s0.t_prev = 0.0;
   s0.dt = dt_factor * dtmin;
   memcpy(&s0.dtmin_bits, &dtmin, sizeof(dtmin));
+  s0.dt_guide = dtmin;
   s0.step = 0;
   s0.done = 0;
   s0.fresh = 0;
@@ -1433,10 +1540,10 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
   // X (3 doubles) + u (3 doubles) per node; Prony history read + write.
   double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0);
   if (g->has_visco) elem += ne * 2.0 * 144.0 * es;
-  // nodal kernel: force read 24 es + CSR entries 32 per element; per node CSR
-  // row pointer 4, mass 8, motion codes 3, X 24 (boundary nodes only: not
-  // counted), u, v_half and fi read + write 3 x 48.
-  double nodal = ne * (24.0 * es + 32.0) + nn * (4.0 + 8.0 + 3.0 + 3.0 * 48.0);
+  // nodal kernel: force read 24 es per element; per node the ELL incidences
+  // 4 width, mass 8, motion code 4, X 24 (boundary nodes only: not counted),
+  // u, v_half and fi read + write 3 x 48.
+  double nodal = ne * (24.0 * es) + nn * (4.0 * g->width + 8.0 + 4.0 + 3.0 * 48.0);
   bytes[0] = elem;
   bytes[1] = nodal;
   return 0;

@@ -31,6 +31,7 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
   pp.t2 = p9[8];
   double nu = 0.5 * pp.lambda / (pp.lambda + pp.mu);
   pp.ce = sqrt(pp.lambda * (1.0 / nu - 1.0) / pp.rho);
+  pp.inv_ce = 1.0 / pp.ce;
   R xv[3][8], uv[3][8];
   for (int p = 0; p < 8; ++p) {
     int a = ft::p2n(p);
