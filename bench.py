@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="also run a plain device copy of this many MB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: the engine holds the whole mesh on one GPU; --gpus %d is not supported "
@@ -137,6 +139,8 @@ def main():
         "energy": {"W_int": E[0], "W_ext": E[1], "KE": E[2], "balance": E[3]},
         "status": status,
     }
+    if args.copy_probe > 0:
+        out["copy_probe_gbs"] = ft.copy_probe(args.copy_probe, 5)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, threads)

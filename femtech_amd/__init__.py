@@ -60,6 +60,7 @@ _SIGS = {
     "ft_set_timing": [_VP, ctypes.c_int],
     "ft_get_timing": [_VP, _D, _I],
     "ft_traffic_model": [_VP, _D],
+    "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
     "ft_mesh_block": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                       ctypes.c_ulonglong, _D, _I],
@@ -117,6 +118,13 @@ def device_count():
     c = ctypes.c_int(0)
     _check(library().ft_device_count(ctypes.byref(c)))
     return c.value
+
+
+def copy_probe(megabytes=256.0, repeats=10):
+    """GB/s of a plain device copy (read + write) of `megabytes`."""
+    out = ctypes.c_double(0)
+    _check(library().ft_copy_probe(float(megabytes), int(repeats), ctypes.byref(out)))
+    return out.value
 
 
 def mesh_block(nx, ny, nz, h=(1e-3, 1e-3, 1e-3), jitter=0.0, seed=0):

@@ -605,6 +605,12 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
   }
 }
 
+// Plain streaming copy (calibration of the HBM traffic counters).
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const float4 *src, float4 *dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // Lumped mass: per-element row sums (Mass3D.cpp:5-67,134-144) ...
 __global__ __launch_bounds__(kBlock) void k_mass_elem(int nn, int ne, const int *conn, const int *einfo,
                                                       const double *X, const PartDev *parts,
@@ -1529,6 +1535,34 @@ int ft_get_timing(ft_engine *g, double ms[3], int launches[3]) {
     if (ms) ms[c] = g->ms[c];
     if (launches) launches[c] = g->launches[c];
   }
+  return 0;
+}
+
+int ft_copy_probe(double megabytes, int repeats, double *gbs) {
+  if (!(megabytes > 0) || repeats < 1) return fail(FT_ERR_ARG, "bad copy probe size");
+  const size_t n = (size_t)(megabytes * 1e6 / 16.0);
+  float4 *a = nullptr, *b = nullptr;
+  int rc;
+  if ((rc = dalloc(&a, n)) || (rc = dalloc(&b, n))) {
+    if (a) (void)hipFree(a);
+    return rc;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipMemset(a, 0, n * sizeof(float4)));
+  hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(kBlock), 0, 0, a, b, n);  // warm
+  HIPCHK(hipEventRecord(e0, 0));
+  for (int r = 0; r < repeats; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(kBlock), 0, 0, a, b, n);
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (gbs) *gbs = 2.0 * n * sizeof(float4) * repeats / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }
 

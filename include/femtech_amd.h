@@ -144,6 +144,10 @@ int ft_get_timing(ft_engine *eng, double ms[3], int launches[3]);
  * kernel, as defined in the design notes (compulsory traffic, each array
  * touched once). */
 int ft_traffic_model(ft_engine *eng, double bytes[2]);
+/* Plain device-to-device streaming copy of `megabytes` (read + write), run
+ * `repeats` times; reports GB/s moved.  Used to calibrate the FETCH_SIZE /
+ * WRITE_SIZE counters against a kernel of known traffic. */
+int ft_copy_probe(double megabytes, int repeats, double *gbs);
 
 /* ---- host-side mesh tools (no GPU needed) -------------------------------
  * Structured hex block nx x ny x nz with spacings h*, nodes numbered
