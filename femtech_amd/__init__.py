@@ -61,6 +61,7 @@ _SIGS = {
     "ft_get_timing": [_VP, _D, _I],
     "ft_traffic_model": [_VP, _D],
     "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
+    "ft_element_strain": [_VP, _D, _D, _D],
     "ft_mesh_block": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                       ctypes.c_ulonglong, _D, _I],
@@ -288,6 +289,15 @@ class Engine:
         s = ctypes.c_int(0)
         _check(self._lib.ft_get_status(self._h, ctypes.byref(s)))
         return s.value
+
+    def element_strain(self, u=None):
+        """Element-averaged Green-Lagrange strain (9 per element, column-major)
+        and principal summary (max principal, min principal, max shear)."""
+        E = np.empty(9 * self.ne, np.float64)
+        P = np.empty(3 * self.ne, np.float64)
+        u = None if u is None else _f64(u)
+        _check(self._lib.ft_element_strain(self._h, _dp(u), _dp(E), _dp(P)))
+        return E, P
 
     def visco_history(self):
         h = np.empty(144 * self.ne, np.float64)

@@ -605,6 +605,99 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
   }
 }
 
+// Element-averaged Green-Lagrange strain (CalculateStrain.cpp:77-97:
+// Eavg = (1/8) sum_gp (F^T F - I) / 2, column-major) and the principal-strain
+// summary of CalculateMaximumPrincipalStrain (CalculateStrain.cpp:8-75: max
+// principal clamped at >= 0, min principal clamped at <= 0, max shear), FP64.
+__global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *conn, const double *X,
+                                                   const double *u, double *eavg, double *princ) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int nd[8];
+  for (int k = 0; k < 8; ++k) nd[k] = conn[8 * (size_t)e + k];
+  double cx[7][3], cu[7][3];
+  for (int d = 0; d < 3; ++d) {
+    double xv[8], uv[8];
+    for (int p = 0; p < 8; ++p) {
+      const int n = nd[p2n(p)];
+      xv[p] = X[(size_t)d * nn + n] - X[(size_t)d * nn + nd[0]];
+      uv[p] = u[(size_t)d * nn + n] - u[(size_t)d * nn + nd[0]];
+    }
+    wht8(xv);
+    wht8(uv);
+    for (int m = 0; m < 7; ++m) {
+      cx[m][d] = xv[m + 1] * 0.125;
+      cu[m][d] = uv[m + 1] * 0.125;
+    }
+  }
+  const double g = kGp, g2 = kGp * kGp;
+  double E[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int gp = 0; gp < 8; ++gp) {
+    const double sx = ((gp & 3) == 1 || (gp & 3) == 2) ? 1.0 : -1.0;
+    const double sy = (gp & 2) ? 1.0 : -1.0;
+    const double sz = gp < 4 ? 1.0 : -1.0;
+    double J0[3][3], Ju[3][3];
+    for (int d = 0; d < 3; ++d) {
+      J0[d][0] = cx[0][d] + (sy * g) * cx[2][d] + (sz * g) * cx[4][d] + (sy * sz * g2) * cx[6][d];
+      J0[d][1] = cx[1][d] + (sx * g) * cx[2][d] + (sz * g) * cx[5][d] + (sx * sz * g2) * cx[6][d];
+      J0[d][2] = cx[3][d] + (sx * g) * cx[4][d] + (sy * g) * cx[5][d] + (sx * sy * g2) * cx[6][d];
+      Ju[d][0] = cu[0][d] + (sy * g) * cu[2][d] + (sz * g) * cu[4][d] + (sy * sz * g2) * cu[6][d];
+      Ju[d][1] = cu[1][d] + (sx * g) * cu[2][d] + (sz * g) * cu[5][d] + (sx * sz * g2) * cu[6][d];
+      Ju[d][2] = cu[3][d] + (sx * g) * cu[4][d] + (sy * g) * cu[5][d] + (sx * sy * g2) * cu[6][d];
+    }
+    double A[3][3];
+    A[0][0] = J0[1][1] * J0[2][2] - J0[1][2] * J0[2][1];
+    A[0][1] = J0[0][2] * J0[2][1] - J0[0][1] * J0[2][2];
+    A[0][2] = J0[0][1] * J0[1][2] - J0[0][2] * J0[1][1];
+    A[1][0] = J0[1][2] * J0[2][0] - J0[1][0] * J0[2][2];
+    A[1][1] = J0[0][0] * J0[2][2] - J0[0][2] * J0[2][0];
+    A[1][2] = J0[0][2] * J0[1][0] - J0[0][0] * J0[1][2];
+    A[2][0] = J0[1][0] * J0[2][1] - J0[1][1] * J0[2][0];
+    A[2][1] = J0[0][1] * J0[2][0] - J0[0][0] * J0[2][1];
+    A[2][2] = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
+    const double det0 = J0[0][0] * A[0][0] + J0[0][1] * A[1][0] + J0[0][2] * A[2][0];
+    double H[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) / det0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        E[i][j] += 0.0625 * (H[i][j] + H[j][i] + H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j]);
+  }
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) eavg[9 * (size_t)e + i + 3 * j] = E[i][j];
+  // principal values: trigonometric solution of the characteristic cubic
+  const double a = E[0][0], b = E[1][0], c = E[2][0], d = E[1][1], ee = E[2][1], f = E[2][2];
+  const double p1 = b * b + c * c + ee * ee;
+  double e1, e2, e3;
+  if (p1 == 0) {
+    e1 = a;
+    e2 = d;
+    e3 = f;
+  } else {
+    const double pi = 4.0 * atan(1.0);
+    double I1 = a + d + f;
+    const double I2 = a * (d + f) + d * f - b * b - c * c - ee * ee;
+    const double I3 = a * d * f + 2.0 * b * c * ee - b * b * f - c * c * d - ee * ee * a;
+    const double Q = (3.0 * I2 - I1 * I1) / 9.0;
+    const double R = (2.0 * I1 * I1 * I1 - 9.0 * I1 * I2 + 27.0 * I3) / 54.0;
+    // The reference takes acos of the raw ratio; round-off near a repeated
+    // eigenvalue can push it outside [-1, 1] (or make Q >= 0) and the
+    // reference then returns NaN.  Clamping gives the limit value instead.
+    const double den = sqrt(fmax(-Q * Q * Q, 0.0));
+    const double ratio = den > 0.0 ? fmin(fmax(R / den, -1.0), 1.0) : 1.0;
+    const double theta = acos(ratio);
+    const double sq = 2.0 * sqrt(fmax(-Q, 0.0));
+    I1 = I1 / 3.0;
+    e1 = sq * cos(theta / 3.0) + I1;
+    e2 = sq * cos((theta + 2.0 * pi) / 3.0) + I1;
+    e3 = sq * cos((theta + 4.0 * pi) / 3.0) + I1;
+  }
+  const double mx = fmax(e3, fmax(e2, e1)), mn = fmin(e3, fmin(e2, e1));
+  princ[3 * (size_t)e + 0] = mx > 0.0 ? mx : 0.0;
+  princ[3 * (size_t)e + 1] = mn < 0.0 ? mn : 0.0;
+  princ[3 * (size_t)e + 2] = 0.5 * (mx - mn);
+}
+
 // Plain streaming copy (calibration of the HBM traffic counters).
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4 *src, float4 *dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1534,6 +1627,33 @@ int ft_get_timing(ft_engine *g, double ms[3], int launches[3]) {
   for (int c = 0; c < 3; ++c) {
     if (ms) ms[c] = g->ms[c];
     if (launches) launches[c] = g->launches[c];
+  }
+  return 0;
+}
+
+int ft_element_strain(ft_engine *g, const double *u, double *eavg, double *principal) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  int rc = upload_u(g, u);
+  if (rc) return rc;
+  double *de = nullptr, *dp = nullptr;
+  if ((rc = dalloc(&de, 9 * (size_t)g->ne)) || (rc = dalloc(&dp, 3 * (size_t)g->ne))) {
+    if (de) (void)hipFree(de);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_strain, dim3((g->ne + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream, g->nn, g->ne,
+                     g->conn, g->X, g->u, de, dp);
+  hipError_t err = hipGetLastError();
+  std::vector<double> he(9 * (size_t)g->ne), hp(3 * (size_t)g->ne);
+  if (err == hipSuccess) err = hipStreamSynchronize(g->stream);
+  if (err == hipSuccess) err = hipMemcpy(he.data(), de, he.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(hp.data(), dp, hp.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(de);
+  (void)hipFree(dp);
+  if (err != hipSuccess) return fail(FT_ERR_HIP, "strain kernel: %s", hipGetErrorString(err));
+  for (int e = 0; e < g->ne; ++e) {
+    const size_t i = g->inv[e];
+    if (eavg) memcpy(eavg + 9 * (size_t)e, &he[9 * i], 9 * sizeof(double));
+    if (principal) memcpy(principal + 3 * (size_t)e, &hp[3 * i], 3 * sizeof(double));
   }
   return 0;
 }

@@ -134,6 +134,17 @@ int ft_get_status(ft_engine *eng, int *status);
  * Voigt components per element, element order), as FP64. */
 int ft_get_visco_history(ft_engine *eng, double *hist);
 
+/* Element-averaged Green-Lagrange strain (CalculateStrain, CalculateStrain.cpp:
+ * 77-97; 9 column-major values per element) and the principal-strain summary
+ * of CalculateMaximumPrincipalStrain (CalculateStrain.cpp:8-75; per element:
+ * max principal clamped >= 0, min principal clamped <= 0, max shear), FP64,
+ * at displacement u (NULL = device displacement).  Either output may be NULL.
+ * Where round-off at a repeated eigenvalue pushes the reference's acos
+ * argument outside [-1, 1] (the reference then yields NaN), the argument is
+ * clamped, which gives the limit eigenvalues. */
+int ft_element_strain(ft_engine *eng, const double *u, double *eavg,
+                      double *principal);
+
 /* ---- instrumentation -----------------------------------------------------
  * When enabled, every kernel launch of the loop is bracketed by HIP events on
  * the engine's stream.  Times are summed in milliseconds per kernel class

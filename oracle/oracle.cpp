@@ -711,6 +711,65 @@ void oracle_set_initial(void *mp, const double *u0, const double *v0) {
   if (v0) m.v0.assign(v0, v0 + 3 * (size_t)m.nn);
 }
 
+/* CalculateStrain.cpp:77-97 and CalculateMaximumPrincipalStrain
+ * (CalculateStrain.cpp:8-75), with F from CalculateDeformationGradient. */
+void oracle_element_strain(void *mp, const double *u, double *eavg, double *principal) {
+  Model &m = *(Model *)mp;
+  const double kPi = 4.0 * atan(1.0);
+  for (int e = 0; e < m.ne; ++e) {
+    const double *dN = &m.dNdX[192 * (size_t)e];
+    double x[24];
+    element_coords(m, e, u, x);
+    double E[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double pre = 0.5 / 8.0;
+    for (int gp = 0; gp < 8; ++gp) {
+      double F[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+          for (int k = 0; k < 8; ++k) s = s + x[3 * k + i] * dN[gp * 24 + 3 * k + j];
+          F[3 * j + i] = s;
+        }
+      // dgemm('T','N', alpha = 1/16, beta = 1): E += pre F^T F
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < 3; ++k) s += F[k + 3 * i] * F[k + 3 * j];
+          E[i + 3 * j] += pre * s;
+        }
+    }
+    E[0] -= 0.5;
+    E[4] -= 0.5;
+    E[8] -= 0.5;
+    if (eavg) memcpy(eavg + 9 * (size_t)e, E, sizeof(E));
+    double a = E[0], b = E[1], c = E[2], d = E[4], ee = E[5], f = E[8];
+    double p1 = b * b + c * c + ee * ee, e1, e2, e3;
+    if (p1 == 0) {
+      e1 = a;
+      e2 = d;
+      e3 = f;
+    } else {
+      double I1 = a + d + f;
+      double I2 = a * (d + f) + d * f - b * b - c * c - ee * ee;
+      double I3 = a * d * f + 2.0 * b * c * ee - b * b * f - c * c * d - ee * ee * a;
+      double Q = (3.0 * I2 - I1 * I1) / 9.0;
+      double R = (2.0 * I1 * I1 * I1 - 9.0 * I1 * I2 + 27.0 * I3) / 54.0;
+      double theta = acos(R / sqrt(-Q * Q * Q));
+      double sq = 2.0 * sqrt(-Q);
+      I1 = I1 / 3.0;
+      e1 = sq * cos(theta / 3.0) + I1;
+      e2 = sq * cos((theta + 2.0 * kPi) / 3.0) + I1;
+      e3 = sq * cos((theta + 4.0 * kPi) / 3.0) + I1;
+    }
+    double mx = fmax(e3, fmax(e2, e1)), mn = fmin(e3, fmin(e2, e1));
+    if (principal) {
+      principal[3 * (size_t)e] = mx > 0.0 ? mx : 0.0;
+      principal[3 * (size_t)e + 1] = mn < 0.0 ? mn : 0.0;
+      principal[3 * (size_t)e + 2] = 0.5 * (mx - mn);
+    }
+  }
+}
+
 void oracle_motion_eval(const double *motion, const double X[3], double t, double u[3],
                         double v[3], double a[3]) {
   motion_eval(motion, X, t, u, v, a);

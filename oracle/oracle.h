@@ -79,6 +79,9 @@ int oracle_run(void *m, int max_steps, double tmax, double dt_factor,
                double *fi_out, double *hist_time, double *hist_dt,
                double *hist_energy, int n_snap, const int *snap_steps,
                double *snap_u);
+/* Element-averaged Green-Lagrange strain (9 column-major per element) and
+ * principal summary (max >= 0, min <= 0, max shear; 3 per element) at u. */
+void oracle_element_strain(void *m, const double *u, double *eavg, double *principal);
 /* Motion evaluation (used by tests of the kinematics). */
 void oracle_motion_eval(const double *motion, const double X[3], double t,
                         double u[3], double v[3], double a[3]);

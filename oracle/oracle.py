@@ -56,6 +56,7 @@ def lib():
         L.oracle_char_length.argtypes = [_D]
         L.oracle_element_mass.argtypes = [_D, ctypes.c_double, _D]
         L.oracle_motion_eval.argtypes = [_D, _D, ctypes.c_double, _D, _D, _D]
+        L.oracle_element_strain.argtypes = [ctypes.c_void_p, _D, _D, _D]
         _lib = L
     return _lib
 
@@ -174,6 +175,12 @@ class Model:
 
     def stable_dt(self, u, boundary):
         return lib().oracle_stable_dt(self.h, _d(f64(u)), _i(i32(boundary)))
+
+    def element_strain(self, u):
+        E = np.zeros(9 * self.ne)
+        P = np.zeros(3 * self.ne)
+        lib().oracle_element_strain(self.h, _d(f64(u)), _d(E), _d(P))
+        return E, P
 
     def history(self):
         h = np.zeros(216 * self.ne)

@@ -255,6 +255,20 @@ int mode_run(const std::string &dir) {
     hist_energy.push_back(fabs(WKE + Wint - Wext));
     snap(step);
   }
+  // CalculateStrain (CalculateStrain.cpp:77-97) and the principal-strain
+  // summary (CalculateStrain.cpp:8-75) at the final state (F from the last
+  // GetForce).
+  CalculateStrain();
+  write_bin("out_eavg.f64", Eavg, 9 * (size_t)nelements);
+  std::vector<double> princ;
+  for (int e = 0; e < nelements; ++e) {
+    double mx, mn, sh;
+    CalculateMaximumPrincipalStrain(e, &mx, &mn, &sh);
+    princ.push_back(mx);
+    princ.push_back(mn);
+    princ.push_back(sh);
+  }
+  write_bin("out_princ.f64", princ.data(), princ.size());
   write_bin("out_u.f64", displacements, nDOF);
   write_bin("out_v.f64", velocities, nDOF);
   write_bin("out_a.f64", accelerations, nDOF);

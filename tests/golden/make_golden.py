@@ -79,7 +79,8 @@ def golden_run(name, c, mesh_file=None):
                    ref_v=r("out_v.f64"), ref_a=r("out_a.f64"), ref_fi=r("out_fi.f64"),
                    ref_time=r("out_time.f64"), ref_dt=r("out_dt.f64"),
                    ref_energy=r("out_energy.f64").reshape(-1, 4),
-                   ref_snap_u=r("out_snap_u.f64").reshape(-1, 3 * nn), ref_steps=np.int64(steps))
+                   ref_snap_u=r("out_snap_u.f64").reshape(-1, 3 * nn), ref_steps=np.int64(steps),
+                   ref_eavg=r("out_eavg.f64"), ref_princ=r("out_princ.f64"))
         rows = []
         with open(os.path.join(d, "energy_golden.dat")) as f:
             for line in f:

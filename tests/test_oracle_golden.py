@@ -109,3 +109,55 @@ def test_ex9_against_abaqus_report():
     t_ab, u_ab = rpt[-1, 0], rpt[-1, 9:12]  # columns X, A(4), RF(4), U mag, U1..U3
     assert abs(r["time"][-1] - t_ab) / t_ab < 0.01
     assert np.max(np.abs(u - rpt[-1, 10:13]) / np.abs(rpt[-1, 10:13])) < 0.05
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_strain_matches_reference(name):
+    """Eavg and the principal summary at the final state of each golden run
+    (CalculateStrain.cpp:8-97 run by the reference build)."""
+    z = golden("run_" + name)
+    o = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    E, P = o.element_strain(z["ref_u"])
+    assert np.abs(E - z["ref_eavg"]).max() <= 1e-14 * max(1.0, np.abs(z["ref_eavg"]).max())
+    ref = z["ref_princ"]
+    np.testing.assert_array_equal(np.isnan(P), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    assert np.abs(P[ok] - ref[ok]).max(initial=0.0) <= 1e-12 * max(1.0, np.abs(ref[ok]).max(initial=0.0))
+
+
+def test_strain_principal_against_eigenvalues():
+    """The trigonometric cubic solution against a symmetric eigen-solver on
+    the reference's own Eavg outputs (finite entries)."""
+    for name in RUNS:
+        z = golden("run_" + name)
+        E = z["ref_eavg"].reshape(-1, 3, 3).transpose(0, 2, 1)
+        P = z["ref_princ"].reshape(-1, 3)
+        w = np.linalg.eigvalsh(0.5 * (E + E.transpose(0, 2, 1)))
+        ok = ~np.isnan(P[:, 2])
+        scale = max(1e-300, np.abs(w).max())
+        assert np.abs(P[ok, 0] - np.maximum(w[ok, 2], 0)).max(initial=0) <= 1e-7 * scale
+        assert np.abs(P[ok, 1] - np.minimum(w[ok, 0], 0)).max(initial=0) <= 1e-7 * scale
+        assert np.abs(P[ok, 2] - 0.5 * (w[ok, 2] - w[ok, 0])).max(initial=0) <= 1e-7 * scale
+
+
+def test_strain_closed_forms():
+    """Homogeneous deformations of a distorted hex: rigid rotation gives
+    E = 0; u = (F - I) X gives E = (F^T F - I) / 2 exactly at every Gauss
+    point, and the principal values are its eigenvalues."""
+    rng = np.random.default_rng(5)
+    X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                  [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float) * 1e-3
+    X = X + rng.uniform(-1e-4, 1e-4, X.shape)
+    conn = np.arange(8, dtype=np.int32)
+    o = O.Model(X.reshape(-1), conn, np.zeros(1, np.int32), np.array([1], np.int32),
+                np.array([1000.0, 1e6, 1e7, 0, 0, 0, 0, 0, 0]))
+    a = 0.7
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    E, P = o.element_strain(((X @ R.T) - X).reshape(-1))
+    assert np.abs(E).max() < 1e-14 and np.abs(P[:2]).max() < 1e-14
+    F = np.eye(3) + np.array([[0.10, 0.03, -0.02], [0.01, -0.05, 0.04], [0.02, 0.00, 0.07]])
+    E, P = o.element_strain(((X @ F.T) - X).reshape(-1))
+    Ex = 0.5 * (F.T @ F - np.eye(3))
+    assert np.abs(E.reshape(3, 3).T - Ex).max() < 1e-14
+    w = np.linalg.eigvalsh(Ex)
+    np.testing.assert_allclose(P, [max(w[2], 0), min(w[0], 0), 0.5 * (w[2] - w[0])], rtol=0, atol=1e-12)
