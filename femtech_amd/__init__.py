@@ -21,6 +21,16 @@ MOTION_ROT_HALFSINE = 3
 STATUS_NONFINITE = 1
 STATUS_INVERTED = 2
 STATUS_DT_COLLAPSE = 4
+STATUS_DT_FP32 = 16
+# error codes (include/femtech_amd.h): FemTech's own, then the engine's
+FT_ERR_MATERIAL = 1
+FT_ERR_INPUT = 3
+FT_ERR_ALLOC = 12
+FT_ERR_DT = 19
+FT_ERR_HIP = 50
+FT_ERR_ARG = 51
+FT_ERR_STATE = 52
+FT_ERR_NUMERIC = 53
 
 # FemTech's termination codes (InitFinalizeFemTech.cpp:82-86 callers) and the
 # engine's own.

@@ -698,6 +698,50 @@ __global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *co
   princ[3 * (size_t)e + 2] = 0.5 * (mx - mn);
 }
 
+// CalculateAccelerations (CalculateAcclerations.cpp:4-12) on node-major host
+// arrays: a = f_net / m on DOFs whose boundary flag is 0.
+__global__ __launch_bounds__(kBlock) void k_accel(int ndof, const double *f_net, const int *boundary,
+                                                  const double *mass, double *a) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndof; i += gridDim.x * blockDim.x)
+    if (!boundary[i]) a[i] = f_net[i] / mass[i / 3];
+}
+
+// CheckEnergy's per-step sums (CheckEnergy.cpp:15-52, one rank) on node-major
+// arrays: [0] sum dd (fi_prev + fi), [1] sum dd (fe_prev + fe) plus the
+// boundary reactions dd (fi_prev + fi + m (a + a_prev)), [2] sum m v^2.
+// Per-block partials in a fixed order; k_sum3 adds them in a fixed order.
+constexpr int kEnergyBlocks = 512;
+struct EnergyIn {
+  const double *u, *u_prev, *v, *a, *a_prev, *fi, *fi_prev, *fe, *fe_prev, *mass;
+  const int *boundary;
+};
+__global__ __launch_bounds__(kBlock) void k_energy_terms(int ndof, EnergyIn in, double *part) {
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndof; i += gridDim.x * blockDim.x) {
+    const double m = in.mass[i / 3], dd = in.u[i] - in.u_prev[i], fsum = in.fi_prev[i] + in.fi[i];
+    s[2] += m * in.v[i] * in.v[i];
+    if (in.boundary[i]) s[1] += dd * (fsum + m * (in.a[i] + in.a_prev[i]));
+    s[0] += dd * fsum;
+    s[1] += dd * (in.fe_prev[i] + in.fe[i]);
+  }
+  __shared__ double sh[3][kBlock];
+  for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void k_sum3(int nblk, const double *part, double *out) {
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[3 * b + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
 // Plain streaming copy (calibration of the HBM traffic counters).
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4 *src, float4 *dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -838,6 +882,9 @@ struct ft_engine {
   // history
   int hist_cap = 0, hist_n = 0;
   double *hist_rows = nullptr;
+  // staging for the entry points that take host arrays every call
+  double *aux = nullptr;
+  size_t aux_len = 0;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
@@ -1047,6 +1094,19 @@ int upload_u(ft_engine *g, const double *u) {
   return 0;
 }
 
+// Device staging of `len` doubles for host-array entry points (kept between
+// calls; grows on demand).
+int aux_reserve(ft_engine *g, size_t len) {
+  if (g->aux_len >= len) return 0;
+  if (g->aux) (void)hipFree(g->aux);
+  g->aux = nullptr;
+  g->aux_len = 0;
+  int rc = dalloc(&g->aux, len);
+  if (rc) return rc;
+  g->aux_len = len;
+  return 0;
+}
+
 int check_status(ft_engine *g) {
   int st = 0;
   HIPCHK(hipMemcpyAsync(&st, g->status, sizeof(int), hipMemcpyDeviceToHost, g->stream));
@@ -1213,7 +1273,7 @@ int ft_engine_destroy(ft_engine *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
-                  g->ell, g->fe, g->hist, g->eacc, g->hist_rows};
+                  g->ell, g->fe, g->hist, g->eacc, g->hist_rows, g->aux};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int c = 0; c < 3; ++c)
@@ -1655,6 +1715,53 @@ int ft_element_strain(ft_engine *g, const double *u, double *eavg, double *princ
     if (eavg) memcpy(eavg + 9 * (size_t)e, &he[9 * i], 9 * sizeof(double));
     if (principal) memcpy(principal + 3 * (size_t)e, &hp[3 * i], 3 * sizeof(double));
   }
+  return 0;
+}
+
+int ft_accelerations(ft_engine *g, const double *f_net, const int *boundary, double *a) {
+  if (!g || !f_net || !boundary || !a) return fail(FT_ERR_ARG, "null argument");
+  if (!g->mass_ready) return fail(FT_ERR_STATE, "lumped mass not assembled");
+  const size_t nd = 3 * (size_t)g->nn;
+  int rc = aux_reserve(g, 3 * nd);
+  if (rc) return rc;
+  double *df = g->aux, *da = g->aux + nd;
+  int *db = (int *)(g->aux + 2 * nd);
+  HIPCHK(hipMemcpyAsync(df, f_net, nd * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(da, a, nd * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIPCHK(hipMemcpyAsync(db, boundary, nd * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(k_accel, dim3(grid_for((long)nd)), dim3(kBlock), 0, g->stream, (int)nd, df, db, g->mass,
+                     da);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(a, da, nd * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int ft_energy_terms(ft_engine *g, const double *u, const double *u_prev, const double *v,
+                    const double *a, const double *a_prev, const double *fi, const double *fi_prev,
+                    const double *fe, const double *fe_prev, const int *boundary, double out3[3]) {
+  const double *in[] = {u, u_prev, v, a, a_prev, fi, fi_prev, fe, fe_prev};
+  if (!g || !boundary || !out3) return fail(FT_ERR_ARG, "null argument");
+  for (const double *p : in)
+    if (!p) return fail(FT_ERR_ARG, "null argument");
+  if (!g->mass_ready) return fail(FT_ERR_STATE, "lumped mass not assembled");
+  const size_t nd = 3 * (size_t)g->nn;
+  int rc = aux_reserve(g, 10 * nd + 3 * kEnergyBlocks + 3);
+  if (rc) return rc;
+  double *d[9];
+  for (int k = 0; k < 9; ++k) {
+    d[k] = g->aux + k * nd;
+    HIPCHK(hipMemcpyAsync(d[k], in[k], nd * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  }
+  int *db = (int *)(g->aux + 9 * nd);
+  double *part = g->aux + 10 * nd, *res = part + 3 * kEnergyBlocks;
+  HIPCHK(hipMemcpyAsync(db, boundary, nd * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  EnergyIn ei{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], g->mass, db};
+  hipLaunchKernelGGL(k_energy_terms, dim3(kEnergyBlocks), dim3(kBlock), 0, g->stream, (int)nd, ei, part);
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(64), 0, g->stream, kEnergyBlocks, part, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out3, res, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
   return 0;
 }
 

@@ -145,6 +145,21 @@ int ft_get_visco_history(ft_engine *eng, double *hist);
 int ft_element_strain(ft_engine *eng, const double *u, double *eavg,
                       double *principal);
 
+/* CalculateAccelerations (CalculateAcclerations.cpp:4-12) on host arrays
+ * (3 per node): a = f_net / mass on DOFs with boundary == 0; the other
+ * entries of a are returned unchanged.  Needs the lumped mass. */
+int ft_accelerations(ft_engine *eng, const double *f_net, const int *boundary,
+                     double *a);
+/* The sums of CheckEnergy (CheckEnergy.cpp:15-52, one rank) over host arrays
+ * (3 per node each): out3[0] = sum dd (fi_prev + fi), out3[1] = sum dd
+ * (fe_prev + fe) plus, on DOFs with boundary != 0, dd (fi_prev + fi +
+ * m (a + a_prev)), out3[2] = sum m v^2, with dd = u - u_prev.  The caller
+ * applies CheckEnergy's factors of 1/2 and accumulates the works. */
+int ft_energy_terms(ft_engine *eng, const double *u, const double *u_prev,
+                    const double *v, const double *a, const double *a_prev,
+                    const double *fi, const double *fi_prev, const double *fe,
+                    const double *fe_prev, const int *boundary, double out3[3]);
+
 /* ---- instrumentation -----------------------------------------------------
  * When enabled, every kernel launch of the loop is bracketed by HIP events on
  * the engine's stream.  Times are summed in milliseconds per kernel class

@@ -1,0 +1,87 @@
+"""FemTech's C++ call surface on the GPU: the in-repo driver with ex9's call
+sequence (femtech_amd/drivers/explicit_tension.cpp) against the reference
+build's runs of the same problems (tests/golden/run_ex9.npz: the reference's
+1-element cube; run_bench10.npz: its 10-element benchmark mesh).
+
+Compatibility mode (host arrays authoritative, every library routine on the
+GPU) and resident mode (the loop on the device) must both reproduce the
+reference's FP64 trajectory: final displacement to 1e-10 relative, the same
+number of steps, and the energy file CheckEnergy writes (%12.6e text) to its
+printed precision.  The VTU output carries the same displacement field.
+"""
+import os
+import subprocess
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden, rel
+from test_compat import DRIVER, write_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"ex9": 1.0, "bench10": 0.1}
+
+
+def run_driver(tmp_path, name, extra):
+    z = golden("run_" + name)
+    write_case(str(tmp_path), z)
+    cmd = [DRIVER, "mesh.inp", "--tmax", repr(CASES[name]), "--dmax", "0.007", "--length", "0.005",
+           "--plots", "1000000"] + extra
+    print("driver:", name, " ".join(extra), flush=True)
+    res = subprocess.run(cmd, cwd=str(tmp_path), capture_output=True, text=True,
+                         env=dict(os.environ, FEMTECH_UID="t"), timeout=300)
+    assert res.returncode == 0, (res.returncode, res.stdout[-2000:], res.stderr[-2000:])
+    u = np.fromfile(str(tmp_path / "tension.u.f64"))
+    summary = dict(line.split() for line in open(tmp_path / "tension.summary.txt"))
+    return z, u, summary
+
+
+def energy_rows(path):
+    return np.array([[float(x) for x in line.split()] for line in open(path)
+                     if line.strip() and not line.startswith("#")])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_compatibility_mode_matches_reference(gpu_ready, tmp_path, name):
+    z, u, summary = run_driver(tmp_path, name, [])
+    assert int(summary["steps"]) == int(z["ref_steps"])
+    assert float(summary["time"]) == pytest.approx(float(z["ref_time"][-1]), rel=1e-13)
+    assert rel(u, z["ref_u"]) < 1e-10
+    rows = energy_rows(tmp_path / "energy_t.dat")
+    ref = z["ref_energy_file"]
+    assert rows.shape == ref.shape
+    scale = np.abs(ref[:, 1:4]).max()
+    np.testing.assert_allclose(rows[:, 0], ref[:, 0], rtol=1e-6)
+    assert np.abs(rows[:, 1:4] - ref[:, 1:4]).max() <= 2e-6 * scale
+    # last VTU piece holds the final displacement (%10.8e text)
+    vtu = sorted(os.listdir(tmp_path / "results" / "vtu"))[-1]
+    root = ET.parse(str(tmp_path / "results" / "vtu" / vtu)).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    assert int(piece.get("NumberOfPoints")) == u.size // 3
+    disp = [a for a in piece.iter("DataArray") if a.get("Name") == "Displacements"][0]
+    vals = np.array(disp.text.split(), float)
+    np.testing.assert_allclose(vals, u, rtol=1e-8, atol=1e-20)
+    assert os.path.exists(tmp_path / "mesh.pvd")
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_resident_mode_matches_reference(gpu_ready, tmp_path, name):
+    z, u, summary = run_driver(tmp_path, name, ["--resident"])
+    assert int(summary["steps"]) == int(z["ref_steps"])
+    assert rel(u, z["ref_u"]) < 1e-10
+    E = np.array([float(summary[k]) for k in ("W_int", "W_ext", "KE")])
+    assert rel(E, z["ref_energy"][-1, :3]) < 1e-9
+
+
+def test_fp32_compatibility_mode_within_split_tolerance(gpu_ready, tmp_path):
+    """FEMTECH_PRECISION=fp32: FP32 element arithmetic under the same driver."""
+    z = golden("run_bench10")
+    write_case(str(tmp_path), z)
+    res = subprocess.run([DRIVER, "mesh.inp", "--tmax", "0.1", "--plots", "5"], cwd=str(tmp_path),
+                         capture_output=True, text=True, timeout=300,
+                         env=dict(os.environ, FEMTECH_UID="t", FEMTECH_PRECISION="fp32"))
+    assert res.returncode == 0, res.stderr[-2000:]
+    u = np.fromfile(str(tmp_path / "tension.u.f64"))
+    assert rel(u, z["ref_u"]) < 1e-5
