@@ -112,6 +112,13 @@ def main():
     elem_gbs = traffic[0] / (elem_ms * 1e-3) / 1e9
     nodal_gbs = traffic[1] / (nodal_ms * 1e-3) / 1e9
     step_bytes = traffic[0] + traffic[1]
+    # Layout-independent compulsory traffic of one step at these precisions
+    # (each array touched once): per element connectivity 32 + part id 4 +
+    # FP32 element-force write 96 and read 96 + node-to-element entries 32 =
+    # 260 B; per node row pointer 4 + X 12 + u read 24 + u read/write 48 +
+    # v read/write 48 + mass 8 + BC flag 1 = 145 B, plus fi_prev read/write 48
+    # for the FP64 energy balance.
+    fixed_bytes = 260.0 * ne + (145.0 + 48.0) * nn
     out = {
         "metric": "element-steps/s on 1M-hex neo-Hookean; surrogate vs plain at <=1% stress err",
         "value": value,
@@ -135,7 +142,9 @@ def main():
                      "avg_launch_ms": elem_ms},
         "nodal_kernel": {"achieved": nodal_gbs, "frac": nodal_gbs / HBM_PEAK_GBS,
                          "algorithmic_bytes_per_launch": traffic[1], "avg_launch_ms": nodal_ms},
-        "step_roofline_frac": step_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        "step_roofline_frac": fixed_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        "step_bytes": {"compulsory_model": fixed_bytes, "this_layout": step_bytes,
+                       "per_element_step_compulsory": fixed_bytes / ne},
         "energy": {"W_int": E[0], "W_ext": E[1], "KE": E[2], "balance": E[3]},
         "status": status,
     }

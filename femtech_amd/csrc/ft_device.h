@@ -45,6 +45,7 @@ template <> struct Num<float> {
     }
     return log1pf(x);
   }
+  __host__ __device__ static float fma_(float a, float b, float c) { return fmaf(a, b, c); }
   __host__ __device__ static float expm1_(float x) { return expm1f(x); }
   __host__ __device__ static float exp_(float x) { return expf(x); }
   // hardware square root (1 ulp) in the FP32 element path
@@ -66,6 +67,7 @@ template <> struct Num<float> {
 };
 template <> struct Num<double> {
   __host__ __device__ static double log1p_(double x) { return log1p(x); }
+  __host__ __device__ static double fma_(double a, double b, double c) { return fma(a, b, c); }
   __host__ __device__ static double expm1_(double x) { return expm1(x); }
   __host__ __device__ static double exp_(double x) { return exp(x); }
   __host__ __device__ static double sqrt_(double x) { return sqrt(x); }
@@ -303,6 +305,49 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
   return false;
 }
 
+// Kirchhoff stress tau = F S F^T = J sigma of the neo-Hookean (MAT 1) and
+// isotropic HGO (MAT 4) laws, from H = grad_X u:
+//   neo-Hookean (CompressibleNeoHookean.cpp:26-55): tau = mu (b - I) + lambda ln J I
+//   HGO (HGOIsotropic.cpp:37-90, kappa = 1/3):
+//     tau = J^-2/3 (mu + fib) dev(b) + K/2 (J^2 - 1) I,  K = lambda + 2 mu / 3
+// with b - I = H + H^T + H H^T formed directly (no cancellation at small
+// strain).  Equal to F S F^T of pk2_stress in exact arithmetic.
+template <typename R, int MAT>
+__host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3][3], R Jm1, Sym<R> &tau) {
+  const R mu = R(pp.mu), lambda = R(pp.lambda);
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+  Sym<R> bm;  // b - I
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = I[k], j = J[k];
+    bm.v[k] = H[i][j] + H[j][i] + (H[i][0] * H[j][0] + H[i][1] * H[j][1] + H[i][2] * H[j][2]);
+  }
+  const R lnJ = Num<R>::log1p_(Jm1);
+  if (MAT == 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau.v[k] = mu * bm.v[k];
+    tau.v[0] += lambda * lnJ;
+    tau.v[1] += lambda * lnJ;
+    tau.v[2] += lambda * lnJ;
+  } else {
+    const R kappa = R(1.0 / 3.0);
+    const R K = lambda + R(2) * mu / R(3);
+    const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * lnJ);
+    const R Jm23 = R(1) + Jm23m1;
+    const R trb = bm.v[0] + bm.v[1] + bm.v[2];  // tr(b - I) = tr(C - I)
+    const R Ea = kappa * (R(3) * Jm23m1 + Jm23 * trb);
+    R fib = R(0);
+    if (Ea > R(0)) fib = R(2) * R(pp.k1) * Num<R>::exp_(R(pp.k2) * Ea * Ea) * Ea * kappa;
+    const R a = Jm23 * (mu + fib);
+    const R p = R(0.5) * K * Jm1 * (Jm1 + R(2)) - a * trb / R(3);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau.v[k] = a * bm.v[k];
+    tau.v[0] += p;
+    tau.v[1] += p;
+    tau.v[2] += p;
+  }
+}
+
 template <typename R>
 __host__ __device__ inline R norm_cross(const R a[3], const R b[3]) {
   // normOfCrossProduct, math.cpp:30-43
@@ -451,6 +496,26 @@ __host__ __device__ inline R hex_part(const R d[7][3], int k) {
   return face_area(cD, e1, e2);
 }
 
+// f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k accumulated in the monomial basis of
+// the element (1, xi, eta, xi.eta, zeta, xi.zeta, eta.zeta, xi.eta.zeta; the
+// constant term is zero).  The signs are +-1, so each product is exact and
+// chaining the updates as FMAs changes only the order of the additions.
+template <typename R>
+__host__ __device__ inline void accumulate_force_basis(R h[7], R q0, R q1, R q2, R sx, R sy, R sz) {
+  h[0] += q0;
+  h[1] += q1;
+  h[3] += q2;
+  h[2] = Num<R>::fma_(sy, q0, h[2]);
+  h[2] = Num<R>::fma_(sx, q1, h[2]);
+  h[4] = Num<R>::fma_(sz, q0, h[4]);
+  h[4] = Num<R>::fma_(sx, q2, h[4]);
+  h[5] = Num<R>::fma_(sz, q1, h[5]);
+  h[5] = Num<R>::fma_(sy, q2, h[5]);
+  h[6] = Num<R>::fma_(sy * sz, q0, h[6]);
+  h[6] = Num<R>::fma_(sx * sz, q1, h[6]);
+  h[6] = Num<R>::fma_(sx * sy, q2, h[6]);
+}
+
 // Status bits of element_eval (FT_STATUS_* of the C ABI plus an internal
 // "unknown material" bit).
 constexpr int kFlagNonFinite = 1;
@@ -542,6 +607,41 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
 #pragma unroll
         for (int j = 0; j < 3; ++j)
           H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
+      if constexpr (MAT == 1 || MAT == 4) {
+        // Spatial form: f_a = sum_gp det0 P J0^-T dN_a/dxi with P = tau F^-T
+        // and F J0 = Jx (the deformed Jacobian), so Q = det0 tau Jx^-T =
+        // tau adj(Jx)^T / J.
+        const R Jm1 = det_ip1_m1(H);
+        if (!(Jm1 > R(-1))) flags |= kFlagInverted;
+        Sym<R> tau;
+        kirchhoff_stress<R, MAT>(pp, H, Jm1, tau);
+        const R iJ = Num<R>::rcp_(R(1) + Jm1);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tau.v[k] *= iJ;
+        R X[3][3], B[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) X[i][j] = J0[i][j] + Ju[i][j];
+        B[0][0] = X[1][1] * X[2][2] - X[1][2] * X[2][1];
+        B[0][1] = X[0][2] * X[2][1] - X[0][1] * X[2][2];
+        B[0][2] = X[0][1] * X[1][2] - X[0][2] * X[1][1];
+        B[1][0] = X[1][2] * X[2][0] - X[1][0] * X[2][2];
+        B[1][1] = X[0][0] * X[2][2] - X[0][2] * X[2][0];
+        B[1][2] = X[0][2] * X[1][0] - X[0][0] * X[1][2];
+        B[2][0] = X[1][0] * X[2][1] - X[1][1] * X[2][0];
+        B[2][1] = X[0][1] * X[2][0] - X[0][0] * X[2][1];
+        B[2][2] = X[0][0] * X[1][1] - X[0][1] * X[1][0];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const R t0 = sym_at(tau, i, 0), t1 = sym_at(tau, i, 1), t2 = sym_at(tau, i, 2);
+          const R q0 = t0 * B[0][0] + t1 * B[0][1] + t2 * B[0][2];
+          const R q1 = t0 * B[1][0] + t1 * B[1][1] + t2 * B[1][2];
+          const R q2 = t0 * B[2][0] + t1 * B[2][1] + t2 * B[2][2];
+          accumulate_force_basis(h[i], q0, q1, q2, sx, sy, sz);
+        }
+        continue;
+      }
       R hloc[18];
       R *hp = nullptr;
       if (mat == 5 && hist.valid()) {
@@ -571,13 +671,7 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
         const R q1 = P[i][0] * A[1][0] + P[i][1] * A[1][1] + P[i][2] * A[1][2];
         const R q2 = P[i][0] * A[2][0] + P[i][1] * A[2][1] + P[i][2] * A[2][2];
         // f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k in the monomial basis
-        h[i][0] += q0;
-        h[i][1] += q1;
-        h[i][3] += q2;
-        h[i][2] += sy * q0 + sx * q1;
-        h[i][4] += sz * q0 + sx * q2;
-        h[i][5] += sz * q1 + sy * q2;
-        h[i][6] += (sy * sz) * q0 + (sx * sz) * q1 + (sx * sy) * q2;
+        accumulate_force_basis(h[i], q0, q1, q2, sx, sy, sz);
       }
     }
     // Back to the nodes: f_p = sum_q c_q prod(signs), c = h scaled by

@@ -66,6 +66,7 @@ constexpr uint64_t kInfBits = 0x7FF0000000000000ull;
 constexpr int kSkipBit = 1 << 30;
 constexpr int kPidMask = (1 << 20) - 1;
 constexpr int kMaxMotions = 64;
+constexpr int kEaccStride = 8;  // doubles per nodal block in the energy accumulators
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // walks one contiguous range of elements or nodes (its own L2 then sees the
@@ -230,7 +231,8 @@ struct ElemArgs {
   const double *X;       // component-major [3][nn]
   const double *u;       // component-major [3][nn]
   const PartDev *parts;
-  R *fe;                 // 24 per element (nullable when only dt is wanted)
+  R *fe;                 // element forces (nullable when only dt is wanted)
+  int corner;            // fe[corner][node][xyz] instead of fe[slot][element][xyz]
   R *hist;               // 144 per element, component-major [c * ne + e]
   const StepSlot *slot;  // nullable: step slot giving dt_k and done
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
@@ -242,8 +244,14 @@ struct ElemArgs {
   int update_hist;
 };
 
+// Occupancy target of the FP32 instances (waves per SIMD; 4 = at most 128
+// VGPRs).  The Prony instance (MAT 5) and the FP64 ones are not held to it.
+#ifndef FT_ELEM_WAVES_FP32
+#define FT_ELEM_WAVES_FP32 4
+#endif
 template <typename R, bool FORCE, bool DT, int MAT>
-__global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
+__global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT != 5) ? FT_ELEM_WAVES_FP32 : 1) void k_element(
+    ElemArgs<R> a) {
   __shared__ double sh_min[kBlock / 64];
   const int L = xcd_block();
   const int e = a.begin + L * kBlock + threadIdx.x;
@@ -290,13 +298,14 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
     element_eval<R, FORCE, DT, MAT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
                                a.update_hist != 0, outv, my_dt, flags);
     if (FORCE) {
-      // Slot-major layout fe[slot][element][xyz]: each of the 8 stores of a
-      // wavefront covers one contiguous run, and so does each of the 8 loads
-      // of the node gather.
+      // Slot-major layouts: fe[slot][element][xyz] or fe[corner][node][xyz].
+      // Each of the 8 stores of a wavefront covers one contiguous run (for
+      // the corner layout: neighbouring elements of a block have
+      // neighbouring corner-k nodes), and so does each load of the node sum.
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        R *dst = a.fe + 3 * (size_t)k * a.ne;  // uniform
-        const unsigned o = 3u * (unsigned)e;
+        R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);  // uniform
+        const unsigned o = 3u * (unsigned)(a.corner ? nd[k] : e);
         dst[o] = outv[3 * k];
         dst[o + 1] = outv[3 * k + 1];
         dst[o + 2] = outv[3 * k + 2];
@@ -361,7 +370,25 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<R> a) {
 template <typename R, int W>
 __device__ inline void gather3(const int *ell, int nn, int width, const R *fe, int n, double f[3]) {
   f[0] = f[1] = f[2] = 0.0;
-  if (W > 0) {
+  if (W < 0) {
+    // Corner-slot layout fe[corner][node][xyz]: the element that has node n
+    // as its local corner k wrote slot k (zero where no element does).
+    R v[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const R *src = fe + 3 * (size_t)k * nn;  // uniform
+      const unsigned o = 3u * (unsigned)n;
+      v[k][0] = src[o];
+      v[k][1] = src[o + 1];
+      v[k][2] = src[o + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      f[0] += (double)v[k][0];
+      f[1] += (double)v[k][1];
+      f[2] += (double)v[k][2];
+    }
+  } else if (W > 0) {
     constexpr int WA = W > 0 ? W : 1;
     int id[WA];
 #pragma unroll
@@ -424,7 +451,8 @@ struct NodalArgs {
   int parity;
   int mode;
   int energy;
-  double *eacc;  // 4 per logical block: W_int, W_ext, KE, unused
+  double *eacc;  // kEaccStride per logical block: W_int, W_ext, KE, and the
+                 // fi_{k-1} halves of the next step's W_int and W_ext
   double tmax, factor, failure_dt;
   int max_steps;
 };
@@ -458,7 +486,12 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
   for (int i = threadIdx.x; i < 3 * a.n_motions; i += blockDim.x)
     ms[i / a.n_motions][i % a.n_motions] = a.mstate[i];
   __syncthreads();
-  double sWint = 0.0, sWext = 0.0, KE = 0.0;
+  // Energy (CheckEnergy.cpp:15-52): the terms of step k pair each DOF's
+  // displacement increment du_k with fi_{k-1} + fi_k.  The fi_{k-1} halves
+  // (pInt, pExt) are added by the launch that knows fi_{k-1} and already
+  // computes du_k, so fi is not carried from step to step; it is stored only
+  // when the run stops.
+  double sWint = 0.0, sWext = 0.0, KE = 0.0, pInt = 0.0, pExt = 0.0;
   if (n < a.nn) {
     double f[3];
     if (a.mode == kResume) {
@@ -497,8 +530,8 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
         v = vb;
       }
       if (a.mode == kStep && a.energy) {
-        // CheckEnergy.cpp:34-46 with Delta u = dt v_half for free DOFs
-        const double fp = a.fi[id];
+        // this step's halves, with du = dt v_half for free DOFs and the
+        // boundary reaction fi + m a at constrained ones (CheckEnergy.cpp:34-46)
         double dd;
         if (mi < 0) {
           dd = dt_k * vso;
@@ -506,35 +539,49 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
           double up, vp, ap;
           motion_apply(mo, ms[0][mi], Xn, d, up, vp, ap);
           dd = ub - up;
-          sWext += dd * (fp + f[d] + m * (acc + ap));
+          sWext += dd * (f[d] + m * acc);
         }
         KE += m * v * v;
-        sWint += dd * (fp + f[d]);
+        sWint += dd * f[d];
       }
-      if (a.mode != kResume) a.fi[id] = f[d];
       if (go) {
         if (mi < 0) {
           const double vh = v + (t_k1half - t_k) * acc;
           a.vs[id] = vh;
           a.u[id] = uo + dt_next * vh;
+          pInt += (dt_next * vh) * f[d];
         } else {
-          motion_apply(mo, ms[2][mi], Xn, d, ub, vb, ab);
-          a.u[id] = ub;
-          a.vs[id] = vb;
+          double un, vn, an;
+          motion_apply(mo, ms[2][mi], Xn, d, un, vn, an);
+          a.u[id] = un;
+          a.vs[id] = vn;
+          const double dd1 = un - ub;
+          pInt += dd1 * f[d];
+          pExt += dd1 * (f[d] + m * acc);
         }
       } else {
         a.vs[id] = v;
+        if (a.mode != kResume) a.fi[id] = f[d];
       }
     }
   }
-  if (a.mode == kStep && a.energy) {
+  if (a.energy && (a.mode == kStep || go)) {  // uniform over the grid
     const double w = block_sum(sWint, sh);
     const double x = block_sum(sWext, sh);
     const double k = block_sum(KE, sh);
+    const double pw = block_sum(pInt, sh);
+    const double px = block_sum(pExt, sh);
     if (threadIdx.x == 0) {
-      a.eacc[4 * L + 0] += 0.5 * w;
-      a.eacc[4 * L + 1] += 0.5 * x;
-      a.eacc[4 * L + 2] = 0.5 * k;
+      double *acc = a.eacc + kEaccStride * L;
+      if (a.mode == kStep) {
+        acc[0] += 0.5 * (acc[3] + w);
+        acc[1] += 0.5 * (acc[4] + x);
+        acc[2] = 0.5 * k;
+      }
+      if (go) {
+        acc[3] = pw;
+        acc[4] = px;
+      }
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -582,9 +629,9 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
   __shared__ double sh[kBlock / 64];
   double w = 0, x = 0, k = 0;
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-    w += eacc[4 * b];
-    x += eacc[4 * b + 1];
-    k += eacc[4 * b + 2];
+    w += eacc[kEaccStride * b];
+    x += eacc[kEaccStride * b + 1];
+    k += eacc[kEaccStride * b + 2];
   }
   w = block_sum(w, sh);
   x = block_sum(x, sh);
@@ -848,6 +895,12 @@ struct ft_engine {
   double *X = nullptr;
   int *conn = nullptr, *einfo = nullptr, *ell = nullptr;
   int width = 8;  // ELL width: max node valence, at least 8
+  // Corner-slot force layout: when no two elements share a node at the same
+  // local corner (every structured block), element forces go straight to
+  // fe[corner][node][xyz] and the node sum reads 8 contiguous slots with no
+  // incidence list.  Otherwise fe[slot][element][xyz] + the ELL.
+  bool corner = false;
+  std::vector<int> h_conn;  // caller's connectivity (element-force read-back)
   PartDev *parts = nullptr;
   std::vector<PartDev> h_parts;
   bool has_visco = false;
@@ -992,6 +1045,7 @@ int launch_element(ft_engine *g, int begin, int end, const StepSlot *slot, doubl
   a.u = g->u;
   a.parts = g->parts;
   a.fe = reinterpret_cast<R *>(g->fe);
+  a.corner = g->corner ? 1 : 0;
   a.hist = reinterpret_cast<R *>(g->hist);
   a.slot = slot;
   a.refine = (sizeof(R) == 4 && slot) ? 1 : 0;
@@ -1067,7 +1121,9 @@ int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
   a.failure_dt = g->failure_dt;
   a.max_steps = max_steps;
   Timed t(g, 1);
-  if (g->width == 8)
+  if (g->corner)
+    hipLaunchKernelGGL((k_nodal<R, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+  else if (g->width == 8)
     hipLaunchKernelGGL((k_nodal<R, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
   else
     hipLaunchKernelGGL((k_nodal<R, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
@@ -1200,12 +1256,25 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       false)
     return bail(rc);
   const size_t esz = precision == FT_FP32 ? 4 : 8;
-  if (hipMalloc(&g->fe, 24 * (size_t)n_elems * esz) != hipSuccess)
+  {
+    std::vector<char> seen(8 * (size_t)n_nodes, 0);
+    g->corner = true;
+    for (long i = 0; i < 8L * n_elems && g->corner; ++i) {
+      char &c = seen[(size_t)(i % 8) * n_nodes + conn[i]];
+      if (c) g->corner = false;
+      c = 1;
+    }
+  }
+  const size_t fe_len = 24 * (size_t)(g->corner ? n_nodes : n_elems);
+  if (hipMalloc(&g->fe, fe_len * esz) != hipSuccess)
     return bail(fail(FT_ERR_ALLOC, "element force buffer allocation failed"));
+  // unwritten corner slots must read as zero for the whole run
+  if (hipMemset(g->fe, 0, fe_len * esz) != hipSuccess) return bail(fail(FT_ERR_HIP, "memset failed"));
+  g->h_conn.assign(conn, conn + 8 * (size_t)n_elems);
   if (g->has_visco && hipMalloc(&g->hist, 144 * (size_t)n_elems * esz) != hipSuccess)
     return bail(fail(FT_ERR_ALLOC, "history allocation failed"));
   g->nodal_grid = grid_for(n_nodes);
-  if ((rc = dalloc(&g->eacc, 4 * (size_t)g->nodal_grid))) return bail(rc);
+  if ((rc = dalloc(&g->eacc, kEaccStride * (size_t)g->nodal_grid))) return bail(rc);
   // Group elements by material (stable counting sort on the material id).
   g->perm.resize(n_elems);
   g->inv.resize(n_elems);
@@ -1411,7 +1480,7 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
   if ((rc = element_pass(g, true, false, nullptr, dt, nullptr, update_history))) return rc;
   if ((rc = check_status(g))) return rc;
   if (fe_out) {
-    const size_t n = 24 * (size_t)g->ne;
+    const size_t n = 24 * (size_t)(g->corner ? g->nn : g->ne);
     std::vector<double> dev(n);
     if (g->precision == FT_FP32) {
       std::vector<float> tmp(n);
@@ -1421,9 +1490,11 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
       HIPCHK(hipMemcpy(dev.data(), g->fe, n * sizeof(double), hipMemcpyDeviceToHost));
     }
     for (int e = 0; e < g->ne; ++e)
-      for (int k = 0; k < 8; ++k)
-        for (int d = 0; d < 3; ++d)
-          fe_out[24 * (size_t)e + 3 * k + d] = dev[3 * ((size_t)k * g->ne + g->inv[e]) + d];
+      for (int k = 0; k < 8; ++k) {
+        const size_t slot = g->corner ? (size_t)k * g->nn + g->h_conn[8 * (size_t)e + k]
+                                      : (size_t)k * g->ne + g->inv[e];
+        for (int d = 0; d < 3; ++d) fe_out[24 * (size_t)e + 3 * k + d] = dev[3 * slot + d];
+      }
   }
   return 0;
 }
@@ -1436,7 +1507,14 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
   {
     Timed t(g, 1);
     const bool w8 = g->width == 8;
-    if (g->precision == FT_FP32) {
+    if (g->corner) {
+      if (g->precision == FT_FP32)
+        hipLaunchKernelGGL((k_gather<float, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const float *)g->fe, g->fi);
+      else
+        hipLaunchKernelGGL((k_gather<double, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const double *)g->fe, g->fi);
+    } else if (g->precision == FT_FP32) {
       if (w8)
         hipLaunchKernelGGL((k_gather<float, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
                            g->ell, g->width, (const float *)g->fe, g->fi);
@@ -1486,7 +1564,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
       }
     if ((rc = h2d_nodal(g, g->u, u.data())) || (rc = h2d_nodal(g, g->vs, v.data()))) return rc;
   }
-  HIPCHK(hipMemset(g->eacc, 0, 4 * (size_t)g->nodal_grid * sizeof(double)));
+  HIPCHK(hipMemset(g->eacc, 0, kEaccStride * (size_t)g->nodal_grid * sizeof(double)));
   HIPCHK(hipMemset(g->status, 0, sizeof(int)));
   Ctl c;
   memset(&c, 0, sizeof(c));
@@ -1587,14 +1665,14 @@ int ft_get_state(ft_engine *g, double *u, double *v, double *a, double *fi) {
 
 int ft_get_energy(ft_engine *g, double out4[4]) {
   if (!g || !out4) return fail(FT_ERR_ARG, "null argument");
-  std::vector<double> e(4 * (size_t)g->nodal_grid);
+  std::vector<double> e(kEaccStride * (size_t)g->nodal_grid);
   HIPCHK(hipStreamSynchronize(g->stream));
   HIPCHK(hipMemcpy(e.data(), g->eacc, e.size() * sizeof(double), hipMemcpyDeviceToHost));
   double w = 0, x = 0, k = 0;
   for (int b = 0; b < g->nodal_grid; ++b) {
-    w += e[4 * b];
-    x += e[4 * b + 1];
-    k += e[4 * b + 2];
+    w += e[kEaccStride * b];
+    x += e[kEaccStride * b + 1];
+    k += e[kEaccStride * b + 2];
   }
   out4[0] = w;
   out4[1] = x;
@@ -1801,10 +1879,12 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
   // X (3 doubles) + u (3 doubles) per node; Prony history read + write.
   double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0);
   if (g->has_visco) elem += ne * 2.0 * 144.0 * es;
-  // nodal kernel: force read 24 es per element; per node the ELL incidences
-  // 4 width, mass 8, motion code 4, X 24 (boundary nodes only: not counted),
-  // u, v_half and fi read + write 3 x 48.
-  double nodal = ne * (24.0 * es) + nn * (4.0 * g->width + 8.0 + 4.0 + 3.0 * 48.0);
+  // nodal kernel: element forces, 8 corner slots of 3 per node (corner
+  // layout) or 24 es per element + the ELL incidences 4 width per node; per
+  // node mass 8, motion code 4, u and v_half read + write 2 x 48 (X only at
+  // constrained nodes and fi only when a run stops: not counted).
+  double nodal = (g->corner ? nn * (24.0 * es) : ne * (24.0 * es) + nn * 4.0 * g->width) +
+                 nn * (8.0 + 4.0 + 2.0 * 48.0);
   bytes[0] = elem;
   bytes[1] = nodal;
   return 0;

@@ -132,3 +132,27 @@ def random_hexes(rng, n):
             x = x @ np.array([[0.8, 0.6, 0.0], [-0.6, 0.8, 0.0], [0.0, 0.0, 1.0]])
         out.append(x.reshape(24))
     return np.array(out)
+
+
+def case_rotation(n=5, h=1e-3, steps=600, peak=6000.0, T=0.01):
+    """Jittered n^3 block whose outer-surface nodes follow a rigid rotation
+    about the x axis through the block centre, with angular acceleration
+    peak*sin(pi t/T) for t <= T and constant angular velocity afterwards
+    (the half-sine pulse of the rotational loading); soft neo-Hookean
+    interior (G = 10 kPa, K = 1 MPa, rho = 1040)."""
+    X, conn = ft.mesh_block(n, n, n, h, jitter=0.15, seed=31)
+    nn = X.size // 3
+    Xr = X.reshape(-1, 3)
+    L = n * h
+    surf = np.any((np.abs(Xr) < TOL) | (np.abs(Xr - L) < TOL), axis=1)
+    dm = -np.ones(3 * nn, np.int32)
+    for d in range(3):
+        dm[3 * np.nonzero(surf)[0] + d] = 0
+    c = 0.5 * L
+    motions = np.stack([ft.motion(ft.MOTION_ROT_HALFSINE, 1.0, 0.0, 0.0, c, c, c, peak, T)])
+    G, K = 1.0e4, 1.0e6
+    props = props_row(1040.0, G, K - 2.0 * G / 3.0)[None, :]
+    return dict(X=X, conn=conn, pid=np.zeros(conn.size // 8, np.int32),
+                material_id=np.array([1], np.int32), props=props, dof_motion=dm, motions=motions,
+                u0=None, v0=None, max_steps=steps, tmax=1e9, dt_factor=0.9, failure_dt=1e-11,
+                snaps=[steps])

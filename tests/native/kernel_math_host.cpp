@@ -46,7 +46,22 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
   R out[24];
   double my = 1e300;
   int flags = 0;
-  ft::element_eval<R, true, true, -1>(xv, uv, pp, false, dt, h, update != 0, out, my, flags);
+  // the per-material instance the element kernel launches for this material
+  switch (mat) {
+#define FT_KM_CASE(m)                                                                     \
+  case m:                                                                                 \
+    ft::element_eval<R, true, true, m>(xv, uv, pp, false, dt, h, update != 0, out, my, flags); \
+    break;
+    FT_KM_CASE(0)
+    FT_KM_CASE(1)
+    FT_KM_CASE(2)
+    FT_KM_CASE(3)
+    FT_KM_CASE(4)
+    FT_KM_CASE(5)
+#undef FT_KM_CASE
+    default:
+      ft::element_eval<R, true, true, -1>(xv, uv, pp, false, dt, h, update != 0, out, my, flags);
+  }
   for (int k = 0; k < 24; ++k) f24[k] = out[k];
   if (hist144)
     for (int k = 0; k < 144; ++k) hist144[k] = hl[k];
