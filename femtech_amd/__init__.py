@@ -71,6 +71,10 @@ _SIGS = {
     "ft_get_timing": [_VP, _D, _I],
     "ft_traffic_model": [_VP, _D],
     "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
+    "ft_injury_enable": [_VP, _I],
+    "ft_injury_disable": [_VP],
+    "ft_injury_update": [_VP, _D, ctypes.c_double, ctypes.c_double],
+    "ft_injury_results": [_VP, _D, _I],
     "ft_element_strain": [_VP, _D, _D, _D],
     "ft_mesh_block": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -299,6 +303,25 @@ class Engine:
         s = ctypes.c_int(0)
         _check(self._lib.ft_get_status(self._h, ctypes.byref(s)))
         return s.value
+
+    def injury_enable(self, include_part=None):
+        """Track the ex5 injury criteria over the following steps."""
+        inc = None if include_part is None else np.ascontiguousarray(include_part, np.int32)
+        _check(self._lib.ft_injury_enable(self._h, _ip(inc)))
+
+    def injury_disable(self):
+        _check(self._lib.ft_injury_disable(self._h))
+
+    def injury_update(self, u, time, dt):
+        u = None if u is None else _f64(u)
+        _check(self._lib.ft_injury_update(self._h, _dp(u), float(time), float(dt)))
+
+    def injury_results(self):
+        """(out[21], flags[ne]) as documented for ft_injury_results."""
+        out = np.empty(21, np.float64)
+        flags = np.empty(self.ne, np.int32)
+        _check(self._lib.ft_injury_results(self._h, _dp(out), _ip(flags)))
+        return out, flags
 
     def element_strain(self, u=None):
         """Element-averaged Green-Lagrange strain (9 per element, column-major)

@@ -656,12 +656,13 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
 // Eavg = (1/8) sum_gp (F^T F - I) / 2, column-major) and the principal-strain
 // summary of CalculateMaximumPrincipalStrain (CalculateStrain.cpp:8-75: max
 // principal clamped at >= 0, min principal clamped at <= 0, max shear), FP64.
-__global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *conn, const double *X,
-                                                   const double *u, double *eavg, double *princ) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  int nd[8];
-  for (int k = 0; k < 8; ++k) nd[k] = conn[8 * (size_t)e + k];
+// One element's averaged Green-Lagrange strain E (row-major 3x3) and
+// principal summary P = (max principal >= 0, min principal <= 0, max shear)
+// in FP64, from H = grad_X u at the 8 Gauss points: E = (1/8) sum (H + H^T +
+// H^T H) / 2 equals the reference's (1/8) sum (F^T F - I) / 2 without the
+// cancellation.
+__device__ inline void element_strain64(int nn, const int *nd, const double *X, const double *u, double E[3][3],
+                                        double P[3]) {
   double cx[7][3], cu[7][3];
   for (int d = 0; d < 3; ++d) {
     double xv[8], uv[8];
@@ -678,7 +679,8 @@ __global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *co
     }
   }
   const double g = kGp, g2 = kGp * kGp;
-  double E[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) E[i][j] = 0.0;
   for (int gp = 0; gp < 8; ++gp) {
     const double sx = ((gp & 3) == 1 || (gp & 3) == 2) ? 1.0 : -1.0;
     const double sy = (gp & 2) ? 1.0 : -1.0;
@@ -710,8 +712,6 @@ __global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *co
       for (int j = 0; j < 3; ++j)
         E[i][j] += 0.0625 * (H[i][j] + H[j][i] + H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j]);
   }
-  for (int j = 0; j < 3; ++j)
-    for (int i = 0; i < 3; ++i) eavg[9 * (size_t)e + i + 3 * j] = E[i][j];
   // principal values: trigonometric solution of the characteristic cubic
   const double a = E[0][0], b = E[1][0], c = E[2][0], d = E[1][1], ee = E[2][1], f = E[2][2];
   const double p1 = b * b + c * c + ee * ee;
@@ -740,9 +740,274 @@ __global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *co
     e3 = sq * cos((theta + 4.0 * pi) / 3.0) + I1;
   }
   const double mx = fmax(e3, fmax(e2, e1)), mn = fmin(e3, fmin(e2, e1));
-  princ[3 * (size_t)e + 0] = mx > 0.0 ? mx : 0.0;
-  princ[3 * (size_t)e + 1] = mn < 0.0 ? mn : 0.0;
-  princ[3 * (size_t)e + 2] = 0.5 * (mx - mn);
+  P[0] = mx > 0.0 ? mx : 0.0;
+  P[1] = mn < 0.0 ? mn : 0.0;
+  P[2] = 0.5 * (mx - mn);
+}
+
+// Element-averaged Green-Lagrange strain (CalculateStrain.cpp:77-97,
+// column-major) and the principal-strain summary of
+// CalculateMaximumPrincipalStrain (CalculateStrain.cpp:8-75), FP64.
+__global__ __launch_bounds__(kBlock) void k_strain(int nn, int ne, const int *conn, const double *X,
+                                                   const double *u, double *eavg, double *princ) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int nd[8];
+  for (int k = 0; k < 8; ++k) nd[k] = conn[8 * (size_t)e + k];
+  double E[3][3], P[3];
+  element_strain64(nn, nd, X, u, E, P);
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) eavg[9 * (size_t)e + i + 3 * j] = E[i][j];
+  for (int k = 0; k < 3; ++k) princ[3 * (size_t)e + k] = P[k];
+}
+
+// ---------------------------------------------------------------------------
+// Injury criteria of FemTech's ex5 (CalculateInjuryCriterions, ex5.cpp:
+// 1311-1430), evaluated after every step at the step's displacement.
+// ---------------------------------------------------------------------------
+// Per element flags (internal element order).
+constexpr int kInjIncluded = 1, kInjGt15 = 2, kInjGt30 = 4, kInjPsr120 = 8, kInjPsxsr28 = 16;
+constexpr int kInjMps95 = 32, kInjMpsxsr95 = 64;
+// Running scalars: (value, original element id, time) for max principal,
+// min principal, max shear and max PS x SR; (value, time) for MPS-95 and
+// MPSxSR-95; then two "list needs refresh" flags.
+constexpr int kInjScal = 20;
+struct InjuryArgs {
+  int nn, ne;
+  const int *conn;
+  const double *X, *u;
+  const int *perm;   // internal -> original element id
+  int *flags;
+  double *ps_old, *psxsr;
+  double *part;      // 8 doubles per block
+  double *scal;      // kInjScal
+  const StepSlot *slot;  // time / dt source (nullable: use time, dt)
+  double time, dt;
+};
+
+struct ArgBest {
+  double v;
+  int id;
+};
+__device__ inline ArgBest better_max(ArgBest a, ArgBest b) {
+  return (b.v > a.v || (b.v == a.v && b.id < a.id)) ? b : a;
+}
+
+__device__ inline ArgBest block_argbest(ArgBest x, bool is_max) {
+  __shared__ double sv[kBlock];
+  __shared__ int si[kBlock];
+  if (!is_max) x.v = -x.v;
+  sv[threadIdx.x] = x.v;
+  si[threadIdx.x] = x.id;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ArgBest o = better_max({sv[threadIdx.x], si[threadIdx.x]}, {sv[threadIdx.x + w], si[threadIdx.x + w]});
+      sv[threadIdx.x] = o.v;
+      si[threadIdx.x] = o.id;
+    }
+    __syncthreads();
+  }
+  ArgBest r{sv[0], si[0]};
+  __syncthreads();
+  if (!is_max) r.v = -r.v;
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_injury_elem(InjuryArgs a) {
+  if (a.slot && a.slot->done) return;  // uniform
+  const double dt = a.slot ? a.slot->dt : a.dt;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const double ninf = -INFINITY;
+  ArgBest bmax{ninf, INT32_MAX}, bmin{INFINITY, INT32_MAX}, bsh{ninf, INT32_MAX}, bx{ninf, INT32_MAX};
+  if (e < a.ne && (a.flags[e] & kInjIncluded)) {
+    int nd[8];
+    for (int k = 0; k < 8; ++k) nd[k] = a.conn[8 * (size_t)e + k];
+    double E[3][3], P[3];
+    element_strain64(a.nn, nd, a.X, a.u, E, P);
+    const int id = a.perm[e];
+    int fl = a.flags[e];
+    if (P[0] > 0.15) fl |= kInjGt15;
+    if (P[0] > 0.30) fl |= kInjGt30;
+    const double psr = (P[0] - a.ps_old[e]) / dt;
+    const double psxsr = P[0] * psr;
+    if (psr > 120.0) fl |= kInjPsr120;
+    if (psxsr > 28.0) fl |= kInjPsxsr28;
+    a.flags[e] = fl;
+    a.ps_old[e] = P[0];
+    a.psxsr[e] = psxsr;
+    bmax = {P[0], id};
+    bmin = {P[1], id};
+    bsh = {P[2], id};
+    bx = {psxsr, id};
+  }
+  bmax = block_argbest(bmax, true);
+  bmin = block_argbest(bmin, false);
+  bsh = block_argbest(bsh, true);
+  bx = block_argbest(bx, true);
+  if (threadIdx.x == 0) {
+    double *p = a.part + 8 * (size_t)blockIdx.x;
+    p[0] = bmax.v, p[1] = bmax.id, p[2] = bmin.v, p[3] = bmin.id;
+    p[4] = bsh.v, p[5] = bsh.id, p[6] = bx.v, p[7] = bx.id;
+  }
+}
+
+// The step's extrema over all blocks (block order, smallest id on ties), then
+// the running maxima with strict comparisons (ex5.cpp:1318-1366).
+__global__ __launch_bounds__(kBlock) void k_injury_reduce(InjuryArgs a, int nblk) {
+  if (a.slot && a.slot->done) return;
+  const double ninf = -INFINITY;
+  ArgBest r[4] = {{ninf, INT32_MAX}, {ninf, INT32_MAX}, {ninf, INT32_MAX}, {ninf, INT32_MAX}};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const double *p = a.part + 8 * (size_t)b;
+    for (int k = 0; k < 4; ++k) {
+      const double v = k == 1 ? -p[2 * k] : p[2 * k];  // the minimum as a maximum of -v
+      r[k] = better_max(r[k], {v, (int)p[2 * k + 1]});
+    }
+  }
+  for (int k = 0; k < 4; ++k) r[k] = block_argbest(r[k], true);
+  if (threadIdx.x == 0) {
+    const double t = a.slot ? a.slot->t : a.time;
+    double *s = a.scal;
+    if (r[0].id != INT32_MAX && s[0] < r[0].v) s[0] = r[0].v, s[1] = r[0].id, s[2] = t;
+    if (r[1].id != INT32_MAX && s[3] > -r[1].v) s[3] = -r[1].v, s[4] = r[1].id, s[5] = t;
+    if (r[2].id != INT32_MAX && s[6] < r[2].v) s[6] = r[2].v, s[7] = r[2].id, s[8] = t;
+    if (r[3].id != INT32_MAX && s[9] < r[3].v) s[9] = r[3].v, s[10] = r[3].id, s[11] = t;
+  }
+}
+
+// 95th percentile (math.cpp:160-196 on one rank: the element at index
+// (int)(0.95 n) - 1 of the ascending order) of ps_old and of psxsr by a
+// radix select over order-preserving 64-bit keys, 8 bits per pass, both
+// arrays in the same launches.  sel[m]: {prefix, remaining rank}.
+__device__ inline unsigned long long order_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double key_value(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+__global__ void k_select_init(unsigned long long *sel, unsigned *hist, long n_incl, const StepSlot *slot) {
+  if (slot && slot->done) return;
+  long k = (long)(n_incl * 0.95) - 1;
+  if (k < 0) k = 0;
+  if (threadIdx.x < 2) {
+    sel[2 * threadIdx.x] = 0;
+    sel[2 * threadIdx.x + 1] = (unsigned long long)k;
+  }
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) hist[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_select_hist(int ne, const int *flags, const double *v0,
+                                                        const double *v1, const unsigned long long *sel,
+                                                        int pass, unsigned *hist, const StepSlot *slot) {
+  if (slot && slot->done) return;
+  __shared__ unsigned h[2][256];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i >> 8][i & 255] = 0;
+  __syncthreads();
+  const int shift = 56 - 8 * pass;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+    if (!(flags[e] & kInjIncluded)) continue;
+    const unsigned long long k0 = order_key(v0[e]), k1 = order_key(v1[e]);
+    if (pass == 0 || (k0 >> (shift + 8)) == sel[0]) atomicAdd(&h[0][(k0 >> shift) & 255], 1u);
+    if (pass == 0 || (k1 >> (shift + 8)) == sel[2]) atomicAdd(&h[1][(k1 >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x)
+    if (h[i >> 8][i & 255]) atomicAdd(&hist[i], h[i >> 8][i & 255]);
+}
+
+// Picks the digit holding the remaining rank, extends the prefix, clears the
+// histogram for the next pass.
+__global__ void k_select_pick(unsigned long long *sel, unsigned *hist, const StepSlot *slot) {
+  if (slot && slot->done) return;
+  if (threadIdx.x < 2) {
+    const int m = threadIdx.x;
+    unsigned long long k = sel[2 * m + 1], cum = 0;
+    int b = 0;
+    for (; b < 255; ++b) {
+      if (k < cum + hist[256 * m + b]) break;
+      cum += hist[256 * m + b];
+    }
+    sel[2 * m] = (sel[2 * m] << 8) | (unsigned long long)b;
+    sel[2 * m + 1] = k - cum;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) hist[i] = 0;
+}
+
+// MPS-95 / MPSxSR-95 running maxima (ex5.cpp:1370-1429).
+__global__ void k_injury_p95(const unsigned long long *sel, double *scal, const StepSlot *slot, double time) {
+  if (slot && slot->done) return;
+  if (threadIdx.x < 2) {
+    const int m = threadIdx.x;
+    const double v = key_value(sel[2 * m]);
+    const double t = slot ? slot->t : time;
+    double *s = scal + 12 + 2 * m;
+    scal[16 + m] = 0.0;
+    if (v > s[0]) {
+      s[0] = v;
+      s[1] = t;
+      scal[16 + m] = 1.0;
+    }
+  }
+}
+
+// Element lists of a new MPS-95 / MPSxSR-95 maximum: value >= the maximum.
+__global__ __launch_bounds__(kBlock) void k_injury_members(int ne, int *flags, const double *ps_old,
+                                                           const double *psxsr, const double *scal,
+                                                           const StepSlot *slot) {
+  if (slot && slot->done) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  int fl = flags[e];
+  if (scal[16] != 0.0) fl = (fl & ~kInjMps95) | ((fl & kInjIncluded) && ps_old[e] >= scal[12] ? kInjMps95 : 0);
+  if (scal[17] != 0.0)
+    fl = (fl & ~kInjMpsxsr95) | ((fl & kInjIncluded) && psxsr[e] >= scal[14] ? kInjMpsxsr95 : 0);
+  flags[e] = fl;
+}
+
+// Reference-configuration element volumes (calculateVolume,
+// CalculateCentroidAndVolume.cpp:26-37) for the CSDM fractions.
+__global__ __launch_bounds__(kBlock) void k_element_volume(int nn, int ne, const int *conn, const double *X,
+                                                           double *vol) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double c[7][3];
+  for (int d = 0; d < 3; ++d) {
+    double v[8];
+    const int n0 = conn[8 * (size_t)e];
+    for (int p = 0; p < 8; ++p) v[p] = X[(size_t)d * nn + conn[8 * (size_t)e + p2n(p)]] - X[(size_t)d * nn + n0];
+    wht8(v);
+    for (int m = 0; m < 7; ++m) c[m][d] = v[m + 1] * 0.125;
+  }
+  vol[e] = hex_part(c, 6);
+}
+
+// Volume sums for CSDM-15, CSDM-30, MPSR-120, MPSxSR-28 (ex5.cpp:1049-1066):
+// per-block partials [included, gt15, gt15 and gt30, psr120, psxsr28].
+__global__ __launch_bounds__(kBlock) void k_injury_volumes(int ne, const int *flags, const double *vol,
+                                                           double *part) {
+  __shared__ double sh[kBlock / 64];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[5] = {0, 0, 0, 0, 0};
+  if (e < ne && (flags[e] & kInjIncluded)) {
+    const int fl = flags[e];
+    const double v = vol[e];
+    s[0] = v;
+    if (fl & kInjGt15) {
+      s[1] = v;
+      if (fl & kInjGt30) s[2] = v;
+    }
+    if (fl & kInjPsr120) s[3] = v;
+    if (fl & kInjPsxsr28) s[4] = v;
+  }
+  for (int k = 0; k < 5; ++k) {
+    const double r = block_sum(s[k], sh);
+    if (threadIdx.x == 0) part[5 * (size_t)blockIdx.x + k] = r;
+  }
 }
 
 // CalculateAccelerations (CalculateAcclerations.cpp:4-12) on node-major host
@@ -780,6 +1045,14 @@ __global__ __launch_bounds__(kBlock) void k_energy_terms(int ndof, EnergyIn in, 
     __syncthreads();
   }
   if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+// Sums of nv-tuples over nblk blocks, in block order (one thread per value).
+__global__ void k_sum_tuples(int nblk, int nv, const double *part, double *out) {
+  if ((int)threadIdx.x < nv) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(size_t)nv * b + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
 }
 __global__ void k_sum3(int nblk, const double *part, double *out) {
   if (threadIdx.x < 3) {
@@ -935,6 +1208,14 @@ struct ft_engine {
   // history
   int hist_cap = 0, hist_n = 0;
   double *hist_rows = nullptr;
+  // injury criteria (optional, per step inside the resident loop)
+  bool inj_on = false;
+  long inj_n = 0;  // included elements
+  int *inj_flags = nullptr, *inj_perm = nullptr;
+  double *inj_ps_old = nullptr, *inj_psxsr = nullptr, *inj_vol = nullptr, *inj_part = nullptr;
+  double *inj_scal = nullptr, *inj_vpart = nullptr, *inj_vsum = nullptr;
+  unsigned long long *inj_sel = nullptr;
+  unsigned *inj_hist = nullptr;
   // staging for the entry points that take host arrays every call
   double *aux = nullptr;
   size_t aux_len = 0;
@@ -1145,6 +1426,47 @@ int launch_nodal(ft_engine *g, int mode, int max_steps) {
   return 0;
 }
 
+InjuryArgs injury_args(ft_engine *g, const StepSlot *slot, double time, double dt) {
+  InjuryArgs a;
+  a.nn = g->nn;
+  a.ne = g->ne;
+  a.conn = g->conn;
+  a.X = g->X;
+  a.u = g->u;
+  a.perm = g->inj_perm;
+  a.flags = g->inj_flags;
+  a.ps_old = g->inj_ps_old;
+  a.psxsr = g->inj_psxsr;
+  a.part = g->inj_part;
+  a.scal = g->inj_scal;
+  a.slot = slot;
+  a.time = time;
+  a.dt = dt;
+  return a;
+}
+
+// One update of the injury criteria (CalculateInjuryCriterions, ex5.cpp:
+// 1311-1430) on the stream: time and dt from `slot` when given.
+int injury_launch(ft_engine *g, const StepSlot *slot, double time, double dt) {
+  Timed t(g, 2);
+  const int nblk = (g->ne + kBlock - 1) / kBlock;
+  const InjuryArgs a = injury_args(g, slot, time, dt);
+  hipLaunchKernelGGL(k_injury_elem, dim3(nblk), dim3(kBlock), 0, g->stream, a);
+  hipLaunchKernelGGL(k_injury_reduce, dim3(1), dim3(kBlock), 0, g->stream, a, nblk);
+  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(kBlock), 0, g->stream, g->inj_sel, g->inj_hist, g->inj_n, slot);
+  const int sgrid = std::min(nblk, 1024);
+  for (int pass = 0; pass < 8; ++pass) {
+    hipLaunchKernelGGL(k_select_hist, dim3(sgrid), dim3(kBlock), 0, g->stream, g->ne, g->inj_flags, g->inj_ps_old,
+                       g->inj_psxsr, g->inj_sel, pass, g->inj_hist, slot);
+    hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(kBlock), 0, g->stream, g->inj_sel, g->inj_hist, slot);
+  }
+  hipLaunchKernelGGL(k_injury_p95, dim3(1), dim3(64), 0, g->stream, g->inj_sel, g->inj_scal, slot, time);
+  hipLaunchKernelGGL(k_injury_members, dim3(nblk), dim3(kBlock), 0, g->stream, g->ne, g->inj_flags, g->inj_ps_old,
+                     g->inj_psxsr, g->inj_scal, slot);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int upload_u(ft_engine *g, const double *u) {
   if (u) return h2d_nodal(g, g->u, u);
   return 0;
@@ -1342,7 +1664,9 @@ int ft_engine_destroy(ft_engine *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
-                  g->ell, g->fe, g->hist, g->eacc, g->hist_rows, g->aux};
+                  g->ell, g->fe, g->hist, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
+                  g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
+                  g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int c = 0; c < 3; ++c)
@@ -1614,6 +1938,9 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
     if ((rc = element_pass(g, true, true, &g->ctl->slot[g->parity], 0.0,
                            &g->ctl->slot[g->parity].dtmin_bits, 1)))
       return rc;
+    // u still holds this step's displacement here (the nodal kernel below
+    // advances it): the injury update of ex5.cpp:240 for t_k
+    if (g->inj_on && (rc = injury_launch(g, &g->ctl->slot[g->parity], 0.0, 0.0))) return rc;
     if ((rc = launch_nodal(g, kStep, target))) return rc;
     if (g->hist_cap > 0) {
       Timed t(g, 2);
@@ -1840,6 +2167,88 @@ int ft_energy_terms(ft_engine *g, const double *u, const double *u_prev, const d
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out3, res, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int ft_injury_enable(ft_engine *g, const int *include_part) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  const size_t ne = g->ne;
+  const int nblk = (g->ne + kBlock - 1) / kBlock;
+  int rc;
+  if (!g->inj_flags) {
+    if ((rc = dalloc(&g->inj_flags, ne)) || (rc = dalloc(&g->inj_perm, ne)) || (rc = dalloc(&g->inj_ps_old, ne)) ||
+        (rc = dalloc(&g->inj_psxsr, ne)) || (rc = dalloc(&g->inj_vol, ne)) ||
+        (rc = dalloc(&g->inj_part, 8 * (size_t)nblk)) || (rc = dalloc(&g->inj_scal, kInjScal)) ||
+        (rc = dalloc(&g->inj_vpart, 5 * (size_t)nblk)) || (rc = dalloc(&g->inj_vsum, 5)) ||
+        (rc = dalloc(&g->inj_sel, 4)) || (rc = dalloc(&g->inj_hist, 512)))
+      return rc;
+  }
+  // included elements: parts with include_part[p] != 0 (ex5.cpp:1262-1281)
+  std::vector<int> fl(ne);
+  long n = 0;
+  {
+    std::vector<int> hinfo(ne);
+    HIPCHK(hipMemcpy(hinfo.data(), g->einfo, ne * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ne; ++i) {
+      const int p = hinfo[i] & kPidMask;
+      fl[i] = (!include_part || include_part[p]) ? kInjIncluded : 0;
+      n += fl[i] ? 1 : 0;
+    }
+  }
+  g->inj_n = n;
+  HIPCHK(hipMemcpy(g->inj_flags, fl.data(), ne * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->inj_perm, g->perm.data(), ne * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(g->inj_ps_old, 0, ne * sizeof(double)));  // ex5 leaves PS_Old unset; 0 here
+  HIPCHK(hipMemset(g->inj_psxsr, 0, ne * sizeof(double)));
+  double sc[kInjScal];
+  for (int k = 0; k < kInjScal; ++k) sc[k] = 0.0;
+  sc[1] = sc[4] = sc[7] = sc[10] = -1.0;  // no element yet
+  HIPCHK(hipMemcpy(g->inj_scal, sc, sizeof(sc), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_element_volume, dim3(nblk), dim3(kBlock), 0, g->stream, g->nn, g->ne, g->conn,
+                     g->X, g->inj_vol);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g->stream));
+  g->inj_on = true;
+  return 0;
+}
+
+int ft_injury_disable(ft_engine *g) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  g->inj_on = false;
+  return 0;
+}
+
+int ft_injury_update(ft_engine *g, const double *u, double time, double dt) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (!g->inj_flags) return fail(FT_ERR_STATE, "ft_injury_enable must run first");
+  if (!(dt > 0)) return fail(FT_ERR_ARG, "dt must be positive");
+  int rc = upload_u(g, u);
+  if (rc) return rc;
+  if ((rc = injury_launch(g, nullptr, time, dt))) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int ft_injury_results(ft_engine *g, double out[21], int *flags) {
+  if (!g || !out) return fail(FT_ERR_ARG, "null argument");
+  if (!g->inj_flags) return fail(FT_ERR_STATE, "ft_injury_enable must run first");
+  const int nblk = (g->ne + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_injury_volumes, dim3(nblk), dim3(kBlock), 0, g->stream, g->ne, g->inj_flags, g->inj_vol,
+                     g->inj_vpart);
+  hipLaunchKernelGGL(k_sum_tuples, dim3(1), dim3(64), 0, g->stream, nblk, 5, g->inj_vpart, g->inj_vsum);
+  HIPCHK(hipGetLastError());
+  double sc[kInjScal], vs[5];
+  HIPCHK(hipMemcpyAsync(sc, g->inj_scal, sizeof(sc), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(vs, g->inj_vsum, sizeof(vs), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (int k = 0; k < 16; ++k) out[k] = sc[k];
+  for (int k = 0; k < 4; ++k) out[16 + k] = vs[1 + k] / vs[0];
+  out[20] = vs[0];
+  if (flags) {
+    std::vector<int> fl(g->ne);
+    HIPCHK(hipMemcpy(fl.data(), g->inj_flags, fl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < g->ne; ++i) flags[g->perm[i]] = fl[i];
+  }
   return 0;
 }
 

@@ -160,6 +160,27 @@ int ft_energy_terms(ft_engine *eng, const double *u, const double *u_prev,
                     const double *fi, const double *fi_prev, const double *fe,
                     const double *fe_prev, const int *boundary, double out3[3]);
 
+/* ---- injury criteria ------------------------------------------------------
+ * CalculateInjuryCriterions of FemTech's ex5 (ex5.cpp:1311-1430) with its
+ * outputs (ex5.cpp:1040-1240), on the device.  ft_injury_enable selects the
+ * included elements (parts with include_part[p] != 0; NULL = all), computes
+ * their reference-configuration volumes and resets the criteria; from then on
+ * every step of ft_run_steps updates them at the step's displacement, time and
+ * step size.  ft_injury_update applies one update at u (NULL = device
+ * displacement) with the given time and dt (host-array drivers).
+ * ft_injury_results: out[0..15] = max principal strain, its element, time;
+ * min principal strain, element, time; max shear, element, time; max PS x SR,
+ * element, time; MPS-95 maximum and its time; MPSxSR-95 maximum and its time;
+ * out[16..19] = CSDM-15, CSDM-30, MPSR-120, MPSxSR-28 (fractions of the
+ * included volume); out[20] = included volume.  flags (per element, may be
+ * NULL): bit 0 included, 1 MPS > 0.15 seen, 2 MPS > 0.30 seen, 3 PSR > 120
+ * seen, 4 PS x SR > 28 seen, 5 in the MPS-95 list, 6 in the MPSxSR-95 list.
+ * The previous principal strain starts at 0 (the reference leaves it unset). */
+int ft_injury_enable(ft_engine *eng, const int *include_part);
+int ft_injury_disable(ft_engine *eng);
+int ft_injury_update(ft_engine *eng, const double *u, double time, double dt);
+int ft_injury_results(ft_engine *eng, double out[21], int *flags);
+
 /* ---- instrumentation -----------------------------------------------------
  * When enabled, every kernel launch of the loop is bracketed by HIP events on
  * the engine's stream.  Times are summed in milliseconds per kernel class

@@ -294,6 +294,17 @@ struct Model {
   std::vector<double> u0, v0;
   // node -> (element*8 + slot), sorted by element: deterministic gather
   std::vector<int> csr_ptr, csr_idx;
+  // injury criteria (ex5.cpp:1253-1430): state of the included elements
+  bool injury = false;
+  std::vector<int> inj_elem;                    // elementIDInjury
+  std::vector<double> inj_ps_old, inj_psxsr;    // PS_Old, PSxSRArray
+  std::vector<int> inj_gt15, inj_gt30, inj_psr120, inj_psxsr28;
+  std::vector<double> inj_volume;               // reference-configuration volumes
+  std::vector<char> inj_in_mps95, inj_in_mpsxsr95;  // latest element lists
+  double maxStrain = 0, maxT = 0, minStrain = 0, minT = 0, maxShear = 0, maxShearT = 0;
+  double maxPSxSR = 0, maxTimePSxSR = 0, maxMPS95 = 0, maxTimeMPS95 = 0;
+  double maxMPSxSR95 = 0, maxTimeMPSxSR95 = 0;
+  int maxElem = -1, minElem = -1, shearElem = -1, maxElemPSxSR = -1;
 };
 
 void build_csr(Model &m) {
@@ -711,62 +722,200 @@ void oracle_set_initial(void *mp, const double *u0, const double *v0) {
   if (v0) m.v0.assign(v0, v0 + 3 * (size_t)m.nn);
 }
 
+namespace {
+/* Eavg of one element (CalculateStrain.cpp:77-97) and its principal summary
+ * (CalculateStrain.cpp:8-75): max principal >= 0, min principal <= 0, max shear. */
+void element_principal(const Model &m, int e, const double *u, double E[9], double P[3]) {
+  const double kPi = 4.0 * atan(1.0);
+  const double *dN = &m.dNdX[192 * (size_t)e];
+  double x[24];
+  element_coords(m, e, u, x);
+  for (int i = 0; i < 9; ++i) E[i] = 0.0;
+  const double pre = 0.5 / 8.0;
+  for (int gp = 0; gp < 8; ++gp) {
+    double F[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 8; ++k) s = s + x[3 * k + i] * dN[gp * 24 + 3 * k + j];
+        F[3 * j + i] = s;
+      }
+    // dgemm('T','N', alpha = 1/16, beta = 1): E += pre F^T F
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += F[k + 3 * i] * F[k + 3 * j];
+        E[i + 3 * j] += pre * s;
+      }
+  }
+  E[0] -= 0.5;
+  E[4] -= 0.5;
+  E[8] -= 0.5;
+  double a = E[0], b = E[1], c = E[2], d = E[4], ee = E[5], f = E[8];
+  double p1 = b * b + c * c + ee * ee, e1, e2, e3;
+  if (p1 == 0) {
+    e1 = a;
+    e2 = d;
+    e3 = f;
+  } else {
+    double I1 = a + d + f;
+    double I2 = a * (d + f) + d * f - b * b - c * c - ee * ee;
+    double I3 = a * d * f + 2.0 * b * c * ee - b * b * f - c * c * d - ee * ee * a;
+    double Q = (3.0 * I2 - I1 * I1) / 9.0;
+    double R = (2.0 * I1 * I1 * I1 - 9.0 * I1 * I2 + 27.0 * I3) / 54.0;
+    double theta = acos(R / sqrt(-Q * Q * Q));
+    double sq = 2.0 * sqrt(-Q);
+    I1 = I1 / 3.0;
+    e1 = sq * cos(theta / 3.0) + I1;
+    e2 = sq * cos((theta + 2.0 * kPi) / 3.0) + I1;
+    e3 = sq * cos((theta + 4.0 * kPi) / 3.0) + I1;
+  }
+  double mx = fmax(e3, fmax(e2, e1)), mn = fmin(e3, fmin(e2, e1));
+  P[0] = mx > 0.0 ? mx : 0.0;
+  P[1] = mn < 0.0 ? mn : 0.0;
+  P[2] = 0.5 * (mx - mn);
+}
+
+/* compute95thPercentileValue on one rank (math.cpp:227-332 falls through to
+ * the brute force of math.cpp:160-196): the element at index
+ * (int)(0.95 n) - 1 of the ascending order (index 0 when that is negative). */
+double percentile95(const std::vector<double> &v) {
+  std::vector<double> w(v);
+  long k = (long)(w.size() * 0.95) - 1;
+  if (k < 0) k = 0;
+  std::nth_element(w.begin(), w.begin() + k, w.end());
+  return w[k];
+}
+}  // namespace
+
+/* InitInjuryCriteria (ex5.cpp:1253-1308): the included elements are those of
+ * parts with include_part[p] != 0 (all parts when NULL); the reference leaves
+ * PS_Old uninitialised, here it starts at 0. */
+void oracle_injury_init(void *mp, const int *include_part) {
+  Model &m = *(Model *)mp;
+  m.injury = true;
+  m.inj_elem.clear();
+  for (int e = 0; e < m.ne; ++e)
+    if (!include_part || include_part[m.pid[e]]) m.inj_elem.push_back(e);
+  const size_t n = m.inj_elem.size();
+  m.inj_ps_old.assign(n, 0.0);
+  m.inj_psxsr.assign(n, 0.0);
+  m.inj_gt15.assign(n, 0);
+  m.inj_gt30.assign(n, 0);
+  m.inj_psr120.assign(n, 0);
+  m.inj_psxsr28.assign(n, 0);
+  m.inj_in_mps95.assign(m.ne, 0);
+  m.inj_in_mpsxsr95.assign(m.ne, 0);
+  m.inj_volume.assign(m.ne, 0.0);
+  for (int e = 0; e < m.ne; ++e) {
+    double xe[24];
+    element_coords(m, e, nullptr, xe);
+    m.inj_volume[e] = oracle_hex_volume(xe);  // calculateVolume, CalculateCentroidAndVolume.cpp:26-37
+  }
+  m.maxStrain = m.maxT = m.minStrain = m.minT = m.maxShear = m.maxShearT = 0.0;
+  m.maxPSxSR = m.maxTimePSxSR = m.maxMPS95 = m.maxTimeMPS95 = 0.0;
+  m.maxMPSxSR95 = m.maxTimeMPSxSR95 = 0.0;
+  m.maxElem = m.minElem = m.shearElem = m.maxElemPSxSR = -1;
+}
+
+/* CalculateInjuryCriterions (ex5.cpp:1311-1430) at displacement u, time and
+ * step size dt. */
+void oracle_injury_update(void *mp, const double *u, double time, double dt) {
+  Model &m = *(Model *)mp;
+  const size_t n = m.inj_elem.size();
+  for (size_t j = 0; j < n; ++j) {
+    const int i = m.inj_elem[j];
+    double E[9], P[3];
+    element_principal(m, i, u, E, P);
+    if (m.maxStrain < P[0]) {
+      m.maxStrain = P[0];
+      m.maxElem = i;
+      m.maxT = time;
+    }
+    if (m.minStrain > P[1]) {
+      m.minStrain = P[1];
+      m.minElem = i;
+      m.minT = time;
+    }
+    if (m.maxShear < P[2]) {
+      m.maxShear = P[2];
+      m.shearElem = i;
+      m.maxShearT = time;
+    }
+    if (!m.inj_gt15[j] && P[0] > 0.15) m.inj_gt15[j] = 1;
+    if (!m.inj_gt30[j] && P[0] > 0.30) m.inj_gt30[j] = 1;
+    const double PSR = (P[0] - m.inj_ps_old[j]) / dt;
+    const double PSxSR = P[0] * PSR;
+    if (m.maxPSxSR < PSxSR) {
+      m.maxPSxSR = PSxSR;
+      m.maxElemPSxSR = i;
+      m.maxTimePSxSR = time;
+    }
+    if (!m.inj_psr120[j] && PSR > 120.0) m.inj_psr120[j] = 1;
+    if (!m.inj_psxsr28[j] && PSxSR > 28.0) m.inj_psxsr28[j] = 1;
+    m.inj_ps_old[j] = P[0];
+    m.inj_psxsr[j] = PSxSR;
+  }
+  if (n == 0) return;
+  const double mps95 = percentile95(m.inj_ps_old);
+  if (mps95 > m.maxMPS95) {
+    m.maxMPS95 = mps95;
+    m.maxTimeMPS95 = time;
+    std::fill(m.inj_in_mps95.begin(), m.inj_in_mps95.end(), 0);
+    for (size_t j = 0; j < n; ++j)
+      if (m.inj_ps_old[j] >= m.maxMPS95) m.inj_in_mps95[m.inj_elem[j]] = 1;
+  }
+  const double x95 = percentile95(m.inj_psxsr);
+  if (x95 > m.maxMPSxSR95) {
+    m.maxMPSxSR95 = x95;
+    m.maxTimeMPSxSR95 = time;
+    std::fill(m.inj_in_mpsxsr95.begin(), m.inj_in_mpsxsr95.end(), 0);
+    for (size_t j = 0; j < n; ++j)
+      if (m.inj_psxsr[j] >= m.maxMPSxSR95) m.inj_in_mpsxsr95[m.inj_elem[j]] = 1;
+  }
+}
+
+/* Injury outputs (ex5.cpp:1040-1240): out[0..20] = maxStrain, its element,
+ * time; minStrain, element, time; maxShear, element, time; maxPSxSR,
+ * element, time; MPS-95 and its time; MPSxSR-95 and its time; CSDM-15,
+ * CSDM-30, MPSR-120, MPSxSR-28 (volume fractions of the included volume);
+ * included volume.  flags (per element, may be NULL): bit 0 included, 1 MPS
+ * > 0.15, 2 MPS > 0.30, 3 PSR > 120, 4 PSxSR > 28, 5 in the MPS-95 list,
+ * 6 in the MPSxSR-95 list. */
+void oracle_injury_results(void *mp, double *out, int *flags) {
+  Model &m = *(Model *)mp;
+  double vt = 0, v15 = 0, v30 = 0, v120 = 0, v28 = 0;
+  if (flags) memset(flags, 0, sizeof(int) * m.ne);
+  for (size_t j = 0; j < m.inj_elem.size(); ++j) {
+    const int e = m.inj_elem[j];
+    const double eV = m.inj_volume[e];
+    vt += eV;
+    if (m.inj_gt15[j]) {
+      v15 += eV;
+      if (m.inj_gt30[j]) v30 += eV;
+    }
+    if (m.inj_psr120[j]) v120 += eV;
+    if (m.inj_psxsr28[j]) v28 += eV;
+    if (flags)
+      flags[e] = 1 | (m.inj_gt15[j] << 1) | (m.inj_gt30[j] << 2) | (m.inj_psr120[j] << 3) |
+                 (m.inj_psxsr28[j] << 4) | (m.inj_in_mps95[e] << 5) | (m.inj_in_mpsxsr95[e] << 6);
+  }
+  const double r[21] = {m.maxStrain, (double)m.maxElem, m.maxT, m.minStrain, (double)m.minElem, m.minT,
+                        m.maxShear, (double)m.shearElem, m.maxShearT, m.maxPSxSR, (double)m.maxElemPSxSR,
+                        m.maxTimePSxSR, m.maxMPS95, m.maxTimeMPS95, m.maxMPSxSR95, m.maxTimeMPSxSR95,
+                        v15 / vt, v30 / vt, v120 / vt, v28 / vt, vt};
+  memcpy(out, r, sizeof(r));
+}
+
 /* CalculateStrain.cpp:77-97 and CalculateMaximumPrincipalStrain
  * (CalculateStrain.cpp:8-75), with F from CalculateDeformationGradient. */
 void oracle_element_strain(void *mp, const double *u, double *eavg, double *principal) {
   Model &m = *(Model *)mp;
-  const double kPi = 4.0 * atan(1.0);
   for (int e = 0; e < m.ne; ++e) {
-    const double *dN = &m.dNdX[192 * (size_t)e];
-    double x[24];
-    element_coords(m, e, u, x);
-    double E[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const double pre = 0.5 / 8.0;
-    for (int gp = 0; gp < 8; ++gp) {
-      double F[9];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          double s = 0.0;
-          for (int k = 0; k < 8; ++k) s = s + x[3 * k + i] * dN[gp * 24 + 3 * k + j];
-          F[3 * j + i] = s;
-        }
-      // dgemm('T','N', alpha = 1/16, beta = 1): E += pre F^T F
-      for (int j = 0; j < 3; ++j)
-        for (int i = 0; i < 3; ++i) {
-          double s = 0.0;
-          for (int k = 0; k < 3; ++k) s += F[k + 3 * i] * F[k + 3 * j];
-          E[i + 3 * j] += pre * s;
-        }
-    }
-    E[0] -= 0.5;
-    E[4] -= 0.5;
-    E[8] -= 0.5;
+    double E[9], P[3];
+    element_principal(m, e, u, E, P);
     if (eavg) memcpy(eavg + 9 * (size_t)e, E, sizeof(E));
-    double a = E[0], b = E[1], c = E[2], d = E[4], ee = E[5], f = E[8];
-    double p1 = b * b + c * c + ee * ee, e1, e2, e3;
-    if (p1 == 0) {
-      e1 = a;
-      e2 = d;
-      e3 = f;
-    } else {
-      double I1 = a + d + f;
-      double I2 = a * (d + f) + d * f - b * b - c * c - ee * ee;
-      double I3 = a * d * f + 2.0 * b * c * ee - b * b * f - c * c * d - ee * ee * a;
-      double Q = (3.0 * I2 - I1 * I1) / 9.0;
-      double R = (2.0 * I1 * I1 * I1 - 9.0 * I1 * I2 + 27.0 * I3) / 54.0;
-      double theta = acos(R / sqrt(-Q * Q * Q));
-      double sq = 2.0 * sqrt(-Q);
-      I1 = I1 / 3.0;
-      e1 = sq * cos(theta / 3.0) + I1;
-      e2 = sq * cos((theta + 2.0 * kPi) / 3.0) + I1;
-      e3 = sq * cos((theta + 4.0 * kPi) / 3.0) + I1;
-    }
-    double mx = fmax(e3, fmax(e2, e1)), mn = fmin(e3, fmin(e2, e1));
-    if (principal) {
-      principal[3 * (size_t)e] = mx > 0.0 ? mx : 0.0;
-      principal[3 * (size_t)e + 1] = mn < 0.0 ? mn : 0.0;
-      principal[3 * (size_t)e + 2] = 0.5 * (mx - mn);
-    }
+    if (principal) memcpy(principal + 3 * (size_t)e, P, sizeof(P));
   }
 }
 
@@ -863,6 +1012,7 @@ int oracle_run(void *mp, int max_steps, double tmax, double dt_factor, double fa
     Wint += sWint;
     Wext += sWext;
     double total = fabs(WKE + Wint - Wext);
+    if (m.injury) oracle_injury_update(mp, u.data(), Time, dt);  // ex5.cpp:240
     ++step;
     double dtn = oracle_stable_dt(mp, u.data(), boundary.data());
     if (dtn < failure_dt) return -19;

@@ -82,6 +82,13 @@ int oracle_run(void *m, int max_steps, double tmax, double dt_factor,
 /* Element-averaged Green-Lagrange strain (9 column-major per element) and
  * principal summary (max >= 0, min <= 0, max shear; 3 per element) at u. */
 void oracle_element_strain(void *m, const double *u, double *eavg, double *principal);
+/* Injury criteria of ex5 (ex5.cpp:1253-1430, 1040-1240).  After
+ * oracle_injury_init, oracle_run updates them after every step.
+ * include_part: per part 0/1, NULL = all parts.  out: 21 doubles, flags: per
+ * element bits (see oracle.cpp). */
+void oracle_injury_init(void *m, const int *include_part);
+void oracle_injury_update(void *m, const double *u, double time, double dt);
+void oracle_injury_results(void *m, double *out, int *flags);
 /* Motion evaluation (used by tests of the kinematics). */
 void oracle_motion_eval(const double *motion, const double X[3], double t,
                         double u[3], double v[3], double a[3]);

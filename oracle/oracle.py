@@ -57,6 +57,9 @@ def lib():
         L.oracle_element_mass.argtypes = [_D, ctypes.c_double, _D]
         L.oracle_motion_eval.argtypes = [_D, _D, ctypes.c_double, _D, _D, _D]
         L.oracle_element_strain.argtypes = [ctypes.c_void_p, _D, _D, _D]
+        L.oracle_injury_init.argtypes = [ctypes.c_void_p, _I]
+        L.oracle_injury_update.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, ctypes.c_double]
+        L.oracle_injury_results.argtypes = [ctypes.c_void_p, _D, _I]
         _lib = L
     return _lib
 
@@ -181,6 +184,18 @@ class Model:
         P = np.zeros(3 * self.ne)
         lib().oracle_element_strain(self.h, _d(f64(u)), _d(E), _d(P))
         return E, P
+
+    def injury_init(self, include_part=None):
+        lib().oracle_injury_init(self.h, _i(None if include_part is None else i32(include_part)))
+
+    def injury_update(self, u, time, dt):
+        lib().oracle_injury_update(self.h, _d(f64(u)), float(time), float(dt))
+
+    def injury_results(self):
+        out = np.zeros(21)
+        flags = np.zeros(self.ne, np.int32)
+        lib().oracle_injury_results(self.h, _d(out), _i(flags))
+        return out, flags
 
     def history(self):
         h = np.zeros(216 * self.ne)

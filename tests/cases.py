@@ -156,3 +156,14 @@ def case_rotation(n=5, h=1e-3, steps=600, peak=6000.0, T=0.01):
                 material_id=np.array([1], np.int32), props=props, dof_motion=dm, motions=motions,
                 u0=None, v0=None, max_steps=steps, tmax=1e9, dt_factor=0.9, failure_dt=1e-11,
                 snaps=[steps])
+
+
+def case_injury(steps=150):
+    """Large-strain variant of the rotational loading (6^3 jittered block,
+    h = 5 mm, G = 500 Pa, K = 20 kPa, peak 20 krad/s^2 over 5 ms): the
+    strains cross the 15 % and 30 % thresholds and the strain rates the
+    120 1/s and 28 1/s thresholds of the injury criteria."""
+    c = case_rotation(n=6, h=5e-3, steps=steps, peak=20000.0, T=0.005)
+    G, K = 500.0, 2.0e4
+    c["props"] = props_row(1040.0, G, K - 2.0 * G / 3.0)[None, :]
+    return c

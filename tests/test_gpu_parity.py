@@ -270,3 +270,62 @@ def test_rotational_halfsine_loading_fp32_split(gpu_ready):
     ref = r["energy"][1:]
     scale = np.abs(ref[:, :3]).max()
     assert np.abs((en[:, 2] + en[:, 0] - en[:, 1]) - (ref[:, 2] + ref[:, 0] - ref[:, 1])).max() / scale < 1e-4
+
+
+def compare_injury(out, fl, ro, rfl):
+    """Running extrema and percentiles (values, element ids, times) and the
+    per-element flags of the injury criteria, FP64 engine vs oracle."""
+    for k in (0, 3, 6, 9, 12, 14):
+        assert out[k] == pytest.approx(ro[k], rel=1e-9, abs=1e-300), k
+    for k in (1, 4, 7, 10):
+        assert out[k] == ro[k], k
+    for k in (2, 5, 8, 11, 13, 15):
+        assert out[k] == pytest.approx(ro[k], rel=1e-13), k
+    np.testing.assert_array_equal(fl, rfl)
+    np.testing.assert_allclose(out[16:21], ro[16:21], rtol=1e-12)
+
+
+def test_injury_criteria_in_resident_loop_match_oracle(gpu_ready):
+    """CalculateInjuryCriterions (ex5.cpp:1311-1430) after every step of a
+    large-strain rotational run, on the device, against the oracle's update
+    after every step of the same run (FP64 engine)."""
+    from cases import case_injury
+    c = case_injury()
+    steps = c["max_steps"]
+    with ft.Engine(c["X"], c["conn"], c["pid"], c["material_id"], c["props"], ft.FT_FP64) as e:
+        e.set_boundary(c["dof_motion"], c["motions"])
+        e.lumped_mass()
+        e.run_init(c["tmax"], c["dt_factor"], c["failure_dt"])
+        e.injury_enable()
+        assert e.run_steps(steps) == steps
+        out, fl = e.injury_results()
+        u = e.state()[0]
+    o = O.Model(c["X"], c["conn"], c["pid"], c["material_id"], c["props"])
+    o.set_bc(c["dof_motion"], c["motions"])
+    o.injury_init()
+    r = o.run(steps, c["tmax"], c["dt_factor"], c["failure_dt"])
+    ro, rfl = o.injury_results()
+    assert rel(u, r["u"]) < 1e-10
+    # every threshold is exercised by this case
+    assert ro[16] > 0.5 and ro[17] > 0.3 and ro[18] > 0 and ro[19] > 0
+    compare_injury(out, fl, ro, rfl)
+
+
+def test_injury_update_on_host_displacements_matches_oracle(gpu_ready):
+    """ft_injury_update with caller-supplied displacement, time and step."""
+    from cases import case_injury
+    c = case_injury()
+    rng = np.random.default_rng(12)
+    Xr = c["X"].reshape(-1, 3)
+    o = O.Model(c["X"], c["conn"], c["pid"], c["material_id"], c["props"])
+    o.injury_init(np.array([1], np.int32))
+    with ft.Engine(c["X"], c["conn"], c["pid"], c["material_id"], c["props"], ft.FT_FP64) as e:
+        e.injury_enable(np.array([1], np.int32))
+        for k in range(4):
+            F = np.eye(3) + (0.1 * (k + 1)) * rng.uniform(-1, 1, (3, 3))
+            u = (Xr @ F.T - Xr).reshape(-1) + 1e-4 * rng.uniform(-1, 1, Xr.size)
+            e.injury_update(u, 0.01 * (k + 1), 1e-3)
+            o.injury_update(u, 0.01 * (k + 1), 1e-3)
+        out, fl = e.injury_results()
+    ro, rfl = o.injury_results()
+    compare_injury(out, fl, ro, rfl)

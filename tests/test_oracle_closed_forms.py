@@ -265,3 +265,65 @@ def test_motion_kinematics():
         _, vp, _ = O.motion_eval(mo, X, t + e)
         _, vm, _ = O.motion_eval(mo, X, t - e)
         np.testing.assert_allclose(a, (vp - vm) / (2 * e), rtol=1e-5, atol=1e-6)
+
+
+def test_injury_criteria_homogeneous_stretch():
+    """ex5.cpp:1311-1430 on a homogeneous stretch F = diag(1.4, 1.1, 0.95)
+    (distinct principal values: the reference's cubic has no degenerate
+    root): every element has max principal Green strain 0.48, so every
+    threshold trips and MPS-95 is 0.48; a second update at the same state has
+    zero strain rate and changes nothing."""
+    X, conn = ft.mesh_block(3, 2, 2, 1e-3, jitter=0.1, seed=3)
+    Xr = X.reshape(-1, 3)
+    ne = conn.size // 8
+    m = O.Model(X, conn, np.zeros(ne, np.int32), np.array([1], np.int32), ft_props(1)[None, :])
+    m.injury_init()
+    F = np.diag([1.4, 1.1, 0.95])
+    u = (Xr @ F.T - Xr).reshape(-1)
+    m.injury_update(u, 0.1, 1e-3)
+    out, fl = m.injury_results()
+    e = 0.5 * (1.4 ** 2 - 1.0)
+    _, P = m.element_strain(u)
+    P = P.reshape(-1, 3)
+    # equal up to round-off in every element: the maximum is the first element
+    # holding the largest computed value
+    assert out[0] == P[:, 0].max() and out[1] == np.argmax(P[:, 0]) and out[2] == 0.1
+    assert out[0] == pytest.approx(e, rel=1e-13)
+    assert out[12] == pytest.approx(e, rel=1e-13) and out[13] == 0.1
+    assert out[9] == pytest.approx(e * e / 1e-3, rel=1e-12)
+    np.testing.assert_allclose(out[16:20], 1.0)
+    assert out[20] == pytest.approx(3 * 2 * 2 * 1e-9, rel=1e-12)
+    assert np.all(fl & 31 == 31)
+    np.testing.assert_array_equal(np.nonzero(fl & 32)[0], np.nonzero(P[:, 0] >= out[12])[0])
+    m.injury_update(u, 0.2, 1e-3)
+    out2, fl2 = m.injury_results()
+    np.testing.assert_array_equal(out2[:16], out[:16])  # nothing exceeds the running maxima
+    np.testing.assert_array_equal(fl2, fl)
+
+
+def test_injury_percentile_and_included_parts():
+    """MPS-95 is the element at index (int)(0.95 n) - 1 of the ascending
+    order of the included elements' max principal strains (math.cpp:160-196);
+    only included parts count toward the volume fractions."""
+    rng = np.random.default_rng(8)
+    X, conn = ft.mesh_block(5, 4, 3, 1e-3, jitter=0.1, seed=4)
+    ne = conn.size // 8
+    pid = (np.arange(ne) % 2).astype(np.int32)
+    props = np.stack([ft_props(1), ft_props(1)])
+    m = O.Model(X, conn, pid, np.array([1, 1], np.int32), props)
+    u = 2e-4 * rng.uniform(-1, 1, X.size)
+    _, P = m.element_strain(u)
+    P = P.reshape(-1, 3)
+    m.injury_init(np.array([1, 0], np.int32))
+    m.injury_update(u, 0.5, 1e-4)
+    out, fl = m.injury_results()
+    inc = np.nonzero(pid == 0)[0]
+    ps = P[inc, 0]
+    assert out[12] == np.sort(ps)[int(0.95 * inc.size) - 1]
+    j = np.argmax(ps)
+    assert out[0] == ps[j] and out[1] == inc[j]
+    assert np.all(fl[pid == 1] == 0) and np.all(fl[pid == 0] & 1)
+    members = np.nonzero(fl & 32)[0]
+    np.testing.assert_array_equal(members, inc[ps >= out[12]])
+    vol = np.array([O.hex_volume(X.reshape(-1, 3)[conn.reshape(-1, 8)[e]]) for e in inc]).sum()
+    assert out[20] == pytest.approx(vol, rel=1e-13)
