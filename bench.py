@@ -14,6 +14,7 @@ on one GPU.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -42,6 +43,23 @@ def workload(n=100, slab=10):
     v0[2::3][Xr[:, 2] >= (n - slab) * 1e-3 - 0.2e-3] = -5.0
     return dict(X=X, conn=conn, pid=np.zeros(ne, np.int32), material_id=np.array([1], np.int32),
                 props=props, dof_motion=dm, motions=motions, v0=v0)
+
+
+PROFILE = os.path.join(ROOT, "profiles", "k_element_k_nodal_1M_hex_neo_hookean.md")
+
+
+def calibrated_traffic(ne, precision):
+    """HBM bytes per launch of k_element / k_nodal from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, calibrated against a
+    plain copy), when it was taken on this workload; else None."""
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"calibrated_traffic_bytes: k_element=(\d+) k_nodal=(\d+) elements=(\d+) precision=(\w+)", text)
+    if not m or int(m.group(3)) != ne or m.group(4) != precision:
+        return None
+    return float(m.group(1)), float(m.group(2))
 
 
 def cpu_baseline(w, steps, threads):
@@ -112,6 +130,7 @@ def main():
     elem_gbs = traffic[0] / (elem_ms * 1e-3) / 1e9
     nodal_gbs = traffic[1] / (nodal_ms * 1e-3) / 1e9
     step_bytes = traffic[0] + traffic[1]
+    measured = calibrated_traffic(ne, args.precision)
     # Layout-independent compulsory traffic of one step at these precisions
     # (each array touched once): per element connectivity 32 + part id 4 +
     # FP32 element-force write 96 and read 96 + node-to-element entries 32 =
@@ -137,10 +156,11 @@ def main():
                    "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k},
         "roofline": {"bound": "hbm", "kernel": "k_element",
                      "achieved": elem_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": elem_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": elem_gbs / HBM_PEAK_GBS, "traffic": measured[0] if measured else None,
                      "algorithmic_bytes_per_launch": traffic[0],
                      "avg_launch_ms": elem_ms},
         "nodal_kernel": {"achieved": nodal_gbs, "frac": nodal_gbs / HBM_PEAK_GBS,
+                         "traffic": measured[1] if measured else None,
                          "algorithmic_bytes_per_launch": traffic[1], "avg_launch_ms": nodal_ms},
         "step_roofline_frac": fixed_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
         "step_bytes": {"compulsory_model": fixed_bytes, "this_layout": step_bytes,
