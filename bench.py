@@ -27,22 +27,11 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
-def workload(n=100, slab=10):
-    import femtech_amd as ft
-    X, conn = ft.mesh_block(n, n, n, 1e-3, jitter=0.15, seed=SEED)
-    nn, ne = X.size // 3, conn.size // 8
-    G, K = 1.0e6, 10.0e6
-    props = np.array([1000.0, G, K - 2.0 * G / 3.0, 0, 0, 0, 0, 0, 0])
-    Xr = X.reshape(-1, 3)
-    dm = -np.ones(3 * nn, np.int32)
-    base = np.nonzero(Xr[:, 2] < 1e-9)[0]
-    for d in range(3):
-        dm[3 * base + d] = 0
-    motions = ft.motion(ft.MOTION_FIXED)[None, :]
-    v0 = np.zeros(3 * nn)
-    v0[2::3][Xr[:, 2] >= (n - slab) * 1e-3 - 0.2e-3] = -5.0
-    return dict(X=X, conn=conn, pid=np.zeros(ne, np.int32), material_id=np.array([1], np.int32),
-                props=props, dof_motion=dm, motions=motions, v0=v0)
+def workload(kind, n):
+    from femtech_amd import workloads
+    if kind == "impact":
+        return workloads.impact_block(n=n or 100)
+    return workloads.ellipsoid_head(n=n or 126)
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_element_k_nodal_1M_hex_neo_hookean.md")
@@ -68,6 +57,8 @@ def cpu_baseline(w, steps, threads):
     workload: `steps` explicit steps of the full 1M-hex mesh."""
     from oracle import oracle as O
     o = O.Model(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], threads=threads)
+    if w.get("fibres") is not None:
+        o.set_fibres(w["fibres"])
     o.set_bc(w["dof_motion"], w["motions"])
     o.set_initial(None, w["v0"])
     t0 = time.perf_counter()
@@ -79,8 +70,8 @@ def cpu_baseline(w, steps, threads):
     ne = w["conn"].size // 8
     o.close()
     return dict(value=ne * steps / loop, unit="element-steps/s", cores=threads, kind="port",
-                sample="%d explicit steps of the 1M-hex block (FP64 oracle, %d OpenMP threads), "
-                       "loop time %.2f s" % (steps, threads, loop))
+                sample="%d explicit steps of the same %d-element mesh (FP64 oracle, %d OpenMP threads), "
+                       "loop time %.2f s" % (steps, ne, threads, loop))
 
 
 def main():
@@ -88,7 +79,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4900)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--n", type=int, default=100, help="elements per block edge")
+    ap.add_argument("--workload", choices=["impact", "ellipsoid"], default="impact",
+                    help="impact: 1M-hex neo-Hookean block (the headline); ellipsoid: 2.0M-hex "
+                         "three-region ellipsoid (shell / Ogden + Prony / anisotropic HGO)")
+    ap.add_argument("--n", type=int, default=0, help="elements per edge (default 100 / 126)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -101,10 +95,12 @@ def main():
         sys.exit(2)
 
     import femtech_amd as ft
-    w = workload(args.n)
+    w = workload(args.workload, args.n)
     ne, nn = w["conn"].size // 8, w["X"].size // 3
     prec = ft.FT_FP32 if args.precision == "fp32" else ft.FT_FP64
     e = ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], prec)
+    if w.get("fibres") is not None:
+        e.set_fibres(w["fibres"])
     e.set_boundary(w["dof_motion"], w["motions"])
     e.set_state(None, w["v0"])
     e.lumped_mass()
@@ -125,12 +121,13 @@ def main():
     elapsed = t1 - t0
     value = ne * args.steps / elapsed
     traffic = e.traffic_model()
-    elem_ms = ms[0] / max(launches[0], 1)
+    # one element pass per step (one launch per material segment)
+    elem_ms = ms[0] / max(launches[0], 1) * (launches[0] / max(launches[1], 1))
     nodal_ms = ms[1] / max(launches[1], 1)
     elem_gbs = traffic[0] / (elem_ms * 1e-3) / 1e9
     nodal_gbs = traffic[1] / (nodal_ms * 1e-3) / 1e9
     step_bytes = traffic[0] + traffic[1]
-    measured = calibrated_traffic(ne, args.precision)
+    measured = calibrated_traffic(ne, args.precision) if args.workload == "impact" else None
     # Layout-independent compulsory traffic of one step at these precisions
     # (each array touched once): per element connectivity 32 + part id 4 +
     # FP32 element-force write 96 and read 96 + node-to-element entries 32 =
@@ -151,10 +148,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 element / f64 nodal" if prec == ft.FT_FP32 else "f64",
         "data": "synthetic",
-        "config": {"workload": "1M-hex jittered neo-Hookean impact block (plain method)",
-                   "elements": ne, "nodes": nn, "edge_elements": args.n,
-                   "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k},
-        "roofline": {"bound": "hbm", "kernel": "k_element",
+        "config": ({"workload": "1M-hex jittered neo-Hookean impact block (plain method)",
+                    "elements": ne, "nodes": nn, "edge_elements": args.n or 100,
+                    "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k}
+                   if args.workload == "impact" else
+                   {"workload": "2.0M-hex ellipsoid 90x75x65 mm: linear-elastic shell, Ogden + Prony, "
+                                "anisotropic HGO core; half-sine rotational surface loading (plain method)",
+                    "elements": ne, "nodes": nn, "edge_elements": args.n or 126,
+                    "simulated_time_s": t_sim, "step_index": k}),
+        "roofline": {"bound": "hbm", "kernel": "k_element (all material segments of a step)",
                      "achieved": elem_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": elem_gbs / HBM_PEAK_GBS, "traffic": measured[0] if measured else None,
                      "algorithmic_bytes_per_launch": traffic[0],

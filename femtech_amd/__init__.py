@@ -71,6 +71,8 @@ _SIGS = {
     "ft_get_timing": [_VP, _D, _I],
     "ft_traffic_model": [_VP, _D],
     "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
+    "ft_mesh_ellipsoid": [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _I, _D],
+    "ft_set_fibres": [_VP, _D],
     "ft_injury_enable": [_VP, _I],
     "ft_injury_disable": [_VP],
     "ft_injury_update": [_VP, _D, ctypes.c_double, ctypes.c_double],
@@ -153,6 +155,18 @@ def mesh_block(nx, ny, nz, h=(1e-3, 1e-3, 1e-3), jitter=0.0, seed=0):
     conn = np.empty(8 * ne, np.int32)
     _check(library().ft_mesh_block(nx, ny, nz, h[0], h[1], h[2], jitter, seed, _dp(X), _ip(conn)))
     return X, conn
+
+
+def mesh_ellipsoid(n, a, b, c):
+    """Ellipsoid of semi-axes a, b, c from an n^3 cube mapped onto the ball;
+    returns (X, conn, radius) with radius the normalised radius of each
+    element's centroid (0 centre, 1 surface)."""
+    nn, ne = (n + 1) ** 3, n ** 3
+    X = np.empty(3 * nn, np.float64)
+    conn = np.empty(8 * ne, np.int32)
+    r = np.empty(ne, np.float64)
+    _check(library().ft_mesh_ellipsoid(n, a, b, c, _dp(X), _ip(conn), _dp(r)))
+    return X, conn, r
 
 
 def write_inp(path, X, conn, pid=None):
@@ -303,6 +317,11 @@ class Engine:
         s = ctypes.c_int(0)
         _check(self._lib.ft_get_status(self._h, ctypes.byref(s)))
         return s.value
+
+    def set_fibres(self, directions):
+        """Fibre direction per element (3 each) for the anisotropic HGO law."""
+        d = _f64(directions)
+        _check(self._lib.ft_set_fibres(self._h, _dp(d)))
 
     def injury_enable(self, include_part=None):
         """Track the ex5 injury criteria over the following steps."""

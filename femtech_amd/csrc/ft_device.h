@@ -26,6 +26,8 @@ struct PartDev {
   double rho, mu, lambda, k1, k2, g1, t1, g2, t2;
   double ce;   // dilatational wave speed, CalculateTimeStep.cpp:13-19
   double inv_ce;
+  double alpha;  // Ogden exponent (materials 6, 7)
+  double kappa;  // fibre dispersion (material 8)
 };
 
 template <typename R> struct Num;
@@ -179,6 +181,96 @@ __host__ __device__ inline Sym<R> sym_mul(const Sym<R> &A, const Sym<R> &T, R s)
   return S;
 }
 
+// Eigenvalues w and eigenvectors (columns of V) of a symmetric 3x3 matrix by
+// cyclic Jacobi rotations, a fixed number of sweeps (converged to round-off
+// for strain-sized matrices; repeated eigenvalues need no special case).
+template <typename R, int SWEEPS>
+__host__ __device__ inline void sym_eig3(const Sym<R> &m, R w[3], R V[3][3]) {
+  R a[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      a[i][j] = sym_at(m, i, j);
+      V[i][j] = i == j ? R(1) : R(0);
+    }
+#pragma unroll 1
+  for (int sweep = 0; sweep < SWEEPS; ++sweep) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2, o = 3 - p - q;
+      const R apq = a[p][q];
+      if (apq == R(0)) continue;
+      const R theta = (a[q][q] - a[p][p]) / (R(2) * apq);
+      // t = tan of the rotation angle, the smaller root (|t| <= 1)
+      R t = R(1) / (fabs(theta) + Num<R>::sqrt_(theta * theta + R(1)));
+      if (theta < R(0)) t = -t;
+      if (!(fabs(theta) < R(1e18))) t = R(0.5) / theta;  // theta^2 would overflow
+      const R c = R(1) / Num<R>::sqrt_(t * t + R(1)), sn = t * c;
+      a[p][p] -= t * apq;
+      a[q][q] += t * apq;
+      a[p][q] = a[q][p] = R(0);
+      const R aop = a[o][p], aoq = a[o][q];
+      a[o][p] = a[p][o] = c * aop - sn * aoq;
+      a[o][q] = a[q][o] = sn * aop + c * aoq;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const R vp = V[k][p], vq = V[k][q];
+        V[k][p] = c * vp - sn * vq;
+        V[k][q] = sn * vp + c * vq;
+      }
+    }
+  }
+  w[0] = a[0][0];
+  w[1] = a[1][1];
+  w[2] = a[2][2];
+}
+
+// b - I = H + H^T + H H^T from H = grad_X u (no cancellation at small strain).
+template <typename R>
+__host__ __device__ inline Sym<R> b_minus_identity(const R H[3][3]) {
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+  Sym<R> bm;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = I[k], j = J[k];
+    bm.v[k] = H[i][j] + H[j][i] + (H[i][0] * H[j][0] + H[i][1] * H[j][1] + H[i][2] * H[j][2]);
+  }
+  return bm;
+}
+
+// 1-term Ogden (materials 6 and 7; no reference counterpart): deviatoric
+// energy (2 mu / alpha^2) sum_a (lbar_a^alpha - 1) on the isochoric principal
+// stretches, volumetric part as the reference's HGO (K/2 (J^2 - 1) in tau,
+// K = lambda + 2 mu / 3).  Kirchhoff stress tau = sum_a tau_a n_a n_a + p I
+// with tau_a = (2 mu / alpha) (lbar_a^alpha - mean_b lbar_b^alpha); the
+// stretches come from the eigenvalues of b - I (ln lbar_a = log1p(.) / 2 -
+// ln J / 3, lbar^alpha - 1 by expm1), so small strains keep full precision.
+// Returns p separately (the Prony update acts on the deviatoric part).
+template <typename R>
+__host__ __device__ inline void ogden_tau(const PartDev &pp, const Sym<R> &bm, R Jm1, Sym<R> &tau, R &p) {
+  const R mu = R(pp.mu), alpha = R(pp.alpha);
+  const R K = R(pp.lambda) + R(2) * mu / R(3);
+  const R lnJ3 = Num<R>::log1p_(Jm1) / R(3);
+  R w[3], V[3][3];
+  sym_eig3<R, sizeof(R) == 4 ? 4 : 6>(bm, w, V);
+  R e[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) e[a] = Num<R>::expm1_(alpha * (R(0.5) * Num<R>::log1p_(w[a]) - lnJ3));
+  const R mean = (e[0] + e[1] + e[2]) / R(3);
+  const R f = R(2) * mu / alpha;
+  R t[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) t[a] = f * (e[a] - mean);
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = I[k], j = J[k];
+    tau.v[k] = t[0] * V[i][0] * V[j][0] + t[1] * V[i][1] * V[j][1] + t[2] * V[i][2] * V[j][2];
+  }
+  p = R(0.5) * K * Jm1 * (Jm1 + R(2));
+}
+
 struct ViscoCoef { double c11, c12, c21, c22; };
 
 // PK2 stress of the reference's material routines, restated in terms of the
@@ -187,7 +279,7 @@ struct ViscoCoef { double c11, c12, c21, c22; };
 // compile time; MAT < 0 dispatches on pp.mat.  Returns false for an unknown
 // id.
 template <typename R, int MAT = -1>
-__host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R *hist,
+__host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R (&hist)[18], bool use_hist,
                                   const ViscoCoef &vc, Sym<R> &S, R &Jm1) {
   Jm1 = det_ip1_m1(H);
   const int mat = MAT >= 0 ? MAT : pp.mat;
@@ -230,6 +322,63 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
     for (int k = 0; k < 6; ++k) {
       int i = I[k], j = J[k];
       S.v[k] = (A[i][0] * P[0][j] + A[i][1] * P[1][j] + A[i][2] * P[2][j]) * iJ;
+    }
+    return true;
+  }
+  if (mat == 6 || mat == 7) {
+    // 1-term Ogden (ogden_tau), pulled back: S = F^-1 tau F^-T.  Material 7
+    // adds the two-term Prony update of HGOIsotropicViscoelastic.cpp:94-159 on
+    // the isochoric part Sdev = F^-1 dev(tau) F^-T.
+    Sym<R> td;
+    R p;
+    ogden_tau(pp, b_minus_identity(H), Jm1, td, p);
+    R Fi[3][3];  // adj(F) / J
+    {
+      R F[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? R(1) : R(0));
+      const R iJ = Num<R>::rcp_(R(1) + Jm1);
+      Fi[0][0] = (F[1][1] * F[2][2] - F[1][2] * F[2][1]) * iJ;
+      Fi[0][1] = (F[0][2] * F[2][1] - F[0][1] * F[2][2]) * iJ;
+      Fi[0][2] = (F[0][1] * F[1][2] - F[0][2] * F[1][1]) * iJ;
+      Fi[1][0] = (F[1][2] * F[2][0] - F[1][0] * F[2][2]) * iJ;
+      Fi[1][1] = (F[0][0] * F[2][2] - F[0][2] * F[2][0]) * iJ;
+      Fi[1][2] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * iJ;
+      Fi[2][0] = (F[1][0] * F[2][1] - F[1][1] * F[2][0]) * iJ;
+      Fi[2][1] = (F[0][1] * F[2][0] - F[0][0] * F[2][1]) * iJ;
+      Fi[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * iJ;
+    }
+    const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+    Sym<R> Sdev, Ci;  // F^-1 dev(tau) F^-T and C^-1 = F^-1 F^-T
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int i = I[k], j = J[k];
+      R sd = R(0), ci = R(0);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ci += Fi[i][a] * Fi[j][a];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) sd += Fi[i][a] * sym_at(td, a, b) * Fi[j][b];
+      }
+      Sdev.v[k] = sd;
+      Ci.v[k] = ci;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.v[k] = Sdev.v[k] + p * Ci.v[k];
+    if (mat == 7 && use_hist) {
+      const R c11 = R(vc.c11), c12 = R(vc.c12), c21 = R(vc.c21), c22 = R(vc.c22);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        R d = Sdev.v[k] - hist[12 + k];
+        R h1 = c11 * hist[k] + c21 * d;
+        R h2 = c12 * hist[6 + k] + c22 * d;
+        hist[k] = h1;
+        hist[6 + k] = h2;
+        hist[12 + k] = Sdev.v[k];
+        S.v[k] = S.v[k] + h1 + h2;
+      }
     }
     return true;
   }
@@ -285,7 +434,7 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
     S.v[3] = Sdev.v[3] + pvol * Ac.v[3];
     S.v[4] = Sdev.v[4] + pvol * Ac.v[4];
     S.v[5] = Sdev.v[5] + pvol * Ac.v[5];
-    if (mat == 5 && hist) {
+    if (mat == 5 && use_hist) {
       // HGOIsotropicViscoelastic.cpp:94-159: Sdev = S - (C:S)/3 C^-1 is the
       // isochoric part above; H_j <- c1j H_j + c2j (Sdev - S0); S += H1 + H2.
       const R c11 = R(vc.c11), c12 = R(vc.c12), c21 = R(vc.c21), c22 = R(vc.c22);
@@ -312,17 +461,58 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
 //     tau = J^-2/3 (mu + fib) dev(b) + K/2 (J^2 - 1) I,  K = lambda + 2 mu / 3
 // with b - I = H + H^T + H H^T formed directly (no cancellation at small
 // strain).  Equal to F S F^T of pk2_stress in exact arithmetic.
+//
+// Also the 1-term Ogden law (MAT 6, ogden_tau) and the anisotropic HGO law
+// (MAT 8; no reference counterpart): W = mu/2 (I1bar - 3) + K/4 (J^2 - 1 -
+// 2 ln J) + k1/(2 k2) (exp(k2 E^2) - 1) for E = kappa (I1bar - 3) +
+// (1 - 3 kappa)(I4bar - 1) > 0, I4bar = J^-2/3 |F a|^2 for the unit fibre a:
+//   tau = dev[(mu + 2 psi kappa) bbar + 2 psi (1 - 3 kappa) abar abar] + K/2 (J^2 - 1) I
+// with psi = k1 E exp(k2 E^2); kappa = 1/3 gives the HGO law above.
 template <typename R, int MAT>
-__host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3][3], R Jm1, Sym<R> &tau) {
+__host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3][3], R Jm1, const R fib[3],
+                                                 Sym<R> &tau) {
   const R mu = R(pp.mu), lambda = R(pp.lambda);
-  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
-  Sym<R> bm;  // b - I
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int i = I[k], j = J[k];
-    bm.v[k] = H[i][j] + H[j][i] + (H[i][0] * H[j][0] + H[i][1] * H[j][1] + H[i][2] * H[j][2]);
+  const Sym<R> bm = b_minus_identity(H);
+  if (MAT == 6) {
+    R p;
+    ogden_tau(pp, bm, Jm1, tau, p);
+    tau.v[0] += p;
+    tau.v[1] += p;
+    tau.v[2] += p;
+    return;
   }
   const R lnJ = Num<R>::log1p_(Jm1);
+  if (MAT == 8) {
+    const R kappa = R(pp.kappa);
+    const R K = lambda + R(2) * mu / R(3);
+    const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * lnJ);
+    const R Jm23 = R(1) + Jm23m1;
+    const R trb = bm.v[0] + bm.v[1] + bm.v[2];
+    R fa[3], ha[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ha[i] = H[i][0] * fib[0] + H[i][1] * fib[1] + H[i][2] * fib[2];
+      fa[i] = fib[i] + ha[i];
+    }
+    // |F a|^2 - 1 = 2 a.Ha + |Ha|^2
+    const R s4 = R(2) * (fib[0] * ha[0] + fib[1] * ha[1] + fib[2] * ha[2]) +
+                 (ha[0] * ha[0] + ha[1] * ha[1] + ha[2] * ha[2]);
+    const R I4m1 = Jm23m1 * (R(1) + s4) + s4;
+    const R I1m3 = R(3) * Jm23m1 + Jm23 * trb;
+    const R E = kappa * I1m3 + (R(1) - R(3) * kappa) * I4m1;
+    R psi = R(0);
+    if (E > R(0)) psi = R(pp.k1) * E * Num<R>::exp_(R(pp.k2) * E * E);
+    const R c1 = Jm23 * (mu + R(2) * psi * kappa);
+    const R c4 = Jm23 * R(2) * psi * (R(1) - R(3) * kappa);
+    const R p = R(0.5) * K * Jm1 * (Jm1 + R(2)) - c1 * trb / R(3) - c4 * (R(1) + s4) / R(3);
+    const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tau.v[k] = c1 * bm.v[k] + c4 * fa[I[k]] * fa[J[k]];
+    tau.v[0] += p;
+    tau.v[1] += p;
+    tau.v[2] += p;
+    return;
+  }
   if (MAT == 1) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) tau.v[k] = mu * bm.v[k];
@@ -532,7 +722,7 @@ template <typename R, bool FORCE, bool DT, int MAT, class Hist>
 __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartDev &pp,
                                              bool skip_dt, double dt_step, Hist &hist,
                                              bool update_hist, R outv[24], double &my_dt,
-                                             int &flags) {
+                                             int &flags, const R fib[3] = nullptr) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     wht8(xv[d]);
@@ -554,7 +744,7 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
     for (int k = 0; k < 24; ++k) outv[k] = R(0);
   } else if (FORCE) {
     ViscoCoef vc = {0, 0, 0, 0};
-    if (mat == 5) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
+    if (mat == 5 || mat == 7) {  // HGOIsotropicViscoelastic.cpp:126-131, global dt
       const double rt1 = dt_step / pp.t1, rt2 = dt_step / pp.t2;
       vc.c11 = exp(-rt1);
       vc.c12 = exp(-rt2);
@@ -607,14 +797,14 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
 #pragma unroll
         for (int j = 0; j < 3; ++j)
           H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
-      if constexpr (MAT == 1 || MAT == 4) {
+      if constexpr (MAT == 1 || MAT == 4 || MAT == 6 || MAT == 8) {
         // Spatial form: f_a = sum_gp det0 P J0^-T dN_a/dxi with P = tau F^-T
         // and F J0 = Jx (the deformed Jacobian), so Q = det0 tau Jx^-T =
         // tau adj(Jx)^T / J.
         const R Jm1 = det_ip1_m1(H);
         if (!(Jm1 > R(-1))) flags |= kFlagInverted;
         Sym<R> tau;
-        kirchhoff_stress<R, MAT>(pp, H, Jm1, tau);
+        kirchhoff_stress<R, MAT>(pp, H, Jm1, fib, tau);
         const R iJ = Num<R>::rcp_(R(1) + Jm1);
 #pragma unroll
         for (int k = 0; k < 6; ++k) tau.v[k] *= iJ;
@@ -642,18 +832,17 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
         }
         continue;
       }
+      // history of this Gauss point, in registers (fixed-size array, static
+      // indices: no private-memory copy)
       R hloc[18];
-      R *hp = nullptr;
-      if (mat == 5 && hist.valid()) {
+      const bool use_hist = (mat == 5 || mat == 7) && hist.valid();
 #pragma unroll
-        for (int k = 0; k < 18; ++k) hloc[k] = hist.load(gp, k);
-        hp = hloc;
-      }
+      for (int k = 0; k < 18; ++k) hloc[k] = use_hist ? hist.load(gp, k) : R(0);
       Sym<R> S;
       R Jm1;
-      if (!pk2_stress<R, MAT>(pp, H, hp, vc, S, Jm1)) flags |= kFlagMaterial;
+      if (!pk2_stress<R, MAT>(pp, H, hloc, use_hist, vc, S, Jm1)) flags |= kFlagMaterial;
       if (!(Jm1 > R(-1))) flags |= kFlagInverted;
-      if (hp && update_hist) {
+      if (use_hist && update_hist) {
 #pragma unroll
         for (int k = 0; k < 18; ++k) hist.store(gp, k, hloc[k]);
       }

@@ -234,6 +234,7 @@ struct ElemArgs {
   R *fe;                 // element forces (nullable when only dt is wanted)
   int corner;            // fe[corner][node][xyz] instead of fe[slot][element][xyz]
   R *hist;               // 144 per element, component-major [c * ne + e]
+  const R *fib;          // fibre directions [3][ne] (material 8)
   const StepSlot *slot;  // nullable: step slot giving dt_k and done
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
   double dt_fixed;       // used when slot == nullptr
@@ -244,13 +245,15 @@ struct ElemArgs {
   int update_hist;
 };
 
-// Occupancy target of the FP32 instances (waves per SIMD; 4 = at most 128
-// VGPRs).  The Prony instance (MAT 5) and the FP64 ones are not held to it.
+// Occupancy target of the FP32 instances of the laws written in closed form
+// (waves per SIMD; 4 = at most 128 VGPRs).  The Prony laws (5, 7), Ogden's
+// eigen-decomposition (6) and the fibre law (8) need more registers and the
+// FP64 instances are not held to it either.
 #ifndef FT_ELEM_WAVES_FP32
 #define FT_ELEM_WAVES_FP32 4
 #endif
 template <typename R, bool FORCE, bool DT, int MAT>
-__global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT != 5) ? FT_ELEM_WAVES_FP32 : 1) void k_element(
+__global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT <= 4 && MAT != -1) ? FT_ELEM_WAVES_FP32 : 1) void k_element(
     ElemArgs<R> a) {
   __shared__ double sh_min[kBlock / 64];
   const int L = xcd_block();
@@ -295,8 +298,14 @@ __global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT != 5) ? FT_ELEM_WAVE
     }
     HistSoA<R> hist{a.hist, a.ne, e};
     R outv[24];
+    R fib[3] = {R(1), R(0), R(0)};
+    if (MAT == 8 && a.fib) {
+      fib[0] = a.fib[e];
+      fib[1] = a.fib[(size_t)a.ne + e];
+      fib[2] = a.fib[2 * (size_t)a.ne + e];
+    }
     element_eval<R, FORCE, DT, MAT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
-                               a.update_hist != 0, outv, my_dt, flags);
+                               a.update_hist != 0, outv, my_dt, flags, fib);
     if (FORCE) {
       // Slot-major layouts: fe[slot][element][xyz] or fe[corner][node][xyz].
       // Each of the 8 stores of a wavefront covers one contiguous run (for
@@ -1187,6 +1196,7 @@ struct ft_engine {
   double *u = nullptr, *vs = nullptr, *fi = nullptr, *mass = nullptr, *tmp3 = nullptr;
   void *fe = nullptr;
   void *hist = nullptr;
+  void *fib = nullptr;  // fibre directions, element type, [3][ne] internal order
   int *node_code = nullptr;
   unsigned char *node_mask = nullptr;
   double *motions = nullptr;
@@ -1328,6 +1338,7 @@ int launch_element(ft_engine *g, int begin, int end, const StepSlot *slot, doubl
   a.fe = reinterpret_cast<R *>(g->fe);
   a.corner = g->corner ? 1 : 0;
   a.hist = reinterpret_cast<R *>(g->hist);
+  a.fib = reinterpret_cast<const R *>(g->fib);
   a.slot = slot;
   a.refine = (sizeof(R) == 4 && slot) ? 1 : 0;
   a.dt_fixed = dt_fixed;
@@ -1353,6 +1364,9 @@ int force_segment(ft_engine *g, const ft_engine::Seg &sg, const StepSlot *slot, 
     case 3: return launch_element<R, true, DT, 3>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
     case 4: return launch_element<R, true, DT, 4>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
     case 5: return launch_element<R, true, DT, 5>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 6: return launch_element<R, true, DT, 6>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 7: return launch_element<R, true, DT, 7>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 8: return launch_element<R, true, DT, 8>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
   }
   return fail(FT_ERR_MATERIAL, "Unknown material type %d", sg.mat);
 }
@@ -1550,8 +1564,8 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
     memset(&q, 0, sizeof(q));
     const double *pr = props + 9 * (size_t)p;
     q.mat = material_id[p];
-    if (q.mat < 0 || q.mat > 5) return bail(fail(FT_ERR_MATERIAL, "Unknown material type %d for part %d", q.mat, p));
-    q.visco = q.mat == 5;
+    if (q.mat < 0 || q.mat > 8) return bail(fail(FT_ERR_MATERIAL, "Unknown material type %d for part %d", q.mat, p));
+    q.visco = q.mat == 5 || q.mat == 7;
     q.rho = pr[0];
     q.mu = pr[1];
     q.lambda = pr[2];
@@ -1561,6 +1575,10 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
     q.t1 = pr[6];
     q.g2 = pr[7];
     q.t2 = pr[8];
+    q.alpha = pr[3];  // Ogden (6, 7)
+    q.kappa = pr[5];  // anisotropic HGO (8)
+    if ((q.mat == 6 || q.mat == 7) && q.alpha == 0.0)
+      return bail(fail(FT_ERR_INPUT, "Ogden exponent of part %d is zero", p));
     double nu = 0.5 * q.lambda / (q.lambda + q.mu);
     q.ce = sqrt(q.lambda * (1.0 / nu - 1.0) / q.rho);
     q.inv_ce = 1.0 / q.ce;
@@ -1601,12 +1619,13 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   g->perm.resize(n_elems);
   g->inv.resize(n_elems);
   {
-    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    constexpr int kMats = 9;
+    int cnt[kMats + 1] = {0};
     for (int e = 0; e < n_elems; ++e) cnt[g->h_parts[pid[e]].mat + 1]++;
-    for (int m = 0; m < 6; ++m) cnt[m + 1] += cnt[m];
-    for (int m = 0; m < 6; ++m)
+    for (int m = 0; m < kMats; ++m) cnt[m + 1] += cnt[m];
+    for (int m = 0; m < kMats; ++m)
       if (cnt[m + 1] > cnt[m]) g->segs.push_back({m, cnt[m], cnt[m + 1]});
-    std::vector<int> fill(cnt, cnt + 6);
+    std::vector<int> fill(cnt, cnt + kMats);
     for (int e = 0; e < n_elems; ++e) {
       int i = fill[g->h_parts[pid[e]].mat]++;
       g->perm[i] = e;
@@ -1664,7 +1683,7 @@ int ft_engine_destroy(ft_engine *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
-                  g->ell, g->fe, g->hist, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
+                  g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -2170,6 +2189,28 @@ int ft_energy_terms(ft_engine *g, const double *u, const double *u_prev, const d
   return 0;
 }
 
+int ft_set_fibres(ft_engine *g, const double *dir) {
+  if (!g || !dir) return fail(FT_ERR_ARG, "null argument");
+  const size_t ne = g->ne;
+  const size_t esz = g->precision == FT_FP32 ? 4 : 8;
+  if (!g->fib && hipMalloc(&g->fib, 3 * ne * esz) != hipSuccess)
+    return fail(FT_ERR_ALLOC, "fibre allocation failed");
+  std::vector<double> soa(3 * ne);
+  for (size_t e = 0; e < ne; ++e) {
+    const double *a = dir + 3 * e;
+    const double n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if (!(n > 0)) return fail(FT_ERR_ARG, "fibre direction of element %zu is zero", e);
+    for (int d = 0; d < 3; ++d) soa[d * ne + g->inv[e]] = a[d] / n;
+  }
+  if (g->precision == FT_FP32) {
+    std::vector<float> f(soa.begin(), soa.end());
+    HIPCHK(hipMemcpy(g->fib, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(g->fib, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
 int ft_injury_enable(ft_engine *g, const int *include_part) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   const size_t ne = g->ne;
@@ -2285,9 +2326,14 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
   const double es = g->precision == FT_FP32 ? 4.0 : 8.0;
   const double ne = g->ne, nn = g->nn;
   // element kernel: connectivity 32 + einfo 4 + force write 24 es per element;
-  // X (3 doubles) + u (3 doubles) per node; Prony history read + write.
+  // X (3 doubles) + u (3 doubles) per node; Prony history read + write and
+  // fibre directions for the elements that carry them.
   double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0);
-  if (g->has_visco) elem += ne * 2.0 * 144.0 * es;
+  for (const auto &sg : g->segs) {
+    const double n = sg.end - sg.begin;
+    if (sg.mat == 5 || sg.mat == 7) elem += n * 2.0 * 144.0 * es;  // Prony history read + write
+    if (sg.mat == 8) elem += n * 3.0 * es;                         // fibre direction
+  }
   // nodal kernel: element forces, 8 corner slots of 3 per node (corner
   // layout) or 24 es per element + the ELL incidences 4 width per node; per
   // node mass 8, motion code 4, u and v_half read + write 2 x 48 (X only at

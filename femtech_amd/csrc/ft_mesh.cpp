@@ -200,6 +200,59 @@ int ft_mesh_block(int nx, int ny, int nz, double hx, double hy, double hz, doubl
   return 0;
 }
 
+// Ellipsoid by a mapped cube: the n^3 block on [-1, 1]^3 (numbering as
+// ft_mesh_block) is mapped onto the unit ball, then scaled by the semi-axes.
+// A node at cube "radius" r = max|x_i| goes to the blend (1 - s) x + s r d
+// with s = r^0.3 and d the unit direction of tan(pi/4 x_i / r) (the
+// equiangular cubed-sphere coordinates of its concentric cube): the cube's
+// faces land exactly on the sphere, concentric cubes become near-spheres
+// toward the surface, and the centre stays a regular cube.  Unlike a direct
+// cube-to-ball formula, which collapses the elements at the 8 cube corners,
+// this keeps every element valid (smallest element volume about half the
+// mean at n = 126).  radius (per element, nullable): the ball radius of the
+// mapped element centre, the normalised radius used to assign regions.
+namespace {
+void ball_map(double x, double y, double z, double out[3]) {
+  const double r = std::max(fabs(x), std::max(fabs(y), fabs(z)));
+  if (r == 0.0) {
+    out[0] = out[1] = out[2] = 0.0;
+    return;
+  }
+  const double q = 0.78539816339744831 / r;  // pi/4 / r
+  const double d[3] = {tan(q * x), tan(q * y), tan(q * z)};
+  const double nd = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double s = pow(r, 0.3);
+  const double c[3] = {x, y, z};
+  for (int k = 0; k < 3; ++k) out[k] = (1.0 - s) * c[k] + s * r * d[k] / nd;
+}
+}  // namespace
+
+int ft_mesh_ellipsoid(int n, double a, double b, double c, double *coords, int *conn, double *radius) {
+  if (n <= 0 || !(a > 0) || !(b > 0) || !(c > 0) || !coords || !conn)
+    return mfail(FT_ERR_ARG, "bad ellipsoid arguments");
+  const double h = 2.0 / n;
+  int rc = ft_mesh_block(n, n, n, h, h, h, 0.0, 0, coords, conn);
+  if (rc) return rc;
+  const long s = n + 1, nn = s * s * s;
+  for (long i = 0; i < nn; ++i) {
+    double q[3];
+    ball_map(coords[3 * i] - 1.0, coords[3 * i + 1] - 1.0, coords[3 * i + 2] - 1.0, q);
+    coords[3 * i] = a * q[0];
+    coords[3 * i + 1] = b * q[1];
+    coords[3 * i + 2] = c * q[2];
+  }
+  if (radius) {
+    const long ne = (long)n * n * n;
+    for (long e = 0; e < ne; ++e) {
+      const long i = e % n, j = (e / n) % n, k = e / ((long)n * n);
+      double q[3];
+      ball_map(-1.0 + (i + 0.5) * h, -1.0 + (j + 0.5) * h, -1.0 + (k + 0.5) * h, q);
+      radius[e] = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    }
+  }
+  return 0;
+}
+
 int ft_write_inp(const char *path, int n_nodes, int n_elems, const double *coords, const int *conn,
                  const int *pid) {
   FILE *f = fopen(path, "w");
@@ -293,7 +346,10 @@ int ft_read_mesh(const char *path, int *n_nodes, int *n_elems, double *coords, i
 int ft_read_materials(const char *path, int n_parts, int *material_id, double *props) {
   FILE *f = fopen(path, "r");
   if (!f) return mfail(FT_ERR_INPUT, "Cannot open materials.dat file");
-  const int nprops[6] = {1, 3, 3, 3, 5, 9};  // ReadMaterials.cpp:43-122
+  // ReadMaterials.cpp:43-122 for 0-5; this engine's laws 6-8: Ogden (rho, mu,
+  // lambda, alpha), Ogden + Prony (rho, mu, lambda, alpha, unused, g1, tau1,
+  // g2, tau2), anisotropic HGO (rho, mu, lambda, k1, k2, kappa)
+  const int nprops[9] = {1, 3, 3, 3, 5, 9, 4, 9, 6};
   std::vector<char> seen(n_parts, 0);
   for (int p = 0; p < n_parts; ++p)
     for (int k = 0; k < 9; ++k) props[9 * p + k] = 0.0;
@@ -307,7 +363,7 @@ int ft_read_materials(const char *path, int n_parts, int *material_id, double *p
       fclose(f);
       return mfail(FT_ERR_INPUT, "materials.dat: part %d out of range", part);
     }
-    if (mat < 0 || mat > 5) {
+    if (mat < 0 || mat > 8) {
       fclose(f);
       return mfail(FT_ERR_INPUT, "Material ID for Part %d not found", part);
     }

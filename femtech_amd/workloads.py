@@ -1,0 +1,74 @@
+"""Synthetic workloads of the benchmark configurations, as engine inputs.
+
+Each builder returns a dict with X, conn, pid, material_id, props (9 per
+part), dof_motion, motions, v0 and, where the laws need them, fibres.
+"""
+import numpy as np
+
+from . import (MOTION_FIXED, MOTION_ROT_HALFSINE, mesh_block, mesh_ellipsoid, motion)
+
+IMPACT_SEED = 20261202
+
+
+def props_row(rho, mu, lam, *rest):
+    p = np.zeros(9)
+    p[:3] = [rho, mu, lam]
+    p[3:3 + len(rest)] = rest
+    return p
+
+
+def impact_block(n=100, slab=10, h=1e-3, jitter=0.15, seed=IMPACT_SEED):
+    """n^3 jittered hex block (h = 1 mm), compressible neo-Hookean G = 1 MPa,
+    K = 10 MPa, rho = 1000; base face fixed; the top `slab` element layers
+    start at v_z = -5 m/s."""
+    X, conn = mesh_block(n, n, n, h, jitter=jitter, seed=seed)
+    nn, ne = X.size // 3, conn.size // 8
+    G, K = 1.0e6, 10.0e6
+    Xr = X.reshape(-1, 3)
+    dm = -np.ones(3 * nn, np.int32)
+    base = np.nonzero(Xr[:, 2] < 1e-9 * h)[0]
+    for d in range(3):
+        dm[3 * base + d] = 0
+    v0 = np.zeros(3 * nn)
+    v0[2::3][Xr[:, 2] >= (n - slab) * h - 0.2 * h] = -5.0
+    return dict(X=X, conn=conn, pid=np.zeros(ne, np.int32), material_id=np.array([1], np.int32),
+                props=props_row(1000.0, G, K - 2.0 * G / 3.0)[None, :], dof_motion=dm,
+                motions=motion(MOTION_FIXED)[None, :], v0=v0, fibres=None)
+
+
+def ellipsoid_head(n=126, semi_axes=(0.090, 0.075, 0.065), peak=6000.0, duration=0.010, seed=7):
+    """Mapped-cube ellipsoid (n^3 elements; n = 126 gives 2.0M) in three
+    regions by normalised radius r of the element centroid:
+      r > 0.93          linear elastic shell, E = 6 GPa, nu = 0.22, rho = 2000
+      0.3 < r <= 0.93   1-term Ogden mu = 530 Pa, alpha = -4.4, K = 2.1 MPa,
+                        rho = 1040, with Prony g = 0.5 / 0.3, tau = 2 / 20 ms
+      r <= 0.3          anisotropic HGO k1 = 1 kPa, k2 = 10, kappa = 0.1
+                        (mu = 530 Pa, K = 2.1 MPa, rho = 1040), seeded unit
+                        fibre directions
+    Outer-surface nodes follow a rigid rotation about the x axis through the
+    centre with angular acceleration peak sin(pi t / duration) for t <=
+    duration."""
+    a, b, c = semi_axes
+    X, conn, r = mesh_ellipsoid(n, a, b, c)
+    nn, ne = X.size // 3, conn.size // 8
+    pid = np.where(r > 0.93, 0, np.where(r > 0.3, 1, 2)).astype(np.int32)
+    E, nu = 6.0e9, 0.22
+    mu_s, lam_s = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu, K = 530.0, 2.1e6
+    props = np.stack([props_row(2000.0, mu_s, lam_s),
+                      props_row(1040.0, mu, K - 2 * mu / 3, -4.4, 0.0, 0.5, 2e-3, 0.3, 2e-2),
+                      props_row(1040.0, mu, K - 2 * mu / 3, 1.0e3, 10.0, 0.1)])
+    # surface nodes: the cube's faces (node grid indices 0 or n)
+    s = n + 1
+    idx = np.arange(nn)
+    i, j, k = idx % s, (idx // s) % s, idx // (s * s)
+    surf = (i == 0) | (i == n) | (j == 0) | (j == n) | (k == 0) | (k == n)
+    dm = -np.ones(3 * nn, np.int32)
+    for d in range(3):
+        dm[3 * np.nonzero(surf)[0] + d] = 0
+    motions = motion(MOTION_ROT_HALFSINE, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, peak, duration)[None, :]
+    rng = np.random.default_rng(seed)
+    fib = rng.normal(size=(ne, 3))
+    fib /= np.linalg.norm(fib, axis=1)[:, None]
+    return dict(X=X, conn=conn, pid=pid, material_id=np.array([3, 7, 8], np.int32), props=props,
+                dof_motion=dm, motions=motions, v0=None, fibres=fib.reshape(-1), radius=r)

@@ -67,7 +67,12 @@ int ft_device_count(int *count);
  * AllocateArrays.cpp:29, ShapeFunctions.cpp:32).  Builds the node-to-element
  * CSR used for the force gather.  material_id per part follows
  * StressUpdate.cpp:7-27 (0 rigid, 1 neo-Hookean, 2 St Venant-Kirchhoff,
- * 3 linear elastic, 4 HGO isotropic, 5 HGO isotropic + Prony). */
+ * 3 linear elastic, 4 HGO isotropic, 5 HGO isotropic + Prony), extended with
+ * laws the reference does not have: 6 1-term Ogden (props rho, mu, lambda,
+ * alpha), 7 1-term Ogden + 2-term Prony (rho, mu, lambda, alpha, -, g1, tau1,
+ * g2, tau2), 8 anisotropic HGO with fibre dispersion (rho, mu, lambda, k1,
+ * k2, kappa; fibres from ft_set_fibres).  K = lambda + 2 mu / 3 is the bulk
+ * modulus of the volumetric term K/4 (J^2 - 1 - 2 ln J) of 4-8. */
 int ft_engine_create(int n_nodes, int n_elems, const double *coords,
                      const int *conn, const int *pid, int n_parts,
                      const int *material_id, const double *props,
@@ -83,6 +88,9 @@ int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
  * StableTimeStep.cpp:12-20. */
 int ft_set_boundary(ft_engine *eng, const int *dof_motion, int n_motions,
                     const double *motions);
+/* Fibre directions of the anisotropic HGO law (material 8), 3 per element in
+ * the reference configuration, normalised here.  Default: (1, 0, 0). */
+int ft_set_fibres(ft_engine *eng, const double *dir);
 /* Displacement and velocity (3 per node); either may be NULL (zero). */
 int ft_set_state(ft_engine *eng, const double *u, const double *v);
 /* Lumped mass: row-sum of the consistent element mass, gathered per node
@@ -204,6 +212,13 @@ int ft_copy_probe(double megabytes, int repeats, double *gbs);
 int ft_mesh_block(int nx, int ny, int nz, double hx, double hy, double hz,
                   double jitter, unsigned long long seed, double *coords,
                   int *conn);
+/* Ellipsoid with semi-axes a, b, c from an n^3 cube mapped onto the ball
+ * (nodes and elements numbered as ft_mesh_block with nx = ny = nz = n; the
+ * cube's faces land on the surface).  radius: per element, the normalised
+ * radius (0 centre, 1 surface) of the centroid, for region assignment; may
+ * be NULL.  coords: 3 (n+1)^3, conn: 8 n^3. */
+int ft_mesh_ellipsoid(int n, double a, double b, double c, double *coords,
+                      int *conn, double *radius);
 /* Abaqus .inp writer readable by ReadAbaqus.cpp:26-277 (one ELSET per part,
  * node ids 1..n in array order, %.17g coordinates). */
 int ft_write_inp(const char *path, int n_nodes, int n_elems, const double *coords,

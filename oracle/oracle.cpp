@@ -214,6 +214,151 @@ void hgo(const double *p, const double *F, double J, bool visco, double dt,
   voigt_from(S, S6);
 }
 
+/* Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (converged
+ * to round-off): eigenvalues w, eigenvectors in the columns of V
+ * (column-major V[3*col + row]). */
+void jacobi3(const double *A, double *w, double *V) {
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = 0.5 * (A[i + 3 * j] + A[j + 3 * i]);
+  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+    const double scale = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+    if (off == 0.0 || off < 1e-300 || off <= 1e-18 * scale) break;
+    for (int r = 0; r < 3; ++r) {
+      const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2;
+      if (a[p][q] == 0.0) continue;
+      const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+      double t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+      if (theta < 0) t = -t;
+      const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+      double b[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) b[i][j] = a[i][j];
+      for (int k = 0; k < 3; ++k) {  // a <- R^T a R
+        b[k][p] = c * a[k][p] - sn * a[k][q];
+        b[k][q] = sn * a[k][p] + c * a[k][q];
+      }
+      for (int k = 0; k < 3; ++k) {
+        a[p][k] = c * b[p][k] - sn * b[q][k];
+        a[q][k] = sn * b[p][k] + c * b[q][k];
+        if (k != p && k != q) a[k][p] = a[p][k], a[k][q] = a[q][k];
+      }
+      a[p][q] = a[q][p] = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double vp = v[k][p], vq = v[k][q];
+        v[k][p] = c * vp - sn * vq;
+        v[k][q] = sn * vp + c * vq;
+      }
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    w[i] = a[i][i];
+    for (int k = 0; k < 3; ++k) V[3 * i + k] = v[k][i];
+  }
+}
+
+/* Kaliske-Rothert two-term Prony update of HGOIsotropicViscoelastic.cpp:
+ * 94-159 on a total PK2 S (column-major), applied in place. */
+void prony_update(const double *p, const double *F, double dt, double *hist, double *S) {
+  const double g1 = p[5], t1 = p[6], g2 = p[7], t2 = p[8];
+  double C[9], fInv[9], J = 0.0;
+  gemm3(true, false, 1.0, F, F, C);
+  {
+    double det;
+    inverse3(F, fInv, &det);
+    J = det;
+  }
+  (void)J;
+  double SddC = 0.0;
+  for (int i = 0; i < 9; ++i) SddC += C[i] * S[i];
+  SddC = SddC / 3.0;
+  double Sic[9], Sdev[9];
+  gemm3(false, true, SddC, fInv, fInv, Sic);
+  for (int i = 0; i < 9; ++i) Sdev[i] = S[i] - Sic[i];
+  double *H1 = hist, *H2 = hist + 9, *S0 = hist + 18;
+  const double rt1 = dt / t1, rt2 = dt / t2;
+  const double c11 = exp(-rt1), c12 = exp(-rt2);
+  const double c21 = g1 * (1 - c11) / rt1, c22 = g2 * (1 - c12) / rt2;
+  for (int i = 0; i < 9; ++i) {
+    H1[i] = c11 * H1[i] + c21 * (Sdev[i] - S0[i]);
+    H2[i] = c12 * H2[i] + c22 * (Sdev[i] - S0[i]);
+    S[i] = S[i] + H1[i] + H2[i];
+  }
+  for (int i = 0; i < 9; ++i) S0[i] = Sdev[i];
+}
+
+/* Pull-back S = F^-1 tau F^-T of a Kirchhoff stress (column-major). */
+void pull_back(const double *F, const double *tau, double *S) {
+  double fInv[9], T[9], det;
+  inverse3(F, fInv, &det);
+  gemm3(false, false, 1.0, fInv, tau, T);
+  gemm3(false, true, 1.0, T, fInv, S);
+}
+
+/* 1-term Ogden (no reference counterpart): W = (2 mu / alpha^2) sum
+ * (lbar_a^alpha - 1) + K/4 (J^2 - 1 - 2 ln J), K = lambda + 2 mu / 3 as in
+ * HGOIsotropic.cpp:44; tau = sum_a (2 mu / alpha)(lbar_a^alpha - mean) n_a
+ * n_a + K/2 (J^2 - 1) I; Prony (material 7) as HGOIsotropicViscoelastic. */
+void ogden(const double *p, const double *F, double J, bool visco, double dt, double *hist, double *S6) {
+  const double mu = p[1], lambda = p[2], alpha = p[3];
+  const double K = lambda + 2.0 * mu / 3.0;
+  double B[9], w[3], V[9];
+  gemm3(false, true, 1.0, F, F, B);
+  jacobi3(B, w, V);
+  double lb[3], mean = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    lb[a] = pow(pow(J, -1.0 / 3.0) * sqrt(w[a]), alpha);
+    mean += lb[a] / 3.0;
+  }
+  double tau[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = 0; a < 3; ++a) {
+    const double ta = 2.0 * mu / alpha * (lb[a] - mean);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) tau[i + 3 * j] += ta * V[3 * a + i] * V[3 * a + j];
+  }
+  const double pv = 0.5 * K * (J * J - 1.0);
+  tau[0] += pv;
+  tau[4] += pv;
+  tau[8] += pv;
+  double S[9];
+  pull_back(F, tau, S);
+  if (visco) prony_update(p, F, dt, hist, S);
+  voigt_from(S, S6);
+}
+
+/* Anisotropic HGO with fibre dispersion (no reference counterpart):
+ * W = mu/2 (I1bar - 3) + K/4 (J^2 - 1 - 2 ln J) + k1/(2 k2)(exp(k2 E^2) - 1),
+ * E = kappa (I1bar - 3) + (1 - 3 kappa)(I4bar - 1) counted for E > 0 only,
+ * I4bar = J^-2/3 |F a|^2; kappa = 1/3 is HGOIsotropic.cpp:21-115. */
+void hgo_aniso(const double *p, const double *fib, const double *F, double J, double *S6) {
+  const double mu = p[1], lambda = p[2], k1 = p[3], k2 = p[4], kappa = p[5];
+  const double K = lambda + 2.0 * mu / 3.0;
+  double B[9];
+  gemm3(false, true, 1.0, F, F, B);
+  double fa[3];
+  for (int i = 0; i < 3; ++i) fa[i] = F[i] * fib[0] + F[i + 3] * fib[1] + F[i + 6] * fib[2];
+  const double Jm23 = pow(J, -2.0 / 3.0);
+  const double I1b = Jm23 * (B[0] + B[4] + B[8]);
+  const double I4b = Jm23 * (fa[0] * fa[0] + fa[1] * fa[1] + fa[2] * fa[2]);
+  const double E = kappa * (I1b - 3.0) + (1.0 - 3.0 * kappa) * (I4b - 1.0);
+  const double psi = E > 0.0 ? k1 * E * exp(k2 * E * E) : 0.0;
+  double tau[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      tau[i + 3 * j] = Jm23 * (mu + 2.0 * psi * kappa) * B[i + 3 * j] +
+                       Jm23 * 2.0 * psi * (1.0 - 3.0 * kappa) * fa[i] * fa[j];
+  const double tr = (tau[0] + tau[4] + tau[8]) / 3.0;
+  const double pv = 0.5 * K * (J * J - 1.0);
+  tau[0] += pv - tr;
+  tau[4] += pv - tr;
+  tau[8] += pv - tr;
+  double S[9];
+  pull_back(F, tau, S);
+  voigt_from(S, S6);
+}
+
 /* tripleProduct, math.cpp:52-56 */
 double triple(const double *s, const double *a, const double *b) {
   return s[2] * (a[0] * b[1] - a[1] * b[0]) + s[0] * (a[1] * b[2] - a[2] * b[1]) -
@@ -288,6 +433,7 @@ struct Model {
   std::vector<double> dNdX, detJ;  // ShapeFunctions output
   std::vector<double> hist;        // 8*27 per element (material 5 only)
   bool has_visco = false;
+  std::vector<double> fibres;  // 3 per element (material 8), unit length
   // boundary conditions
   std::vector<int> dof_motion;
   std::vector<double> motions;
@@ -349,7 +495,8 @@ int element_force(const Model &m, int e, const double *u, double dt,
       }
     double J = oracle_det3(F);
     double S6[6] = {0, 0, 0, 0, 0, 0};
-    int rc = oracle_stress(mat, p, F, J, dt, hist_e ? hist_e + 27 * gp : nullptr, S6);
+    int rc = oracle_stress_fibre(mat, p, m.fibres.empty() ? nullptr : &m.fibres[3 * (size_t)e], F, J, dt,
+                                 hist_e ? hist_e + 27 * gp : nullptr, S6);
     if (rc) return rc;
     // InternalForceUpdate.cpp:9-26 with StrainDisplacementMatrix.cpp:27-76
     const double pre = 1.0 * dJ[gp];
@@ -462,7 +609,22 @@ double oracle_det3(const double *F) {
 /* StressUpdate.cpp:5-29 */
 int oracle_stress(int material, const double props[9], const double F[9], double J,
                   double dt, double *hist, double S6[6]) {
+  return oracle_stress_fibre(material, props, nullptr, F, J, dt, hist, S6);
+}
+
+int oracle_stress_fibre(int material, const double props[9], const double fibre[3], const double F[9],
+                        double J, double dt, double *hist, double S6[6]) {
+  static const double kDefaultFibre[3] = {1.0, 0.0, 0.0};
   switch (material) {
+    case 6:
+      ogden(props, F, J, false, dt, nullptr, S6);
+      return 0;
+    case 7:
+      ogden(props, F, J, true, dt, hist, S6);
+      return 0;
+    case 8:
+      hgo_aniso(props, fibre ? fibre : kDefaultFibre, F, J, S6);
+      return 0;
     case 0:
       return 0;  // rigid: stress untouched (zero)
     case 1:
@@ -590,7 +752,7 @@ void *oracle_create(int n_nodes, int n_elems, const double *X, const int *conn,
   m->matid.assign(material_id, material_id + n_parts);
   m->props.assign(props, props + 9 * (size_t)n_parts);
   for (int p = 0; p < n_parts; ++p)
-    if (material_id[p] == 5) m->has_visco = true;
+    if (material_id[p] == 5 || material_id[p] == 7) m->has_visco = true;
   m->hist.assign(m->has_visco ? (size_t)n_elems * 8 * 27 : 0, 0.0);
   m->dof_motion.assign(3 * (size_t)n_nodes, -1);
   m->u0.assign(3 * (size_t)n_nodes, 0.0);
@@ -714,6 +876,16 @@ void oracle_set_bc(void *mp, const int *dof_motion, int n_motions, const double 
   Model &m = *(Model *)mp;
   m.dof_motion.assign(dof_motion, dof_motion + 3 * (size_t)m.nn);
   m.motions.assign(motions, motions + ORACLE_MOTION_STRIDE * (size_t)n_motions);
+}
+
+void oracle_set_fibres(void *mp, const double *dir) {
+  Model &m = *(Model *)mp;
+  m.fibres.assign(3 * (size_t)m.ne, 0.0);
+  for (int e = 0; e < m.ne; ++e) {
+    const double *a = dir + 3 * (size_t)e;
+    const double n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int d = 0; d < 3; ++d) m.fibres[3 * (size_t)e + d] = a[d] / n;
+  }
 }
 
 void oracle_set_initial(void *mp, const double *u0, const double *v0) {

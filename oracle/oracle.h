@@ -36,6 +36,11 @@ void oracle_shape_c3d8(const double xe[24], double dNdX[192], double detJ[8]);
  * reference's error code 1 for an unknown material. */
 int oracle_stress(int material, const double props[9], const double F[9],
                   double J, double dt, double *hist, double S6[6]);
+/* Also materials 6 (1-term Ogden), 7 (Ogden + Prony) and 8 (anisotropic HGO
+ * with fibre direction `fibre`, NULL = (1, 0, 0)); see oracle.cpp. */
+int oracle_stress_fibre(int material, const double props[9], const double fibre[3],
+                        const double F[9], double J, double dt, double *hist,
+                        double S6[6]);
 double oracle_det3(const double F[9]);
 double oracle_hex_volume(const double xe[24]);
 double oracle_face_area(const double xe[24], const int idx[4]);
@@ -69,6 +74,8 @@ void oracle_set_history(void *m, const double *hist);
 void oracle_set_bc(void *m, const int *dof_motion, int n_motions,
                    const double *motions);
 void oracle_set_initial(void *m, const double *u0, const double *v0);
+/* Fibre directions (3 per element, normalised) for material 8. */
+void oracle_set_fibres(void *m, const double *dir);
 /* ex9-style explicit loop (ex9.cpp:79-175).  Runs while time < tmax and
  * steps < max_steps.  Outputs may be NULL.  hist_* have room for
  * max_steps+1 entries (entry 0 = state after the initial GetForce).

@@ -58,6 +58,8 @@ def lib():
         L.oracle_motion_eval.argtypes = [_D, _D, ctypes.c_double, _D, _D, _D]
         L.oracle_element_strain.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.oracle_injury_init.argtypes = [ctypes.c_void_p, _I]
+        L.oracle_stress_fibre.argtypes = [ctypes.c_int, _D, _D, _D, ctypes.c_double, ctypes.c_double, _D, _D]
+        L.oracle_set_fibres.argtypes = [ctypes.c_void_p, _D]
         L.oracle_injury_update.argtypes = [ctypes.c_void_p, _D, ctypes.c_double, ctypes.c_double]
         L.oracle_injury_results.argtypes = [ctypes.c_void_p, _D, _I]
         _lib = L
@@ -80,13 +82,15 @@ def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
-def stress(material, props, F, dt=0.0, hist=None):
-    """PK2 (Voigt 6) of StressUpdate for column-major F; hist (27) updated."""
+def stress(material, props, F, dt=0.0, hist=None, fibre=None):
+    """PK2 (Voigt 6) of StressUpdate for column-major F; hist (27) updated;
+    fibre: direction for material 8."""
     F = f64(F).reshape(9)
     p = f64(props).reshape(9)
     S = np.zeros(6)
     J = lib().oracle_det3(_d(F))
-    rc = lib().oracle_stress(int(material), _d(p), _d(F), J, dt, _d(hist), _d(S))
+    fb = None if fibre is None else f64(fibre).reshape(3)
+    rc = lib().oracle_stress_fibre(int(material), _d(p), _d(fb), _d(F), J, dt, _d(hist), _d(S))
     if rc:
         raise RuntimeError("oracle_stress error %d" % rc)
     return S, J
@@ -184,6 +188,9 @@ class Model:
         P = np.zeros(3 * self.ne)
         lib().oracle_element_strain(self.h, _d(f64(u)), _d(E), _d(P))
         return E, P
+
+    def set_fibres(self, directions):
+        lib().oracle_set_fibres(self.h, _d(f64(directions)))
 
     def injury_init(self, include_part=None):
         lib().oracle_injury_init(self.h, _i(None if include_part is None else i32(include_part)))

@@ -16,7 +16,7 @@ struct HistArr {
 
 template <typename R>
 int run(const double *X24, const double *U24, int mat, const double *p9, double dt, double *hist144,
-        int update, double *f24, double *dte) {
+        int update, double *f24, double *dte, const double *fib3) {
   ft::PartDev pp;
   memset(&pp, 0, sizeof(pp));
   pp.mat = mat;
@@ -29,6 +29,9 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
   pp.t1 = p9[6];
   pp.g2 = p9[7];
   pp.t2 = p9[8];
+  pp.alpha = p9[3];
+  pp.kappa = p9[5];
+  pp.visco = mat == 5 || mat == 7;
   double nu = 0.5 * pp.lambda / (pp.lambda + pp.mu);
   pp.ce = sqrt(pp.lambda * (1.0 / nu - 1.0) / pp.rho);
   pp.inv_ce = 1.0 / pp.ce;
@@ -46,11 +49,16 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
   R out[24];
   double my = 1e300;
   int flags = 0;
+  R fib[3] = {R(1), R(0), R(0)};
+  if (fib3) {
+    const double n = sqrt(fib3[0] * fib3[0] + fib3[1] * fib3[1] + fib3[2] * fib3[2]);
+    for (int d = 0; d < 3; ++d) fib[d] = R(fib3[d] / n);
+  }
   // the per-material instance the element kernel launches for this material
   switch (mat) {
 #define FT_KM_CASE(m)                                                                     \
   case m:                                                                                 \
-    ft::element_eval<R, true, true, m>(xv, uv, pp, false, dt, h, update != 0, out, my, flags); \
+    ft::element_eval<R, true, true, m>(xv, uv, pp, false, dt, h, update != 0, out, my, flags, fib); \
     break;
     FT_KM_CASE(0)
     FT_KM_CASE(1)
@@ -58,6 +66,9 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
     FT_KM_CASE(3)
     FT_KM_CASE(4)
     FT_KM_CASE(5)
+    FT_KM_CASE(6)
+    FT_KM_CASE(7)
+    FT_KM_CASE(8)
 #undef FT_KM_CASE
     default:
       ft::element_eval<R, true, true, -1>(xv, uv, pp, false, dt, h, update != 0, out, my, flags);
@@ -71,7 +82,8 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
 }  // namespace
 
 extern "C" int km_element(int fp32, const double *X24, const double *U24, int mat, const double *p9,
-                          double dt, double *hist144, int update, double *f24, double *dte) {
-  return fp32 ? run<float>(X24, U24, mat, p9, dt, hist144, update, f24, dte)
-              : run<double>(X24, U24, mat, p9, dt, hist144, update, f24, dte);
+                          double dt, double *hist144, int update, double *f24, double *dte,
+                          const double *fib3) {
+  return fp32 ? run<float>(X24, U24, mat, p9, dt, hist144, update, f24, dte, fib3)
+              : run<double>(X24, U24, mat, p9, dt, hist144, update, f24, dte, fib3);
 }

@@ -119,3 +119,30 @@ def test_read_materials(tmp_path):
     p.write_text("0 1 1 2 3\n")
     with pytest.raises(ft.FemTechError):
         ft.read_materials(str(p), 2)  # part 1 missing
+
+
+@pytest.mark.parametrize("n", [8, 24])
+def test_ellipsoid_generator(n):
+    """Mapped-cube ellipsoid: every Gauss-point Jacobian positive, smallest
+    element not degenerate, volume converging to 4/3 pi abc, surface nodes on
+    the ellipsoid, regions by normalised radius."""
+    from femtech_amd import workloads as W
+    a, b, c = 0.09, 0.075, 0.065
+    X, conn, r = ft.mesh_ellipsoid(n, a, b, c)
+    ne = conn.size // 8
+    assert X.size == 3 * (n + 1) ** 3 and ne == n ** 3
+    m = O.Model(X, conn, np.zeros(ne, np.int32), np.array([1], np.int32),
+                np.array([[1000.0, 1.0, 1.0, 0, 0, 0, 0, 0, 0]]))
+    _, dJ = m.shape_functions()
+    dJ = dJ.reshape(-1, 8)
+    vol = dJ.sum(axis=1)
+    assert dJ.min() > 0 and vol.min() > 0.3 * vol.mean()
+    exact = 4.0 / 3.0 * np.pi * a * b * c
+    assert abs(vol.sum() / exact - 1) < 2.0 / n ** 2
+    w = W.ellipsoid_head(n=n)
+    Xr = w["X"].reshape(-1, 3)
+    surf = w["dof_motion"].reshape(-1, 3)[:, 0] == 0
+    np.testing.assert_allclose(((Xr[surf] / np.array([a, b, c])) ** 2).sum(axis=1), 1.0, atol=1e-14)
+    assert np.all((r >= 0) & (r <= 1.0 + 1e-12))
+    assert set(np.unique(w["pid"])) <= {0, 1, 2}
+    np.testing.assert_array_equal(w["pid"], np.where(r > 0.93, 0, np.where(r > 0.3, 1, 2)))

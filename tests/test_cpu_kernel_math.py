@@ -24,7 +24,7 @@ def km():
                     "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", SRC, "-o", OUT], check=True)
     lib = ctypes.CDLL(OUT)
     lib.km_element.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int, _D, ctypes.c_double, _D,
-                               ctypes.c_int, _D, _D]
+                               ctypes.c_int, _D, _D, _D]
     return lib
 
 
@@ -32,11 +32,12 @@ def dp(a):
     return None if a is None else a.ctypes.data_as(_D)
 
 
-def elem(km, fp32, x24, u24, mat, p9, dt=1e-5, hist=None, update=1):
+def elem(km, fp32, x24, u24, mat, p9, dt=1e-5, hist=None, update=1, fibre=None):
     f = np.zeros(24)
     dte = np.zeros(1)
+    fb = None if fibre is None else np.ascontiguousarray(fibre, np.float64)
     flags = km.km_element(int(fp32), dp(np.ascontiguousarray(x24)), dp(np.ascontiguousarray(u24)),
-                          int(mat), dp(np.ascontiguousarray(p9)), dt, dp(hist), update, dp(f), dp(dte))
+                          int(mat), dp(np.ascontiguousarray(p9)), dt, dp(hist), update, dp(f), dp(dte), dp(fb))
     return f, dte[0], flags
 
 
@@ -111,3 +112,44 @@ def test_rigid_rotation_gives_zero_force(km):
         f, _, _ = elem(km, 0, x.reshape(24), u, mat, p, hist=np.zeros(144))
         f_ref, _, _ = elem(km, 0, x.reshape(24), 1e-3 * u, mat, p, hist=np.zeros(144))
         assert np.abs(f).max() <= 1e-9 * max(np.abs(f_ref).max(), p[1] * 1e-6)
+
+
+def new_law_block():
+    """3x3x3 jittered block in three parts: 1-term Ogden (alpha = -4.4),
+    Ogden + Prony, anisotropic HGO (kappa = 0.1, seeded fibres)."""
+    import femtech_amd as ft
+    X, conn = ft.mesh_block(3, 3, 3, 1e-3, jitter=0.15, seed=17)
+    ne = conn.size // 8
+    pid = (np.arange(ne) % 3).astype(np.int32)
+    mu, lam = 2.0e4, 2.0e5
+    props = np.zeros((3, 9))
+    props[0, :4] = [1040.0, mu, lam, -4.4]
+    props[1, :9] = [1040.0, mu, lam, 3.0, 0.0, 0.5, 2e-3, 0.3, 2e-2]
+    props[2, :6] = [1000.0, mu, lam, 3.0e4, 8.0, 0.1]
+    rng = np.random.default_rng(5)
+    fib = rng.normal(size=(ne, 3))
+    fib /= np.linalg.norm(fib, axis=1)[:, None]
+    return X, conn, pid, np.array([6, 7, 8], np.int32), props, fib
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_new_laws_element_forces(km, fp32):
+    """Materials 6-8 (no reference counterpart) in the element kernel's math
+    against the oracle's independent restatement (itself pinned by
+    reductions, energy derivatives and objectivity)."""
+    X, conn, pid, mid, props, fib = new_law_block()
+    o = O.Model(X, conn, pid, mid, props)
+    o.set_fibres(fib)
+    Xr, cr = X.reshape(-1, 3), conn.reshape(-1, 8)
+    for scale, seed in ((1e-3, 3), (8e-2, 4)):
+        u = field(Xr, scale, seed)
+        fo = o.element_forces(u, 1e-5).reshape(-1, 24)
+        num = den = 0.0
+        for e in range(len(cr)):
+            f, _, flags = elem(km, fp32, Xr[cr[e]].reshape(24), u.reshape(-1, 3)[cr[e]].reshape(24),
+                               mid[pid[e]], props[pid[e]], hist=np.zeros(144), fibre=fib[e])
+            assert flags == 0
+            num += np.sum((f - fo[e]) ** 2)
+            den += np.sum(fo[e] ** 2)
+        err = np.sqrt(num / den)
+        assert err < (1e-6 if fp32 else 1e-11), (scale, err)

@@ -329,3 +329,57 @@ def test_injury_update_on_host_displacements_matches_oracle(gpu_ready):
         out, fl = e.injury_results()
     ro, rfl = o.injury_results()
     compare_injury(out, fl, ro, rfl)
+
+
+def ellipsoid_run(precision, steps=300):
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], precision) as e:
+        e.set_boundary(w["dof_motion"], w["motions"])
+        e.set_fibres(w["fibres"])
+        e.lumped_mass()
+        e.set_history(steps + 4)
+        e.run_init(1e9, 0.9, 1e-11)
+        assert e.run_steps(steps) == steps
+        out = e.state(), e.history(), e.visco_history()
+    o = O.Model(w["X"], w["conn"], w["pid"], w["material_id"], w["props"])
+    o.set_fibres(w["fibres"])
+    o.set_bc(w["dof_motion"], w["motions"])
+    r = o.run(steps, 1e9, 0.9, 1e-11)
+    return w, out, r, o
+
+
+def test_ellipsoid_three_regions_fp64_matches_oracle(gpu_ready):
+    """The ellipsoid workload in miniature (1,728 elements): linear-elastic
+    shell, Ogden + Prony middle (strains up to ~23 %), anisotropic HGO core
+    with seeded fibres, rotational surface loading; FP64 engine against the
+    oracle."""
+    w, ((u, v, a, fi), (th, dth, en), hv), r, o = ellipsoid_run(ft.FT_FP64)
+    assert rel(u, r["u"]) < 1e-10
+    assert rel(v, r["v"]) < 1e-9
+    assert rel(th, r["time"][1:]) < 1e-13
+    assert rel(en[:, :3], r["energy"][1:, :3]) < 1e-9
+    ho = o.history().reshape(-1, 8, 3, 9)[..., [0, 4, 8, 7, 6, 3]]
+    mask = w["material_id"][w["pid"]] == 7
+    assert rel(hv.reshape(-1, 8, 3, 6)[mask], ho[mask]) < 1e-9
+
+
+def test_ellipsoid_three_regions_fp32_split(gpu_ready):
+    w, ((u, v, a, fi), (th, dth, en), hv), r, o = ellipsoid_run(ft.FT_FP32)
+    assert rel(u, r["u"]) < 1e-5
+    ref = r["energy"][1:]
+    scale = np.abs(ref[:, :3]).max()
+    assert np.abs((en[:, 2] + en[:, 0] - en[:, 1]) - (ref[:, 2] + ref[:, 0] - ref[:, 1])).max() / scale < 1e-4
+
+
+def test_new_laws_element_forces_fp64(gpu_ready):
+    """Materials 6-8 in the device kernels against the oracle."""
+    from test_cpu_kernel_math import new_law_block, field
+    X, conn, pid, mid, props, fib = new_law_block()
+    o = O.Model(X, conn, pid, mid, props)
+    o.set_fibres(fib)
+    with ft.Engine(X, conn, pid, mid, props, ft.FT_FP64) as e:
+        e.set_fibres(fib)
+        for scale, seed in ((1e-3, 3), (8e-2, 4)):
+            u = field(X.reshape(-1, 3), scale, seed)
+            assert rel(e.element_forces(u, dt=1e-5), o.element_forces(u, 1e-5)) < 1e-11

@@ -327,3 +327,148 @@ def test_injury_percentile_and_included_parts():
     np.testing.assert_array_equal(members, inc[ps >= out[12]])
     vol = np.array([O.hex_volume(X.reshape(-1, 3)[conn.reshape(-1, 8)[e]]) for e in inc]).sum()
     assert out[20] == pytest.approx(vol, rel=1e-13)
+
+
+# ---- laws without a reference counterpart: Ogden (6), Ogden + Prony (7),
+# anisotropic HGO (8).  Pinned by reductions to reference-pinned laws, an
+# independent (numpy) evaluation, finite differences of the strain energy,
+# objectivity and fibre symmetry.
+
+def ogden_props(mu=530.0, lam=2.1e6 - 2 * 530.0 / 3, alpha=-4.4, prony=None):
+    p = np.zeros(9)
+    p[:4] = [1040.0, mu, lam, alpha]
+    if prony:
+        p[5:9] = prony
+    return p
+
+
+def aniso_props(mu=2.0e4, lam=2.0e5, k1=3.0e4, k2=8.0, kappa=0.1):
+    p = np.zeros(9)
+    p[:6] = [1000.0, mu, lam, k1, k2, kappa]
+    return p
+
+
+def ogden_energy(F, mu, lam, alpha):
+    J = np.linalg.det(F)
+    lam2 = np.linalg.eigvalsh(F @ F.T)
+    lb = J ** (-1.0 / 3.0) * np.sqrt(lam2)
+    K = lam + 2 * mu / 3
+    return 2 * mu / alpha ** 2 * np.sum(lb ** alpha - 1) + K / 4 * (J * J - 1 - 2 * np.log(J))
+
+
+def aniso_energy(F, a, mu, lam, k1, k2, kappa):
+    J = np.linalg.det(F)
+    K = lam + 2 * mu / 3
+    I1b = J ** (-2 / 3) * np.trace(F.T @ F)
+    I4b = J ** (-2 / 3) * np.dot(F @ a, F @ a)
+    E = kappa * (I1b - 3) + (1 - 3 * kappa) * (I4b - 1)
+    Wf = k1 / (2 * k2) * (np.exp(k2 * E * E) - 1) if E > 0 else 0.0
+    return mu / 2 * (I1b - 3) + K / 4 * (J * J - 1 - 2 * np.log(J)) + Wf
+
+
+def fd_first_piola(W, F, h=1e-6):
+    P = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            Fp, Fm = F.copy(), F.copy()
+            Fp[i, j] += h
+            Fm[i, j] -= h
+            P[i, j] = (W(Fp) - W(Fm)) / (2 * h)
+    return P
+
+
+def S_matrix(S6):
+    s = S6
+    return np.array([[s[0], s[5], s[4]], [s[5], s[1], s[3]], [s[4], s[3], s[2]]])
+
+
+def test_ogden_alpha2_is_the_isochoric_neo_hookean_of_hgo():
+    """alpha = 2: (2 mu/4) sum(lbar^2 - 1) = mu/2 (I1bar - 3), i.e. the
+    reference's HGOIsotropic with k1 = 0 (reference-pinned by the KATs)."""
+    rng = np.random.default_rng(21)
+    for _ in range(8):
+        F = np.eye(3) + 0.3 * rng.uniform(-1, 1, (3, 3))
+        p6 = ogden_props(mu=MU, lam=LAM, alpha=2.0)
+        p4 = ft_props(4, k1=0.0, k2=1.0)
+        S6, _ = O.stress(6, p6, col(F))
+        S4, _ = O.stress(4, p4, col(F))
+        np.testing.assert_allclose(S6, S4, rtol=1e-10, atol=1e-9 * np.abs(S4).max())
+
+
+def test_ogden_against_independent_evaluation_and_energy():
+    rng = np.random.default_rng(22)
+    mu, lam, alpha = 530.0, 2.1e6 - 2 * 530.0 / 3, -4.4
+    for amp in (0.02, 0.2, 0.4):
+        F = np.eye(3) + amp * rng.uniform(-1, 1, (3, 3))
+        J = np.linalg.det(F)
+        w, V = np.linalg.eigh(F @ F.T)
+        lb = J ** (-1 / 3) * np.sqrt(w)
+        ta = 2 * mu / alpha * (lb ** alpha - np.mean(lb ** alpha))
+        tau = (V * ta) @ V.T + 0.5 * (lam + 2 * mu / 3) * (J * J - 1) * np.eye(3)
+        Fi = np.linalg.inv(F)
+        S_want = Fi @ tau @ Fi.T
+        S, _ = O.stress(6, ogden_props(mu, lam, alpha), col(F))
+        np.testing.assert_allclose(S_matrix(S), S_want, rtol=1e-9, atol=1e-11 * np.abs(S_want).max())
+        P_fd = fd_first_piola(lambda G: ogden_energy(G, mu, lam, alpha), F)
+        np.testing.assert_allclose(F @ S_matrix(S), P_fd, rtol=1e-5, atol=1e-6 * np.abs(P_fd).max())
+
+
+def test_anisotropic_hgo_reductions_energy_and_symmetry():
+    rng = np.random.default_rng(23)
+    for _ in range(6):
+        F = np.eye(3) + 0.25 * rng.uniform(-1, 1, (3, 3))
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        # kappa = 1/3: the reference's HGOIsotropic
+        S8, _ = O.stress(8, aniso_props(MU, LAM, 3.0e4, 8.0, 1.0 / 3.0), col(F), fibre=a)
+        S4, _ = O.stress(4, ft_props(4), col(F))
+        np.testing.assert_allclose(S8, S4, rtol=1e-10, atol=1e-9 * np.abs(S4).max())
+        # kappa = 0.1: energy derivative, and a -> -a changes nothing
+        p = aniso_props(kappa=0.1)
+        S, _ = O.stress(8, p, col(F), fibre=a)
+        Sm, _ = O.stress(8, p, col(F), fibre=-a)
+        np.testing.assert_allclose(Sm, S, rtol=1e-13, atol=1e-12 * np.abs(S).max())
+        P_fd = fd_first_piola(lambda G: aniso_energy(G, a, MU, LAM, 3.0e4, 8.0, 0.1), F)
+        np.testing.assert_allclose(F @ S_matrix(S), P_fd, rtol=1e-5, atol=1e-6 * np.abs(P_fd).max())
+
+
+def test_fibre_stiffens_only_in_tension_along_it():
+    """Uniaxial stretch along the fibre engages the exponential term (kappa <
+    1/3 puts weight on I4); the same stretch across it engages less."""
+    p = aniso_props(kappa=0.0)
+    a = np.array([1.0, 0.0, 0.0])
+    S_along, _ = O.stress(8, p, col(np.diag([1.2, 1.0, 1.0])), fibre=a)
+    S_across, _ = O.stress(8, p, col(np.diag([1.0, 1.2, 1.0])), fibre=a)
+    p0 = aniso_props(k1=0.0, kappa=0.0)
+    S_iso, _ = O.stress(8, p0, col(np.diag([1.2, 1.0, 1.0])), fibre=a)
+    assert S_along[0] > S_iso[0] * 1.2
+    assert S_across[1] < S_along[0]
+
+
+@pytest.mark.parametrize("mat", [6, 8])
+def test_objectivity_new_laws(mat):
+    rng = np.random.default_rng(30 + mat)
+    p = ogden_props() if mat == 6 else aniso_props()
+    a = np.array([0.6, 0.8, 0.0])
+    for _ in range(5):
+        F = np.eye(3) + 0.2 * rng.uniform(-1, 1, (3, 3))
+        Q = rand_rotation(rng)
+        S1, _ = O.stress(mat, p, col(F), fibre=a)
+        S2, _ = O.stress(mat, p, col(Q @ F), fibre=a)
+        np.testing.assert_allclose(S2, S1, rtol=1e-9, atol=1e-9 * np.abs(S1).max())
+
+
+def test_ogden_prony_reduces_and_relaxes():
+    rng = np.random.default_rng(24)
+    F = np.eye(3) + 0.1 * rng.uniform(-1, 1, (3, 3))
+    # moderate bulk modulus so that the deviatoric (relaxing) part is visible
+    S6, _ = O.stress(6, ogden_props(mu=MU, lam=LAM), col(F))
+    S7, _ = O.stress(7, ogden_props(mu=MU, lam=LAM, prony=[0.0, 2e-3, 0.0, 2e-2]), col(F), 1e-4, np.zeros(27))
+    np.testing.assert_allclose(S7, S6, rtol=1e-12, atol=1e-12 * np.abs(S6).max())
+    h = np.zeros(27)
+    p7 = ogden_props(mu=MU, lam=LAM, prony=[0.5, 2e-3, 0.3, 2e-3])
+    first, _ = O.stress(7, p7, col(F), 1e-4, h)
+    assert np.abs(first - S6).max() > 1e-3 * np.abs(S6).max()
+    for _ in range(3000):
+        late, _ = O.stress(7, p7, col(F), 1e-4, h)
+    np.testing.assert_allclose(late, S6, rtol=1e-6, atol=1e-6 * np.abs(S6).max())
