@@ -72,6 +72,9 @@ _SIGS = {
     "ft_traffic_model": [_VP, _D],
     "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
     "ft_mesh_ellipsoid": [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _I, _D],
+    "ft_set_instances": [_VP, ctypes.c_int, _I, _I],
+    "ft_set_instance_end_times": [_VP, _D],
+    "ft_get_instance": [_VP, ctypes.c_int, _D],
     "ft_set_fibres": [_VP, _D],
     "ft_injury_enable": [_VP, _I],
     "ft_injury_disable": [_VP],
@@ -317,6 +320,23 @@ class Engine:
         s = ctypes.c_int(0)
         _check(self._lib.ft_get_status(self._h, ctypes.byref(s)))
         return s.value
+
+    def set_instances(self, node_offsets, elem_offsets):
+        """Declare independent instances (batched simulations) by their node
+        and element ranges; each then has its own dt, time and energy."""
+        no = np.ascontiguousarray(node_offsets, np.int32)
+        eo = np.ascontiguousarray(elem_offsets, np.int32)
+        _check(self._lib.ft_set_instances(self._h, no.size - 1, _ip(no), _ip(eo)))
+        self.n_inst = no.size - 1
+
+    def set_instance_end_times(self, tmax):
+        _check(self._lib.ft_set_instance_end_times(self._h, _dp(_f64(tmax))))
+
+    def instance(self, i):
+        """(time, next dt, step, finished, W_int, W_ext, KE, |balance|)."""
+        out = np.empty(8, np.float64)
+        _check(self._lib.ft_get_instance(self._h, int(i), _dp(out)))
+        return out
 
     def set_fibres(self, directions):
         """Fibre direction per element (3 each) for the anisotropic HGO law."""

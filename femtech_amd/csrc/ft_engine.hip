@@ -65,7 +65,7 @@ constexpr int kBlock = 256;
 constexpr uint64_t kInfBits = 0x7FF0000000000000ull;
 constexpr int kSkipBit = 1 << 30;
 constexpr int kPidMask = (1 << 20) - 1;
-constexpr int kMaxMotions = 64;
+constexpr int kMaxMotions = 64;  // distinct motions per instance (held in LDS by the nodal kernel)
 constexpr int kEaccStride = 8;  // doubles per nodal block in the energy accumulators
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
@@ -79,6 +79,12 @@ __device__ inline int xcd_block() {
 int grid_for(long n) {
   long nb = (n + kBlock - 1) / kBlock;
   nb = (nb + 7) & ~7L;
+  return (int)std::max(nb, 8L);
+}
+
+// Blocks for n tiles: a multiple of 8 (the XCD remap), the excess exits.
+int grid_for_tiles(long n) {
+  long nb = (n + 7) & ~7L;
   return (int)std::max(nb, 8L);
 }
 
@@ -137,11 +143,13 @@ __device__ inline double block_sum(double v, double *sh) {
 // per block and per time into LDS; the per-node part is cheap and inline.
 struct MotionT {
   double s0, s1, s2, s3;  // LINEAR/RAMP: u, v scalars; ROT: cos, sin, omega, alpha
+  double R[9];            // ROT: rotation matrix (row-major) of the angle at this time
 };
 
 __host__ __device__ inline void motion_time(const double *mo, double t, MotionT &m) {
   const int type = (int)mo[0];
   m.s0 = m.s1 = m.s2 = m.s3 = 0.0;
+  for (int k = 0; k < 9; ++k) m.R[k] = 0.0;
   if (type == FT_MOTION_LINEAR) {  // ex9.cpp:191-243: AppliedDisp = Time*dMax/tMax
     m.s0 = t * mo[1];
     m.s1 = mo[1];
@@ -166,6 +174,17 @@ __host__ __device__ inline void motion_time(const double *mo, double t, MotionT 
     m.s0 = cos(theta);
     m.s1 = sin(theta);
     m.s2 = omega;
+    // Rodrigues: R = cos I + sin [n]x + (1 - cos) n n^T
+    const double n[3] = {mo[1], mo[2], mo[3]};
+    const double c = m.s0, sn = m.s1;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) m.R[3 * i + j] = (i == j ? c : 0.0) + (1.0 - c) * n[i] * n[j];
+    m.R[1] -= sn * n[2];
+    m.R[2] += sn * n[1];
+    m.R[3] += sn * n[2];
+    m.R[5] -= sn * n[0];
+    m.R[6] -= sn * n[1];
+    m.R[7] += sn * n[0];
     m.s3 = alpha;
   }
 }
@@ -180,22 +199,17 @@ __host__ __device__ inline void motion_apply(const double *mo, const MotionT &m,
     u = m.s0;
     v = m.s1;
   } else if (type == FT_MOTION_ROT_HALFSINE) {
-    // Rodrigues: r = r0 cos + (n x r0) sin + n (n.r0)(1 - cos); v = w n x r;
-    // a = alpha n x r + w^2 n x (n x r)
+    // r = R r0 (R from motion_time); v = w n x r; a = alpha n x r + w^2 n x (n x r)
     const double n0 = mo[1], n1 = mo[2], n2 = mo[3];
     const double r0[3] = {X[0] - mo[4], X[1] - mo[5], X[2] - mo[6]};
-    const double ct = m.s0, st = m.s1;
-    const double nxr0[3] = {n1 * r0[2] - n2 * r0[1], n2 * r0[0] - n0 * r0[2], n0 * r0[1] - n1 * r0[0]};
-    const double ndr0 = n0 * r0[0] + n1 * r0[1] + n2 * r0[2];
-    const double n[3] = {n0, n1, n2};
     double r[3];
-    for (int k = 0; k < 3; ++k) r[k] = r0[k] * ct + nxr0[k] * st + n[k] * ndr0 * (1.0 - ct);
+    for (int k = 0; k < 3; ++k) r[k] = m.R[3 * k] * r0[0] + m.R[3 * k + 1] * r0[1] + m.R[3 * k + 2] * r0[2];
     const double nxr[3] = {n1 * r[2] - n2 * r[1], n2 * r[0] - n0 * r[2], n0 * r[1] - n1 * r[0]};
-    const double nxnxr[3] = {n1 * nxr[2] - n2 * nxr[1], n2 * nxr[0] - n0 * nxr[2],
-                             n0 * nxr[1] - n1 * nxr[0]};
+    const double n[3] = {n0, n1, n2};
+    const int d1 = d == 2 ? 0 : d + 1, d2 = d == 0 ? 2 : d - 1;
     u = r[d] - r0[d];
     v = m.s2 * nxr[d];
-    a = m.s3 * nxr[d] + m.s2 * m.s2 * nxnxr[d];
+    a = m.s3 * nxr[d] + m.s2 * m.s2 * (n[d1] * nxr[d2] - n[d2] * nxr[d1]);
   }
 }
 
@@ -221,11 +235,17 @@ struct HistSoA {
   __device__ void store(int gp, int k, R v) { base[(size_t)(gp * 18 + k) * ne + e] = v; }
 };
 
+// A tile is the work of one block: a range of at most kBlock consecutive
+// elements (or nodes) of one instance.  int4 {begin, end, instance, flags};
+// node tiles carry kTileFirst on the first tile of their instance.
+constexpr int kTileFirst = 1;
+
 template <typename R>
 struct ElemArgs {
   int nn;                // nodes (stride of the component-major nodal arrays)
   int ne;                // all elements (history stride)
-  int begin, end;        // this launch's element range (one material)
+  const int4 *tiles;     // this launch's tiles (one material segment, or all)
+  int n_tiles;
   const int *conn;       // 8 per element
   const int *einfo;      // part id | skip-dt flag
   const double *X;       // component-major [3][nn]
@@ -235,10 +255,11 @@ struct ElemArgs {
   int corner;            // fe[corner][node][xyz] instead of fe[slot][element][xyz]
   R *hist;               // 144 per element, component-major [c * ne + e]
   const R *fib;          // fibre directions [3][ne] (material 8)
-  const StepSlot *slot;  // nullable: step slot giving dt_k and done
+  Ctl *ctl;              // run loop: per-instance control, slot `parity` gives dt_k and done
+  int parity;
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
-  double dt_fixed;       // used when slot == nullptr
-  unsigned long long *dtmin_bits;  // nullable
+  double dt_fixed;       // used outside the run loop (ctl == nullptr)
+  unsigned long long *dtbits;  // outside the run loop: per-instance minimum (nullable)
   int *status;
   int want_force;
   int want_dt;
@@ -257,17 +278,23 @@ __global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT <= 4 && MAT != -1) ?
     ElemArgs<R> a) {
   __shared__ double sh_min[kBlock / 64];
   const int L = xcd_block();
-  const int e = a.begin + L * kBlock + threadIdx.x;
+  if (L >= a.n_tiles) return;  // grid padding (uniform over the block)
+  const int4 tile = a.tiles[L];
+  const int e = tile.x + threadIdx.x;
+  const int end = tile.y;
   double dt_step = a.dt_fixed;
   double dt_guide = 1e300;  // no previous minimum: refine every element
-  if (a.slot) {
-    if (a.slot->done) return;  // uniform over the grid
-    dt_step = a.slot->dt;
-    dt_guide = a.slot->dt_guide;
+  unsigned long long *dtmin = a.dtbits ? a.dtbits + tile.z : nullptr;
+  if (a.ctl) {
+    const StepSlot *slot = &a.ctl[tile.z].slot[a.parity];
+    if (slot->done) return;  // uniform over the block
+    dt_step = slot->dt;
+    dt_guide = slot->dt_guide;
+    dtmin = &a.ctl[tile.z].slot[a.parity].dtmin_bits;
   }
   double my_dt = 1e300;
   int flags = 0;
-  if (e < a.end) {
+  if (e < end) {
     const int4 c0 = reinterpret_cast<const int4 *>(a.conn)[2 * e];
     const int4 c1 = reinterpret_cast<const int4 *>(a.conn)[2 * e + 1];
     const int nd[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
@@ -334,7 +361,7 @@ __global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT <= 4 && MAT != -1) ?
       __shared__ double sh_a[7];
       if (threadIdx.x == 0) sh_arg = INT32_MAX;
       __syncthreads();
-      if (e < a.end && my_dt == bmin) atomicMin(&sh_arg, e);
+      if (e < end && my_dt == bmin) atomicMin(&sh_arg, e);
       __syncthreads();
       const int es = sh_arg;
       if (threadIdx.x < 3) {
@@ -359,10 +386,10 @@ __global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT <= 4 && MAT != -1) ?
         for (int f = 0; f < 6; ++f)
           if (sh_a[f] > amax) amax = sh_a[f];
         const double dte = sh_a[6] / amax / a.parts[a.einfo[es] & kPidMask].ce;
-        atomicMin(a.dtmin_bits, (unsigned long long)__double_as_longlong(dte));
+        atomicMin(dtmin, (unsigned long long)__double_as_longlong(dte));
       }
-    } else if (threadIdx.x == 0 && bmin < 1e300) {
-      atomicMin(a.dtmin_bits, (unsigned long long)__double_as_longlong(bmin));
+    } else if (threadIdx.x == 0 && bmin < 1e300 && dtmin) {
+      atomicMin(dtmin, (unsigned long long)__double_as_longlong(bmin));
     }
   }
   if (flags) atomicOr(a.status, flags);
@@ -453,17 +480,21 @@ struct NodalArgs {
   double *u, *vs, *fi;
   const double *mass;  // 1 per node
   const int *node_code;  // per node: byte d = motion index + 1 of DOF d (0 free)
-  const double *motions;
-  int n_motions;
-  const MotionT *mstate;  // 3 x n_motions, from k_motion_states
-  Ctl *ctl;
+  const double *motions;  // per-instance copies, instance i at motion mrange[2 i]
+  int n_motions;          // total (stride of mstate)
+  const int *mrange;      // per instance: first motion, count (<= kMaxMotions)
+  const MotionT *mstate;  // [3][n_motions], from k_motion_states
+  const int4 *tiles;      // node tiles
+  int n_tiles;
+  Ctl *ctl;               // per instance; ctl[0] also holds the status word
   int parity;
   int mode;
   int energy;
-  double *eacc;  // kEaccStride per logical block: W_int, W_ext, KE, and the
-                 // fi_{k-1} halves of the next step's W_int and W_ext
-  double tmax, factor, failure_dt;
-  int max_steps;
+  double *eacc;  // kEaccStride per tile: W_int, W_ext, KE, and the fi_{k-1}
+                 // halves of the next step's W_int and W_ext
+  const double *tmax;     // per instance
+  const int *cap;         // per instance: last step this call may complete
+  double factor, failure_dt;
 };
 
 // End of step k (gather, a = f/m, v = v_half + dt/2 a, energy) fused with the
@@ -473,35 +504,41 @@ struct NodalArgs {
 template <typename R, int W>
 __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
   __shared__ double sh[kBlock / 64];
-  const StepSlot cur = a.ctl->slot[a.parity];
+  const int L = xcd_block();
+  if (L >= a.n_tiles) return;  // grid padding
+  const int4 tile = a.tiles[L];
+  const int inst = tile.z;
+  Ctl *ic = a.ctl + inst;
+  const StepSlot cur = ic->slot[a.parity];
   if (cur.done && a.mode != kResume) {
     // keep both ping-pong slots stopped so later launches stay no-ops
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if ((tile.w & kTileFirst) && threadIdx.x == 0) {
       StepSlot nx = cur;
       nx.fresh = 0;
-      a.ctl->slot[a.parity ^ 1] = nx;
+      ic->slot[a.parity ^ 1] = nx;
     }
     return;
   }
-  const int L = xcd_block();
-  const int n = L * kBlock + threadIdx.x;
+  const int n = tile.x + threadIdx.x;
   const double t_k = cur.t, t_km1 = cur.t_prev, dt_k = cur.dt;
   const double dtmin = bits_to_double(cur.dtmin_bits);
   const double dt_next = a.factor * dtmin;
   const bool collapse = !(dtmin >= a.failure_dt);
-  const bool go = (t_k < a.tmax) && (cur.step < a.max_steps) && !collapse;
-  // motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
+  const bool go = (t_k < a.tmax[inst]) && (cur.step < a.cap[inst]) && !collapse;
+  // this instance's motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
   __shared__ MotionT ms[3][kMaxMotions];
-  for (int i = threadIdx.x; i < 3 * a.n_motions; i += blockDim.x)
-    ms[i / a.n_motions][i % a.n_motions] = a.mstate[i];
+  const int mb = a.mrange[2 * inst], mc = a.mrange[2 * inst + 1];
+  for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x)
+    ms[i / mc][i % mc] = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
   __syncthreads();
+  const double *motions = a.motions + (size_t)FT_MOTION_STRIDE * mb;
   // Energy (CheckEnergy.cpp:15-52): the terms of step k pair each DOF's
   // displacement increment du_k with fi_{k-1} + fi_k.  The fi_{k-1} halves
   // (pInt, pExt) are added by the launch that knows fi_{k-1} and already
   // computes du_k, so fi is not carried from step to step; it is stored only
   // when the run stops.
   double sWint = 0.0, sWext = 0.0, KE = 0.0, pInt = 0.0, pExt = 0.0;
-  if (n < a.nn) {
+  if (n < tile.y) {
     double f[3];
     if (a.mode == kResume) {
       f[0] = a.fi[n];
@@ -515,6 +552,12 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
     const double t_khalf = 0.5 * (t_k + t_km1);
     const double t_k1 = t_k + dt_next;
     const double t_k1half = 0.5 * (t_k1 + t_k);
+    double Xn[3] = {0.0, 0.0, 0.0};
+    if (code) {  // reference position, read for constrained nodes only
+      Xn[0] = a.X[n];
+      Xn[1] = a.X[(size_t)a.nn + n];
+      Xn[2] = a.X[2 * (size_t)a.nn + n];
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const int mi = ((code >> (8 * d)) & 0xff) - 1;
@@ -523,13 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
       const double vso = a.vs[id];
       double acc, v;
       double ub = 0.0, vb = 0.0, ab = 0.0;
-      const double *mo = a.motions + FT_MOTION_STRIDE * (mi < 0 ? 0 : mi);
-      double Xn[3] = {0.0, 0.0, 0.0};
-      if (mi >= 0) {  // reference position, read for constrained DOFs only
-        Xn[0] = a.X[n];
-        Xn[1] = a.X[(size_t)a.nn + n];
-        Xn[2] = a.X[2 * (size_t)a.nn + n];
-      }
+      const double *mo = motions + FT_MOTION_STRIDE * (mi < 0 ? 0 : mi);
       if (mi < 0) {
         acc = (0.0 - f[d]) / m;
         v = (a.mode == kStep) ? vso + (t_k - t_khalf) * acc : vso;
@@ -593,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
       }
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if ((tile.w & kTileFirst) && threadIdx.x == 0) {
     StepSlot nx;
     if (go) {
       nx.t = t_k + dt_next;
@@ -605,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
     } else {
       nx = cur;
       nx.done = 1;
-      if (collapse && cur.step < a.max_steps && t_k < a.tmax) atomicOr(&a.ctl->status, FT_STATUS_DT_COLLAPSE);
+      if (collapse && cur.step < a.cap[inst] && t_k < a.tmax[inst]) atomicOr(&a.ctl->status, FT_STATUS_DT_COLLAPSE);
     }
     nx.t_done = t_k;
     nx.dt_done = dt_next;
@@ -613,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
     if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(&a.ctl->status, kStatusDtFromFp32);
     nx.fresh = a.mode == kStep;
     nx.pad = 0;
-    a.ctl->slot[a.parity ^ 1] = nx;
+    ic->slot[a.parity ^ 1] = nx;
   }
 }
 
@@ -621,11 +658,11 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
 // for the nodal kernel of step k (one small block; the trigonometry of the
 // rotation motions is kept out of the per-node kernel).
 __global__ void k_motion_states(const Ctl *ctl, int parity, double factor, const double *motions,
-                                int nm, MotionT *out) {
-  const StepSlot cur = ctl->slot[parity];
-  const double t_next = cur.t + factor * bits_to_double(cur.dtmin_bits);
-  for (int i = threadIdx.x; i < 3 * nm; i += blockDim.x) {
+                                const int *motion_inst, int nm, MotionT *out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nm; i += gridDim.x * blockDim.x) {
     const int w = i / nm, k = i % nm;
+    const StepSlot cur = ctl[motion_inst[k]].slot[parity];
+    const double t_next = cur.t + factor * bits_to_double(cur.dtmin_bits);
     const double tt = w == 0 ? cur.t_prev : (w == 1 ? cur.t : t_next);
     motion_time(motions + FT_MOTION_STRIDE * k, tt, out[i]);
   }
@@ -1145,11 +1182,16 @@ __global__ void k_elem_flags(int ne, const int *conn, const unsigned char *node_
 }
 
 // Velocity / acceleration read-out for ft_get_state.
-__global__ void k_state_out(int nn, const double *fi, const double *mass, const double *X,
-                            const int *node_code, const double *motions, double t,
-                            double *a_out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
+__global__ void k_state_out(int nn, const int4 *tiles, int n_tiles, const Ctl *ctl, int parity,
+                            const double *fi, const double *mass, const double *X, const int *node_code,
+                            const double *motions, const int *mrange, double *a_out) {
+  const int L = blockIdx.x;
+  if (L >= n_tiles) return;
+  const int4 tile = tiles[L];
+  const int n = tile.x + threadIdx.x;
+  if (n >= tile.y) return;
+  const double t = ctl[tile.z].slot[parity].t;
+  const double *mo = motions + (size_t)FT_MOTION_STRIDE * mrange[2 * tile.z];
   for (int d = 0; d < 3; ++d) {
     const int mi = ((node_code[n] >> (8 * d)) & 0xff) - 1;
     const size_t id = (size_t)d * nn + n;
@@ -1158,7 +1200,7 @@ __global__ void k_state_out(int nn, const double *fi, const double *mass, const 
     } else {
       double ub[3], vb[3], ab[3];
       const double Xn[3] = {X[n], X[(size_t)nn + n], X[2 * (size_t)nn + n]};
-      motion_eval(motions + FT_MOTION_STRIDE * mi, Xn, t, ub, vb, ab);
+      motion_eval(mo + FT_MOTION_STRIDE * mi, Xn, t, ub, vb, ab);
       a_out[id] = ab[d];
     }
   }
@@ -1190,8 +1232,21 @@ struct ft_engine {
   // material runs its own specialised element kernel over a contiguous range.
   std::vector<int> perm;  // internal -> original element index
   std::vector<int> inv;   // original -> internal
-  struct Seg { int mat, begin, end; };
+  struct Seg { int mat, begin, end, tile_begin, tile_end; };
   std::vector<Seg> segs;
+  // Independent instances (batched simulations): node range and element
+  // range of each in the caller's numbering; a single mesh is 1 instance.
+  int n_inst = 1;
+  std::vector<int> inst_node, inst_elem;   // n_inst + 1 offsets each
+  int4 *etiles = nullptr, *ntiles = nullptr;
+  int n_etiles = 0, n_ntiles = 0;
+  std::vector<int> ntile_of_inst;          // n_inst + 1: node tiles per instance
+  int *mrange = nullptr;                   // per instance: first motion, count
+  int *motion_inst = nullptr;              // per (expanded) motion: instance
+  int *inst_cap = nullptr;                 // per instance: step cap of the current call
+  double *inst_tmax = nullptr;             // per instance: end time
+  std::vector<double> h_tmax;
+  bool tmax_per_instance = false;          // end times from ft_set_instance_end_times
   // state
   double *u = nullptr, *vs = nullptr, *fi = nullptr, *mass = nullptr, *tmp3 = nullptr;
   void *fe = nullptr;
@@ -1321,15 +1376,18 @@ int d2h_nodal(ft_engine *g, double *aos, const double *dev) {
   return 0;
 }
 
+// Element launches cover tile ranges [t0, t1) of g->etiles.  In the run loop
+// (in_loop) each tile reads its instance's step slot; elsewhere dt_fixed is
+// the step size and dtbits (per instance, nullable) collects the minimum.
 template <typename R, bool FORCE, bool DT, int MAT>
-int launch_element(ft_engine *g, int begin, int end, const StepSlot *slot, double dt_fixed,
-                   unsigned long long *dtbits, int update_hist) {
-  if (end <= begin) return 0;
+int launch_element(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixed, unsigned long long *dtbits,
+                   int update_hist) {
+  if (t1 <= t0) return 0;
   ElemArgs<R> a;
   a.nn = g->nn;
   a.ne = g->ne;
-  a.begin = begin;
-  a.end = end;
+  a.tiles = g->etiles + t0;
+  a.n_tiles = t1 - t0;
   a.conn = g->conn;
   a.einfo = g->einfo;
   a.X = g->X;
@@ -1339,59 +1397,62 @@ int launch_element(ft_engine *g, int begin, int end, const StepSlot *slot, doubl
   a.corner = g->corner ? 1 : 0;
   a.hist = reinterpret_cast<R *>(g->hist);
   a.fib = reinterpret_cast<const R *>(g->fib);
-  a.slot = slot;
-  a.refine = (sizeof(R) == 4 && slot) ? 1 : 0;
+  a.ctl = in_loop ? g->ctl : nullptr;
+  a.parity = g->parity;
+  a.refine = (sizeof(R) == 4 && in_loop) ? 1 : 0;
   a.dt_fixed = dt_fixed;
-  a.dtmin_bits = dtbits;
+  a.dtbits = in_loop ? nullptr : dtbits;
   a.status = g->status;
   a.want_force = FORCE;
   a.want_dt = DT;
   a.update_hist = update_hist;
   Timed t(g, 0);
-  hipLaunchKernelGGL((k_element<R, FORCE, DT, MAT>), dim3(grid_for(end - begin)), dim3(kBlock), 0,
-                     g->stream, a);
+  hipLaunchKernelGGL((k_element<R, FORCE, DT, MAT>), dim3(grid_for_tiles(t1 - t0)), dim3(kBlock), 0, g->stream, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 template <typename R, bool DT>
-int force_segment(ft_engine *g, const ft_engine::Seg &sg, const StepSlot *slot, double dt_fixed,
+int force_segment(ft_engine *g, const ft_engine::Seg &sg, bool in_loop, double dt_fixed,
                   unsigned long long *dtbits, int update_hist) {
+  const int t0 = sg.tile_begin, t1 = sg.tile_end;
   switch (sg.mat) {
-    case 0: return launch_element<R, true, DT, 0>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 1: return launch_element<R, true, DT, 1>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 2: return launch_element<R, true, DT, 2>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 3: return launch_element<R, true, DT, 3>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 4: return launch_element<R, true, DT, 4>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 5: return launch_element<R, true, DT, 5>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 6: return launch_element<R, true, DT, 6>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 7: return launch_element<R, true, DT, 7>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
-    case 8: return launch_element<R, true, DT, 8>(g, sg.begin, sg.end, slot, dt_fixed, dtbits, update_hist);
+    case 0: return launch_element<R, true, DT, 0>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 1: return launch_element<R, true, DT, 1>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 2: return launch_element<R, true, DT, 2>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 3: return launch_element<R, true, DT, 3>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 4: return launch_element<R, true, DT, 4>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 5: return launch_element<R, true, DT, 5>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 6: return launch_element<R, true, DT, 6>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 7: return launch_element<R, true, DT, 7>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 8: return launch_element<R, true, DT, 8>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
   }
   return fail(FT_ERR_MATERIAL, "Unknown material type %d", sg.mat);
 }
 
 template <typename R>
-int element_pass_t(ft_engine *g, bool force, bool dt, const StepSlot *slot, double dt_fixed,
+int element_pass_t(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fixed,
                    unsigned long long *dtbits, int update_hist) {
   // A stand-alone stable-time-step pass always runs in FP64 (exact minimum).
-  if (!force) return launch_element<double, false, true, -1>(g, 0, g->ne, slot, dt_fixed, dtbits, 0);
+  if (!force) return launch_element<double, false, true, -1>(g, 0, g->n_etiles, false, dt_fixed, dtbits, 0);
   for (const auto &sg : g->segs) {
-    int rc = dt ? force_segment<R, true>(g, sg, slot, dt_fixed, dtbits, update_hist)
-                : force_segment<R, false>(g, sg, slot, dt_fixed, dtbits, update_hist);
+    int rc = dt ? force_segment<R, true>(g, sg, in_loop, dt_fixed, dtbits, update_hist)
+                : force_segment<R, false>(g, sg, in_loop, dt_fixed, dtbits, update_hist);
     if (rc) return rc;
   }
   return 0;
 }
 
-int element_pass(ft_engine *g, bool force, bool dt, const StepSlot *slot, double dt_fixed,
-                 unsigned long long *dtbits, int update_hist) {
-  if (g->precision == FT_FP32) return element_pass_t<float>(g, force, dt, slot, dt_fixed, dtbits, update_hist);
-  return element_pass_t<double>(g, force, dt, slot, dt_fixed, dtbits, update_hist);
+int element_pass(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fixed, unsigned long long *dtbits,
+                 int update_hist) {
+  if (g->precision == FT_FP32) return element_pass_t<float>(g, force, dt, in_loop, dt_fixed, dtbits, update_hist);
+  return element_pass_t<double>(g, force, dt, in_loop, dt_fixed, dtbits, update_hist);
 }
 
+// Nodal launches read the per-instance step caps and end times from
+// g->inst_cap / g->inst_tmax (set by the host before a run call).
 template <typename R>
-int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
+int launch_nodal_t(ft_engine *g, int mode) {
   NodalArgs<R> a;
   a.nn = g->nn;
   a.ell = g->ell;
@@ -1405,36 +1466,40 @@ int launch_nodal_t(ft_engine *g, int mode, int max_steps) {
   a.node_code = g->node_code;
   a.motions = g->motions;
   a.n_motions = g->n_motions;
+  a.mrange = g->mrange;
   a.mstate = g->mstate;
+  a.tiles = g->ntiles;
+  a.n_tiles = g->n_ntiles;
   a.ctl = g->ctl;
   a.parity = g->parity;
   a.mode = mode;
   a.energy = 1;
   a.eacc = g->eacc;
-  a.tmax = g->tmax;
+  a.tmax = g->inst_tmax;
+  a.cap = g->inst_cap;
   a.factor = g->factor;
   a.failure_dt = g->failure_dt;
-  a.max_steps = max_steps;
   Timed t(g, 1);
+  const dim3 grid(grid_for_tiles(g->n_ntiles));
   if (g->corner)
-    hipLaunchKernelGGL((k_nodal<R, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+    hipLaunchKernelGGL((k_nodal<R, -1>), grid, dim3(kBlock), 0, g->stream, a);
   else if (g->width == 8)
-    hipLaunchKernelGGL((k_nodal<R, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+    hipLaunchKernelGGL((k_nodal<R, 8>), grid, dim3(kBlock), 0, g->stream, a);
   else
-    hipLaunchKernelGGL((k_nodal<R, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, a);
+    hipLaunchKernelGGL((k_nodal<R, 0>), grid, dim3(kBlock), 0, g->stream, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int launch_nodal(ft_engine *g, int mode, int max_steps) {
+int launch_nodal(ft_engine *g, int mode) {
   if (g->timed_motions) {
     Timed t(g, 2);
-    hipLaunchKernelGGL(k_motion_states, dim3(1), dim3(kBlock), 0, g->stream, g->ctl, g->parity,
-                       g->factor, g->motions, g->n_motions, g->mstate);
+    const int nb = std::max(1, std::min(64, (3 * g->n_motions + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_motion_states, dim3(nb), dim3(kBlock), 0, g->stream, g->ctl, g->parity, g->factor,
+                       g->motions, g->motion_inst, g->n_motions, g->mstate);
     HIPCHK(hipGetLastError());
   }
-  int rc = g->precision == FT_FP32 ? launch_nodal_t<float>(g, mode, max_steps)
-                                   : launch_nodal_t<double>(g, mode, max_steps);
+  int rc = g->precision == FT_FP32 ? launch_nodal_t<float>(g, mode) : launch_nodal_t<double>(g, mode);
   if (rc) return rc;
   g->parity ^= 1;
   return 0;
@@ -1478,6 +1543,145 @@ int injury_launch(ft_engine *g, const StepSlot *slot, double time, double dt) {
   hipLaunchKernelGGL(k_injury_members, dim3(nblk), dim3(kBlock), 0, g->stream, g->ne, g->inj_flags, g->inj_ps_old,
                      g->inj_psxsr, g->inj_scal, slot);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Tiles of at most kBlock consecutive entries that never span two
+// instances; elements per material segment (internal order), nodes in order.
+int build_tiles(ft_engine *g) {
+  std::vector<int> inst_of_elem(g->ne);
+  for (int i = 0; i < g->n_inst; ++i)
+    for (int e = g->inst_elem[i]; e < g->inst_elem[i + 1]; ++e) inst_of_elem[e] = i;
+  std::vector<int4> et;
+  for (auto &sg : g->segs) {
+    sg.tile_begin = (int)et.size();
+    for (int i = sg.begin; i < sg.end;) {
+      const int inst = inst_of_elem[g->perm[i]];
+      int j = i + 1;
+      while (j < sg.end && j - i < kBlock && inst_of_elem[g->perm[j]] == inst) ++j;
+      et.push_back(make_int4(i, j, inst, 0));
+      i = j;
+    }
+    sg.tile_end = (int)et.size();
+  }
+  std::vector<int4> nt;
+  g->ntile_of_inst.assign(g->n_inst + 1, 0);
+  for (int i = 0; i < g->n_inst; ++i) {
+    g->ntile_of_inst[i] = (int)nt.size();
+    for (int n = g->inst_node[i]; n < g->inst_node[i + 1]; n += kBlock)
+      nt.push_back(make_int4(n, std::min(n + kBlock, g->inst_node[i + 1]), i,
+                             n == g->inst_node[i] ? kTileFirst : 0));
+  }
+  g->ntile_of_inst[g->n_inst] = (int)nt.size();
+  for (void *p : {(void *)g->etiles, (void *)g->ntiles, (void *)g->eacc})
+    if (p) (void)hipFree(p);
+  g->etiles = g->ntiles = nullptr;
+  g->eacc = nullptr;
+  g->n_etiles = (int)et.size();
+  g->n_ntiles = (int)nt.size();
+  int rc;
+  if ((rc = dalloc(&g->etiles, et.size())) || (rc = dalloc(&g->ntiles, nt.size())) ||
+      (rc = dalloc(&g->eacc, kEaccStride * (size_t)std::max(1, g->n_ntiles))))
+    return rc;
+  HIPCHK(hipMemcpy(g->etiles, et.data(), et.size() * sizeof(int4), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->ntiles, nt.data(), nt.size() * sizeof(int4), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(g->eacc, 0, kEaccStride * (size_t)std::max(1, g->n_ntiles) * sizeof(double)));
+  return 0;
+}
+
+// Per-instance arrays of the control state (re)allocated for g->n_inst.
+int alloc_instances(ft_engine *g) {
+  for (void *p : {(void *)g->ctl, (void *)g->dtbits, (void *)g->mrange, (void *)g->inst_cap, (void *)g->inst_tmax})
+    if (p) (void)hipFree(p);
+  g->ctl = nullptr;
+  g->dtbits = nullptr;
+  g->mrange = nullptr;
+  g->inst_cap = nullptr;
+  g->inst_tmax = nullptr;
+  const size_t n = g->n_inst;
+  int rc;
+  if ((rc = dalloc(&g->ctl, n)) || (rc = dalloc(&g->dtbits, n)) || (rc = dalloc(&g->mrange, 2 * n)) ||
+      (rc = dalloc(&g->inst_cap, n)) || (rc = dalloc(&g->inst_tmax, n)))
+    return rc;
+  HIPCHK(hipMemset(g->ctl, 0, n * sizeof(Ctl)));
+  HIPCHK(hipMemset(g->mrange, 0, 2 * n * sizeof(int)));
+  HIPCHK(hipMemset(g->inst_cap, 0, n * sizeof(int)));
+  g->h_tmax.assign(n, 1e300);
+  return 0;
+}
+
+// Boundary codes from the caller's table: each instance gets its own copies
+// of the motions its DOFs use (the motion states depend on the instance's
+// time), at most kMaxMotions of them, addressed instance-locally by the node
+// codes (byte d = local index + 1).
+int rebuild_boundary(ft_engine *g) {
+  const int nn = g->nn;
+  std::vector<int> code(nn, 0);
+  std::vector<unsigned char> mask(nn, 0);
+  std::vector<double> mexp;
+  std::vector<int> minst, mr(2 * g->n_inst, 0);
+  const int nm = (int)(g->h_motions.size() / FT_MOTION_STRIDE);
+  std::vector<int> local(nm, -1);
+  for (int i = 0; i < g->n_inst; ++i) {
+    std::fill(local.begin(), local.end(), -1);
+    mr[2 * i] = (int)minst.size();
+    for (int n = g->inst_node[i]; n < g->inst_node[i + 1]; ++n)
+      for (int d = 0; d < 3; ++d) {
+        const int k = g->h_dof_motion.empty() ? -1 : g->h_dof_motion[3 * (size_t)n + d];
+        if (k < 0) continue;
+        if (local[k] < 0) {
+          if (mr[2 * i + 1] == kMaxMotions)
+            return fail(FT_ERR_ARG, "instance %d uses more than %d distinct motions", i, kMaxMotions);
+          local[k] = mr[2 * i + 1]++;
+          mexp.insert(mexp.end(), g->h_motions.begin() + FT_MOTION_STRIDE * (size_t)k,
+                      g->h_motions.begin() + FT_MOTION_STRIDE * (size_t)(k + 1));
+          minst.push_back(i);
+        }
+        mask[n] |= (unsigned char)(1 << d);
+        code[n] |= (local[k] + 1) << (8 * d);
+      }
+  }
+  g->n_motions = (int)minst.size();
+  g->timed_motions = false;
+  for (int k = 0; k < g->n_motions; ++k)
+    if ((int)mexp[FT_MOTION_STRIDE * (size_t)k] != FT_MOTION_FIXED) g->timed_motions = true;
+  for (void *p : {(void *)g->motions, (void *)g->motion_inst, (void *)g->mstate})
+    if (p) (void)hipFree(p);
+  g->motions = nullptr;
+  g->motion_inst = nullptr;
+  g->mstate = nullptr;
+  int rc;
+  if ((rc = dalloc(&g->motions, std::max<size_t>(1, mexp.size()))) ||
+      (rc = dalloc(&g->motion_inst, std::max<size_t>(1, minst.size()))) ||
+      (rc = dalloc(&g->mstate, 3 * std::max<size_t>(1, minst.size()))))
+    return rc;
+  if (!mexp.empty()) {
+    HIPCHK(hipMemcpy(g->motions, mexp.data(), mexp.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g->motion_inst, minst.data(), minst.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemset(g->mstate, 0, 3 * std::max<size_t>(1, minst.size()) * sizeof(MotionT)));
+  HIPCHK(hipMemcpy(g->mrange, mr.data(), mr.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->node_code, code.data(), nn * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g->node_mask, mask.data(), nn, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_elem_flags, dim3((g->ne + 255) / 256), dim3(256), 0, g->stream, g->ne, g->conn,
+                     g->node_mask, g->einfo);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+// Step caps of the next launches: cap[i] (host) for every instance.
+int upload_caps(ft_engine *g, const std::vector<int> &cap) {
+  HIPCHK(hipMemcpyAsync(g->inst_cap, cap.data(), cap.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+int read_slots(ft_engine *g, int parity, std::vector<StepSlot> &s) {
+  std::vector<Ctl> c(g->n_inst);
+  HIPCHK(hipMemcpyAsync(c.data(), g->ctl, c.size() * sizeof(Ctl), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  s.resize(g->n_inst);
+  for (int i = 0; i < g->n_inst; ++i) s[i] = c[i].slot[parity];
   return 0;
 }
 
@@ -1590,10 +1794,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       (rc = dalloc(&g->u, nd)) || (rc = dalloc(&g->vs, nd)) || (rc = dalloc(&g->fi, nd)) ||
       (rc = dalloc(&g->tmp3, nd)) || (rc = dalloc(&g->mass, (size_t)n_nodes)) ||
       (rc = dalloc(&g->node_code, (size_t)n_nodes)) || (rc = dalloc(&g->node_mask, (size_t)n_nodes)) ||
-      (rc = dalloc(&g->motions, (size_t)FT_MOTION_STRIDE * kMaxMotions)) ||
-      (rc = dalloc(&g->mstate, (size_t)3 * kMaxMotions)) ||
-      (rc = dalloc(&g->ctl, 1)) || (rc = dalloc(&g->dtbits, 1)) || (rc = dalloc(&g->status, 1)) ||
-      false)
+      (rc = dalloc(&g->status, 1)))
     return bail(rc);
   const size_t esz = precision == FT_FP32 ? 4 : 8;
   {
@@ -1614,7 +1815,6 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   if (g->has_visco && hipMalloc(&g->hist, 144 * (size_t)n_elems * esz) != hipSuccess)
     return bail(fail(FT_ERR_ALLOC, "history allocation failed"));
   g->nodal_grid = grid_for(n_nodes);
-  if ((rc = dalloc(&g->eacc, kEaccStride * (size_t)g->nodal_grid))) return bail(rc);
   // Group elements by material (stable counting sort on the material id).
   g->perm.resize(n_elems);
   g->inv.resize(n_elems);
@@ -1669,10 +1869,13 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   HIPCHK(hipMemset(g->node_code, 0, (size_t)n_nodes * sizeof(int)));
   HIPCHK(hipMemset(g->node_mask, 0, (size_t)n_nodes));
   HIPCHK(hipMemset(g->status, 0, sizeof(int)));
-  HIPCHK(hipMemset(g->ctl, 0, sizeof(Ctl)));
-  HIPCHK(hipMemset(g->mstate, 0, 3 * kMaxMotions * sizeof(MotionT)));
   if (g->hist) HIPCHK(hipMemset(g->hist, 0, 144 * (size_t)n_elems * esz));
   g->h_dof_motion.assign(nd, -1);
+  // one instance covering the whole mesh
+  g->n_inst = 1;
+  g->inst_node = {0, n_nodes};
+  g->inst_elem = {0, n_elems};
+  if ((rc = alloc_instances(g)) || (rc = build_tiles(g)) || (rc = rebuild_boundary(g))) return bail(rc);
   HIPCHK(hipDeviceSynchronize());
   *out = g;
   return 0;
@@ -1684,6 +1887,7 @@ int ft_engine_destroy(ft_engine *g) {
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
+                  g->etiles, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -1709,20 +1913,14 @@ int ft_engine_info(ft_engine *g, int *n_nodes, int *n_elems, int *precision) {
 
 int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const double *motions) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
-  if (n_motions < 0 || n_motions > kMaxMotions) return fail(FT_ERR_ARG, "at most %d motions", kMaxMotions);
+  if (n_motions < 0) return fail(FT_ERR_ARG, "negative motion count");
   if (n_motions > 0 && !motions) return fail(FT_ERR_ARG, "motions is null");
   const size_t nd = 3 * (size_t)g->nn;
   std::vector<int> dm(nd, -1);
-  std::vector<int> code(g->nn, 0);
-  std::vector<unsigned char> mask(g->nn, 0);
   for (size_t i = 0; i < nd; ++i) {
     int k = dof_motion ? dof_motion[i] : -1;
     if (k >= n_motions || k < -1) return fail(FT_ERR_ARG, "dof %zu: motion index %d out of range", i, k);
     dm[i] = k;
-    if (k >= 0) {
-      mask[i / 3] |= (unsigned char)(1 << (i % 3));
-      code[i / 3] |= (k + 1) << (8 * (i % 3));
-    }
   }
   for (int k = 0; k < n_motions; ++k) {
     int type = (int)motions[FT_MOTION_STRIDE * k];
@@ -1730,19 +1928,39 @@ int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const do
   }
   g->h_dof_motion.assign(dm.begin(), dm.end());
   g->h_motions.assign(motions, motions + FT_MOTION_STRIDE * (size_t)n_motions);
-  g->n_motions = n_motions;
-  g->timed_motions = false;
-  for (int k = 0; k < n_motions; ++k)
-    if ((int)motions[FT_MOTION_STRIDE * k] != FT_MOTION_FIXED) g->timed_motions = true;
-  HIPCHK(hipMemcpyAsync(g->node_code, code.data(), g->nn * sizeof(int), hipMemcpyHostToDevice, g->stream));
-  HIPCHK(hipMemcpyAsync(g->node_mask, mask.data(), g->nn, hipMemcpyHostToDevice, g->stream));
-  if (n_motions)
-    HIPCHK(hipMemcpyAsync(g->motions, motions, FT_MOTION_STRIDE * (size_t)n_motions * sizeof(double),
-                          hipMemcpyHostToDevice, g->stream));
-  hipLaunchKernelGGL(k_elem_flags, dim3((g->ne + 255) / 256), dim3(256), 0, g->stream, g->ne, g->conn,
-                     g->node_mask, g->einfo);
-  HIPCHK(hipGetLastError());
+  return rebuild_boundary(g);
+}
+
+int ft_set_instances(ft_engine *g, int n_inst, const int *node_offsets, const int *elem_offsets) {
+  if (!g || !node_offsets || !elem_offsets) return fail(FT_ERR_ARG, "null argument");
+  if (n_inst < 1) return fail(FT_ERR_ARG, "at least one instance");
+  if (node_offsets[0] != 0 || node_offsets[n_inst] != g->nn || elem_offsets[0] != 0 ||
+      elem_offsets[n_inst] != g->ne)
+    return fail(FT_ERR_ARG, "offsets must span all nodes and elements");
+  for (int i = 0; i < n_inst; ++i)
+    if (node_offsets[i + 1] <= node_offsets[i] || elem_offsets[i + 1] <= elem_offsets[i])
+      return fail(FT_ERR_ARG, "instance %d is empty or the offsets decrease", i);
+  for (int i = 0; i < n_inst; ++i)
+    for (long e = elem_offsets[i]; e < elem_offsets[i + 1]; ++e)
+      for (int k = 0; k < 8; ++k) {
+        const int n = g->h_conn[8 * e + k];
+        if (n < node_offsets[i] || n >= node_offsets[i + 1])
+          return fail(FT_ERR_ARG, "element %ld of instance %d uses node %d of another instance", e, i, n);
+      }
   HIPCHK(hipStreamSynchronize(g->stream));
+  g->n_inst = n_inst;
+  g->inst_node.assign(node_offsets, node_offsets + n_inst + 1);
+  g->inst_elem.assign(elem_offsets, elem_offsets + n_inst + 1);
+  g->run_active = false;
+  int rc;
+  if ((rc = alloc_instances(g)) || (rc = build_tiles(g)) || (rc = rebuild_boundary(g))) return rc;
+  return 0;
+}
+
+int ft_set_instance_end_times(ft_engine *g, const double *tmax) {
+  if (!g || !tmax) return fail(FT_ERR_ARG, "null argument");
+  g->h_tmax.assign(tmax, tmax + g->n_inst);
+  g->tmax_per_instance = true;
   return 0;
 }
 
@@ -1802,15 +2020,15 @@ int ft_stable_time_step(ft_engine *g, const double *u, const int *boundary, doub
                        g->node_mask, g->einfo);
     HIPCHK(hipGetLastError());
   }
-  const unsigned long long inf = kInfBits;
-  HIPCHK(hipMemcpyAsync(g->dtbits, &inf, sizeof(inf), hipMemcpyHostToDevice, g->stream));
-  if ((rc = element_pass(g, false, true, nullptr, 0.0, g->dtbits, 0))) return rc;
-  unsigned long long bits = 0;
-  HIPCHK(hipMemcpyAsync(&bits, g->dtbits, sizeof(bits), hipMemcpyDeviceToHost, g->stream));
+  std::vector<unsigned long long> bits(g->n_inst, kInfBits);
+  HIPCHK(hipMemcpyAsync(g->dtbits, bits.data(), bits.size() * sizeof(bits[0]), hipMemcpyHostToDevice, g->stream));
+  if ((rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0))) return rc;
+  HIPCHK(hipMemcpyAsync(bits.data(), g->dtbits, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
+  const unsigned long long bmin = *std::min_element(bits.begin(), bits.end());
   double dt;
-  memcpy(&dt, &bits, sizeof(dt));
-  if (bits == kInfBits) dt = 1e20;  // every element skipped: GlobalVariables.h huge
+  memcpy(&dt, &bmin, sizeof(dt));
+  if (bmin == kInfBits) dt = 1e20;  // every element skipped: GlobalVariables.h huge
   *dt_out = dt;
   if (dt < failure_dt) return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", dt);
   return 0;
@@ -1820,7 +2038,7 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = upload_u(g, u);
   if (rc) return rc;
-  if ((rc = element_pass(g, true, false, nullptr, dt, nullptr, update_history))) return rc;
+  if ((rc = element_pass(g, true, false, false, dt, nullptr, update_history))) return rc;
   if ((rc = check_status(g))) return rc;
   if (fe_out) {
     const size_t n = 24 * (size_t)(g->corner ? g->nn : g->ne);
@@ -1846,7 +2064,7 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = upload_u(g, u);
   if (rc) return rc;
-  if ((rc = element_pass(g, true, false, nullptr, dt, nullptr, 1))) return rc;
+  if ((rc = element_pass(g, true, false, false, dt, nullptr, 1))) return rc;
   {
     Timed t(g, 1);
     const bool w8 = g->width == 8;
@@ -1907,36 +2125,57 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
       }
     if ((rc = h2d_nodal(g, g->u, u.data())) || (rc = h2d_nodal(g, g->vs, v.data()))) return rc;
   }
-  HIPCHK(hipMemset(g->eacc, 0, kEaccStride * (size_t)g->nodal_grid * sizeof(double)));
+  HIPCHK(hipMemset(g->eacc, 0, kEaccStride * (size_t)std::max(1, g->n_ntiles) * sizeof(double)));
   HIPCHK(hipMemset(g->status, 0, sizeof(int)));
-  Ctl c;
-  memset(&c, 0, sizeof(c));
-  c.slot[0].dtmin_bits = kInfBits;
-  c.slot[1].dtmin_bits = kInfBits;
-  HIPCHK(hipMemcpy(g->ctl, &c, sizeof(c), hipMemcpyHostToDevice));
-  // dt_1 = factor * StableTimeStep() at u_0 (ex9.cpp:87)
+  if (!g->tmax_per_instance) g->h_tmax.assign(g->n_inst, tmax);
+  HIPCHK(hipMemcpy(g->inst_tmax, g->h_tmax.data(), g->n_inst * sizeof(double), hipMemcpyHostToDevice));
+  // dt_1 = factor * StableTimeStep() at u_0 (ex9.cpp:87), per instance
   int rc;
-  if ((rc = element_pass(g, false, true, nullptr, 0.0, &g->ctl->slot[0].dtmin_bits, 0))) return rc;
-  StepSlot s0;
-  if ((rc = read_slot(g, 0, &s0))) return rc;
-  double dtmin = 1e20;
-  if (s0.dtmin_bits != kInfBits) memcpy(&dtmin, &s0.dtmin_bits, sizeof(dtmin));
-  if (dtmin < failure_dt) return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", dtmin);
-  s0.t = 0.0;
-  s0.t_prev = 0.0;
-  s0.dt = dt_factor * dtmin;
-  memcpy(&s0.dtmin_bits, &dtmin, sizeof(dtmin));
-  s0.dt_guide = dtmin;
-  s0.step = 0;
-  s0.done = 0;
-  s0.fresh = 0;
-  HIPCHK(hipMemcpy(&g->ctl->slot[0], &s0, sizeof(s0), hipMemcpyHostToDevice));
+  std::vector<unsigned long long> bits(g->n_inst, kInfBits);
+  HIPCHK(hipMemcpy(g->dtbits, bits.data(), bits.size() * sizeof(bits[0]), hipMemcpyHostToDevice));
+  if ((rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0))) return rc;
+  HIPCHK(hipMemcpy(bits.data(), g->dtbits, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost));
+  std::vector<Ctl> c(g->n_inst);
+  for (int i = 0; i < g->n_inst; ++i) {
+    memset(&c[i], 0, sizeof(Ctl));
+    double dtmin = 1e20;
+    if (bits[i] != kInfBits) memcpy(&dtmin, &bits[i], sizeof(dtmin));
+    if (dtmin < failure_dt)
+      return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e (instance %d)", dtmin, i);
+    StepSlot &s0 = c[i].slot[0];
+    s0.t = 0.0;
+    s0.t_prev = 0.0;
+    s0.dt = dt_factor * dtmin;
+    memcpy(&s0.dtmin_bits, &dtmin, sizeof(dtmin));
+    s0.dt_guide = dtmin;
+    c[i].slot[1].dtmin_bits = kInfBits;
+  }
+  HIPCHK(hipMemcpy(g->ctl, c.data(), c.size() * sizeof(Ctl), hipMemcpyHostToDevice));
   // Initial GetForce + CalculateAccelerations (ex9.cpp:89-92); the Prony
   // update of this pass uses dt_1 like the reference's global dt.
-  if ((rc = element_pass(g, true, false, &g->ctl->slot[0], 0.0, nullptr, 1))) return rc;
-  if ((rc = launch_nodal(g, kInitial, 0))) return rc;  // stops: state of step 0
+  if ((rc = element_pass(g, true, false, true, 0.0, nullptr, 1))) return rc;
+  if ((rc = upload_caps(g, std::vector<int>(g->n_inst, 0)))) return rc;
+  if ((rc = launch_nodal(g, kInitial))) return rc;  // stops: state of step 0
   if ((rc = check_status(g))) return rc;
   g->run_active = true;
+  return 0;
+}
+
+// One step of every running instance: element pass (forces at u_k, the next
+// stable step), optional injury update, nodal update.
+int loop_body(ft_engine *g) {
+  int rc;
+  if ((rc = element_pass(g, true, true, true, 0.0, nullptr, 1))) return rc;
+  // u still holds this step's displacement here (the nodal kernel below
+  // advances it): the injury update of ex5.cpp:240 for t_k
+  if (g->inj_on && (rc = injury_launch(g, &g->ctl->slot[g->parity], 0.0, 0.0))) return rc;
+  if ((rc = launch_nodal(g, kStep))) return rc;
+  if (g->hist_cap > 0) {
+    Timed t(g, 2);
+    hipLaunchKernelGGL(k_history, dim3(1), dim3(kBlock), 0, g->stream, g->eacc, g->ntile_of_inst[1], g->ctl,
+                       g->parity, g->hist_rows, g->hist_cap);
+    HIPCHK(hipGetLastError());
+  }
   return 0;
 }
 
@@ -1945,35 +2184,35 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   if (steps_done) *steps_done = 0;
   if (!g->run_active) return fail(FT_ERR_STATE, "ft_run_init must run first");
   if (max_steps < 0) return fail(FT_ERR_ARG, "max_steps < 0");
-  StepSlot s;
+  std::vector<StepSlot> s;
   int rc;
-  if ((rc = read_slot(g, g->parity, &s))) return rc;
-  const int before = s.step;  // the slot is always stopped between calls
-  if (max_steps == 0 || !(s.t < g->tmax)) return 0;
-  const long target_l = (long)before + max_steps;
-  const int target = (int)std::min<long>(target_l, INT32_MAX - 1);
-  if ((rc = launch_nodal(g, kResume, target))) return rc;  // start step before+1
-  for (int k = before + 1; k <= target; ++k) {
-    if ((rc = element_pass(g, true, true, &g->ctl->slot[g->parity], 0.0,
-                           &g->ctl->slot[g->parity].dtmin_bits, 1)))
-      return rc;
-    // u still holds this step's displacement here (the nodal kernel below
-    // advances it): the injury update of ex5.cpp:240 for t_k
-    if (g->inj_on && (rc = injury_launch(g, &g->ctl->slot[g->parity], 0.0, 0.0))) return rc;
-    if ((rc = launch_nodal(g, kStep, target))) return rc;
-    if (g->hist_cap > 0) {
-      Timed t(g, 2);
-      hipLaunchKernelGGL(k_history, dim3(1), dim3(kBlock), 0, g->stream, g->eacc, g->nodal_grid,
-                         g->ctl, g->parity, g->hist_rows, g->hist_cap);
-      HIPCHK(hipGetLastError());
-    }
+  if ((rc = read_slots(g, g->parity, s))) return rc;  // all stopped between calls
+  std::vector<int> before(g->n_inst), cap(g->n_inst);
+  bool any = false;
+  for (int i = 0; i < g->n_inst; ++i) {
+    before[i] = s[i].step;
+    const bool runs = max_steps > 0 && s[i].t < g->h_tmax[i];
+    cap[i] = runs ? (int)std::min<long>((long)before[i] + max_steps, INT32_MAX - 1) : before[i];
+    any = any || runs;
   }
-  if ((rc = read_slot(g, g->parity, &s))) return rc;
-  if (steps_done) *steps_done = s.step - before;
+  if (!any) return 0;
+  if ((rc = upload_caps(g, cap))) return rc;
+  if ((rc = launch_nodal(g, kResume))) return rc;  // start step before + 1
+  for (int k = 0; k < max_steps; ++k)
+    if ((rc = loop_body(g))) return rc;
+  if ((rc = read_slots(g, g->parity, s))) return rc;
+  int done = 0;
+  for (int i = 0; i < g->n_inst; ++i) done = std::max(done, s[i].step - before[i]);
+  if (steps_done) *steps_done = done;
   int st = 0;
   HIPCHK(hipMemcpy(&st, g->status, sizeof(int), hipMemcpyDeviceToHost));
   if (st & 8) return fail(FT_ERR_MATERIAL, "Unknown material type");
-  if (st & FT_STATUS_DT_COLLAPSE) return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", s.dt_done / g->factor);
+  if (st & FT_STATUS_DT_COLLAPSE) {
+    for (int i = 0; i < g->n_inst; ++i)
+      if (s[i].dt_done / g->factor < g->failure_dt)
+        return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e (instance %d)", s[i].dt_done / g->factor, i);
+    return fail(FT_ERR_DT, "Timestep too small");
+  }
   return 0;
 }
 
@@ -1998,24 +2237,25 @@ int ft_get_state(ft_engine *g, double *u, double *v, double *a, double *fi) {
   if (v && (rc0 = d2h_nodal(g, v, g->vs))) return rc0;
   if (fi && (rc0 = d2h_nodal(g, fi, g->fi))) return rc0;
   if (a) {
-    StepSlot s;
-    int rc = read_slot(g, g->parity, &s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_state_out, dim3((g->nn + 255) / 256), dim3(256), 0, g->stream, g->nn, g->fi,
-                       g->mass, g->X, g->node_code, g->motions, s.t, g->tmp3);
+    int rc;
+    hipLaunchKernelGGL(k_state_out, dim3(std::max(1, g->n_ntiles)), dim3(kBlock), 0, g->stream, g->nn, g->ntiles,
+                       g->n_ntiles, g->ctl, g->parity, g->fi, g->mass, g->X, g->node_code, g->motions, g->mrange,
+                       g->tmp3);
     HIPCHK(hipGetLastError());
     if ((rc = d2h_nodal(g, a, g->tmp3))) return rc;
   }
   return 0;
 }
 
-int ft_get_energy(ft_engine *g, double out4[4]) {
-  if (!g || !out4) return fail(FT_ERR_ARG, "null argument");
-  std::vector<double> e(kEaccStride * (size_t)g->nodal_grid);
+namespace {
+// Energy sums of instance i: its node tiles' accumulators in tile order.
+int instance_energy(ft_engine *g, int i, double out4[4]) {
+  const int t0 = g->ntile_of_inst[i], t1 = g->ntile_of_inst[i + 1];
+  std::vector<double> e(kEaccStride * (size_t)(t1 - t0));
   HIPCHK(hipStreamSynchronize(g->stream));
-  HIPCHK(hipMemcpy(e.data(), g->eacc, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(e.data(), g->eacc + kEaccStride * (size_t)t0, e.size() * sizeof(double), hipMemcpyDeviceToHost));
   double w = 0, x = 0, k = 0;
-  for (int b = 0; b < g->nodal_grid; ++b) {
+  for (int b = 0; b < t1 - t0; ++b) {
     w += e[kEaccStride * b];
     x += e[kEaccStride * b + 1];
     k += e[kEaccStride * b + 2];
@@ -2026,10 +2266,31 @@ int ft_get_energy(ft_engine *g, double out4[4]) {
   out4[3] = fabs(k + w - x);
   return 0;
 }
+}  // namespace
+
+int ft_get_energy(ft_engine *g, double out4[4]) {
+  if (!g || !out4) return fail(FT_ERR_ARG, "null argument");
+  return instance_energy(g, 0, out4);
+}
+
+int ft_get_instance(ft_engine *g, int i, double out8[8]) {
+  if (!g || !out8) return fail(FT_ERR_ARG, "null argument");
+  if (i < 0 || i >= g->n_inst) return fail(FT_ERR_ARG, "instance %d out of range", i);
+  std::vector<Ctl> c(1);
+  HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipMemcpy(c.data(), g->ctl + i, sizeof(Ctl), hipMemcpyDeviceToHost));
+  const StepSlot &s = c[0].slot[g->parity];
+  out8[0] = s.t;
+  out8[1] = g->run_active ? s.dt_done : 0.0;
+  out8[2] = s.step;
+  out8[3] = (s.t >= g->h_tmax[i]) ? 1.0 : 0.0;
+  return instance_energy(g, i, out8 + 4);
+}
 
 int ft_set_history(ft_engine *g, int capacity) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   if (capacity < 0) return fail(FT_ERR_ARG, "capacity < 0");
+  if (capacity > 0 && g->n_inst != 1) return fail(FT_ERR_STATE, "the step history is a single-instance feature");
   HIPCHK(hipStreamSynchronize(g->stream));
   if (g->hist_rows) HIPCHK(hipFree(g->hist_rows));
   g->hist_rows = nullptr;
@@ -2213,6 +2474,7 @@ int ft_set_fibres(ft_engine *g, const double *dir) {
 
 int ft_injury_enable(ft_engine *g, const int *include_part) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (g->n_inst != 1) return fail(FT_ERR_STATE, "the injury criteria are a single-instance feature");
   const size_t ne = g->ne;
   const int nblk = (g->ne + kBlock - 1) / kBlock;
   int rc;

@@ -72,3 +72,37 @@ def ellipsoid_head(n=126, semi_axes=(0.090, 0.075, 0.065), peak=6000.0, duration
     fib /= np.linalg.norm(fib, axis=1)[:, None]
     return dict(X=X, conn=conn, pid=pid, material_id=np.array([3, 7, 8], np.int32), props=props,
                 dof_motion=dm, motions=motions, v0=None, fibres=fib.reshape(-1), radius=r)
+
+
+def concatenate(instances):
+    """One engine input from independent instances: nodes, elements, parts
+    and motions renumbered; returns (merged dict, node_offsets, elem_offsets)."""
+    X, conn, pid, mid, props, dm, motions, v0, fib = [], [], [], [], [], [], [], [], []
+    node_off, elem_off = [0], [0]
+    n_parts = n_motions = 0
+    any_fib = any(w.get("fibres") is not None for w in instances)
+    for w in instances:
+        nn, ne = w["X"].size // 3, w["conn"].size // 8
+        X.append(w["X"])
+        conn.append(w["conn"] + node_off[-1])
+        pid.append(w["pid"] + n_parts)
+        mid.append(w["material_id"])
+        props.append(np.asarray(w["props"]).reshape(-1, 9))
+        d = np.asarray(w["dof_motion"]).copy()
+        d[d >= 0] += n_motions
+        dm.append(d)
+        motions.append(np.asarray(w["motions"]).reshape(-1, 16))
+        v0.append(w["v0"] if w.get("v0") is not None else np.zeros(3 * nn))
+        if any_fib:
+            f = w.get("fibres")
+            fib.append(f if f is not None else np.tile([1.0, 0.0, 0.0], ne))
+        n_parts += len(w["material_id"])
+        n_motions += motions[-1].shape[0]
+        node_off.append(node_off[-1] + nn)
+        elem_off.append(elem_off[-1] + ne)
+    merged = dict(X=np.concatenate(X), conn=np.concatenate(conn).astype(np.int32),
+                  pid=np.concatenate(pid).astype(np.int32), material_id=np.concatenate(mid).astype(np.int32),
+                  props=np.concatenate(props), dof_motion=np.concatenate(dm).astype(np.int32),
+                  motions=np.concatenate(motions), v0=np.concatenate(v0),
+                  fibres=np.concatenate(fib) if any_fib else None)
+    return merged, np.array(node_off, np.int32), np.array(elem_off, np.int32)

@@ -130,8 +130,27 @@ int ft_get_time(ft_engine *eng, double *time, double *dt, int *step);
  * completed step (CheckEnergy-time values). */
 int ft_get_state(ft_engine *eng, double *u, double *v, double *a, double *fi);
 /* Energy balance (CheckEnergy.cpp:3-85): W_int, W_ext, KE, |KE+W_int-W_ext|
- * after the last completed step. */
+ * after the last completed step (of instance 0). */
 int ft_get_energy(ft_engine *eng, double out4[4]);
+
+/* ---- batched independent simulations (no reference counterpart) ---------
+ * The mesh given to ft_engine_create may hold several independent meshes
+ * ("instances"): instance i owns nodes [node_offsets[i], node_offsets[i+1])
+ * and elements [elem_offsets[i], elem_offsets[i+1]) in the caller's
+ * numbering, and its elements use only its nodes.  Each instance then has its
+ * own stable step, time and step count, energy balance and motion states
+ * (at most 64 distinct motions per instance); one launch per kernel advances
+ * all of them (blocks never span two instances).  ft_run_init starts every
+ * instance; ft_run_steps(n) advances every unfinished instance by up to n
+ * steps and reports the largest count.  ft_set_instance_end_times sets a
+ * per-instance end time (otherwise the tmax of ft_run_init).
+ * ft_get_instance: time, next dt, step, finished flag, W_int, W_ext, KE,
+ * |balance| of instance i.  Per-step history and the injury criteria are
+ * single-instance features. */
+int ft_set_instances(ft_engine *eng, int n_inst, const int *node_offsets,
+                     const int *elem_offsets);
+int ft_set_instance_end_times(ft_engine *eng, const double *tmax);
+int ft_get_instance(ft_engine *eng, int i, double out8[8]);
 /* Per-step history (time, dt, energy[4]) of the steps run after this call;
  * capacity 0 disables.  Costs one small reduction kernel per step. */
 int ft_set_history(ft_engine *eng, int capacity);

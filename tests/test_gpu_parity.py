@@ -383,3 +383,73 @@ def test_new_laws_element_forces_fp64(gpu_ready):
         for scale, seed in ((1e-3, 3), (8e-2, 4)):
             u = field(X.reshape(-1, 3), scale, seed)
             assert rel(e.element_forces(u, dt=1e-5), o.element_forces(u, 1e-5)) < 1e-11
+
+
+def batch_instances():
+    """Independent instances of different sizes, laws, motions and step
+    sizes: jittered impact blocks, a rotating ellipsoid and the
+    rotational-loading block."""
+    from femtech_amd import workloads as W
+    from cases import case_rotation
+    a = W.impact_block(n=4, slab=1, seed=11)
+    b = W.impact_block(n=6, slab=2, seed=12)
+    b["props"] = b["props"].copy()
+    b["props"][0, 1] *= 1.3  # stiffer: smaller dt
+    c = W.ellipsoid_head(n=8, peak=4e5, duration=2e-3)
+    r = case_rotation(n=4, steps=0)
+    r = {k: r[k] for k in ("X", "conn", "pid", "material_id", "props", "dof_motion", "motions")}
+    r["v0"] = None
+    return [a, b, c, r]
+
+
+def test_batched_instances_match_separate_runs(gpu_ready):
+    """Instances batched in one engine, each with its own stable step, time,
+    motions and energy, against the oracle run of each instance alone (FP64
+    engine; end times differ per instance)."""
+    from femtech_amd import workloads as W
+    insts = batch_instances()
+    m, no, eo = W.concatenate(insts)
+    tmax = np.array([4e-4, 3e-4, 2e-4, 5e-4])
+    steps = 400
+    with ft.Engine(m["X"], m["conn"], m["pid"], m["material_id"], m["props"], ft.FT_FP64) as e:
+        e.set_instances(no, eo)
+        e.set_fibres(m["fibres"])
+        e.set_boundary(m["dof_motion"], m["motions"])
+        e.set_state(None, m["v0"])
+        e.lumped_mass()
+        e.set_instance_end_times(tmax)
+        e.run_init(1e9, 0.9, 1e-11)
+        done = e.run_steps(100)
+        done += e.run_steps(steps)
+        u = e.state()[0]
+        info = [e.instance(i) for i in range(len(insts))]
+    assert done > 0
+    for i, w in enumerate(insts):
+        o = O.Model(w["X"], w["conn"], w["pid"], w["material_id"], w["props"])
+        if w.get("fibres") is not None:
+            o.set_fibres(w["fibres"])
+        o.set_bc(w["dof_motion"], w["motions"])
+        o.set_initial(None, w.get("v0"))
+        r = o.run(100000, tmax[i], 0.9, 1e-11)
+        assert info[i][3] == 1.0 and int(info[i][2]) == r["steps"], (i, info[i], r["steps"])
+        assert info[i][0] == pytest.approx(r["time"][-1], rel=1e-13)
+        ui = u[3 * no[i]:3 * no[i + 1]]
+        assert rel(ui, r["u"]) < 1e-10, i
+        assert rel(info[i][4:7], r["energy"][-1, :3]) < 1e-9, i
+
+
+def test_single_instance_declared_equals_default(gpu_ready):
+    """ft_set_instances with one instance changes nothing."""
+    z = golden("run_impact")
+    outs = []
+    for declare in (False, True):
+        with engine_for(z, ft.FT_FP32) as e:
+            if declare:
+                e.set_instances([0, z["X"].size // 3], [0, z["conn"].size // 8])
+                e.set_boundary(z["dof_motion"], z["motions"])
+            e.set_state(None, z["v0"])
+            e.lumped_mass()
+            e.run_init(1e9, float(z["dt_factor"]), float(z["failure_dt"]))
+            e.run_steps(50)
+            outs.append(e.state()[0])
+    assert np.array_equal(outs[0], outs[1])
