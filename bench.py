@@ -27,10 +27,16 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
-def workload(kind, n):
+def workload(kind, n, count=512):
+    """Engine input of the workload; a batch also carries node_off, elem_off,
+    end_times and info (per-instance kind, n, elements, peak, duration)."""
     from femtech_amd import workloads
     if kind == "impact":
         return workloads.impact_block(n=n or 100)
+    if kind == "batch":
+        w, no, eo, tmax, info = workloads.batch_instances(count=count)
+        w.update(node_off=no, elem_off=eo, end_times=tmax, info=info)
+        return w
     return workloads.ellipsoid_head(n=n or 126)
 
 
@@ -79,9 +85,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4900)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", choices=["impact", "ellipsoid"], default="impact",
+    ap.add_argument("--workload", choices=["impact", "ellipsoid", "batch"], default="impact",
                     help="impact: 1M-hex neo-Hookean block (the headline); ellipsoid: 2.0M-hex "
-                         "three-region ellipsoid (shell / Ogden + Prony / anisotropic HGO)")
+                         "three-region ellipsoid (shell / Ogden + Prony / anisotropic HGO); batch: "
+                         "independent blocks and ellipsoids of 2k-80k elements in one run")
+    ap.add_argument("--count", type=int, default=512, help="instances of the batch workload")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="steps per captured HIP graph (0: one launch per kernel)")
+    ap.add_argument("--autotune", type=int, default=5,
+                    help="timed launches per element-kernel shape in the autotuner (0: default shapes)")
+    ap.add_argument("--kernel-timing", type=int, default=50,
+                    help="N > 0: HIP events around the launches of every N-th step of the timed region "
+                         "(per-kernel averages; launches are not captured in graphs meanwhile); 0: none")
     ap.add_argument("--n", type=int, default=0, help="elements per edge (default 100 / 126)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-steps", type=int, default=10)
@@ -95,19 +110,28 @@ def main():
         sys.exit(2)
 
     import femtech_amd as ft
-    w = workload(args.workload, args.n)
+    w = workload(args.workload, args.n, args.count)
     ne, nn = w["conn"].size // 8, w["X"].size // 3
     prec = ft.FT_FP32 if args.precision == "fp32" else ft.FT_FP64
     e = ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], prec)
+    batch = args.workload == "batch"
+    if batch:
+        e.set_instances(w["node_off"], w["elem_off"])
+        e.set_instance_end_times(w["end_times"])
     if w.get("fibres") is not None:
         e.set_fibres(w["fibres"])
     e.set_boundary(w["dof_motion"], w["motions"])
     e.set_state(None, w["v0"])
     e.lumped_mass()
-    e.run_init(1e9, 0.9, 1e-11)
+    e.run_init(float(np.max(w["end_times"])) if batch else 1e9, 0.9, 1e-11)
+    tune = e.autotune(args.autotune) if args.autotune > 0 else None
+    e.set_graph(args.graph)
     if args.warmup:
         e.run_steps(args.warmup)
-    e.set_timing(1)
+    n_inst = len(w["info"]) if batch else 1
+    ne_inst = np.diff(w["elem_off"]) if batch else np.array([ne])
+    steps0 = np.array([e.instance(i)[2] for i in range(n_inst)]) if batch else None
+    e.set_timing(args.kernel_timing)
     e.time()  # synchronises the stream
     t0 = time.perf_counter()
     done = e.run_steps(args.steps)
@@ -117,15 +141,29 @@ def main():
     e.set_timing(0)
     E = e.energy()
     status = e.status()
-    assert done == args.steps, (done, args.steps)
     elapsed = t1 - t0
-    value = ne * args.steps / elapsed
+    if batch:
+        # every instance advances by its own dt; instances that reached their
+        # end time stop, so count the element-steps actually taken
+        steps1 = np.array([e.instance(i)[2] for i in range(n_inst)])
+        elem_steps = float(np.sum(ne_inst * (steps1 - steps0)))
+        running = int(np.sum(steps1 - steps0 == args.steps))
+    else:
+        assert done == args.steps, (done, args.steps)
+        elem_steps = float(ne) * args.steps
+    value = elem_steps / elapsed
     traffic = e.traffic_model()
     # one element pass per step (one launch per material segment)
-    elem_ms = ms[0] / max(launches[0], 1) * (launches[0] / max(launches[1], 1))
-    nodal_ms = ms[1] / max(launches[1], 1)
-    elem_gbs = traffic[0] / (elem_ms * 1e-3) / 1e9
-    nodal_gbs = traffic[1] / (nodal_ms * 1e-3) / 1e9
+    timed = args.kernel_timing and launches[1] > 0
+    # sampled steps: one nodal launch each, plus the resume launch of the run
+    # call; times summed over material segments give the element time of a
+    # step
+    sampled = max(int(launches[1]) - 1, 1)
+    elem_ms = ms[0] / sampled if timed else None
+    nodal_ms = ms[1] / max(launches[1], 1) if timed else None
+    elem_gbs = traffic[0] / (elem_ms * 1e-3) / 1e9 if timed else None
+    nodal_gbs = traffic[1] / (nodal_ms * 1e-3) / 1e9 if timed else None
+    frac = (lambda x: None if x is None else x / HBM_PEAK_GBS)
     step_bytes = traffic[0] + traffic[1]
     measured = calibrated_traffic(ne, args.precision) if args.workload == "impact" else None
     # Layout-independent compulsory traffic of one step at these precisions
@@ -135,6 +173,24 @@ def main():
     # v read/write 48 + mass 8 + BC flag 1 = 145 B, plus fi_prev read/write 48
     # for the FP64 energy balance.
     fixed_bytes = 260.0 * ne + (145.0 + 48.0) * nn
+    if args.workload == "impact":
+        config = {"workload": "1M-hex jittered neo-Hookean impact block (plain method)",
+                  "elements": ne, "nodes": nn, "edge_elements": args.n or 100,
+                  "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k}
+    elif args.workload == "ellipsoid":
+        config = {"workload": "2.0M-hex ellipsoid 90x75x65 mm: linear-elastic shell, Ogden + Prony, "
+                              "anisotropic HGO core; half-sine rotational surface loading (plain method)",
+                  "elements": ne, "nodes": nn, "edge_elements": args.n or 126,
+                  "simulated_time_s": t_sim, "step_index": k}
+    else:
+        kinds = [i[0] for i in w["info"]]
+        config = {"workload": "%d independent impact simulations in one run (jittered neo-Hookean blocks and "
+                              "three-region ellipsoids, 2k-80k elements each, moduli +-30%%, half-sine "
+                              "rotational pulses 2-8 krad/s^2 over 5-20 ms, 20 ms each, own dt each)" % n_inst,
+                  "instances": n_inst, "blocks": kinds.count("block"), "ellipsoids": kinds.count("ellipsoid"),
+                  "elements": ne, "nodes": nn, "min_instance_elements": int(ne_inst.min()),
+                  "max_instance_elements": int(ne_inst.max()), "running_whole_window": running,
+                  "element_steps": elem_steps}
     out = {
         "metric": "element-steps/s on 1M-hex neo-Hookean; surrogate vs plain at <=1% stress err",
         "value": value,
@@ -148,23 +204,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 element / f64 nodal" if prec == ft.FT_FP32 else "f64",
         "data": "synthetic",
-        "config": ({"workload": "1M-hex jittered neo-Hookean impact block (plain method)",
-                    "elements": ne, "nodes": nn, "edge_elements": args.n or 100,
-                    "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k}
-                   if args.workload == "impact" else
-                   {"workload": "2.0M-hex ellipsoid 90x75x65 mm: linear-elastic shell, Ogden + Prony, "
-                                "anisotropic HGO core; half-sine rotational surface loading (plain method)",
-                    "elements": ne, "nodes": nn, "edge_elements": args.n or 126,
-                    "simulated_time_s": t_sim, "step_index": k}),
+        "config": config,
         "roofline": {"bound": "hbm", "kernel": "k_element (all material segments of a step)",
                      "achieved": elem_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": elem_gbs / HBM_PEAK_GBS, "traffic": measured[0] if measured else None,
+                     "frac": frac(elem_gbs), "traffic": measured[0] if measured else None,
                      "algorithmic_bytes_per_launch": traffic[0],
                      "avg_launch_ms": elem_ms},
-        "nodal_kernel": {"achieved": nodal_gbs, "frac": nodal_gbs / HBM_PEAK_GBS,
+        "nodal_kernel": {"achieved": nodal_gbs, "frac": frac(nodal_gbs),
                          "traffic": measured[1] if measured else None,
                          "algorithmic_bytes_per_launch": traffic[1], "avg_launch_ms": nodal_ms},
-        "step_roofline_frac": fixed_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        "launch": {"graph_steps": args.graph, "kernel_timing_every": args.kernel_timing,
+                   "sampled_steps": sampled if timed else 0,
+                   "autotune": None if tune is None else
+                   [{"material": int(r[0]), "elements": int(r[1]), "waves": int(r[2]), "tile": int(r[3]),
+                     "best_us": r[4], "default_us": r[5]} for r in tune]},
+        "step_roofline_frac": fixed_bytes * elem_steps / ne / elapsed / 1e9 / HBM_PEAK_GBS,
         "step_bytes": {"compulsory_model": fixed_bytes, "this_layout": step_bytes,
                        "per_element_step_compulsory": fixed_bytes / ne},
         "energy": {"W_int": E[0], "W_ext": E[1], "KE": E[2], "balance": E[3]},
@@ -172,7 +226,7 @@ def main():
     }
     if args.copy_probe > 0:
         out["copy_probe_gbs"] = ft.copy_probe(args.copy_probe, 5)
-    if not args.no_cpu:
+    if not args.no_cpu and not batch:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, threads)
     print(json.dumps(out), flush=True)

@@ -68,6 +68,10 @@ _SIGS = {
     "ft_get_status": [_VP, _I],
     "ft_get_visco_history": [_VP, _D],
     "ft_set_timing": [_VP, ctypes.c_int],
+    "ft_set_graph": [_VP, ctypes.c_int],
+    "ft_autotune": [_VP, ctypes.c_int, _D, _I],
+    "ft_get_segments": [_VP, _I, _I],
+    "ft_set_segment_shape": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "ft_get_timing": [_VP, _D, _I],
     "ft_traffic_model": [_VP, _D],
     "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
@@ -376,10 +380,36 @@ class Engine:
         _check(self._lib.ft_get_visco_history(self._h, _dp(h)))
         return h
 
+    def set_graph(self, steps):
+        """Steps per captured HIP graph in run_steps (0: separate launches)."""
+        _check(self._lib.ft_set_graph(self._h, int(steps)))
+
+    def segments(self):
+        """Material segments: rows (material, begin, end, waves, tile)."""
+        n = ctypes.c_int(0)
+        _check(self._lib.ft_get_segments(self._h, ctypes.byref(n), None))
+        d = np.zeros(5 * n.value, np.int32)
+        _check(self._lib.ft_get_segments(self._h, ctypes.byref(n), _ip(d)))
+        return d.reshape(n.value, 5)
+
+    def set_segment_shape(self, segment, waves, tile):
+        _check(self._lib.ft_set_segment_shape(self._h, int(segment), int(waves), int(tile)))
+
+    def autotune(self, repeats=5):
+        """Times every element-kernel launch shape per material segment and
+        keeps the fastest.  Returns rows (material, elements, waves, tile,
+        best us, default-shape us)."""
+        n = len(self.segments())
+        rep = np.zeros(6 * n)
+        cnt = ctypes.c_int(0)
+        _check(self._lib.ft_autotune(self._h, int(repeats), _dp(rep), ctypes.byref(cnt)))
+        return rep.reshape(n, 6)
+
     def set_timing(self, enable):
         _check(self._lib.ft_set_timing(self._h, int(enable)))
 
     def timing(self):
+        """Summed ms and launch counts per class (element, nodal, other)."""
         ms = np.zeros(3)
         n = np.zeros(3, np.int32)
         _check(self._lib.ft_get_timing(self._h, _dp(ms), _ip(n)))

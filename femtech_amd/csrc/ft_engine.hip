@@ -67,6 +67,8 @@ constexpr int kSkipBit = 1 << 30;
 constexpr int kPidMask = (1 << 20) - 1;
 constexpr int kMaxMotions = 64;  // distinct motions per instance (held in LDS by the nodal kernel)
 constexpr int kEaccStride = 8;  // doubles per nodal block in the energy accumulators
+// launch classes of the timing: element, nodal/gather, other
+constexpr int kTimingClasses = 3;
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // walks one contiguous range of elements or nodes (its own L2 then sees the
@@ -104,7 +106,7 @@ struct StepSlot {
 
 struct Ctl {
   StepSlot slot[2];
-  int status;
+  int reserved;
   int hist_count;
   int pad[2];
 };
@@ -112,6 +114,16 @@ struct Ctl {
 constexpr int kStatusDtFromFp32 = FT_STATUS_DT_FP32;  // no element refined in FP64
 
 __device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Key of an element time step for the unsigned atomicMin: the bits of a
+// positive double order like the values; a step <= 0 (an inverted element:
+// negative volume) maps to +0, below every positive step, so the minimum
+// collapses as in the reference (dt < FailureTimeStep, StableTimeStep.cpp:
+// 33-38); NaN keeps the minimum.
+__device__ inline unsigned long long dt_key(double d) {
+  if (d > 0.0) return (unsigned long long)__double_as_longlong(d);
+  return d <= 0.0 ? 0ull : kInfBits;
+}
 
 template <typename T>
 __device__ inline T block_min(T v, T *sh) {
@@ -266,30 +278,61 @@ struct ElemArgs {
   int update_hist;
 };
 
-// Occupancy target of the FP32 instances of the laws written in closed form
-// (waves per SIMD; 4 = at most 128 VGPRs).  The Prony laws (5, 7), Ogden's
-// eigen-decomposition (6) and the fibre law (8) need more registers and the
-// FP64 instances are not held to it either.
+// Occupancy target (waves per SIMD; 4 = at most 128 VGPRs) of the default
+// instances: the FP32 laws written in closed form are held to 4; the Prony
+// laws (5, 7), Ogden's eigen-decomposition (6), the fibre law (8) and the
+// FP64 instances are not held.  WAVES > 0 overrides it (the FP32 run-loop
+// instances are compiled for WAVES = 1..5 and the tuner picks one per
+// material segment).
 #ifndef FT_ELEM_WAVES_FP32
 #define FT_ELEM_WAVES_FP32 4
 #endif
+template <typename R, int MAT, int WAVES>
+constexpr int elem_waves() {
+  return WAVES > 0 ? WAVES : ((sizeof(R) == 4 && MAT <= 4 && MAT != -1) ? FT_ELEM_WAVES_FP32 : 1);
+}
+constexpr int kMaxTuneWaves = 5;
+
+// Le / ce of element e from the FP64 coordinates and displacements
+// (StableTimeStep.cpp:22-27; the coefficients of x - x_0 as in the FP64
+// kernel's refinement of the reference's dt).
+template <typename R>
+__device__ double element_dt_fp64(const ElemArgs<R> &a, int e) {
+  const int *ce = a.conn + 8 * (size_t)e;
+  const int n0 = ce[0];
+  double c[7][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double *Xc = a.X + (size_t)d * a.nn, *Uc = a.u + (size_t)d * a.nn;
+    double v[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int n = ce[p2n(p)];
+      v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
+    }
+    wht8(v);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) c[m][d] = v[m + 1] * 0.125;
+  }
+  return char_length(c) / a.parts[a.einfo[e] & kPidMask].ce;
+}
+
+// Element tile L of a launch (one block; a tile holds at most blockDim.x =
+// 64, 128 or 256 elements).  Returns early, uniformly over the block, past
+// the last tile or when the instance has stopped.
 template <typename R, bool FORCE, bool DT, int MAT>
-__global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT <= 4 && MAT != -1) ? FT_ELEM_WAVES_FP32 : 1) void k_element(
-    ElemArgs<R> a) {
+__device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) {
   __shared__ double sh_min[kBlock / 64];
-  const int L = xcd_block();
   if (L >= a.n_tiles) return;  // grid padding (uniform over the block)
   const int4 tile = a.tiles[L];
   const int e = tile.x + threadIdx.x;
   const int end = tile.y;
   double dt_step = a.dt_fixed;
-  double dt_guide = 1e300;  // no previous minimum: refine every element
   unsigned long long *dtmin = a.dtbits ? a.dtbits + tile.z : nullptr;
   if (a.ctl) {
     const StepSlot *slot = &a.ctl[tile.z].slot[a.parity];
     if (slot->done) return;  // uniform over the block
     dt_step = slot->dt;
-    dt_guide = slot->dt_guide;
     dtmin = &a.ctl[tile.z].slot[a.parity].dtmin_bits;
   }
   double my_dt = 1e300;
@@ -350,49 +393,31 @@ __global__ __launch_bounds__(kBlock, (sizeof(R) == 4 && MAT <= 4 && MAT != -1) ?
   }
   if (DT) {
     const double bmin = block_min(my_dt, sh_min);
+    // the last minimum (read here: not held over the element computation)
+    const double dt_guide = a.ctl ? a.ctl[tile.z].slot[a.parity].dt_guide : 1e300;
     if (sizeof(R) == 4 && a.refine && bmin <= dt_guide * 1.001) {
-      // The step size sets the time grid and is sensitive to round-off: the
-      // block's minimising element (smallest id on ties) recomputes Le / ce
-      // from FP64 geometry (StableTimeStep.cpp:22-27).  Rare (blocks near the
-      // last minimum only) and spread over a few threads through LDS, so it
-      // adds no registers to the kernel.
-      __shared__ int sh_arg;
-      __shared__ double sh_c[7][3];
-      __shared__ double sh_a[7];
-      if (threadIdx.x == 0) sh_arg = INT32_MAX;
-      __syncthreads();
-      if (e < end && my_dt == bmin) atomicMin(&sh_arg, e);
-      __syncthreads();
-      const int es = sh_arg;
-      if (threadIdx.x < 3) {
-        const int d = threadIdx.x;
-        const double *Xc = a.X + (size_t)d * a.nn, *Uc = a.u + (size_t)d * a.nn;
-        const int n0 = a.conn[8 * (size_t)es];
-        double v[8];
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-          const int n = a.conn[8 * (size_t)es + p2n(p)];
-          v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
-        }
-        wht8(v);
-#pragma unroll
-        for (int m = 0; m < 7; ++m) sh_c[m][d] = v[m + 1] * 0.125;
-      }
-      __syncthreads();
-      if (threadIdx.x < 7) sh_a[threadIdx.x] = hex_part(sh_c, threadIdx.x);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double amax = 0.0;
-        for (int f = 0; f < 6; ++f)
-          if (sh_a[f] > amax) amax = sh_a[f];
-        const double dte = sh_a[6] / amax / a.parts[a.einfo[es] & kPidMask].ce;
-        atomicMin(dtmin, (unsigned long long)__double_as_longlong(dte));
-      }
+      // The step size sets the time grid and is sensitive to round-off: in
+      // blocks near the last minimum, every element within 4e-6 (about 32
+      // FP32 ulps) of the block's FP32 minimum recomputes Le / ce from the
+      // FP64 state (StableTimeStep.cpp:22-27).  The FP32 values of the
+      // step's minimiser and of its block's minimiser differ by less than
+      // twice the FP32 error of Le / ce, so the minimiser is among them and
+      // the step does not depend on the tiling.
+      __shared__ double sh_min2[kBlock / 64];
+      double d64 = 1e300;
+      if (e < end && my_dt <= bmin + fabs(bmin) * 4e-6) d64 = element_dt_fp64(a, e);
+      const double b64 = block_min(d64, sh_min2);
+      if (threadIdx.x == 0 && b64 < 1e300) atomicMin(dtmin, dt_key(b64));
     } else if (threadIdx.x == 0 && bmin < 1e300 && dtmin) {
-      atomicMin(dtmin, (unsigned long long)__double_as_longlong(bmin));
+      atomicMin(dtmin, dt_key(bmin));
     }
   }
   if (flags) atomicOr(a.status, flags);
+}
+
+template <typename R, bool FORCE, bool DT, int MAT, int WAVES>
+__global__ __launch_bounds__(kBlock, (elem_waves<R, MAT, WAVES>())) void k_element(ElemArgs<R> a) {
+  element_tile<R, FORCE, DT, MAT>(a, xcd_block());
 }
 
 // ---------------------------------------------------------------------------
@@ -495,6 +520,7 @@ struct NodalArgs {
   const double *tmax;     // per instance
   const int *cap;         // per instance: last step this call may complete
   double factor, failure_dt;
+  int *status;            // engine status word (FT_STATUS_* bits)
 };
 
 // End of step k (gather, a = f/m, v = v_half + dt/2 a, energy) fused with the
@@ -502,9 +528,8 @@ struct NodalArgs {
 // ex9.cpp:107-175.  vs holds v_half(k) for free DOFs and the prescribed v
 // for constrained ones while a run is in progress, v(k) once it stops.
 template <typename R, int W>
-__global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
+__device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   __shared__ double sh[kBlock / 64];
-  const int L = xcd_block();
   if (L >= a.n_tiles) return;  // grid padding
   const int4 tile = a.tiles[L];
   const int inst = tile.z;
@@ -642,16 +667,21 @@ __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
     } else {
       nx = cur;
       nx.done = 1;
-      if (collapse && cur.step < a.cap[inst] && t_k < a.tmax[inst]) atomicOr(&a.ctl->status, FT_STATUS_DT_COLLAPSE);
+      if (collapse && cur.step < a.cap[inst] && t_k < a.tmax[inst]) atomicOr(a.status, FT_STATUS_DT_COLLAPSE);
     }
     nx.t_done = t_k;
     nx.dt_done = dt_next;
     nx.dt_guide = dtmin;
-    if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(&a.ctl->status, kStatusDtFromFp32);
+    if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(a.status, kStatusDtFromFp32);
     nx.fresh = a.mode == kStep;
     nx.pad = 0;
     ic->slot[a.parity ^ 1] = nx;
   }
+}
+
+template <typename R, int W>
+__global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
+  nodal_tile<R, W>(a, xcd_block());
 }
 
 // Scalar state of every motion at t_{k-1}, t_k and t_{k+1} = t_k + dt_{k+1}
@@ -1232,7 +1262,9 @@ struct ft_engine {
   // material runs its own specialised element kernel over a contiguous range.
   std::vector<int> perm;  // internal -> original element index
   std::vector<int> inv;   // original -> internal
-  struct Seg { int mat, begin, end, tile_begin, tile_end; };
+  // waves: occupancy variant of the FP32 run-loop kernel (0: default);
+  // tile: elements per tile = threads per block.  Both set by ft_autotune.
+  struct Seg { int mat, begin, end, tile_begin = 0, tile_end = 0, waves = 0, tile = kBlock; };
   std::vector<Seg> segs;
   // Independent instances (batched simulations): node range and element
   // range of each in the caller's numbering; a single mesh is 1 instance.
@@ -1284,12 +1316,21 @@ struct ft_engine {
   // staging for the entry points that take host arrays every call
   double *aux = nullptr;
   size_t aux_len = 0;
-  // timing
+  // Run-loop chunks captured as HIP graphs, one per starting parity;
+  // graph_steps == 0 launches every step eagerly.
+  int graph_steps = 0;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // stream of the next launch (the engine's stream)
+  hipStream_t lstream = nullptr;
+  // timing: every `timing_every`-th step of the run loop is timed (sample)
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  int timing_every = 1;
+  long loop_iter = 0;
+  bool sample = true;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kTimingClasses];
   std::vector<hipEvent_t> ev_pool;
-  double ms[3] = {0, 0, 0};
-  int launches[3] = {0, 0, 0};
+  double ms[kTimingClasses] = {0, 0, 0};
+  int launches[kTimingClasses] = {0, 0, 0};
 };
 
 namespace {
@@ -1319,16 +1360,17 @@ struct Timed {
   ft_engine *g;
   int cls;
   hipEvent_t a = nullptr, b = nullptr;
-  Timed(ft_engine *g_, int c) : g(g_), cls(c) {
-    if (g->timing) {
+  hipStream_t st;
+  Timed(ft_engine *g_, int c) : g(g_), cls(c), st(g_->lstream) {
+    if (g->timing && g->sample) {
       a = take_event(g);
       b = take_event(g);
-      if (a) (void)hipEventRecord(a, g->stream);
+      if (a) (void)hipEventRecord(a, st);
     }
   }
   ~Timed() {
-    if (g->timing && a && b) {
-      (void)hipEventRecord(b, g->stream);
+    if (a && b) {
+      (void)hipEventRecord(b, st);
       g->ev[cls].push_back({a, b});
     }
   }
@@ -1336,7 +1378,7 @@ struct Timed {
 
 int collect_timing(ft_engine *g) {
   HIPCHK(hipStreamSynchronize(g->stream));
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kTimingClasses; ++c) {
     for (auto &pr : g->ev[c]) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
@@ -1348,6 +1390,16 @@ int collect_timing(ft_engine *g) {
     g->ev[c].clear();
   }
   return 0;
+}
+
+// Captured graphs hold kernel arguments (pointers, tile ranges, launch
+// shapes, run parameters): every change of those drops them.
+void drop_graphs(ft_engine *g) {
+  for (auto &x : g->gexec)
+    if (x) {
+      (void)hipGraphExecDestroy(x);
+      x = nullptr;
+    }
 }
 
 // The device keeps nodal arrays component-major ([3][nn]); the C ABI uses the
@@ -1379,10 +1431,9 @@ int d2h_nodal(ft_engine *g, double *aos, const double *dev) {
 // Element launches cover tile ranges [t0, t1) of g->etiles.  In the run loop
 // (in_loop) each tile reads its instance's step slot; elsewhere dt_fixed is
 // the step size and dtbits (per instance, nullable) collects the minimum.
-template <typename R, bool FORCE, bool DT, int MAT>
-int launch_element(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixed, unsigned long long *dtbits,
-                   int update_hist) {
-  if (t1 <= t0) return 0;
+template <typename R>
+ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixed, unsigned long long *dtbits,
+                      int update_hist) {
   ElemArgs<R> a;
   a.nn = g->nn;
   a.ne = g->ne;
@@ -1403,29 +1454,59 @@ int launch_element(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixed, 
   a.dt_fixed = dt_fixed;
   a.dtbits = in_loop ? nullptr : dtbits;
   a.status = g->status;
+  a.want_force = 1;
+  a.want_dt = 1;
+  a.update_hist = update_hist;
+  return a;
+}
+
+template <typename R, bool FORCE, bool DT, int MAT, int WAVES = 0>
+int launch_element(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixed, unsigned long long *dtbits,
+                   int update_hist, int block = kBlock) {
+  if (t1 <= t0) return 0;
+  ElemArgs<R> a = elem_args<R>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
   a.want_force = FORCE;
   a.want_dt = DT;
-  a.update_hist = update_hist;
   Timed t(g, 0);
-  hipLaunchKernelGGL((k_element<R, FORCE, DT, MAT>), dim3(grid_for_tiles(t1 - t0)), dim3(kBlock), 0, g->stream, a);
+  hipLaunchKernelGGL((k_element<R, FORCE, DT, MAT, WAVES>), dim3(grid_for_tiles(t1 - t0)), dim3(block), 0,
+                     g->lstream, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+// A material segment's force kernel with the segment's launch shape: its
+// tile size is the block size; its occupancy variant applies to the FP32
+// force + dt kernel (the run loop's), the others use the default.
+template <typename R, bool DT, int MAT>
+int force_segment_mat(ft_engine *g, const ft_engine::Seg &sg, int waves, bool in_loop, double dt_fixed,
+                      unsigned long long *dtbits, int update_hist) {
+  const int t0 = sg.tile_begin, t1 = sg.tile_end, b = sg.tile;
+  if constexpr (sizeof(R) == 4 && DT) {
+    switch (waves) {
+      case 1: return launch_element<R, true, DT, MAT, 1>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
+      case 2: return launch_element<R, true, DT, MAT, 2>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
+      case 3: return launch_element<R, true, DT, MAT, 3>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
+      case 4: return launch_element<R, true, DT, MAT, 4>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
+      case 5: return launch_element<R, true, DT, MAT, 5>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
+      default: break;
+    }
+  }
+  return launch_element<R, true, DT, MAT>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
+}
+
 template <typename R, bool DT>
-int force_segment(ft_engine *g, const ft_engine::Seg &sg, bool in_loop, double dt_fixed,
+int force_segment(ft_engine *g, const ft_engine::Seg &sg, int waves, bool in_loop, double dt_fixed,
                   unsigned long long *dtbits, int update_hist) {
-  const int t0 = sg.tile_begin, t1 = sg.tile_end;
   switch (sg.mat) {
-    case 0: return launch_element<R, true, DT, 0>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 1: return launch_element<R, true, DT, 1>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 2: return launch_element<R, true, DT, 2>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 3: return launch_element<R, true, DT, 3>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 4: return launch_element<R, true, DT, 4>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 5: return launch_element<R, true, DT, 5>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 6: return launch_element<R, true, DT, 6>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 7: return launch_element<R, true, DT, 7>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
-    case 8: return launch_element<R, true, DT, 8>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist);
+    case 0: return force_segment_mat<R, DT, 0>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 1: return force_segment_mat<R, DT, 1>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 2: return force_segment_mat<R, DT, 2>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 3: return force_segment_mat<R, DT, 3>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 4: return force_segment_mat<R, DT, 4>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 5: return force_segment_mat<R, DT, 5>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 6: return force_segment_mat<R, DT, 6>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 7: return force_segment_mat<R, DT, 7>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
+    case 8: return force_segment_mat<R, DT, 8>(g, sg, waves, in_loop, dt_fixed, dtbits, update_hist);
   }
   return fail(FT_ERR_MATERIAL, "Unknown material type %d", sg.mat);
 }
@@ -1434,10 +1515,17 @@ template <typename R>
 int element_pass_t(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fixed,
                    unsigned long long *dtbits, int update_hist) {
   // A stand-alone stable-time-step pass always runs in FP64 (exact minimum).
-  if (!force) return launch_element<double, false, true, -1>(g, 0, g->n_etiles, false, dt_fixed, dtbits, 0);
+  if (!force) {
+    for (const auto &sg : g->segs) {
+      int rc = launch_element<double, false, true, -1>(g, sg.tile_begin, sg.tile_end, false, dt_fixed, dtbits, 0,
+                                                       sg.tile);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   for (const auto &sg : g->segs) {
-    int rc = dt ? force_segment<R, true>(g, sg, in_loop, dt_fixed, dtbits, update_hist)
-                : force_segment<R, false>(g, sg, in_loop, dt_fixed, dtbits, update_hist);
+    int rc = dt ? force_segment<R, true>(g, sg, sg.waves, in_loop, dt_fixed, dtbits, update_hist)
+                : force_segment<R, false>(g, sg, sg.waves, in_loop, dt_fixed, dtbits, update_hist);
     if (rc) return rc;
   }
   return 0;
@@ -1451,8 +1539,9 @@ int element_pass(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fixe
 
 // Nodal launches read the per-instance step caps and end times from
 // g->inst_cap / g->inst_tmax (set by the host before a run call).
+// Node tiles [t0, t1) (their energy accumulators at the same positions).
 template <typename R>
-int launch_nodal_t(ft_engine *g, int mode) {
+NodalArgs<R> nodal_args(ft_engine *g, int mode, int t0, int t1) {
   NodalArgs<R> a;
   a.nn = g->nn;
   a.ell = g->ell;
@@ -1468,25 +1557,33 @@ int launch_nodal_t(ft_engine *g, int mode) {
   a.n_motions = g->n_motions;
   a.mrange = g->mrange;
   a.mstate = g->mstate;
-  a.tiles = g->ntiles;
-  a.n_tiles = g->n_ntiles;
+  a.tiles = g->ntiles + t0;
+  a.n_tiles = t1 - t0;
   a.ctl = g->ctl;
   a.parity = g->parity;
   a.mode = mode;
   a.energy = 1;
-  a.eacc = g->eacc;
+  a.eacc = g->eacc + kEaccStride * (size_t)t0;
   a.tmax = g->inst_tmax;
   a.cap = g->inst_cap;
   a.factor = g->factor;
   a.failure_dt = g->failure_dt;
+  a.status = g->status;
+  return a;
+}
+
+template <typename R>
+int launch_nodal_t(ft_engine *g, int mode, int t0, int t1) {
+  if (t1 <= t0) return 0;
+  const NodalArgs<R> a = nodal_args<R>(g, mode, t0, t1);
   Timed t(g, 1);
-  const dim3 grid(grid_for_tiles(g->n_ntiles));
+  const dim3 grid(grid_for_tiles(t1 - t0));
   if (g->corner)
-    hipLaunchKernelGGL((k_nodal<R, -1>), grid, dim3(kBlock), 0, g->stream, a);
+    hipLaunchKernelGGL((k_nodal<R, -1>), grid, dim3(kBlock), 0, g->lstream, a);
   else if (g->width == 8)
-    hipLaunchKernelGGL((k_nodal<R, 8>), grid, dim3(kBlock), 0, g->stream, a);
+    hipLaunchKernelGGL((k_nodal<R, 8>), grid, dim3(kBlock), 0, g->lstream, a);
   else
-    hipLaunchKernelGGL((k_nodal<R, 0>), grid, dim3(kBlock), 0, g->stream, a);
+    hipLaunchKernelGGL((k_nodal<R, 0>), grid, dim3(kBlock), 0, g->lstream, a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1495,11 +1592,12 @@ int launch_nodal(ft_engine *g, int mode) {
   if (g->timed_motions) {
     Timed t(g, 2);
     const int nb = std::max(1, std::min(64, (3 * g->n_motions + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_motion_states, dim3(nb), dim3(kBlock), 0, g->stream, g->ctl, g->parity, g->factor,
+    hipLaunchKernelGGL(k_motion_states, dim3(nb), dim3(kBlock), 0, g->lstream, g->ctl, g->parity, g->factor,
                        g->motions, g->motion_inst, g->n_motions, g->mstate);
     HIPCHK(hipGetLastError());
   }
-  int rc = g->precision == FT_FP32 ? launch_nodal_t<float>(g, mode) : launch_nodal_t<double>(g, mode);
+  const int nt = g->n_ntiles;
+  int rc = g->precision == FT_FP32 ? launch_nodal_t<float>(g, mode, 0, nt) : launch_nodal_t<double>(g, mode, 0, nt);
   if (rc) return rc;
   g->parity ^= 1;
   return 0;
@@ -1546,9 +1644,9 @@ int injury_launch(ft_engine *g, const StepSlot *slot, double time, double dt) {
   return 0;
 }
 
-// Tiles of at most kBlock consecutive entries that never span two
-// instances; elements per material segment (internal order), nodes in order.
-int build_tiles(ft_engine *g) {
+// Element tiles: at most sg.tile consecutive elements of one material
+// segment (internal order) and one instance.
+int build_element_tiles(ft_engine *g) {
   std::vector<int> inst_of_elem(g->ne);
   for (int i = 0; i < g->n_inst; ++i)
     for (int e = g->inst_elem[i]; e < g->inst_elem[i + 1]; ++e) inst_of_elem[e] = i;
@@ -1558,12 +1656,26 @@ int build_tiles(ft_engine *g) {
     for (int i = sg.begin; i < sg.end;) {
       const int inst = inst_of_elem[g->perm[i]];
       int j = i + 1;
-      while (j < sg.end && j - i < kBlock && inst_of_elem[g->perm[j]] == inst) ++j;
+      while (j < sg.end && j - i < sg.tile && inst_of_elem[g->perm[j]] == inst) ++j;
       et.push_back(make_int4(i, j, inst, 0));
       i = j;
     }
     sg.tile_end = (int)et.size();
   }
+  if (g->etiles) (void)hipFree(g->etiles);
+  g->etiles = nullptr;
+  g->n_etiles = (int)et.size();
+  int rc;
+  if ((rc = dalloc(&g->etiles, et.size()))) return rc;
+  HIPCHK(hipMemcpy(g->etiles, et.data(), et.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Tiles of consecutive entries that never span two instances: elements per
+// material segment, nodes (kBlock per tile) in order.
+int build_tiles(ft_engine *g) {
+  int rc;
+  if ((rc = build_element_tiles(g))) return rc;
   std::vector<int4> nt;
   g->ntile_of_inst.assign(g->n_inst + 1, 0);
   for (int i = 0; i < g->n_inst; ++i) {
@@ -1573,17 +1685,14 @@ int build_tiles(ft_engine *g) {
                              n == g->inst_node[i] ? kTileFirst : 0));
   }
   g->ntile_of_inst[g->n_inst] = (int)nt.size();
-  for (void *p : {(void *)g->etiles, (void *)g->ntiles, (void *)g->eacc})
+  for (void *p : {(void *)g->ntiles, (void *)g->eacc})
     if (p) (void)hipFree(p);
-  g->etiles = g->ntiles = nullptr;
+  g->ntiles = nullptr;
   g->eacc = nullptr;
-  g->n_etiles = (int)et.size();
   g->n_ntiles = (int)nt.size();
-  int rc;
-  if ((rc = dalloc(&g->etiles, et.size())) || (rc = dalloc(&g->ntiles, nt.size())) ||
+  if ((rc = dalloc(&g->ntiles, nt.size())) ||
       (rc = dalloc(&g->eacc, kEaccStride * (size_t)std::max(1, g->n_ntiles))))
     return rc;
-  HIPCHK(hipMemcpy(g->etiles, et.data(), et.size() * sizeof(int4), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->ntiles, nt.data(), nt.size() * sizeof(int4), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(g->eacc, 0, kEaccStride * (size_t)std::max(1, g->n_ntiles) * sizeof(double)));
   return 0;
@@ -1761,6 +1870,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
     return code;
   };
   if (hipStreamCreate(&g->stream) != hipSuccess) return bail(fail(FT_ERR_HIP, "stream creation failed"));
+  g->lstream = g->stream;
   // parts (CalculateTimeStep.cpp:13-19 wave speed kept in FP64)
   g->h_parts.resize(n_parts);
   for (int p = 0; p < n_parts; ++p) {
@@ -1884,6 +1994,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
 int ft_engine_destroy(ft_engine *g) {
   if (!g) return 0;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  drop_graphs(g);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
@@ -1892,7 +2003,7 @@ int ft_engine_destroy(ft_engine *g) {
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  for (int c = 0; c < 3; ++c)
+  for (int c = 0; c < kTimingClasses; ++c)
     for (auto &pr : g->ev[c]) {
       (void)hipEventDestroy(pr.first);
       (void)hipEventDestroy(pr.second);
@@ -1913,6 +2024,7 @@ int ft_engine_info(ft_engine *g, int *n_nodes, int *n_elems, int *precision) {
 
 int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const double *motions) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
   if (n_motions < 0) return fail(FT_ERR_ARG, "negative motion count");
   if (n_motions > 0 && !motions) return fail(FT_ERR_ARG, "motions is null");
   const size_t nd = 3 * (size_t)g->nn;
@@ -1933,6 +2045,7 @@ int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const do
 
 int ft_set_instances(ft_engine *g, int n_inst, const int *node_offsets, const int *elem_offsets) {
   if (!g || !node_offsets || !elem_offsets) return fail(FT_ERR_ARG, "null argument");
+  drop_graphs(g);
   if (n_inst < 1) return fail(FT_ERR_ARG, "at least one instance");
   if (node_offsets[0] != 0 || node_offsets[n_inst] != g->nn || elem_offsets[0] != 0 ||
       elem_offsets[n_inst] != g->ne)
@@ -2099,6 +2212,7 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
 
 int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
   if (!g->mass_ready) return fail(FT_ERR_STATE, "ft_assemble_lumped_mass must run before ft_run_init");
   if (!(dt_factor > 0)) return fail(FT_ERR_ARG, "dt_factor must be positive");
   g->tmax = tmax;
@@ -2164,6 +2278,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
 // One step of every running instance: element pass (forces at u_k, the next
 // stable step), optional injury update, nodal update.
 int loop_body(ft_engine *g) {
+  g->sample = (g->loop_iter++ % g->timing_every) == 0;
   int rc;
   if ((rc = element_pass(g, true, true, true, 0.0, nullptr, 1))) return rc;
   // u still holds this step's displacement here (the nodal kernel below
@@ -2175,6 +2290,30 @@ int loop_body(ft_engine *g) {
     hipLaunchKernelGGL(k_history, dim3(1), dim3(kBlock), 0, g->stream, g->eacc, g->ntile_of_inst[1], g->ctl,
                        g->parity, g->hist_rows, g->hist_cap);
     HIPCHK(hipGetLastError());
+  }
+  g->sample = true;
+  return 0;
+}
+
+// graph_steps loop bodies from the current parity, captured and
+// instantiated once (the loop issues no memset and no host synchronisation).
+int capture_chunk(ft_engine *g) {
+  const int p0 = g->parity;
+  HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < g->graph_steps && !rc; ++k) rc = loop_body(g);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+  g->parity = p0;
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? rc : fail(FT_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&g->gexec[p0], graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    g->gexec[p0] = nullptr;
+    return fail(FT_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
   }
   return 0;
 }
@@ -2198,7 +2337,15 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   if (!any) return 0;
   if ((rc = upload_caps(g, cap))) return rc;
   if ((rc = launch_nodal(g, kResume))) return rc;  // start step before + 1
-  for (int k = 0; k < max_steps; ++k)
+  int k = 0;
+  if (g->graph_steps > 0 && !g->timing) {
+    for (; k + g->graph_steps <= max_steps; k += g->graph_steps) {
+      if (!g->gexec[g->parity] && (rc = capture_chunk(g))) return rc;
+      HIPCHK(hipGraphLaunch(g->gexec[g->parity], g->stream));
+      g->parity ^= g->graph_steps & 1;
+    }
+  }
+  for (; k < max_steps; ++k)
     if ((rc = loop_body(g))) return rc;
   if ((rc = read_slots(g, g->parity, s))) return rc;
   int done = 0;
@@ -2289,6 +2436,7 @@ int ft_get_instance(ft_engine *g, int i, double out8[8]) {
 
 int ft_set_history(ft_engine *g, int capacity) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
   if (capacity < 0) return fail(FT_ERR_ARG, "capacity < 0");
   if (capacity > 0 && g->n_inst != 1) return fail(FT_ERR_STATE, "the step history is a single-instance feature");
   HIPCHK(hipStreamSynchronize(g->stream));
@@ -2353,12 +2501,152 @@ int ft_get_visco_history(ft_engine *g, double *hist) {
   return 0;
 }
 
+int ft_set_graph(ft_engine *g, int steps) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (steps < 0) return fail(FT_ERR_ARG, "steps < 0");
+  drop_graphs(g);
+  g->graph_steps = steps;
+  return 0;
+}
+
+// Element-kernel launch shapes by measurement on this mesh: per material
+// segment, every tile size (64, 128, 256 elements) and, for the FP32 kernel,
+// every occupancy variant (1..5 waves per SIMD) runs `repeats` timed
+// launches of the run loop's force + dt kernel; the fastest is kept.  The
+// launches write only the element forces (rewritten by every step), a
+// scratch minimum and a scratch copy of the Prony history (updated as in the
+// loop, so its traffic is part of the timing); the status word is restored,
+// so tuning between run calls leaves the run unchanged.
+int ft_autotune(ft_engine *g, int repeats, double *report, int *n_segments) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (repeats < 1) return fail(FT_ERR_ARG, "repeats < 1");
+  drop_graphs(g);
+  int rc;
+  if ((rc = collect_timing(g))) return rc;
+  const bool timing = g->timing;
+  g->timing = false;
+  int status0 = 0;
+  HIPCHK(hipMemcpy(&status0, g->status, sizeof(int), hipMemcpyDeviceToHost));
+  const bool fp32 = g->precision == FT_FP32;
+  // the history the timed launches update: a copy of the engine's
+  void *hist_run = g->hist, *hist_copy = nullptr;
+  if (hist_run) {
+    const size_t bytes = 144 * (size_t)g->ne * (fp32 ? 4 : 8);
+    if ((rc = dalloc((char **)&hist_copy, bytes))) {
+      g->timing = timing;
+      return rc;
+    }
+    if (hipMemcpy(hist_copy, hist_run, bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
+      (void)hipFree(hist_copy);
+      g->timing = timing;
+      return fail(FT_ERR_HIP, "history copy failed");
+    }
+    g->hist = hist_copy;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(FT_ERR_HIP, "event creation failed");
+  auto time_one = [&](size_t si, int waves, double &us) -> int {
+    const ft_engine::Seg sg = g->segs[si];
+    auto launch = [&]() {
+      return fp32 ? force_segment<float, true>(g, sg, waves, false, 1e-7, g->dtbits, 1)
+                  : force_segment<double, true>(g, sg, waves, false, 1e-7, g->dtbits, 1);
+    };
+    int r;
+    if ((r = launch())) return r;  // warm-up
+    HIPCHK(hipEventRecord(e0, g->stream));
+    for (int k = 0; k < repeats; ++k)
+      if ((r = launch())) return r;
+    HIPCHK(hipEventRecord(e1, g->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    us = 1e3 * ms / repeats;
+    return 0;
+  };
+  const int tiles[3] = {64, 128, 256};
+  const size_t ns = g->segs.size();
+  std::vector<double> rows(6 * ns, 0.0);
+  for (size_t si = 0; si < ns && !rc; ++si) {
+    auto &sg = g->segs[si];
+    double best = 1e300, t_def = 0;
+    int bw = 0, bt = kBlock;
+    for (int tile : tiles) {
+      sg.tile = tile;
+      if ((rc = build_element_tiles(g))) break;
+      if (tile == kBlock && (rc = time_one(si, 0, t_def))) break;
+      for (int w = fp32 ? 1 : 0; w <= (fp32 ? kMaxTuneWaves : 0) && !rc; ++w) {
+        double us = 0;
+        if ((rc = time_one(si, w, us))) break;
+        if (us < best) {
+          best = us;
+          bw = w;
+          bt = tile;
+        }
+      }
+      if (rc) break;
+    }
+    sg.tile = bt;
+    sg.waves = bw;
+    double *r = &rows[6 * si];
+    r[0] = sg.mat;
+    r[1] = sg.end - sg.begin;
+    r[2] = bw;
+    r[3] = bt;
+    r[4] = best;
+    r[5] = t_def;
+  }
+  if (!rc) rc = build_element_tiles(g);
+  if (hist_copy) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(hist_copy);
+    g->hist = hist_run;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  g->timing = timing;
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(g->status, &status0, sizeof(int), hipMemcpyHostToDevice));
+  if (report) memcpy(report, rows.data(), rows.size() * sizeof(double));
+  if (n_segments) *n_segments = (int)ns;
+  return 0;
+}
+
+int ft_get_segments(ft_engine *g, int *n_segments, int *desc) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (n_segments) *n_segments = (int)g->segs.size();
+  if (desc)
+    for (size_t i = 0; i < g->segs.size(); ++i) {
+      const auto &sg = g->segs[i];
+      int *d = desc + 5 * i;
+      d[0] = sg.mat;
+      d[1] = sg.begin;
+      d[2] = sg.end;
+      d[3] = sg.waves;
+      d[4] = sg.tile;
+    }
+  return 0;
+}
+
+int ft_set_segment_shape(ft_engine *g, int segment, int waves, int tile) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (segment < 0 || segment >= (int)g->segs.size()) return fail(FT_ERR_ARG, "segment %d out of range", segment);
+  if (waves < 0 || waves > kMaxTuneWaves || (waves > 0 && g->precision != FT_FP32))
+    return fail(FT_ERR_ARG, "occupancy variant %d not available", waves);
+  if (tile != 64 && tile != 128 && tile != 256) return fail(FT_ERR_ARG, "tile size must be 64, 128 or 256");
+  drop_graphs(g);
+  g->segs[segment].waves = waves;
+  g->segs[segment].tile = tile;
+  return build_element_tiles(g);
+}
+
 int ft_set_timing(ft_engine *g, int enable) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = collect_timing(g);
   if (rc) return rc;
   g->timing = enable != 0;
-  for (int c = 0; c < 3; ++c) {
+  g->timing_every = std::max(1, enable);
+  g->loop_iter = 0;
+  for (int c = 0; c < kTimingClasses; ++c) {
     g->ms[c] = 0;
     g->launches[c] = 0;
   }
@@ -2369,7 +2657,7 @@ int ft_get_timing(ft_engine *g, double ms[3], int launches[3]) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = collect_timing(g);
   if (rc) return rc;
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < kTimingClasses; ++c) {
     if (ms) ms[c] = g->ms[c];
     if (launches) launches[c] = g->launches[c];
   }
@@ -2452,6 +2740,7 @@ int ft_energy_terms(ft_engine *g, const double *u, const double *u_prev, const d
 
 int ft_set_fibres(ft_engine *g, const double *dir) {
   if (!g || !dir) return fail(FT_ERR_ARG, "null argument");
+  drop_graphs(g);
   const size_t ne = g->ne;
   const size_t esz = g->precision == FT_FP32 ? 4 : 8;
   if (!g->fib && hipMalloc(&g->fib, 3 * ne * esz) != hipSuccess)
@@ -2474,6 +2763,7 @@ int ft_set_fibres(ft_engine *g, const double *dir) {
 
 int ft_injury_enable(ft_engine *g, const int *include_part) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
   if (g->n_inst != 1) return fail(FT_ERR_STATE, "the injury criteria are a single-instance feature");
   const size_t ne = g->ne;
   const int nblk = (g->ne + kBlock - 1) / kBlock;
@@ -2517,6 +2807,7 @@ int ft_injury_enable(ft_engine *g, const int *include_part) {
 
 int ft_injury_disable(ft_engine *g) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
   g->inj_on = false;
   return 0;
 }

@@ -106,3 +106,63 @@ def concatenate(instances):
                   motions=np.concatenate(motions), v0=np.concatenate(v0),
                   fibres=np.concatenate(fib) if any_fib else None)
     return merged, np.array(node_off, np.int32), np.array(elem_off, np.int32)
+
+
+BATCH_SEED = 4242
+
+
+def batch_instances(count=512, ne_range=(2000, 80000), t_end=0.020, seed=BATCH_SEED):
+    """`count` independent impact simulations for one batched run.
+
+    Each instance is, with equal odds, a jittered hex block (h = 1 mm,
+    jitter 0.15, its own seed; compressible neo-Hookean) or a three-region
+    ellipsoid (the regions and laws of ellipsoid_head).  Its element count is
+    n^3 with n = round(Ne^(1/3)) and Ne drawn log-uniformly from ne_range.
+    Every modulus (neo-Hookean G and K; shell E; Ogden / HGO mu and K; HGO
+    k1) is scaled by its own uniform factor in [0.7, 1.3]; densities, the
+    Ogden exponent, the Prony terms, k2 and kappa keep their nominal values.
+    Loading: a half-sine angular acceleration about the x axis with peak
+    drawn from [2, 8] krad/s^2 and duration from [5, 20] ms, applied to the
+    ellipsoid's outer-surface nodes or to the block's base face (rotation
+    about the base face's centre line).  Every instance runs to t_end.
+
+    Returns (merged, node_offsets, elem_offsets, end_times, info) where info
+    lists (kind, n, elements, peak, duration) per instance."""
+    rng = np.random.default_rng(seed)
+    inst, info = [], []
+    lo, hi = np.log(ne_range[0]), np.log(ne_range[1])
+    for i in range(count):
+        ne_target = float(np.exp(rng.uniform(lo, hi)))
+        n = max(2, int(round(ne_target ** (1.0 / 3.0))))
+        peak = float(rng.uniform(2000.0, 8000.0))
+        duration = float(rng.uniform(0.005, 0.020))
+        f = rng.uniform(0.7, 1.3, size=4)
+        if rng.uniform() < 0.5:
+            w = impact_block(n=n, slab=0, seed=int(rng.integers(1, 2**62)))
+            G, K = 1.0e6 * f[0], 10.0e6 * f[1]
+            w["props"] = props_row(1000.0, G, K - 2.0 * G / 3.0)[None, :]
+            Xr = w["X"].reshape(-1, 3)
+            base = np.nonzero(Xr[:, 2] < 1e-9)[0]
+            dm = -np.ones(Xr.shape[0] * 3, np.int32)
+            for d in range(3):
+                dm[3 * base + d] = 0
+            c = 0.5 * n * 1e-3
+            w["dof_motion"] = dm
+            w["motions"] = motion(MOTION_ROT_HALFSINE, 1.0, 0.0, 0.0, 0.0, c, 0.0, peak, duration)[None, :]
+            w["v0"] = None
+            kind = "block"
+        else:
+            w = ellipsoid_head(n=n, peak=peak, duration=duration, seed=int(rng.integers(1, 2**62)))
+            p = np.array(w["props"])
+            E, nu = 6.0e9 * f[0], 0.22
+            p[0, 1], p[0, 2] = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+            mu, K = 530.0 * f[1], 2.1e6 * f[2]
+            for r in (1, 2):
+                p[r, 1], p[r, 2] = mu, K - 2 * mu / 3
+            p[2, 3] = 1.0e3 * f[3]
+            w["props"] = p
+            kind = "ellipsoid"
+        info.append((kind, n, w["conn"].size // 8, peak, duration))
+        inst.append(w)
+    merged, node_off, elem_off = concatenate(inst)
+    return merged, node_off, elem_off, np.full(count, t_end), info

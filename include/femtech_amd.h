@@ -208,9 +208,32 @@ int ft_injury_disable(ft_engine *eng);
 int ft_injury_update(ft_engine *eng, const double *u, double time, double dt);
 int ft_injury_results(ft_engine *eng, double out[21], int *flags);
 
+/* ---- launch shapes and captured steps ------------------------------------
+ * No counterpart in the reference (its loop is host code, ex9.cpp:107-175).
+ * ft_set_graph: ft_run_steps captures `steps` consecutive steps of the loop
+ * as one HIP graph (one per starting parity, built on first use) and
+ * launches it for every whole chunk of its step count; the rest run as
+ * separate launches.  0 (the default) launches every step separately.  Any
+ * call that changes the engine's configuration drops the captured graphs;
+ * timing (ft_set_timing) uses separate launches.
+ * ft_autotune: per material segment, times the run loop's element kernel at
+ * every tile size (64, 128, 256 elements per block) and, in FP32, every
+ * occupancy variant (1..5 waves per SIMD), `repeats` launches each, and keeps
+ * the fastest.  report (may be NULL): 6 doubles per segment = material,
+ * elements, waves, tile, best us per launch, default-shape us per launch.
+ * The run state is unchanged (the Prony history is not advanced).
+ * ft_get_segments: desc (may be NULL) gets 5 ints per segment = material,
+ * first and end element (internal, grouped order), waves (0 = default), tile.
+ * ft_set_segment_shape sets one segment's shape by hand. */
+int ft_set_graph(ft_engine *eng, int steps);
+int ft_autotune(ft_engine *eng, int repeats, double *report, int *n_segments);
+int ft_get_segments(ft_engine *eng, int *n_segments, int *desc);
+int ft_set_segment_shape(ft_engine *eng, int segment, int waves, int tile);
+
 /* ---- instrumentation -----------------------------------------------------
- * When enabled, every kernel launch of the loop is bracketed by HIP events on
- * the engine's stream.  Times are summed in milliseconds per kernel class
+ * When enabled (enable = N > 0), the kernel launches of every N-th step of
+ * the run loop, and every launch outside it, are bracketed by HIP events on
+ * their stream.  Times are summed in milliseconds per kernel class
  * (0 element, 1 nodal/gather, 2 other) together with launch counts. */
 int ft_set_timing(ft_engine *eng, int enable);
 int ft_get_timing(ft_engine *eng, double ms[3], int launches[3]);

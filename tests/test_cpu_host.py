@@ -146,3 +146,34 @@ def test_ellipsoid_generator(n):
     assert np.all((r >= 0) & (r <= 1.0 + 1e-12))
     assert set(np.unique(w["pid"])) <= {0, 1, 2}
     np.testing.assert_array_equal(w["pid"], np.where(r > 0.93, 0, np.where(r > 0.3, 1, 2)))
+
+
+def test_batch_generator():
+    """The batched workload: log-uniform sizes within range (n^3 elements),
+    instance-local renumbering, moduli within +-30 %, pulses within their
+    ranges, one rotational motion per instance acting on its own nodes."""
+    from femtech_amd import workloads as W
+    m, no, eo, tmax, info = W.batch_instances(count=24)
+    ne = np.diff(eo)
+    assert len(info) == 24 and np.all(tmax == 0.020)
+    assert np.all(ne == np.array([i[1] ** 3 for i in info]))
+    assert ne.min() >= 12 ** 3 and ne.max() <= 44 ** 3
+    assert {i[0] for i in info} == {"block", "ellipsoid"}
+    conn = m["conn"].reshape(-1, 8)
+    for k in range(24):
+        c = conn[eo[k]:eo[k + 1]]
+        assert c.min() >= no[k] and c.max() < no[k + 1]
+        d = m["dof_motion"][3 * no[k]:3 * no[k + 1]]
+        assert set(np.unique(d)) == {-1, k}
+        mo = m["motions"][k]
+        assert mo[0] == ft.MOTION_ROT_HALFSINE and 2000 <= mo[7] <= 8000 and 0.005 <= mo[8] <= 0.020
+    props = m["props"]
+    mid = m["material_id"]
+    nh = props[mid == 1]
+    assert np.all((nh[:, 1] >= 0.7e6) & (nh[:, 1] <= 1.3e6))
+    K = nh[:, 2] + 2 * nh[:, 1] / 3
+    assert np.all((K >= 7e6 * (1 - 1e-12)) & (K <= 13e6 * (1 + 1e-12)))
+    og = props[mid == 7]
+    assert np.all((og[:, 1] >= 0.7 * 530) & (og[:, 1] <= 1.3 * 530)) and np.all(og[:, 3] == -4.4)
+    hgo = props[mid == 8]
+    assert np.all((hgo[:, 3] >= 700) & (hgo[:, 3] <= 1300)) and np.all(hgo[:, 5] == 0.1)
