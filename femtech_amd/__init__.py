@@ -42,6 +42,7 @@ _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _VP = ctypes.c_void_p
+_F = ctypes.POINTER(ctypes.c_float)
 
 # name -> argtypes (all return int unless listed in _RET)
 _SIGS = {
@@ -91,6 +92,15 @@ _SIGS = {
     "ft_write_inp": [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, _I, _I],
     "ft_read_mesh": [ctypes.c_char_p, _I, _I, _D, _I, _I, _I],
     "ft_read_materials": [ctypes.c_char_p, ctypes.c_int, _I, _D],
+    "ft_mlp_param_count": [],
+    "ft_mlp_create": [ctypes.c_ulonglong, ctypes.POINTER(_VP)],
+    "ft_mlp_destroy": [_VP],
+    "ft_mlp_get_params": [_VP, _F],
+    "ft_mlp_set_params": [_VP, _F],
+    "ft_mlp_forward": [_VP, _F, ctypes.c_int, _F],
+    "ft_mlp_gradients": [_VP, _F, _F, ctypes.c_int, _D, _F],
+    "ft_mlp_train": [_VP, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                     ctypes.c_ulonglong, _D],
 }
 _RET = {"ft_last_error": ctypes.c_char_p}
 
@@ -419,3 +429,63 @@ class Engine:
         b = np.zeros(2)
         _check(self._lib.ft_traffic_model(self._h, _dp(b)))
         return b
+
+
+def _fp(a):
+    return a.ctypes.data_as(_F)
+
+
+class Mlp:
+    """The constitutive surrogate network on its own (BF16 matrix-core
+    kernels): 16 inputs (12 used) -> 4 x 128 SiLU -> 8 outputs (6 used)."""
+
+    def __init__(self, seed=1):
+        self._lib = library()
+        h = _VP()
+        _check(self._lib.ft_mlp_create(ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        self._h = h
+        self.n_params = self._lib.ft_mlp_param_count()
+
+    def close(self):
+        if self._h:
+            self._lib.ft_mlp_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def params(self):
+        p = np.empty(self.n_params, np.float32)
+        _check(self._lib.ft_mlp_get_params(self._h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _check(self._lib.ft_mlp_set_params(self._h, _fp(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        y = np.empty((x.shape[0], 8), np.float32)
+        _check(self._lib.ft_mlp_forward(self._h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 8)
+        g = np.empty(self.n_params, np.float32)
+        loss = ctypes.c_double(0)
+        _check(self._lib.ft_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0], ctypes.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, x, t, steps, batch=4096, lr=2e-3, seed=1):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 8)
+        loss = ctypes.c_double(0)
+        _check(self._lib.ft_mlp_train(self._h, _fp(x), _fp(t), x.shape[0], int(steps), int(batch), float(lr),
+                                      ctypes.c_ulonglong(seed), ctypes.byref(loss)))
+        return loss.value
+

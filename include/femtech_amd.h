@@ -230,6 +230,29 @@ int ft_autotune(ft_engine *eng, int repeats, double *report, int *n_segments);
 int ft_get_segments(ft_engine *eng, int *n_segments, int *desc);
 int ft_set_segment_shape(ft_engine *eng, int segment, int waves, int tile);
 
+/* ---- constitutive surrogate network --------------------------------------
+ * No counterpart in the reference.  A multilayer perceptron of 16 inputs (12
+ * used) -> 4 hidden layers of 128 SiLU units -> 8 outputs (6 used), with BF16
+ * weights and activations on the matrix cores and FP32 accumulation; FP32
+ * master weights and Adam moments.  Parameters: one FP32 vector of
+ * ft_mlp_param_count() values, per layer W [out][in] row-major then b [out].
+ * Host arrays: x [n][16] features, t and y [n][8].
+ * ft_mlp_forward: y = network(x) (one fused launch, all layers on chip).
+ * ft_mlp_gradients: mean squared error over the 6 used outputs on the batch
+ * and its gradient (same layout as the parameters).
+ * ft_mlp_train: `steps` Adam steps (cosine-decayed rate from lr) on batches
+ * drawn on the device from (x, t); final_loss: mean loss of the last 1 %. */
+typedef struct ft_mlp ft_mlp;
+int ft_mlp_param_count(void);
+int ft_mlp_create(unsigned long long seed, ft_mlp **out);
+int ft_mlp_destroy(ft_mlp *mlp);
+int ft_mlp_get_params(ft_mlp *mlp, float *params);
+int ft_mlp_set_params(ft_mlp *mlp, const float *params);
+int ft_mlp_forward(ft_mlp *mlp, const float *x, int n, float *y);
+int ft_mlp_gradients(ft_mlp *mlp, const float *x, const float *t, int n, double *loss, float *grad);
+int ft_mlp_train(ft_mlp *mlp, const float *x, const float *t, int n_data, int steps, int batch,
+                 double lr, unsigned long long seed, double *final_loss);
+
 /* ---- instrumentation -----------------------------------------------------
  * When enabled (enable = N > 0), the kernel launches of every N-th step of
  * the run loop, and every launch outside it, are bracketed by HIP events on

@@ -1,0 +1,85 @@
+"""The surrogate network's BF16 matrix-core kernels against the numpy
+restatement (same BF16 rounding points): the fused forward launch, the
+training kernels' loss and gradient, and training on the device."""
+import numpy as np
+import pytest
+
+import femtech_amd as ft
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 16)).astype(np.float32)
+    x[:, 12:] = 0.0
+    t = rng.normal(size=(n, 8)).astype(np.float32)
+    t[:, 6:] = 0.0
+    return x, t
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 200, 4097])
+def test_forward_matches_oracle(gpu_ready, n):
+    """All five layers in one launch (ragged sample counts: partial
+    wavefronts and column tiles)."""
+    x, _ = data(n, 1)
+    with ft.Mlp(seed=11) as net:
+        p = net.params().astype(np.float64)
+        y = net.forward(x)
+    ref = M.forward(p, x)
+    assert rel(y[:, :6], ref[:, :6]) < 2e-3
+    assert np.all(np.abs(y[:, 6:]) < 1e-30 + 1e-6 * np.abs(ref[:, :6]).max())
+
+
+def test_forward_with_trained_scale_weights(gpu_ready):
+    """Larger weights (saturating SiLU, wide activation range)."""
+    x, _ = data(512, 2)
+    rng = np.random.default_rng(3)
+    with ft.Mlp(seed=1) as net:
+        p = net.params() * 3.0
+        p[M.B_OFF[0]:M.B_OFF[0] + 128] = rng.normal(size=128)
+        net.set_params(p)
+        y = net.forward(x)
+    ref = M.forward(p.astype(np.float64), x)
+    assert rel(y[:, :6], ref[:, :6]) < 3e-3
+
+
+@pytest.mark.parametrize("n", [256, 1000])
+def test_gradients_match_oracle(gpu_ready, n):
+    """Loss and every layer's weight and bias gradient of the MFMA training
+    kernels against the restatement's."""
+    x, t = data(n, 4)
+    with ft.Mlp(seed=5) as net:
+        p = net.params().astype(np.float64)
+        loss, g = net.gradients(x, t)
+    lref, gref = M.loss_and_grad(p, x, t)
+    assert abs(loss / lref - 1) < 1e-3
+    for l in range(M.LAYERS):
+        w = slice(M.W_OFF[l], M.B_OFF[l])
+        b = slice(M.B_OFF[l], M.B_OFF[l] + M.layer_out(l))
+        assert rel(g[w], gref[w]) < 1e-2, ("W", l, rel(g[w], gref[w]))
+        assert rel(g[b], gref[b]) < 1e-2, ("b", l, rel(g[b], gref[b]))
+
+
+def test_training_fits_a_smooth_map(gpu_ready):
+    """Device training (gathered batches, Adam) fits a smooth 12 -> 6 map to
+    well below the error of the untrained network."""
+    rng = np.random.default_rng(9)
+    n = 200000
+    x = rng.uniform(-1, 1, size=(n, 16)).astype(np.float32)
+    x[:, 12:] = 0.0
+    A = rng.normal(size=(12, 6)) / 3
+    t = np.zeros((n, 8), np.float32)
+    t[:, :6] = np.tanh(x[:, :12] @ A) + 0.3 * (x[:, :6] ** 2)
+    with ft.Mlp(seed=2) as net:
+        y0 = net.forward(x[:4096])
+        loss = net.train(x, t, steps=1500, batch=4096, lr=2e-3, seed=3)
+        y1 = net.forward(x[:4096])
+    e0 = rel(y0[:, :6], t[:4096, :6])
+    e1 = rel(y1[:, :6], t[:4096, :6])
+    assert loss < 1e-3 and e1 < 0.05 and e1 < 0.1 * e0, (loss, e0, e1)
