@@ -55,6 +55,7 @@ _SIGS = {
     "ft_engine_info": [_VP, _I, _I, _I],
     "ft_set_boundary": [_VP, _I, ctypes.c_int, _D],
     "ft_set_state": [_VP, _D, _D],
+    "ft_reset_history": [_VP],
     "ft_assemble_lumped_mass": [_VP, _D],
     "ft_stable_time_step": [_VP, _D, _I, ctypes.c_double, _D],
     "ft_get_force": [_VP, _D, ctypes.c_double, _D],
@@ -92,6 +93,14 @@ _SIGS = {
     "ft_write_inp": [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, _I, _I],
     "ft_read_mesh": [ctypes.c_char_p, _I, _I, _D, _I, _I, _I],
     "ft_read_materials": [ctypes.c_char_p, ctypes.c_int, _I, _D],
+    "ft_surrogate_reserve": [_VP, ctypes.c_long],
+    "ft_surrogate_collect": [_VP, ctypes.POINTER(ctypes.c_long)],
+    "ft_surrogate_samples": [_VP, ctypes.c_long, ctypes.c_long, _F, _F],
+    "ft_surrogate_train": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _D],
+    "ft_surrogate_get": [_VP, _F, _F],
+    "ft_surrogate_set": [_VP, _F, _F],
+    "ft_surrogate_enable": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "ft_surrogate_stats": [_VP, _D],
     "ft_mlp_param_count": [],
     "ft_mlp_create": [ctypes.c_ulonglong, ctypes.POINTER(_VP)],
     "ft_mlp_destroy": [_VP],
@@ -414,6 +423,56 @@ class Engine:
         cnt = ctypes.c_int(0)
         _check(self._lib.ft_autotune(self._h, int(repeats), _dp(rep), ctypes.byref(cnt)))
         return rep.reshape(n, 6)
+
+    # ---- surrogate constitutive path (materials 7 and 8) ----
+    def surrogate_reserve(self, capacity):
+        _check(self._lib.ft_surrogate_reserve(self._h, int(capacity)))
+
+    def surrogate_collect(self):
+        """Appends the training samples of the current state; returns the
+        number held."""
+        n = ctypes.c_long(0)
+        _check(self._lib.ft_surrogate_collect(self._h, ctypes.byref(n)))
+        return n.value
+
+    def surrogate_samples(self, first, count):
+        x = np.empty((count, 16), np.float32)
+        t = np.empty((count, 8), np.float32)
+        _check(self._lib.ft_surrogate_samples(self._h, int(first), int(count), _fp(x), _fp(t)))
+        return x, t
+
+    def surrogate_train(self, steps, batch=8192, lr=2e-3, seed=1):
+        """Trains on the collected samples; returns (final loss, samples,
+        relative L2 error of S_iso on a spread subset, steps)."""
+        out = np.zeros(4)
+        _check(self._lib.ft_surrogate_train(self._h, int(steps), int(batch), float(lr), ctypes.c_ulonglong(seed),
+                                            _dp(out)))
+        return out
+
+    def surrogate_get(self):
+        p = np.empty(self._lib.ft_mlp_param_count(), np.float32)
+        nrm = np.empty(80, np.float32)
+        _check(self._lib.ft_surrogate_get(self._h, _fp(p), _fp(nrm)))
+        return p, nrm
+
+    def surrogate_set(self, params, norm):
+        p = np.ascontiguousarray(params, np.float32)
+        nrm = np.ascontiguousarray(norm, np.float32)
+        _check(self._lib.ft_surrogate_set(self._h, _fp(p), _fp(nrm)))
+
+    def surrogate_enable(self, enable=True, audit_every=100, audit_permille=10):
+        _check(self._lib.ft_surrogate_enable(self._h, int(enable), int(audit_every), int(audit_permille)))
+
+    def surrogate_stats(self):
+        """(audited rel. L2 error of S_iso, the same relative to the full
+        PK2, fallback share, audited, surrogate and fallback evaluations)."""
+        out = np.zeros(6)
+        _check(self._lib.ft_surrogate_stats(self._h, _dp(out)))
+        return out
+
+    def reset_history(self):
+        """Prony history back to zero (a new run from the same engine)."""
+        _check(self._lib.ft_reset_history(self._h))
 
     def set_timing(self, enable):
         _check(self._lib.ft_set_timing(self._h, int(enable)))

@@ -706,6 +706,254 @@ __host__ __device__ inline void accumulate_force_basis(R h[7], R q0, R q1, R q2,
   h[6] = Num<R>::fma_(sx * sy, q2, h[6]);
 }
 
+// ---------------------------------------------------------------------------
+// Constitutive surrogate of materials 7 and 8 (no reference counterpart).
+// The network replaces the isochoric part S_iso of the PK2 stress (Ogden's
+// eigen-decomposition, the fibre law); the volumetric part p C^-1 and the
+// Prony recurrence on S_iso stay analytic.  Inputs and outputs are in the
+// material frame: the fibre and a right-handed completion (material 8), the
+// reference axes (material 7).
+// ---------------------------------------------------------------------------
+constexpr int kSurrFeat = 16;   // features per sample (12 used)
+constexpr int kSurrOut = 8;     // outputs per sample (6 used)
+// normalisation block: in mean [16], in scale [16], box low [16], box high
+// [16] (raw features), out mean [8], out scale [8]
+constexpr int kSurrNorm = 80;
+
+template <typename R>
+__host__ __device__ inline void material_frame(int mat, const R fib[3], R Q[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Q[i][j] = i == j ? R(1) : R(0);
+  if (mat != 8 || !fib) return;
+  const R a[3] = {fib[0], fib[1], fib[2]};
+  // the reference axis least aligned with the fibre
+  const R ax = fabs(a[0]), ay = fabs(a[1]), az = fabs(a[2]);
+  const int k = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
+  R b[3] = {k == 1 ? -a[2] : (k == 2 ? a[1] : R(0)), k == 0 ? a[2] : (k == 2 ? -a[0] : R(0)),
+            k == 0 ? -a[1] : (k == 1 ? a[0] : R(0))};  // a x e_k
+  const R nb = Num<R>::sqrt_(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  b[0] /= nb;
+  b[1] /= nb;
+  b[2] /= nb;
+  const R c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Q[i][0] = a[i];
+    Q[i][1] = b[i];
+    Q[i][2] = c[i];
+  }
+}
+
+// Q^T S Q (to = true: reference -> material frame) or Q S Q^T.
+template <typename R>
+__host__ __device__ inline Sym<R> rotate_sym(const Sym<R> &S, const R Q[3][3], bool to_material) {
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+  Sym<R> out;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = I[k], j = J[k];
+    R acc = R(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const R qa = to_material ? Q[a][i] : Q[i][a];
+        const R qb = to_material ? Q[b][j] : Q[j][b];
+        acc += qa * sym_at(S, a, b) * qb;
+      }
+    out.v[k] = acc;
+  }
+  return out;
+}
+
+// F^-1 = adj(I + H) / J.
+template <typename R>
+__host__ __device__ inline void inverse_f(const R H[3][3], R Jm1, R Fi[3][3]) {
+  R F[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) F[i][j] = H[i][j] + (i == j ? R(1) : R(0));
+  const R iJ = Num<R>::rcp_(R(1) + Jm1);
+  Fi[0][0] = (F[1][1] * F[2][2] - F[1][2] * F[2][1]) * iJ;
+  Fi[0][1] = (F[0][2] * F[2][1] - F[0][1] * F[2][2]) * iJ;
+  Fi[0][2] = (F[0][1] * F[1][2] - F[0][2] * F[1][1]) * iJ;
+  Fi[1][0] = (F[1][2] * F[2][0] - F[1][0] * F[2][2]) * iJ;
+  Fi[1][1] = (F[0][0] * F[2][2] - F[0][2] * F[2][0]) * iJ;
+  Fi[1][2] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * iJ;
+  Fi[2][0] = (F[1][0] * F[2][1] - F[1][1] * F[2][0]) * iJ;
+  Fi[2][1] = (F[0][1] * F[2][0] - F[0][0] * F[2][1]) * iJ;
+  Fi[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * iJ;
+}
+
+// Isochoric PK2 stress S_iso = F^-1 tau_iso F^-T of material 7 (Ogden) or
+// 8 (anisotropic HGO) and the volumetric factor p = K/2 (J^2 - 1): the
+// elastic PK2 stress is S_iso + p C^-1 (as in pk2_stress and
+// kirchhoff_stress).
+template <typename R, int MAT>
+__host__ __device__ inline void iso_pk2(const PartDev &pp, const R H[3][3], R Jm1, const R fib[3], Sym<R> &Siso,
+                                        R &p) {
+  Sym<R> tau;
+  if (MAT == 7) {
+    ogden_tau(pp, b_minus_identity(H), Jm1, tau, p);
+  } else {
+    kirchhoff_stress<R, 8>(pp, H, Jm1, fib, tau);
+    const R K = R(pp.lambda) + R(2) * R(pp.mu) / R(3);
+    p = R(0.5) * K * Jm1 * (Jm1 + R(2));
+    tau.v[0] -= p;
+    tau.v[1] -= p;
+    tau.v[2] -= p;
+  }
+  R Fi[3][3];
+  inverse_f(H, Jm1, Fi);
+  const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int i = I[k], j = J[k];
+    R acc = R(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc += Fi[i][a] * sym_at(tau, a, b) * Fi[j][b];
+    Siso.v[k] = acc;
+  }
+}
+
+// Raw features of a Gauss point: Dbar = Cbar - I in the material frame
+// (Voigt 11, 22, 33, 23, 13, 12), J - 1, Ibar1 - 3, Ibar2 - 3, Ibar4 - 1
+// (frame axis 1: the fibre), Ibar6 - 1 (axis 2), the law (0: material 7,
+// 1: material 8); Cbar = J^-2/3 C formed from H without cancellation.
+template <typename R>
+__host__ __device__ inline void surr_features(const R H[3][3], R Jm1, const R Q[3][3], int mat, R f[kSurrFeat]) {
+  R D0[3][3];  // C - I = H + H^T + H^T H
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) D0[i][j] = H[i][j] + H[j][i] + (H[0][i] * H[0][j] + H[1][i] * H[1][j] + H[2][i] * H[2][j]);
+  Sym<R> D;
+  {
+    const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) D.v[k] = D0[I[k]][J[k]];
+  }
+  D = rotate_sym(D, Q, true);
+  const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * Num<R>::log1p_(Jm1));
+  const R Jm23 = R(1) + Jm23m1;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) D.v[k] *= Jm23;
+  D.v[0] += Jm23m1;
+  D.v[1] += Jm23m1;
+  D.v[2] += Jm23m1;
+  const R tr = D.v[0] + D.v[1] + D.v[2];
+  const R tr2 = D.v[0] * D.v[0] + D.v[1] * D.v[1] + D.v[2] * D.v[2] +
+                R(2) * (D.v[3] * D.v[3] + D.v[4] * D.v[4] + D.v[5] * D.v[5]);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) f[k] = D.v[k];
+  f[6] = Jm1;
+  f[7] = tr;
+  f[8] = R(2) * tr + R(0.5) * (tr * tr - tr2);
+  f[9] = D.v[0];
+  f[10] = D.v[1];
+  f[11] = mat == 8 ? R(1) : R(0);
+#pragma unroll
+  for (int k = 12; k < kSurrFeat; ++k) f[k] = R(0);
+}
+
+// Per-element inputs of the surrogate stress: the network outputs of its 8
+// Gauss points (normalised, stride gp_stride floats between Gauss points),
+// the normalisation block, and whether this element is audited.
+struct SurrIn {
+  const float *y;
+  size_t gp_stride;
+  const float *norm;
+  bool audit;
+};
+
+// Per-thread tallies: audited squared error of S_iso, squared norms of the
+// analytic S_iso and full PK2, fallback Gauss points, surrogate Gauss
+// points, audited Gauss points.
+struct SurrTally {
+  double v[6];
+};
+
+// PK2 stress of material 7 or 8 with the network's S_iso (the analytic law
+// outside the training box) plus p C^-1 and, for material 7, the Prony
+// update on S_iso (HGOIsotropicViscoelastic.cpp:94-159 semantics).
+template <typename R, int MAT>
+__host__ __device__ inline void surr_pk2(const PartDev &pp, const R H[3][3], const R fib[3], const SurrIn &in, int gp,
+                                         R (&hist)[18], bool use_hist, const ViscoCoef &vc, Sym<R> &S, R &Jm1,
+                                         SurrTally &t) {
+  Jm1 = det_ip1_m1(H);
+  R Q[3][3];
+  material_frame<R>(MAT, fib, Q);
+  R f[kSurrFeat];
+  surr_features<R>(H, Jm1, Q, MAT, f);
+  bool inside = true;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) inside = inside && f[k] >= R(in.norm[32 + k]) && f[k] <= R(in.norm[48 + k]);
+  const R K = R(pp.lambda) + R(2) * R(pp.mu) / R(3);
+  R p = R(0.5) * K * Jm1 * (Jm1 + R(2));
+  Sym<R> Siso;
+  if (inside) {
+    const float *y = in.y + (size_t)gp * in.gp_stride;
+    Sym<R> Sm;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Sm.v[k] = R(y[k]) * R(in.norm[72 + k]) + R(in.norm[64 + k]);
+    Siso = rotate_sym(Sm, Q, false);
+  } else {
+    iso_pk2<R, MAT>(pp, H, Jm1, fib, Siso, p);
+    t.v[3] += 1.0;
+  }
+  t.v[4] += 1.0;
+  R Fi[3][3];
+  inverse_f(H, Jm1, Fi);
+  Sym<R> Ci;
+  {
+    const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int i = I[k], j = J[k];
+      Ci.v[k] = Fi[i][0] * Fi[j][0] + Fi[i][1] * Fi[j][1] + Fi[i][2] * Fi[j][2];
+    }
+  }
+  if (in.audit) {
+    Sym<R> Sref;
+    R pr;
+    iso_pk2<R, MAT>(pp, H, Jm1, fib, Sref, pr);
+    double e2 = 0.0, r2 = 0.0, f2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double w = k < 3 ? 1.0 : 2.0;  // Frobenius norm of the symmetric tensor
+      const double d = (double)Siso.v[k] - (double)Sref.v[k];
+      const double full = (double)Sref.v[k] + (double)pr * (double)Ci.v[k];
+      e2 += w * d * d;
+      r2 += w * (double)Sref.v[k] * (double)Sref.v[k];
+      f2 += w * full * full;
+    }
+    t.v[0] += e2;
+    t.v[1] += r2;
+    t.v[2] += f2;
+    t.v[5] += 1.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) S.v[k] = Siso.v[k] + p * Ci.v[k];
+  if (MAT == 7 && use_hist) {
+    const R c11 = R(vc.c11), c12 = R(vc.c12), c21 = R(vc.c21), c22 = R(vc.c22);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      R d = Siso.v[k] - hist[12 + k];
+      R h1 = c11 * hist[k] + c21 * d;
+      R h2 = c12 * hist[6 + k] + c22 * d;
+      hist[k] = h1;
+      hist[6 + k] = h2;
+      hist[12 + k] = Siso.v[k];
+      S.v[k] = S.v[k] + h1 + h2;
+    }
+  }
+}
+
 // Status bits of element_eval (FT_STATUS_* of the C ABI plus an internal
 // "unknown material" bit).
 constexpr int kFlagNonFinite = 1;
@@ -718,11 +966,12 @@ constexpr int kFlagMaterial = 8;
 // lowers dt_e when DT is set and the element is not skipped.  `hist` is an
 // accessor with load(gp, k) / store(gp, k, v) for the 18 Prony values of a
 // Gauss point (material 5 only).
-template <typename R, bool FORCE, bool DT, int MAT, class Hist>
+template <typename R, bool FORCE, bool DT, int MAT, class Hist, bool SURR = false>
 __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartDev &pp,
                                              bool skip_dt, double dt_step, Hist &hist,
                                              bool update_hist, R outv[24], double &my_dt,
-                                             int &flags, const R fib[3] = nullptr) {
+                                             int &flags, const R fib[3] = nullptr,
+                                             const SurrIn *sin = nullptr, SurrTally *tally = nullptr) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     wht8(xv[d]);
@@ -797,7 +1046,7 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
 #pragma unroll
         for (int j = 0; j < 3; ++j)
           H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
-      if constexpr (MAT == 1 || MAT == 4 || MAT == 6 || MAT == 8) {
+      if constexpr ((MAT == 1 || MAT == 4 || MAT == 6 || MAT == 8) && !SURR) {
         // Spatial form: f_a = sum_gp det0 P J0^-T dN_a/dxi with P = tau F^-T
         // and F J0 = Jx (the deformed Jacobian), so Q = det0 tau Jx^-T =
         // tau adj(Jx)^T / J.
@@ -840,7 +1089,11 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
       for (int k = 0; k < 18; ++k) hloc[k] = use_hist ? hist.load(gp, k) : R(0);
       Sym<R> S;
       R Jm1;
-      if (!pk2_stress<R, MAT>(pp, H, hloc, use_hist, vc, S, Jm1)) flags |= kFlagMaterial;
+      if constexpr (SURR) {
+        surr_pk2<R, MAT>(pp, H, fib, *sin, gp, hloc, use_hist, vc, S, Jm1, *tally);
+      } else {
+        if (!pk2_stress<R, MAT>(pp, H, hloc, use_hist, vc, S, Jm1)) flags |= kFlagMaterial;
+      }
       if (!(Jm1 > R(-1))) flags |= kFlagInverted;
       if (use_hist && update_hist) {
 #pragma unroll

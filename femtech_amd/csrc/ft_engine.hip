@@ -26,6 +26,7 @@
 
 #include "../../include/femtech_amd.h"
 #include "ft_device.h"
+#include "ft_mlp.h"
 
 using namespace ft;
 
@@ -276,6 +277,17 @@ struct ElemArgs {
   int want_force;
   int want_dt;
   int update_hist;
+  // constitutive surrogate (materials 7, 8): network outputs of this
+  // segment's Gauss points [8][n_seg][8], its first element, the
+  // normalisation block, audit switch and share, per-tile tallies
+  const float *surr_y;
+  size_t surr_stride;
+  int surr_e0;
+  const float *surr_norm;
+  int surr_audit;
+  unsigned surr_permille;
+  double *surr_tally;
+  int tile0;
 };
 
 // Occupancy target (waves per SIMD; 4 = at most 128 VGPRs) of the default
@@ -320,7 +332,13 @@ __device__ double element_dt_fp64(const ElemArgs<R> &a, int e) {
 // Element tile L of a launch (one block; a tile holds at most blockDim.x =
 // 64, 128 or 256 elements).  Returns early, uniformly over the block, past
 // the last tile or when the instance has stopped.
-template <typename R, bool FORCE, bool DT, int MAT>
+// Elements audited against the analytic law (surrogate runs): a fixed
+// pseudo-random share of permille / 1000.
+__device__ inline bool audited(int e, unsigned permille) {
+  return ((unsigned)e * 2654435761u >> 8) % 1000u < permille;
+}
+
+template <typename R, bool FORCE, bool DT, int MAT, bool SURR = false>
 __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) {
   __shared__ double sh_min[kBlock / 64];
   if (L >= a.n_tiles) return;  // grid padding (uniform over the block)
@@ -337,6 +355,7 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
   }
   double my_dt = 1e300;
   int flags = 0;
+  SurrTally tally = {{0, 0, 0, 0, 0, 0}};
   if (e < end) {
     const int4 c0 = reinterpret_cast<const int4 *>(a.conn)[2 * e];
     const int4 c1 = reinterpret_cast<const int4 *>(a.conn)[2 * e + 1];
@@ -374,8 +393,15 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       fib[1] = a.fib[(size_t)a.ne + e];
       fib[2] = a.fib[2 * (size_t)a.ne + e];
     }
-    element_eval<R, FORCE, DT, MAT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
-                               a.update_hist != 0, outv, my_dt, flags, fib);
+    if constexpr (SURR) {
+      const SurrIn sin{a.surr_y + (size_t)(e - a.surr_e0) * kSurrOut, a.surr_stride, a.surr_norm,
+                       a.surr_audit != 0 && audited(e, a.surr_permille)};
+      element_eval<R, FORCE, DT, MAT, HistSoA<R>, true>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
+                                                       a.update_hist != 0, outv, my_dt, flags, fib, &sin, &tally);
+    } else {
+      element_eval<R, FORCE, DT, MAT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
+                                 a.update_hist != 0, outv, my_dt, flags, fib);
+    }
     if (FORCE) {
       // Slot-major layouts: fe[slot][element][xyz] or fe[corner][node][xyz].
       // Each of the 8 stores of a wavefront covers one contiguous run (for
@@ -412,12 +438,155 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       atomicMin(dtmin, dt_key(bmin));
     }
   }
+  if constexpr (SURR) {
+    // per-tile tallies (each tile is one block's: no atomics)
+    __shared__ double sh_t[kBlock / 64];
+    for (int k = 0; k < 6; ++k) {
+      const double v = block_sum(tally.v[k], sh_t);
+      if (threadIdx.x == 0) a.surr_tally[(size_t)(a.tile0 + L) * 6 + k] += v;
+    }
+  }
   if (flags) atomicOr(a.status, flags);
 }
 
 template <typename R, bool FORCE, bool DT, int MAT, int WAVES>
 __global__ __launch_bounds__(kBlock, (elem_waves<R, MAT, WAVES>())) void k_element(ElemArgs<R> a) {
   element_tile<R, FORCE, DT, MAT>(a, xcd_block());
+}
+
+// Run-loop element kernel of a surrogate segment (FP32, force + dt).
+template <int MAT>
+__global__ __launch_bounds__(kBlock) void k_element_surr(ElemArgs<float> a) {
+  element_tile<float, true, true, MAT, true>(a, xcd_block());
+}
+
+// grad_X u at Gauss point gp from the normalised coefficients of x and u
+// (the Jacobians of element_eval).
+template <typename R>
+__device__ inline void gp_grad_u(const R cx[7][3], const R cu[7][3], int gp, R H[3][3]) {
+  const R g = R(kGp), g2 = R(kGp * kGp);
+  const R sx = ((gp & 3) == 1 || (gp & 3) == 2) ? R(1) : R(-1);
+  const R sy = (gp & 2) ? R(1) : R(-1);
+  const R sz = gp < 4 ? R(1) : R(-1);
+  R J0[3][3], Ju[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    J0[d][0] = cx[0][d] + (sy * g) * cx[2][d] + (sz * g) * cx[4][d] + (sy * sz * g2) * cx[6][d];
+    J0[d][1] = cx[1][d] + (sx * g) * cx[2][d] + (sz * g) * cx[5][d] + (sx * sz * g2) * cx[6][d];
+    J0[d][2] = cx[3][d] + (sx * g) * cx[4][d] + (sy * g) * cx[5][d] + (sx * sy * g2) * cx[6][d];
+    Ju[d][0] = cu[0][d] + (sy * g) * cu[2][d] + (sz * g) * cu[4][d] + (sy * sz * g2) * cu[6][d];
+    Ju[d][1] = cu[1][d] + (sx * g) * cu[2][d] + (sz * g) * cu[5][d] + (sx * sz * g2) * cu[6][d];
+    Ju[d][2] = cu[3][d] + (sx * g) * cu[4][d] + (sy * g) * cu[5][d] + (sx * sy * g2) * cu[6][d];
+  }
+  R A[3][3];
+  A[0][0] = J0[1][1] * J0[2][2] - J0[1][2] * J0[2][1];
+  A[0][1] = J0[0][2] * J0[2][1] - J0[0][1] * J0[2][2];
+  A[0][2] = J0[0][1] * J0[1][2] - J0[0][2] * J0[1][1];
+  A[1][0] = J0[1][2] * J0[2][0] - J0[1][0] * J0[2][2];
+  A[1][1] = J0[0][0] * J0[2][2] - J0[0][2] * J0[2][0];
+  A[1][2] = J0[0][2] * J0[1][0] - J0[0][0] * J0[1][2];
+  A[2][0] = J0[1][0] * J0[2][1] - J0[1][1] * J0[2][0];
+  A[2][1] = J0[0][1] * J0[2][0] - J0[0][0] * J0[2][1];
+  A[2][2] = J0[0][0] * J0[1][1] - J0[0][1] * J0[1][0];
+  const R det0 = J0[0][0] * A[0][0] + J0[0][1] * A[1][0] + J0[0][2] * A[2][0];
+  const R idet0 = R(1) / det0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
+}
+
+// Coefficients of x - x_0 and u - u_0 of element e (node-0 relative, FP64
+// differences rounded to R) and its fibre.
+template <typename R>
+__device__ inline void element_coeffs(int nn, int ne, const int *conn, const double *X, const double *u,
+                                      const float *fibf, const double *fibd, int e, R cx[7][3], R cu[7][3],
+                                      R fib[3]) {
+  const int *ce = conn + 8 * (size_t)e;
+  const int n0 = ce[0];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double *Xc = X + (size_t)d * nn, *Uc = u + (size_t)d * nn;
+    R xv[8], uv[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int n = ce[p2n(p)];
+      xv[p] = p == 0 ? R(0) : R(Xc[n] - Xc[n0]);
+      uv[p] = p == 0 ? R(0) : R(Uc[n] - Uc[n0]);
+    }
+    wht8(xv);
+    wht8(uv);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      cx[m][d] = xv[m + 1] * R(0.125);
+      cu[m][d] = uv[m + 1] * R(0.125);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    fib[d] = fibd ? R(fibd[(size_t)d * ne + e]) : (fibf ? R(fibf[(size_t)d * ne + e]) : (d == 0 ? R(1) : R(0)));
+}
+
+// Training samples of the surrogate at the current displacement: raw
+// features and the analytic S_iso in the material frame (FP64 arithmetic) of
+// every Gauss point of elements [e0, e1) of material MAT, Gauss-point major
+// from `out0` (out index out0 + gp (e1 - e0) + e - e0).
+template <int MAT>
+__global__ __launch_bounds__(kBlock) void k_surr_sample(int nn, int ne, const int *conn, const double *X,
+                                                        const double *u, const float *fibf, const double *fibd,
+                                                        const PartDev *parts, const int *einfo, int e0, int e1,
+                                                        long out0, float *xs, float *ts) {
+  const int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
+  double cx[7][3], cu[7][3], fib[3];
+  element_coeffs<double>(nn, ne, conn, X, u, fibf, fibd, e, cx, cu, fib);
+  const PartDev pp = parts[einfo[e] & kPidMask];
+  double Q[3][3];
+  material_frame<double>(MAT, fib, Q);
+  for (int gp = 0; gp < 8; ++gp) {
+    double H[3][3];
+    gp_grad_u<double>(cx, cu, gp, H);
+    const double Jm1 = det_ip1_m1(H);
+    double f[kSurrFeat];
+    surr_features<double>(H, Jm1, Q, MAT, f);
+    Sym<double> Siso;
+    double p;
+    iso_pk2<double, MAT>(pp, H, Jm1, fib, Siso, p);
+    const Sym<double> Sm = rotate_sym(Siso, Q, true);
+    const size_t o = (size_t)out0 + (size_t)gp * (e1 - e0) + (e - e0);
+#pragma unroll
+    for (int k = 0; k < kSurrFeat; ++k) xs[o * kSurrFeat + k] = (float)f[k];
+#pragma unroll
+    for (int k = 0; k < kSurrOut; ++k) ts[o * kSurrOut + k] = k < 6 ? (float)Sm.v[k] : 0.0f;
+  }
+}
+
+// Network inputs of a surrogate segment's Gauss points at the run loop's
+// displacement (FP32, as the element kernel forms them), normalised, BF16,
+// Gauss-point major.
+template <int MAT>
+__global__ __launch_bounds__(kBlock) void k_surr_features(int nn, int ne, const int *conn, const double *X,
+                                                          const double *u, const float *fibf, int e0, int e1,
+                                                          const float *norm, __bf16 *feat, const Ctl *ctl,
+                                                          int parity) {
+  const int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
+  if (ctl[0].slot[parity].done) return;
+  float cx[7][3], cu[7][3], fib[3];
+  element_coeffs<float>(nn, ne, conn, X, u, fibf, nullptr, e, cx, cu, fib);
+  float Q[3][3];
+  material_frame<float>(MAT, fib, Q);
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    float H[3][3];
+    gp_grad_u<float>(cx, cu, gp, H);
+    const float Jm1 = det_ip1_m1(H);
+    float f[kSurrFeat];
+    surr_features<float>(H, Jm1, Q, MAT, f);
+    __bf16 *dst = feat + ((size_t)gp * (e1 - e0) + (e - e0)) * kSurrFeat;
+#pragma unroll
+    for (int k = 0; k < kSurrFeat; ++k) dst[k] = (__bf16)((f[k] - norm[k]) / norm[16 + k]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1236,6 +1405,55 @@ __global__ void k_state_out(int nn, const int4 *tiles, int n_tiles, const Ctl *c
   }
 }
 
+// Per (column, chunk): sum, sum of squares, min, max of column `blockIdx.x`
+// of x[n][w] over rows [chunk len, +len) -> part[(chunk w + col) 4 + k].
+__global__ __launch_bounds__(kBlock) void k_col_stats(const float *x, long n, int w, long len, double *part) {
+  __shared__ double sh[kBlock / 64];
+  const int col = blockIdx.x;
+  const long r0 = (long)blockIdx.y * len, r1 = r0 + len < n ? r0 + len : n;
+  double s = 0, q = 0, lo = 1e300, hi = -1e300;
+  for (long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const double v = x[r * w + col];
+    s += v;
+    q += v * v;
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+  s = block_sum(s, sh);
+  q = block_sum(q, sh);
+  __syncthreads();
+  lo = block_min(lo, sh);
+  __syncthreads();
+  hi = -block_min(-hi, sh);
+  if (threadIdx.x == 0) {
+    double *o = part + ((size_t)blockIdx.y * w + col) * 4;
+    o[0] = s;
+    o[1] = q;
+    o[2] = lo;
+    o[3] = hi;
+  }
+}
+
+// Normalised training arrays: x -> BF16 (x - mean) / scale, t -> (t - mean) / scale.
+__global__ void k_surr_normalise(const float *xs, const float *ts, long n, const float *norm, __bf16 *xb, float *tb) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < kSurrFeat; ++k) xb[i * kSurrFeat + k] = (__bf16)((xs[i * kSurrFeat + k] - norm[k]) / norm[16 + k]);
+#pragma unroll
+  for (int k = 0; k < kSurrOut; ++k) tb[i * kSurrOut + k] = (ts[i * kSurrOut + k] - norm[64 + k]) / norm[72 + k];
+}
+
+// Every stride-th sample of (xb, tb) into (vx, vt), m samples.
+__global__ void k_gather_stride(const __bf16 *xb, const float *tb, long stride, long m, __bf16 *vx, float *vt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+#pragma unroll
+  for (int k = 0; k < kSurrFeat; ++k) vx[i * kSurrFeat + k] = xb[i * stride * kSurrFeat + k];
+#pragma unroll
+  for (int k = 0; k < kSurrOut; ++k) vt[i * kSurrOut + k] = tb[i * stride * kSurrOut + k];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1322,6 +1540,22 @@ struct ft_engine {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   // stream of the next launch (the engine's stream)
   hipStream_t lstream = nullptr;
+  // Constitutive surrogate of materials 7 and 8 (FP32 run loop): network,
+  // normalisation block, per-step inputs and outputs of the surrogate
+  // segments' Gauss points, audit schedule and per-tile tallies, training
+  // samples.
+  ft::MlpNet *mlp = nullptr;
+  bool surr_ready = false, surr_on = false;
+  float *surr_norm = nullptr;
+  std::vector<float> h_surr_norm;
+  __bf16 *surr_feat = nullptr;
+  float *surr_y = nullptr;
+  long surr_gps = 0;
+  int surr_audit_every = 100, surr_permille = 10;
+  long surr_iter = 0;
+  double *surr_tally = nullptr;
+  float *ds_x = nullptr, *ds_t = nullptr;
+  long ds_n = 0, ds_cap = 0;
   // timing: every `timing_every`-th step of the run loop is timed (sample)
   bool timing = false;
   int timing_every = 1;
@@ -1457,6 +1691,14 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.want_force = 1;
   a.want_dt = 1;
   a.update_hist = update_hist;
+  a.surr_y = nullptr;
+  a.surr_stride = 0;
+  a.surr_e0 = 0;
+  a.surr_norm = nullptr;
+  a.surr_audit = 0;
+  a.surr_permille = 0;
+  a.surr_tally = nullptr;
+  a.tile0 = t0;
   return a;
 }
 
@@ -1531,8 +1773,68 @@ int element_pass_t(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fi
   return 0;
 }
 
+bool is_surr_mat(int mat) { return mat == 7 || mat == 8; }
+
+// The run loop's element pass with the surrogate: network inputs of every
+// surrogate segment, one inference launch over all their Gauss points, then
+// each segment's element kernel (surrogate stress, or the plain kernel for
+// the other laws).
+int element_pass_surr(ft_engine *g) {
+  const bool audit = g->surr_iter % g->surr_audit_every == 0;
+  g->surr_iter += 1;
+  long off = 0;
+  const float *fibf = reinterpret_cast<const float *>(g->fib);
+  {
+    Timed t(g, 2);
+    for (const auto &sg : g->segs) {
+      if (!is_surr_mat(sg.mat)) continue;
+      const int n = sg.end - sg.begin;
+      __bf16 *feat = g->surr_feat + (size_t)off * kSurrFeat;
+      const dim3 grid((n + kBlock - 1) / kBlock);
+      if (sg.mat == 7)
+        hipLaunchKernelGGL(k_surr_features<7>, grid, dim3(kBlock), 0, g->lstream, g->nn, g->ne, g->conn, g->X, g->u,
+                           fibf, sg.begin, sg.end, g->surr_norm, feat, g->ctl, g->parity);
+      else
+        hipLaunchKernelGGL(k_surr_features<8>, grid, dim3(kBlock), 0, g->lstream, g->nn, g->ne, g->conn, g->X, g->u,
+                           fibf, sg.begin, sg.end, g->surr_norm, feat, g->ctl, g->parity);
+      HIPCHK(hipGetLastError());
+      off += 8L * n;
+    }
+    int rc = ft::mlp_infer(g->mlp, g->surr_feat, g->surr_gps, g->surr_y, g->lstream);
+    if (rc) return rc;
+  }
+  off = 0;
+  for (const auto &sg : g->segs) {
+    int rc;
+    if (!is_surr_mat(sg.mat)) {
+      if ((rc = force_segment<float, true>(g, sg, sg.waves, true, 0.0, nullptr, 1))) return rc;
+      continue;
+    }
+    const int n = sg.end - sg.begin;
+    ElemArgs<float> a = elem_args<float>(g, sg.tile_begin, sg.tile_end, true, 0.0, nullptr, 1);
+    a.surr_y = g->surr_y + (size_t)off * kSurrOut;
+    a.surr_stride = (size_t)n * kSurrOut;
+    a.surr_e0 = sg.begin;
+    a.surr_norm = g->surr_norm;
+    a.surr_audit = audit ? 1 : 0;
+    a.surr_permille = (unsigned)g->surr_permille;
+    a.surr_tally = g->surr_tally;
+    a.tile0 = sg.tile_begin;
+    Timed t(g, 0);
+    const dim3 grid(grid_for_tiles(sg.tile_end - sg.tile_begin));
+    if (sg.mat == 7)
+      hipLaunchKernelGGL(k_element_surr<7>, grid, dim3(sg.tile), 0, g->lstream, a);
+    else
+      hipLaunchKernelGGL(k_element_surr<8>, grid, dim3(sg.tile), 0, g->lstream, a);
+    HIPCHK(hipGetLastError());
+    off += 8L * n;
+  }
+  return 0;
+}
+
 int element_pass(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fixed, unsigned long long *dtbits,
                  int update_hist) {
+  if (g->surr_on && force && dt && in_loop && update_hist) return element_pass_surr(g);
   if (g->precision == FT_FP32) return element_pass_t<float>(g, force, dt, in_loop, dt_fixed, dtbits, update_hist);
   return element_pass_t<double>(g, force, dt, in_loop, dt_fixed, dtbits, update_hist);
 }
@@ -1995,6 +2297,10 @@ int ft_engine_destroy(ft_engine *g) {
   if (!g) return 0;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   drop_graphs(g);
+  if (g->mlp) ft::mlp_destroy(g->mlp);
+  for (void *p : {(void *)g->surr_norm, (void *)g->surr_feat, (void *)g->surr_y, (void *)g->surr_tally,
+                  (void *)g->ds_x, (void *)g->ds_t})
+    if (p) (void)hipFree(p);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
@@ -2074,6 +2380,16 @@ int ft_set_instance_end_times(ft_engine *g, const double *tmax) {
   if (!g || !tmax) return fail(FT_ERR_ARG, "null argument");
   g->h_tmax.assign(tmax, tmax + g->n_inst);
   g->tmax_per_instance = true;
+  return 0;
+}
+
+int ft_reset_history(ft_engine *g) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (g->hist) {
+    const size_t esz = g->precision == FT_FP32 ? 4 : 8;
+    HIPCHK(hipMemsetAsync(g->hist, 0, 144 * (size_t)g->ne * esz, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+  }
   return 0;
 }
 
@@ -2338,7 +2654,7 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   if ((rc = upload_caps(g, cap))) return rc;
   if ((rc = launch_nodal(g, kResume))) return rc;  // start step before + 1
   int k = 0;
-  if (g->graph_steps > 0 && !g->timing) {
+  if (g->graph_steps > 0 && !g->timing && !g->surr_on) {
     for (; k + g->graph_steps <= max_steps; k += g->graph_steps) {
       if (!g->gexec[g->parity] && (rc = capture_chunk(g))) return rc;
       HIPCHK(hipGraphLaunch(g->gexec[g->parity], g->stream));
@@ -2498,6 +2814,257 @@ int ft_get_visco_history(ft_engine *g, double *hist) {
   // component-major [c * ne + internal e] -> element-major original order
   for (size_t c = 0; c < 144; ++c)
     for (size_t e = 0; e < (size_t)g->ne; ++e) hist[e * 144 + c] = cm[c * g->ne + g->inv[e]];
+  return 0;
+}
+
+// ---- constitutive surrogate ----------------------------------------------
+int ft_surrogate_reserve(ft_engine *g, long capacity) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (capacity < 0) return fail(FT_ERR_ARG, "capacity < 0");
+  for (void *p : {(void *)g->ds_x, (void *)g->ds_t})
+    if (p) (void)hipFree(p);
+  g->ds_x = g->ds_t = nullptr;
+  g->ds_n = g->ds_cap = 0;
+  if (capacity == 0) return 0;
+  int rc;
+  if ((rc = dalloc(&g->ds_x, (size_t)capacity * kSurrFeat)) || (rc = dalloc(&g->ds_t, (size_t)capacity * kSurrOut)))
+    return rc;
+  g->ds_cap = capacity;
+  return 0;
+}
+
+int ft_surrogate_collect(ft_engine *g, long *total) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (!g->ds_cap) return fail(FT_ERR_STATE, "ft_surrogate_reserve must run first");
+  const float *fibf = g->precision == FT_FP32 ? reinterpret_cast<const float *>(g->fib) : nullptr;
+  const double *fibd = g->precision == FT_FP64 ? reinterpret_cast<const double *>(g->fib) : nullptr;
+  for (const auto &sg : g->segs) {
+    if (!is_surr_mat(sg.mat)) continue;
+    const int n = sg.end - sg.begin;
+    if (g->ds_n + 8L * n > g->ds_cap) break;
+    const dim3 grid((n + kBlock - 1) / kBlock);
+    if (sg.mat == 7)
+      hipLaunchKernelGGL(k_surr_sample<7>, grid, dim3(kBlock), 0, g->stream, g->nn, g->ne, g->conn, g->X, g->u, fibf,
+                         fibd, g->parts, g->einfo, sg.begin, sg.end, g->ds_n, g->ds_x, g->ds_t);
+    else
+      hipLaunchKernelGGL(k_surr_sample<8>, grid, dim3(kBlock), 0, g->stream, g->nn, g->ne, g->conn, g->X, g->u, fibf,
+                         fibd, g->parts, g->einfo, sg.begin, sg.end, g->ds_n, g->ds_x, g->ds_t);
+    HIPCHK(hipGetLastError());
+    g->ds_n += 8L * n;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (total) *total = g->ds_n;
+  return 0;
+}
+
+namespace {
+
+// Normalisation block from the collected samples: feature and output mean
+// and spread, and the training box (per-feature range of the raw features).
+int surr_statistics(ft_engine *g, std::vector<float> &norm) {
+  const long n = g->ds_n;
+  const long len = 1L << 16;
+  const int nch = (int)((n + len - 1) / len);
+  double *part = nullptr;
+  int rc;
+  if ((rc = dalloc(&part, (size_t)nch * (kSurrFeat + kSurrOut) * 4))) return rc;
+  hipLaunchKernelGGL(k_col_stats, dim3(kSurrFeat, nch), dim3(kBlock), 0, g->stream, g->ds_x, n, kSurrFeat, len, part);
+  hipLaunchKernelGGL(k_col_stats, dim3(kSurrOut, nch), dim3(kBlock), 0, g->stream, g->ds_t, n, kSurrOut, len,
+                     part + (size_t)nch * kSurrFeat * 4);
+  std::vector<double> h((size_t)nch * (kSurrFeat + kSurrOut) * 4);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(FT_ERR_HIP, "statistics failed: %s", hipGetErrorString(e));
+  norm.assign(kSurrNorm, 0.0f);
+  auto column = [&](const double *base, int w, int col, double &mean, double &sd, double &lo, double &hi) {
+    double s = 0, q = 0;
+    lo = 1e300;
+    hi = -1e300;
+    for (int c = 0; c < nch; ++c) {
+      const double *o = base + ((size_t)c * w + col) * 4;
+      s += o[0];
+      q += o[1];
+      lo = std::min(lo, o[2]);
+      hi = std::max(hi, o[3]);
+    }
+    mean = s / n;
+    sd = sqrt(std::max(q / n - mean * mean, 0.0));
+  };
+  for (int k = 0; k < kSurrFeat; ++k) {
+    double mean, sd, lo, hi;
+    column(h.data(), kSurrFeat, k, mean, sd, lo, hi);
+    const bool used = k < 12 && sd > 0;
+    norm[k] = used ? (float)mean : 0.0f;
+    norm[16 + k] = used ? (float)sd : 1.0f;
+    norm[32 + k] = (float)lo;
+    norm[48 + k] = (float)hi;
+  }
+  for (int k = 0; k < kSurrOut; ++k) {
+    double mean, sd, lo, hi;
+    column(h.data() + (size_t)nch * kSurrFeat * 4, kSurrOut, k, mean, sd, lo, hi);
+    norm[64 + k] = k < 6 ? (float)mean : 0.0f;
+    norm[72 + k] = (k < 6 && sd > 0) ? (float)sd : 1.0f;
+  }
+  return 0;
+}
+
+int surr_upload_norm(ft_engine *g) {
+  if (!g->surr_norm) {
+    int rc = dalloc(&g->surr_norm, kSurrNorm);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpy(g->surr_norm, g->h_surr_norm.data(), kSurrNorm * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+int ft_surrogate_train(ft_engine *g, int steps, int batch, double lr, unsigned long long seed, double out[4]) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (g->ds_n < 1024) return fail(FT_ERR_STATE, "too few collected samples (ft_surrogate_collect)");
+  if (steps < 0 || batch < 1) return fail(FT_ERR_ARG, "steps < 0 or batch < 1");
+  drop_graphs(g);
+  int rc;
+  std::vector<float> norm;
+  if ((rc = surr_statistics(g, norm))) return rc;
+  g->h_surr_norm = norm;
+  if ((rc = surr_upload_norm(g))) return rc;
+  const long n = g->ds_n;
+  __bf16 *xb = nullptr;
+  float *tb = nullptr;
+  if ((rc = dalloc(&xb, (size_t)n * kSurrFeat)) || (rc = dalloc(&tb, (size_t)n * kSurrOut))) {
+    if (xb) (void)hipFree(xb);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_surr_normalise, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, g->stream,
+                     g->ds_x, g->ds_t, n, g->surr_norm, xb, tb);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) rc = fail(FT_ERR_HIP, "normalisation failed: %s", hipGetErrorString(e));
+  if (!rc && !g->mlp) rc = ft::mlp_create(seed, &g->mlp);
+  double loss = 0.0;
+  if (!rc) rc = ft::mlp_train(g->mlp, xb, tb, n, steps, batch, (float)lr, seed, g->stream, &loss);
+  // error of the trained network on (up to) 2^16 samples spread over the set
+  double e2 = 0, r2 = 0;
+  const long m = std::min(n, 65536L), stride = n / m;
+  __bf16 *vx = nullptr;
+  float *vt = nullptr, *vy = nullptr;
+  if (!rc && !(rc = dalloc(&vx, (size_t)m * kSurrFeat)) && !(rc = dalloc(&vt, (size_t)m * kSurrOut)) &&
+      !(rc = dalloc(&vy, (size_t)m * kSurrOut))) {
+    hipLaunchKernelGGL(k_gather_stride, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, g->stream, xb,
+                       tb, stride, m, vx, vt);
+    rc = ft::mlp_infer(g->mlp, vx, m, vy, g->stream);
+    std::vector<float> ht((size_t)m * kSurrOut), hy((size_t)m * kSurrOut);
+    if (!rc && (hipMemcpyAsync(ht.data(), vt, ht.size() * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                hipMemcpyAsync(hy.data(), vy, hy.size() * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                hipStreamSynchronize(g->stream) != hipSuccess))
+      rc = fail(FT_ERR_HIP, "validation failed");
+    for (long i = 0; i < m && !rc; ++i)
+      for (int k = 0; k < 6; ++k) {
+        const double w = k < 3 ? 1.0 : 2.0, sc = norm[72 + k], mu = norm[64 + k];
+        const double yv = hy[i * kSurrOut + k] * sc + mu, tv = ht[i * kSurrOut + k] * sc + mu;
+        e2 += w * (yv - tv) * (yv - tv);
+        r2 += w * tv * tv;
+      }
+  }
+  for (void *p : {(void *)vx, (void *)vt, (void *)vy})
+    if (p) (void)hipFree(p);
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(xb);
+  (void)hipFree(tb);
+  if (rc) return rc;
+  g->surr_ready = true;
+  if (out) {
+    out[0] = loss;
+    out[1] = (double)n;
+    out[2] = r2 > 0 ? sqrt(e2 / r2) : 0.0;
+    out[3] = steps;
+  }
+  return 0;
+}
+
+int ft_surrogate_samples(ft_engine *g, long first, long count, float *x, float *t) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (first < 0 || count < 0 || first + count > g->ds_n) return fail(FT_ERR_ARG, "sample range outside the set");
+  if (count == 0) return 0;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (x) HIPCHK(hipMemcpy(x, g->ds_x + first * kSurrFeat, count * kSurrFeat * sizeof(float), hipMemcpyDeviceToHost));
+  if (t) HIPCHK(hipMemcpy(t, g->ds_t + first * kSurrOut, count * kSurrOut * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int ft_surrogate_get(ft_engine *g, float *params, float *norm) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (!g->surr_ready) return fail(FT_ERR_STATE, "no trained surrogate");
+  if (norm) memcpy(norm, g->h_surr_norm.data(), kSurrNorm * sizeof(float));
+  return params ? ft::mlp_get_params(g->mlp, params) : 0;
+}
+
+int ft_surrogate_set(ft_engine *g, const float *params, const float *norm) {
+  if (!g || !params || !norm) return fail(FT_ERR_ARG, "null argument");
+  drop_graphs(g);
+  int rc;
+  if (!g->mlp && (rc = ft::mlp_create(1, &g->mlp))) return rc;
+  if ((rc = ft::mlp_set_params(g->mlp, params))) return rc;
+  g->h_surr_norm.assign(norm, norm + kSurrNorm);
+  if ((rc = surr_upload_norm(g))) return rc;
+  g->surr_ready = true;
+  return 0;
+}
+
+int ft_surrogate_enable(ft_engine *g, int enable, int audit_every, int audit_permille) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
+  if (!enable) {
+    g->surr_on = false;
+    return 0;
+  }
+  if (!g->surr_ready) return fail(FT_ERR_STATE, "no trained surrogate (ft_surrogate_train or ft_surrogate_set)");
+  if (g->precision != FT_FP32) return fail(FT_ERR_STATE, "the surrogate runs in the FP32 engine");
+  if (g->n_inst != 1) return fail(FT_ERR_STATE, "the surrogate is a single-instance mode");
+  if (audit_every < 1 || audit_permille < 0 || audit_permille > 1000) return fail(FT_ERR_ARG, "bad audit schedule");
+  long gps = 0;
+  for (const auto &sg : g->segs)
+    if (is_surr_mat(sg.mat)) gps += 8L * (sg.end - sg.begin);
+  if (gps == 0) return fail(FT_ERR_STATE, "no element of material 7 or 8");
+  if (gps != g->surr_gps) {
+    for (void *p : {(void *)g->surr_feat, (void *)g->surr_y})
+      if (p) (void)hipFree(p);
+    g->surr_feat = nullptr;
+    g->surr_y = nullptr;
+    int rc;
+    if ((rc = dalloc(&g->surr_feat, (size_t)gps * kSurrFeat)) || (rc = dalloc(&g->surr_y, (size_t)gps * kSurrOut)))
+      return rc;
+    g->surr_gps = gps;
+  }
+  if (g->surr_tally) (void)hipFree(g->surr_tally);
+  g->surr_tally = nullptr;
+  int rc;
+  if ((rc = dalloc(&g->surr_tally, (size_t)std::max(1, g->n_etiles) * 6))) return rc;
+  HIPCHK(hipMemset(g->surr_tally, 0, (size_t)std::max(1, g->n_etiles) * 6 * sizeof(double)));
+  g->surr_audit_every = audit_every;
+  g->surr_permille = audit_permille;
+  g->surr_iter = 0;
+  g->surr_on = true;
+  return 0;
+}
+
+int ft_surrogate_stats(ft_engine *g, double out[6]) {
+  if (!g || !out) return fail(FT_ERR_ARG, "null argument");
+  for (int k = 0; k < 6; ++k) out[k] = 0.0;
+  if (!g->surr_tally) return 0;
+  std::vector<double> t((size_t)std::max(1, g->n_etiles) * 6);
+  HIPCHK(hipStreamSynchronize(g->stream));
+  HIPCHK(hipMemcpy(t.data(), g->surr_tally, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (size_t i = 0; i < t.size() / 6; ++i)
+    for (int k = 0; k < 6; ++k) v[k] += t[i * 6 + k];
+  out[0] = v[1] > 0 ? sqrt(v[0] / v[1]) : 0.0;  // audited relative error of S_iso
+  out[1] = v[2] > 0 ? sqrt(v[0] / v[2]) : 0.0;  // the same relative to the full PK2 stress
+  out[2] = v[4] > 0 ? v[3] / v[4] : 0.0;        // fallback share of the Gauss-point evaluations
+  out[3] = v[5];                                // audited Gauss-point evaluations
+  out[4] = v[4];                                // surrogate-segment Gauss-point evaluations
+  out[5] = v[3];                                // evaluations that fell back to the analytic law
   return 0;
 }
 

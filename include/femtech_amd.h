@@ -93,6 +93,9 @@ int ft_set_boundary(ft_engine *eng, const int *dof_motion, int n_motions,
 int ft_set_fibres(ft_engine *eng, const double *dir);
 /* Displacement and velocity (3 per node); either may be NULL (zero). */
 int ft_set_state(ft_engine *eng, const double *u, const double *v);
+/* Prony history of every Gauss point back to zero (the state of a new run,
+ * as after ShapeFunctions.cpp:245-252). */
+int ft_reset_history(ft_engine *eng);
 /* Lumped mass: row-sum of the consistent element mass, gathered per node
  * (AssembleLumpedMass, Mass3D.cpp:127-157).  mass_out: 3 per node or NULL. */
 int ft_assemble_lumped_mass(ft_engine *eng, double *mass_out);
@@ -252,6 +255,43 @@ int ft_mlp_forward(ft_mlp *mlp, const float *x, int n, float *y);
 int ft_mlp_gradients(ft_mlp *mlp, const float *x, const float *t, int n, double *loss, float *grad);
 int ft_mlp_train(ft_mlp *mlp, const float *x, const float *t, int n_data, int steps, int batch,
                  double lr, unsigned long long seed, double *final_loss);
+
+/* ---- surrogate constitutive path (materials 7 and 8) ----------------------
+ * No counterpart in the reference.  The network of ft_mlp_* replaces the
+ * isochoric part of the PK2 stress of materials 7 (Ogden + Prony) and 8
+ * (anisotropic HGO) in the FP32 run loop; the volumetric part and the Prony
+ * recurrence stay analytic.  Inputs (12): Cbar - I in the material frame
+ * (fibre frame for 8; Voigt 11, 22, 33, 23, 13, 12), J - 1, Ibar1 - 3,
+ * Ibar2 - 3, Ibar4 - 1, Ibar6 - 1, law; outputs (6): S_iso in the material
+ * frame.  A Gauss point whose inputs leave the training box (per-input
+ * range of the training samples) uses the analytic law.
+ * ft_surrogate_reserve / ft_surrogate_collect: room for `capacity` training
+ * samples; each collect appends the inputs and analytic S_iso (FP64) of
+ * every Gauss point of the material 7 / 8 elements at the current device
+ * displacement (*total: samples held).
+ * ft_surrogate_train: normalises the samples and trains the network on the
+ * device (ft_mlp_train); out = final loss, samples, relative L2 error of
+ * S_iso on 2^16 samples spread over the set, steps.
+ * ft_surrogate_get / ft_surrogate_set: the trained network's parameters
+ * (ft_mlp_param_count floats) and normalisation block (80 floats).
+ * ft_surrogate_enable: on/off for the run loop (FP32 engine, one instance);
+ * every `audit_every`-th step, a fixed share audit_permille / 1000 of the
+ * elements also evaluates the analytic law and the differences are tallied.
+ * ft_surrogate_stats: out = audited relative L2 error of S_iso, the same
+ * relative to the full PK2 stress, fallback share of the Gauss-point
+ * evaluations, audited evaluations, surrogate evaluations, fallbacks. */
+int ft_surrogate_reserve(ft_engine *eng, long capacity);
+int ft_surrogate_collect(ft_engine *eng, long *total);
+/* Collected samples [first, first + count): x 16 and t 8 floats each (raw
+ * inputs, S_iso in the material frame); either may be NULL.  Sample order
+ * per collect: segment by segment, Gauss-point major within a segment. */
+int ft_surrogate_samples(ft_engine *eng, long first, long count, float *x, float *t);
+int ft_surrogate_train(ft_engine *eng, int steps, int batch, double lr, unsigned long long seed,
+                       double out[4]);
+int ft_surrogate_get(ft_engine *eng, float *params, float *norm);
+int ft_surrogate_set(ft_engine *eng, const float *params, const float *norm);
+int ft_surrogate_enable(ft_engine *eng, int enable, int audit_every, int audit_permille);
+int ft_surrogate_stats(ft_engine *eng, double out[6]);
 
 /* ---- instrumentation -----------------------------------------------------
  * When enabled (enable = N > 0), the kernel launches of every N-th step of

@@ -1,0 +1,161 @@
+"""The surrogate constitutive path of materials 7 and 8: training samples
+against the oracle's laws, device training from plain runs, the surrogate
+run loop against the plain one, the fallback to the analytic law outside
+the training box, and the audit tallies."""
+import numpy as np
+import pytest
+
+import femtech_amd as ft
+from conftest import rel
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VOIGT = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+
+
+def sym(v):
+    S = np.zeros((3, 3))
+    for k, (i, j) in enumerate(VOIGT):
+        S[i, j] = S[j, i] = v[k]
+    return S
+
+
+def voigt(S):
+    return np.array([S[i, j] for i, j in VOIGT])
+
+
+def frame(mat, a):
+    if mat != 8:
+        return np.eye(3)
+    k = int(np.argmin(np.abs(a)))
+    b = np.cross(a, np.eye(3)[k])
+    b /= np.linalg.norm(b)
+    return np.stack([a, b, np.cross(a, b)], axis=1)
+
+
+def expected_sample(mat, props, F, fib):
+    """Raw inputs and S_iso (material frame) of one Gauss point from the
+    oracle's PK2 stress."""
+    H = F - np.eye(3)
+    J = np.linalg.det(F)
+    Q = frame(mat, fib)
+    D = Q.T @ (H + H.T + H.T @ H) @ Q
+    j23 = J ** (-2.0 / 3.0)
+    Db = j23 * D + (j23 - 1.0) * np.eye(3)
+    tr, tr2 = np.trace(Db), np.sum(Db * Db)
+    f = np.zeros(16)
+    f[:6] = voigt(Db)
+    f[6], f[7], f[8] = J - 1.0, tr, 2 * tr + 0.5 * (tr * tr - tr2)
+    f[9], f[10], f[11] = Db[0, 0], Db[1, 1], 1.0 if mat == 8 else 0.0
+    S, _ = O.stress(mat if mat == 8 else 6, props, F.T.reshape(-1), fibre=fib if mat == 8 else None)
+    K = props[2] + 2 * props[1] / 3
+    Ci = np.linalg.inv(F.T @ F)
+    Siso = sym(S) - 0.5 * K * (J * J - 1.0) * Ci
+    return f, voigt(Q.T @ Siso @ Q)
+
+
+def test_training_samples_match_oracle(gpu_ready):
+    from test_cpu_kernel_math import new_law_block, field
+    X, conn, pid, mid, props, fib = new_law_block()
+    ne = conn.size // 8
+    u = field(X.reshape(-1, 3), 8e-2, 4)
+    dN, _ = O.Model(X, conn, pid, mid, props).shape_functions()
+    dN = dN.reshape(ne, 8, 8, 3)
+    cr = conn.reshape(-1, 8)
+    ur = u.reshape(-1, 3)
+    with ft.Engine(X, conn, pid, mid, props, ft.FT_FP64) as e:
+        e.set_fibres(fib)
+        e.set_state(u, None)
+        e.surrogate_reserve(8 * ne)
+        held = e.surrogate_collect()
+        x, t = e.surrogate_samples(0, held)
+    off = 0
+    for m in (7, 8):
+        els = np.nonzero(mid[pid] == m)[0]
+        assert held >= off + 8 * len(els)
+        for k, el in enumerate(els):
+            for gp in range(8):
+                F = np.eye(3) + ur[cr[el]].T @ dN[el, gp]
+                f, s = expected_sample(m, props[pid[el]], F, fib[el])
+                i = off + gp * len(els) + k
+                np.testing.assert_allclose(x[i], f, rtol=2e-6, atol=2e-7 * np.abs(f[:11]).max())
+                np.testing.assert_allclose(t[i, :6], s, rtol=2e-5, atol=2e-6 * np.abs(s).max())
+        off += 8 * len(els)
+    assert held == off
+
+
+@pytest.fixture(scope="module")
+def small_surrogate():
+    """Trained on plain runs of a 1,728-element ellipsoid under two pulses."""
+    from femtech_amd.surrogate import collect_and_train
+    params, norm, rep = collect_and_train(n=12, pulses=[(5.0e4, 2.0e-3), (9.0e4, 2.0e-3)], t_end=4e-3,
+                                          snapshots=16, steps=3000, batch=4096, lr=2e-3, seed=3)
+    return params, norm, rep
+
+
+def run_ellipsoid(n, pulse, steps, surrogate=None, audit=(10, 200)):
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=n, peak=pulse[0], duration=pulse[1])
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP32) as e:
+        e.set_fibres(w["fibres"])
+        e.set_boundary(w["dof_motion"], w["motions"])
+        e.lumped_mass()
+        e.run_init(1e9, 0.9, 1e-11)
+        if surrogate is not None:
+            e.surrogate_set(*surrogate)
+            e.surrogate_enable(True, *audit)
+        assert e.run_steps(steps) == steps
+        stats = e.surrogate_stats() if surrogate is not None else None
+        return e.state()[0], e.energy(), stats
+
+
+def test_device_training_report(gpu_ready, small_surrogate):
+    params, norm, rep = small_surrogate
+    assert rep["samples"] > 100000
+    assert rep["rel_err"] < 0.05, rep
+    assert params.size == 52744
+    lo, hi = norm[32:44], norm[48:60]
+    assert np.all(hi >= lo)
+
+
+def test_surrogate_run_close_to_plain(gpu_ready, small_surrogate):
+    """A pulse between the training pulses: the surrogate run follows the
+    plain run, the audit measures the stress error on the audited share and
+    most Gauss points stay inside the training box."""
+    params, norm, _ = small_surrogate
+    u0, e0, _ = run_ellipsoid(12, (7.0e4, 2.0e-3), 800)
+    u1, e1, st = run_ellipsoid(12, (7.0e4, 2.0e-3), 800, (params, norm))
+    assert st[3] > 0 and st[4] > 0
+    assert st[0] < 0.1, st
+    assert st[2] < 0.5, st
+    assert rel(u1, u0) < 1e-2
+    assert abs(e1[3]) <= 1e-2 * max(abs(e1[:3]).max(), 1e-30) + abs(e0[3])
+
+
+def test_fallback_outside_training_box(gpu_ready, small_surrogate):
+    """With an empty training box every Gauss point falls back to the
+    analytic law: the run then follows the plain run to FP32 round-off."""
+    params, norm, _ = small_surrogate
+    nb = norm.copy()
+    nb[32:48] = 1.0e30   # low bounds above every input
+    u0, _, _ = run_ellipsoid(12, (7.0e4, 2.0e-3), 300)
+    u1, _, st = run_ellipsoid(12, (7.0e4, 2.0e-3), 300, (params, nb))
+    assert st[2] == 1.0 and st[5] == st[4]
+    assert st[0] < 1e-12  # audited fallback points are the analytic law itself
+    assert rel(u1, u0) < 1e-4
+
+
+def test_surrogate_requires_training_and_fp32(gpu_ready):
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=6)
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP32) as e:
+        with pytest.raises(ft.FemTechError):
+            e.surrogate_enable(True)
+        with pytest.raises(ft.FemTechError):
+            e.surrogate_train(10)
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP64) as e:
+        e.surrogate_set(np.zeros(52744, np.float32), np.concatenate([np.zeros(16), np.ones(16), np.zeros(32),
+                                                                       np.zeros(8), np.ones(8)]).astype(np.float32))
+        with pytest.raises(ft.FemTechError):
+            e.surrogate_enable(True)
