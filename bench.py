@@ -80,6 +80,84 @@ def cpu_baseline(w, steps, threads):
                        "loop time %.2f s" % (steps, ne, threads, loop))
 
 
+def surrogate_bench(args):
+    """Surrogate vs plain FP32 on the ellipsoid: the network is trained on the
+    device from plain runs of a smaller ellipsoid (seeded pulse variants),
+    then the same run loop is timed with the analytic Ogden / HGO laws and
+    with the surrogate (fallback outside the training box, audit every 100
+    steps on 1 % of the elements)."""
+    import femtech_amd as ft
+    from femtech_amd.surrogate import collect_and_train, training_pulses
+    t0 = time.perf_counter()
+    pulses = training_pulses(3)
+    params, norm, rep = collect_and_train(n=args.train_n, pulses=pulses, t_end=0.004, snapshots=40,
+                                          steps=args.train_steps, batch=8192, lr=2e-3, seed=1,
+                                          log=lambda m: print(m, file=sys.stderr, flush=True))
+    t_train = time.perf_counter() - t0
+    w = workload("ellipsoid", args.n)
+    ne, nn = w["conn"].size // 8, w["X"].size // 3
+    res = {}
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP32) as e:
+        e.set_fibres(w["fibres"])
+        e.set_boundary(w["dof_motion"], w["motions"])
+        e.lumped_mass()
+        if args.autotune > 0:
+            e.autotune(args.autotune)
+        for mode in ("plain", "surrogate"):
+            e.set_state(None, None)
+            e.reset_history()
+            e.run_init(1e9, 0.9, 1e-11)
+            if mode == "surrogate":
+                e.surrogate_set(params, norm)
+                e.surrogate_enable(True, 100, 10)
+            if args.warmup:
+                e.run_steps(args.warmup)
+            e.set_timing(args.kernel_timing)
+            e.time()
+            ta = time.perf_counter()
+            done = e.run_steps(args.steps)
+            e.time()
+            tb = time.perf_counter()
+            ms, launches = e.timing()
+            e.set_timing(0)
+            assert done == args.steps
+            res[mode] = dict(value=ne * args.steps / (tb - ta), ms_per_step=1e3 * (tb - ta) / args.steps,
+                             u=e.state()[0], stats=e.surrogate_stats() if mode == "surrogate" else None,
+                             kernel_ms=[float(x) for x in ms], launches=[int(x) for x in launches])
+            if mode == "surrogate":
+                e.surrogate_enable(False)
+    st = res["surrogate"]["stats"]
+    du = float(np.linalg.norm(res["surrogate"]["u"] - res["plain"]["u"]) / np.linalg.norm(res["plain"]["u"]))
+    out = {
+        "metric": "element-steps/s on 1M-hex neo-Hookean; surrogate vs plain at <=1% stress err",
+        "value": res["surrogate"]["value"],
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["surrogate"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 surrogate MLP (fp32 accumulate) / f32 element / f64 nodal",
+        "data": "synthetic",
+        "config": {"workload": "2.0M-hex ellipsoid 90x75x65 mm, Ogden + Prony and anisotropic HGO regions by the "
+                               "MLP surrogate (12 -> 4 x 128 SiLU -> 6), linear-elastic shell analytic",
+                   "elements": ne, "nodes": nn, "edge_elements": args.n or 126},
+        "surrogate": {"plain_value": res["plain"]["value"], "plain_ms_per_step": res["plain"]["ms_per_step"],
+                      "speedup_vs_plain": res["surrogate"]["value"] / res["plain"]["value"],
+                      "audited_rel_l2_S_iso": float(st[0]), "audited_rel_l2_pk2": float(st[1]),
+                      "fallback_share": float(st[2]), "audited_gp_evaluations": int(st[3]),
+                      "surrogate_gp_evaluations": int(st[4]), "rel_l2_displacement_vs_plain": du,
+                      "training": {"pulses": pulses, "elements": rep["elements"], "samples": rep["samples"],
+                                   "steps": rep["steps"], "final_loss": rep["loss"],
+                                   "rel_l2_S_iso_on_samples": rep["rel_err"], "seconds": t_train},
+                      "kernel_ms": {"plain": res["plain"]["kernel_ms"], "surrogate": res["surrogate"]["kernel_ms"]},
+                      "launches": {"plain": res["plain"]["launches"], "surrogate": res["surrogate"]["launches"]}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +168,11 @@ def main():
                          "three-region ellipsoid (shell / Ogden + Prony / anisotropic HGO); batch: "
                          "independent blocks and ellipsoids of 2k-80k elements in one run")
     ap.add_argument("--count", type=int, default=512, help="instances of the batch workload")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="ellipsoid only: train the constitutive surrogate from plain runs of a 64k-element "
+                         "ellipsoid under seeded pulse variants, then time the plain and the surrogate run loops")
+    ap.add_argument("--train-n", type=int, default=40, help="elements per edge of the training ellipsoid")
+    ap.add_argument("--train-steps", type=int, default=8000, help="Adam steps of the surrogate training")
     ap.add_argument("--graph", type=int, default=0,
                     help="steps per captured HIP graph (0: one launch per kernel)")
     ap.add_argument("--autotune", type=int, default=5,
@@ -110,6 +193,12 @@ def main():
         sys.exit(2)
 
     import femtech_amd as ft
+    if args.surrogate:
+        if args.workload != "ellipsoid":
+            print("bench.py: --surrogate applies to the ellipsoid workload", file=sys.stderr)
+            sys.exit(2)
+        surrogate_bench(args)
+        return
     w = workload(args.workload, args.n, args.count)
     ne, nn = w["conn"].size // 8, w["X"].size // 3
     prec = ft.FT_FP32 if args.precision == "fp32" else ft.FT_FP64

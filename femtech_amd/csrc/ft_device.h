@@ -821,10 +821,13 @@ __host__ __device__ inline void iso_pk2(const PartDev &pp, const R H[3][3], R Jm
   }
 }
 
-// Raw features of a Gauss point: Dbar = Cbar - I in the material frame
-// (Voigt 11, 22, 33, 23, 13, 12), J - 1, Ibar1 - 3, Ibar2 - 3, Ibar4 - 1
-// (frame axis 1: the fibre), Ibar6 - 1 (axis 2), the law (0: material 7,
-// 1: material 8); Cbar = J^-2/3 C formed from H without cancellation.
+// Raw features of a Gauss point, from Dbar = Cbar - I in the material frame
+// (Cbar = J^-2/3 C formed from H without cancellation) and its Frobenius
+// norm s: the direction Dbar / s (Voigt 11, 22, 33, 23, 13, 12), s, J - 1,
+// Ibar1 - 3, Ibar2 - 3, Ibar4 - 1 (frame axis 1: the fibre), the law (0:
+// material 7, 1: material 8).  The network predicts S_iso / s, so small
+// strains keep the network's relative precision.
+constexpr double kSurrSmin = 1e-12;
 template <typename R>
 __host__ __device__ inline void surr_features(const R H[3][3], R Jm1, const R Q[3][3], int mat, R f[kSurrFeat]) {
   R D0[3][3];  // C - I = H + H^T + H^T H
@@ -849,13 +852,15 @@ __host__ __device__ inline void surr_features(const R H[3][3], R Jm1, const R Q[
   const R tr = D.v[0] + D.v[1] + D.v[2];
   const R tr2 = D.v[0] * D.v[0] + D.v[1] * D.v[1] + D.v[2] * D.v[2] +
                 R(2) * (D.v[3] * D.v[3] + D.v[4] * D.v[4] + D.v[5] * D.v[5]);
+  const R sn = Num<R>::sqrt_(tr2);
+  const R is = sn > R(kSurrSmin) ? R(1) / sn : R(0);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) f[k] = D.v[k];
-  f[6] = Jm1;
-  f[7] = tr;
-  f[8] = R(2) * tr + R(0.5) * (tr * tr - tr2);
-  f[9] = D.v[0];
-  f[10] = D.v[1];
+  for (int k = 0; k < 6; ++k) f[k] = D.v[k] * is;
+  f[6] = sn;
+  f[7] = Jm1;
+  f[8] = tr;
+  f[9] = R(2) * tr + R(0.5) * (tr * tr - tr2);
+  f[10] = D.v[0];
   f[11] = mat == 8 ? R(1) : R(0);
 #pragma unroll
   for (int k = 12; k < kSurrFeat; ++k) f[k] = R(0);
@@ -900,7 +905,7 @@ __host__ __device__ inline void surr_pk2(const PartDev &pp, const R H[3][3], con
     const float *y = in.y + (size_t)gp * in.gp_stride;
     Sym<R> Sm;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) Sm.v[k] = R(y[k]) * R(in.norm[72 + k]) + R(in.norm[64 + k]);
+    for (int k = 0; k < 6; ++k) Sm.v[k] = (R(y[k]) * R(in.norm[72 + k]) + R(in.norm[64 + k])) * f[6];
     Siso = rotate_sym(Sm, Q, false);
   } else {
     iso_pk2<R, MAT>(pp, H, Jm1, fib, Siso, p);

@@ -1434,6 +1434,17 @@ __global__ __launch_bounds__(kBlock) void k_col_stats(const float *x, long n, in
   }
 }
 
+// Network targets of the samples: S_iso / s (s = raw feature 6; 0 where
+// s vanishes), in place.
+__global__ void k_surr_per_strain(const float *xs, float *ts, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float sn = xs[i * kSurrFeat + 6];
+  const float is = sn > (float)kSurrSmin ? 1.0f / sn : 0.0f;
+#pragma unroll
+  for (int k = 0; k < kSurrOut; ++k) ts[i * kSurrOut + k] *= is;
+}
+
 // Normalised training arrays: x -> BF16 (x - mean) / scale, t -> (t - mean) / scale.
 __global__ void k_surr_normalise(const float *xs, const float *ts, long n, const float *norm, __bf16 *xb, float *tb) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1444,10 +1455,13 @@ __global__ void k_surr_normalise(const float *xs, const float *ts, long n, const
   for (int k = 0; k < kSurrOut; ++k) tb[i * kSurrOut + k] = (ts[i * kSurrOut + k] - norm[64 + k]) / norm[72 + k];
 }
 
-// Every stride-th sample of (xb, tb) into (vx, vt), m samples.
-__global__ void k_gather_stride(const __bf16 *xb, const float *tb, long stride, long m, __bf16 *vx, float *vt) {
+// Every stride-th sample of (xb, tb) into (vx, vt), and its strain
+// magnitude (raw feature 6 of xs) into vs; m samples.
+__global__ void k_gather_stride(const __bf16 *xb, const float *tb, const float *xs, long stride, long m, __bf16 *vx,
+                                float *vt, float *vs) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
+  vs[i] = xs[i * stride * kSurrFeat + 6];
 #pragma unroll
   for (int k = 0; k < kSurrFeat; ++k) vx[i * kSurrFeat + k] = xb[i * stride * kSurrFeat + k];
 #pragma unroll
@@ -2676,6 +2690,8 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
         return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e (instance %d)", s[i].dt_done / g->factor, i);
     return fail(FT_ERR_DT, "Timestep too small");
   }
+  if (st & FT_STATUS_NONFINITE)
+    return fail(FT_ERR_NUMERIC, "non-finite element force (an element inverted or a law left its domain)");
   return 0;
 }
 
@@ -2926,8 +2942,21 @@ int ft_surrogate_train(ft_engine *g, int steps, int batch, double lr, unsigned l
   if (steps < 0 || batch < 1) return fail(FT_ERR_ARG, "steps < 0 or batch < 1");
   drop_graphs(g);
   int rc;
+  // targets per unit strain magnitude (the samples keep S_iso; a copy)
+  float *ts_raw = g->ds_t;
+  float *ts_per = nullptr;
+  if ((rc = dalloc(&ts_per, (size_t)g->ds_n * kSurrOut))) return rc;
+  HIPCHK(hipMemcpy(ts_per, ts_raw, (size_t)g->ds_n * kSurrOut * sizeof(float), hipMemcpyDeviceToDevice));
+  hipLaunchKernelGGL(k_surr_per_strain, dim3((unsigned)((g->ds_n + kBlock - 1) / kBlock)), dim3(kBlock), 0, g->stream,
+                     g->ds_x, ts_per, g->ds_n);
+  g->ds_t = ts_per;
   std::vector<float> norm;
-  if ((rc = surr_statistics(g, norm))) return rc;
+  rc = surr_statistics(g, norm);
+  if (rc) {
+    g->ds_t = ts_raw;
+    (void)hipFree(ts_per);
+    return rc;
+  }
   g->h_surr_norm = norm;
   if ((rc = surr_upload_norm(g))) return rc;
   const long n = g->ds_n;
@@ -2939,6 +2968,9 @@ int ft_surrogate_train(ft_engine *g, int steps, int batch, double lr, unsigned l
   }
   hipLaunchKernelGGL(k_surr_normalise, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, g->stream,
                      g->ds_x, g->ds_t, n, g->surr_norm, xb, tb);
+  (void)hipStreamSynchronize(g->stream);
+  g->ds_t = ts_raw;
+  (void)hipFree(ts_per);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) rc = fail(FT_ERR_HIP, "normalisation failed: %s", hipGetErrorString(e));
   if (!rc && !g->mlp) rc = ft::mlp_create(seed, &g->mlp);
@@ -2948,26 +2980,27 @@ int ft_surrogate_train(ft_engine *g, int steps, int batch, double lr, unsigned l
   double e2 = 0, r2 = 0;
   const long m = std::min(n, 65536L), stride = n / m;
   __bf16 *vx = nullptr;
-  float *vt = nullptr, *vy = nullptr;
+  float *vt = nullptr, *vy = nullptr, *vs = nullptr;
   if (!rc && !(rc = dalloc(&vx, (size_t)m * kSurrFeat)) && !(rc = dalloc(&vt, (size_t)m * kSurrOut)) &&
-      !(rc = dalloc(&vy, (size_t)m * kSurrOut))) {
+      !(rc = dalloc(&vy, (size_t)m * kSurrOut)) && !(rc = dalloc(&vs, (size_t)m))) {
     hipLaunchKernelGGL(k_gather_stride, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, g->stream, xb,
-                       tb, stride, m, vx, vt);
+                       tb, g->ds_x, stride, m, vx, vt, vs);
     rc = ft::mlp_infer(g->mlp, vx, m, vy, g->stream);
-    std::vector<float> ht((size_t)m * kSurrOut), hy((size_t)m * kSurrOut);
+    std::vector<float> ht((size_t)m * kSurrOut), hy((size_t)m * kSurrOut), hs((size_t)m);
     if (!rc && (hipMemcpyAsync(ht.data(), vt, ht.size() * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
                 hipMemcpyAsync(hy.data(), vy, hy.size() * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                hipMemcpyAsync(hs.data(), vs, hs.size() * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
                 hipStreamSynchronize(g->stream) != hipSuccess))
       rc = fail(FT_ERR_HIP, "validation failed");
     for (long i = 0; i < m && !rc; ++i)
       for (int k = 0; k < 6; ++k) {
         const double w = k < 3 ? 1.0 : 2.0, sc = norm[72 + k], mu = norm[64 + k];
-        const double yv = hy[i * kSurrOut + k] * sc + mu, tv = ht[i * kSurrOut + k] * sc + mu;
+        const double yv = (hy[i * kSurrOut + k] * sc + mu) * hs[i], tv = (ht[i * kSurrOut + k] * sc + mu) * hs[i];
         e2 += w * (yv - tv) * (yv - tv);
         r2 += w * tv * tv;
       }
   }
-  for (void *p : {(void *)vx, (void *)vt, (void *)vy})
+  for (void *p : {(void *)vx, (void *)vt, (void *)vy, (void *)vs})
     if (p) (void)hipFree(p);
   (void)hipStreamSynchronize(g->stream);
   (void)hipFree(xb);

@@ -120,66 +120,87 @@ __device__ inline void act_chain(const f32x16 (&acc)[4], const float *b, int h, 
       }
 }
 
-// y[n][8] = network(x[n][16]); each wavefront takes 64 samples (two column
-// tiles), weights come through the caches.
-__global__ __launch_bounds__(256) void k_mlp_infer(const __bf16 *x, long n, const __bf16 *wi, const float *P,
-                                                   float *y) {
+// y[n][8] = network(x[n][16]).  Persistent blocks of 4 wavefronts: the
+// block stages all weights (BF16, 108 KB) and biases in LDS once, then each
+// wavefront walks 64-sample slabs (two column tiles), the A operands coming
+// from LDS.
+constexpr int kInferThreads = 256;
+constexpr int kBiases = b_off(kMlpLayers - 1) + kMlpOut;  // span of P holding every bias
+
+__global__ __launch_bounds__(kInferThreads) void k_mlp_infer(const __bf16 *x, long n, const __bf16 *wi,
+                                                             const float *P, float *y) {
+  __shared__ __attribute__((aligned(16))) __bf16 sw[kWi];
+  __shared__ float sb[kMlpLayers * kMlpHid];
+  for (int i = threadIdx.x; i < kWi / 8; i += blockDim.x)
+    reinterpret_cast<bf16x8 *>(sw)[i] = reinterpret_cast<const bf16x8 *>(wi)[i];
+  for (int i = threadIdx.x; i < kMlpLayers * kMlpHid; i += blockDim.x) {
+    const int l = i / kMlpHid, r = i % kMlpHid;
+    sb[i] = r < layer_out(l) ? P[b_off(l) + r] : 0.0f;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, h = lane >> 5, c32 = lane & 31;
-  const long base = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  if (base >= n) return;  // uniform over the wavefront; no block barrier below
-  bf16x8 B[2][8];
+  const int waves = kInferThreads / 64;
+  const long slabs = (n + 63) / 64;
   const bf16x8 zero8 = {};
   const f32x16 zero16 = {};
+  for (long slab = (long)blockIdx.x * waves + (threadIdx.x >> 6); slab < slabs; slab += (long)gridDim.x * waves) {
+    const long base = slab * 64;
+    bf16x8 B[2][8];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const long s = base + 32 * c + c32;
-    B[c][0] = s < n ? *reinterpret_cast<const bf16x8 *>(x + s * kMlpIn + 8 * h) : zero8;
-  }
-  f32x16 acc[2][4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(wi + (32 * t + c32) * kMlpIn + 8 * h);
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[c][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[c][0], zero16, 0, 0, 0);
-  }
-#pragma unroll
-  for (int c = 0; c < 2; ++c) act_chain(acc[c], P + b_off(0), h, B[c]);
-#pragma unroll 1
-  for (int l = 1; l < kMlpLayers - 1; ++l) {
-    const __bf16 *w = wi + wi_off(l);
+    for (int c = 0; c < 2; ++c) {
+      const long s = base + 32 * c + c32;
+      B[c][0] = s < n ? *reinterpret_cast<const bf16x8 *>(x + s * kMlpIn + 8 * h) : zero8;
+    }
+    f32x16 acc[2][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      acc[0][t] = zero16;
-      acc[1][t] = zero16;
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(sw + (32 * t + c32) * kMlpIn + 8 * h);
 #pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + (32 * t + c32) * kMlpHid + 16 * s + 8 * h);
-        acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[0][s], acc[0][t], 0, 0, 0);
-        acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[1][s], acc[1][t], 0, 0, 0);
-      }
+      for (int c = 0; c < 2; ++c) acc[c][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[c][0], zero16, 0, 0, 0);
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) act_chain(acc[c], P + b_off(l), h, B[c]);
-  }
-  const __bf16 *w = wi + wi_off(kMlpLayers - 1);
-  f32x16 o[2] = {zero16, zero16};
+    for (int c = 0; c < 2; ++c) act_chain(acc[c], sb, h, B[c]);
+#pragma unroll 1
+    for (int l = 1; l < kMlpLayers - 1; ++l) {
+      const __bf16 *w = sw + wi_off(l);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + c32 * kMlpHid + 16 * s + 8 * h);
-    o[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[0][s], o[0], 0, 0, 0);
-    o[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[1][s], o[1], 0, 0, 0);
-  }
-  const float *b4 = P + b_off(kMlpLayers - 1);
+      for (int t = 0; t < 4; ++t) {
+        acc[0][t] = zero16;
+        acc[1][t] = zero16;
+      }
+      // depth-step outer, tiles inner: eight independent accumulation
+      // chains in flight behind each other on the matrix core
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const long s = base + 32 * c + c32;
-    if (s < n) {  // rows 4 h + r of the first row group are outputs 4 h + r
-      float4 v;
-      v.x = o[c][0] + b4[4 * h];
-      v.y = o[c][1] + b4[4 * h + 1];
-      v.z = o[c][2] + b4[4 * h + 2];
-      v.w = o[c][3] + b4[4 * h + 3];
-      *reinterpret_cast<float4 *>(y + s * kMlpOut + 4 * h) = v;
+      for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + (32 * t + c32) * kMlpHid + 16 * s + 8 * h);
+          acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[0][s], acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[1][s], acc[1][t], 0, 0, 0);
+        }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) act_chain(acc[c], sb + l * kMlpHid, h, B[c]);
+    }
+    const __bf16 *w = sw + wi_off(kMlpLayers - 1);
+    f32x16 o[2] = {zero16, zero16};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + c32 * kMlpHid + 16 * s + 8 * h);
+      o[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[0][s], o[0], 0, 0, 0);
+      o[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[1][s], o[1], 0, 0, 0);
+    }
+    const float *b4 = sb + (kMlpLayers - 1) * kMlpHid;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const long s = base + 32 * c + c32;
+      if (s < n) {  // rows 4 h + r of the first row group are outputs 4 h + r
+        float4 v;
+        v.x = o[c][0] + b4[4 * h];
+        v.y = o[c][1] + b4[4 * h + 1];
+        v.z = o[c][2] + b4[4 * h + 2];
+        v.w = o[c][3] + b4[4 * h + 3];
+        *reinterpret_cast<float4 *>(y + s * kMlpOut + 4 * h) = v;
+      }
     }
   }
 }
@@ -333,6 +354,7 @@ struct MlpNet {
   float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr;
   __bf16 *Pb = nullptr, *Wi = nullptr;
   int adam_t = 0;
+  int n_cu = 0;  // compute units (persistent inference grid)
   // training workspace for batches up to `cap` samples
   int cap = 0;
   float *Z[kMlpLayers] = {};
@@ -479,8 +501,15 @@ int mlp_copy_gradients(MlpNet *net, float *host) {
 
 int mlp_infer(MlpNet *net, const __bf16 *x, long n, float *y, hipStream_t st) {
   if (n <= 0) return 0;
-  const long waves = (n + 63) / 64;
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, x, n, net->Wi, net->P, y);
+  if (!net->n_cu) {
+    int dev = 0;
+    MCHK(hipGetDevice(&dev));
+    MCHK(hipDeviceGetAttribute(&net->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  const long slabs = (n + 63) / 64, per_block = kInferThreads / 64;
+  const long blocks_needed = (slabs + per_block - 1) / per_block;
+  const unsigned grid = (unsigned)std::min<long>(blocks_needed, (long)net->n_cu);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kInferThreads), 0, st, x, n, net->Wi, net->P, y);
   MCHK(hipGetLastError());
   return 0;
 }

@@ -21,11 +21,12 @@ def training_pulses(count=3, seed=7, peak=(4000.0, 8000.0), duration=(0.006, 0.0
     return [(float(rng.uniform(*peak)), float(rng.uniform(*duration))) for _ in range(count)]
 
 
-def collect_and_train(n=40, pulses=None, t_end=0.030, snapshots=40, capacity=None, steps=6000, batch=8192,
-                      lr=2e-3, seed=1, semi_axes=(0.090, 0.075, 0.065)):
+def collect_and_train(n=40, pulses=None, t_end=0.004, snapshots=40, capacity=None, steps=6000, batch=8192,
+                      lr=2e-3, seed=1, semi_axes=(0.090, 0.075, 0.065), log=None):
     """Plain runs of an n^3-element ellipsoid, one per pulse, to t_end with
     `snapshots` collections each; then device training.  Returns (params,
-    norm, report) with report = dict(loss, samples, rel_err, steps)."""
+    norm, report) with report = dict(loss, samples, rel_err, steps).  `log`
+    (optional) receives a line after each run and after training."""
     pulses = pulses if pulses is not None else training_pulses()
     w = ellipsoid_head(n=n, semi_axes=semi_axes, peak=pulses[0][0], duration=pulses[0][1])
     ne = w["conn"].size // 8
@@ -52,7 +53,12 @@ def collect_and_train(n=40, pulses=None, t_end=0.030, snapshots=40, capacity=Non
                         break
                     t, dt, _ = e.time()
                 held = e.surrogate_collect()
+            if log:
+                log("surrogate training data: pulse %.0f rad/s^2 over %.1f ms, %d steps, %d samples held"
+                    % (peak, 1e3 * duration, e.time()[2], held))
         rep = e.surrogate_train(steps, batch=batch, lr=lr, seed=seed)
+        if log:
+            log("surrogate trained: %d steps, loss %.3e, rel. L2 error of S_iso %.3e" % (steps, rep[0], rep[2]))
         params, norm = e.surrogate_get()
     report = dict(loss=float(rep[0]), samples=int(rep[1]), rel_err=float(rep[2]), steps=int(rep[3]),
                   elements=ne, held=held)

@@ -44,10 +44,11 @@ def expected_sample(mat, props, F, fib):
     j23 = J ** (-2.0 / 3.0)
     Db = j23 * D + (j23 - 1.0) * np.eye(3)
     tr, tr2 = np.trace(Db), np.sum(Db * Db)
+    sn = np.sqrt(tr2)
     f = np.zeros(16)
-    f[:6] = voigt(Db)
-    f[6], f[7], f[8] = J - 1.0, tr, 2 * tr + 0.5 * (tr * tr - tr2)
-    f[9], f[10], f[11] = Db[0, 0], Db[1, 1], 1.0 if mat == 8 else 0.0
+    f[:6] = voigt(Db) / sn
+    f[6], f[7], f[8], f[9] = sn, J - 1.0, tr, 2 * tr + 0.5 * (tr * tr - tr2)
+    f[10], f[11] = Db[0, 0], 1.0 if mat == 8 else 0.0
     S, _ = O.stress(mat if mat == 8 else 6, props, F.T.reshape(-1), fibre=fib if mat == 8 else None)
     K = props[2] + 2 * props[1] / 3
     Ci = np.linalg.inv(F.T @ F)
@@ -79,7 +80,7 @@ def test_training_samples_match_oracle(gpu_ready):
                 F = np.eye(3) + ur[cr[el]].T @ dN[el, gp]
                 f, s = expected_sample(m, props[pid[el]], F, fib[el])
                 i = off + gp * len(els) + k
-                np.testing.assert_allclose(x[i], f, rtol=2e-6, atol=2e-7 * np.abs(f[:11]).max())
+                np.testing.assert_allclose(x[i], f, rtol=2e-6, atol=2e-7 * max(np.abs(f[6:11]).max(), 1.0))
                 np.testing.assert_allclose(t[i, :6], s, rtol=2e-5, atol=2e-6 * np.abs(s).max())
         off += 8 * len(els)
     assert held == off
