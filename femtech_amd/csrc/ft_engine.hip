@@ -305,33 +305,6 @@ constexpr int elem_waves() {
 }
 constexpr int kMaxTuneWaves = 5;
 
-// Le / ce of element e from the FP64 coordinates and displacements
-// (StableTimeStep.cpp:22-27; the coefficients of x - x_0 as in the FP64
-// kernel's refinement of the reference's dt).
-template <typename R>
-__device__ double element_dt_fp64(const ElemArgs<R> &a, int e) {
-  const int *ce = a.conn + 8 * (size_t)e;
-  const int n0 = ce[0];
-  double c[7][3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double *Xc = a.X + (size_t)d * a.nn, *Uc = a.u + (size_t)d * a.nn;
-    double v[8];
-#pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      const int n = ce[p2n(p)];
-      v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
-    }
-    wht8(v);
-#pragma unroll
-    for (int m = 0; m < 7; ++m) c[m][d] = v[m + 1] * 0.125;
-  }
-  return char_length(c) / a.parts[a.einfo[e] & kPidMask].ce;
-}
-
-// Element tile L of a launch (one block; a tile holds at most blockDim.x =
-// 64, 128 or 256 elements).  Returns early, uniformly over the block, past
-// the last tile or when the instance has stopped.
 // Elements audited against the analytic law (surrogate runs): a fixed
 // pseudo-random share of permille / 1000.
 __device__ inline bool audited(int e, unsigned permille) {
@@ -346,11 +319,13 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
   const int e = tile.x + threadIdx.x;
   const int end = tile.y;
   double dt_step = a.dt_fixed;
+  double dt_guide = 1e300;  // no previous minimum: refine every element
   unsigned long long *dtmin = a.dtbits ? a.dtbits + tile.z : nullptr;
   if (a.ctl) {
     const StepSlot *slot = &a.ctl[tile.z].slot[a.parity];
     if (slot->done) return;  // uniform over the block
     dt_step = slot->dt;
+    dt_guide = slot->dt_guide;
     dtmin = &a.ctl[tile.z].slot[a.parity].dtmin_bits;
   }
   double my_dt = 1e300;
@@ -419,21 +394,44 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
   }
   if (DT) {
     const double bmin = block_min(my_dt, sh_min);
-    // the last minimum (read here: not held over the element computation)
-    const double dt_guide = a.ctl ? a.ctl[tile.z].slot[a.parity].dt_guide : 1e300;
     if (sizeof(R) == 4 && a.refine && bmin <= dt_guide * 1.001) {
-      // The step size sets the time grid and is sensitive to round-off: in
-      // blocks near the last minimum, every element within 4e-6 (about 32
-      // FP32 ulps) of the block's FP32 minimum recomputes Le / ce from the
-      // FP64 state (StableTimeStep.cpp:22-27).  The FP32 values of the
-      // step's minimiser and of its block's minimiser differ by less than
-      // twice the FP32 error of Le / ce, so the minimiser is among them and
-      // the step does not depend on the tiling.
-      __shared__ double sh_min2[kBlock / 64];
-      double d64 = 1e300;
-      if (e < end && my_dt <= bmin + fabs(bmin) * 4e-6) d64 = element_dt_fp64(a, e);
-      const double b64 = block_min(d64, sh_min2);
-      if (threadIdx.x == 0 && b64 < 1e300) atomicMin(dtmin, dt_key(b64));
+      // The step size sets the time grid and is sensitive to round-off: the
+      // block's minimising element (smallest id on ties) recomputes Le / ce
+      // from FP64 geometry (StableTimeStep.cpp:22-27).  Rare (blocks near the
+      // last minimum only) and spread over a few threads through LDS, so it
+      // adds no registers to the kernel.
+      __shared__ int sh_arg;
+      __shared__ double sh_c[7][3];
+      __shared__ double sh_a[7];
+      if (threadIdx.x == 0) sh_arg = INT32_MAX;
+      __syncthreads();
+      if (e < end && my_dt == bmin) atomicMin(&sh_arg, e);
+      __syncthreads();
+      const int es = sh_arg;
+      if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        const double *Xc = a.X + (size_t)d * a.nn, *Uc = a.u + (size_t)d * a.nn;
+        const int n0 = a.conn[8 * (size_t)es];
+        double v[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          const int n = a.conn[8 * (size_t)es + p2n(p)];
+          v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
+        }
+        wht8(v);
+#pragma unroll
+        for (int m = 0; m < 7; ++m) sh_c[m][d] = v[m + 1] * 0.125;
+      }
+      __syncthreads();
+      if (threadIdx.x < 7) sh_a[threadIdx.x] = hex_part(sh_c, threadIdx.x);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double amax = 0.0;
+        for (int f = 0; f < 6; ++f)
+          if (sh_a[f] > amax) amax = sh_a[f];
+        const double dte = sh_a[6] / amax / a.parts[a.einfo[es] & kPidMask].ce;
+        atomicMin(dtmin, dt_key(dte));
+      }
     } else if (threadIdx.x == 0 && bmin < 1e300 && dtmin) {
       atomicMin(dtmin, dt_key(bmin));
     }
