@@ -356,7 +356,8 @@ class Engine:
         _check(self._lib.ft_set_instance_end_times(self._h, _dp(_f64(tmax))))
 
     def instance(self, i):
-        """(time, next dt, step, finished, W_int, W_ext, KE, |balance|)."""
+        """(time, next dt, step, state, W_int, W_ext, KE, |balance|); state 0
+        running, 1 reached its end time, 2 stopped without a stable step."""
         out = np.empty(8, np.float64)
         _check(self._lib.ft_get_instance(self._h, int(i), _dp(out)))
         return out

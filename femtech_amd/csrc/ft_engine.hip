@@ -102,7 +102,7 @@ struct StepSlot {
   int step;        // k: step in progress (done == 0) or last completed (done == 1)
   int done;        // 1 when the run has stopped at the end of step k
   int fresh;       // 1 if the last nodal launch completed a step
-  int pad;
+  int failed;      // 1 when it stopped because no stable step was left
 };
 
 struct Ctl {
@@ -715,7 +715,9 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   const double t_k = cur.t, t_km1 = cur.t_prev, dt_k = cur.dt;
   const double dtmin = bits_to_double(cur.dtmin_bits);
   const double dt_next = a.factor * dtmin;
-  const bool collapse = !(dtmin >= a.failure_dt);
+  // no stable step left: below the failure step (an inverted element's is
+  // negative and counts as 0), or none finite at all (non-finite forces)
+  const bool collapse = !(dtmin >= a.failure_dt) || !(dtmin < 1e308);
   const bool go = (t_k < a.tmax[inst]) && (cur.step < a.cap[inst]) && !collapse;
   // this instance's motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
   __shared__ MotionT ms[3][kMaxMotions];
@@ -831,17 +833,20 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
       nx.dtmin_bits = kInfBits;
       nx.step = cur.step + 1;
       nx.done = 0;
+      nx.failed = 0;
     } else {
       nx = cur;
       nx.done = 1;
-      if (collapse && cur.step < a.cap[inst] && t_k < a.tmax[inst]) atomicOr(a.status, FT_STATUS_DT_COLLAPSE);
+      if (collapse && cur.step < a.cap[inst] && t_k < a.tmax[inst]) {
+        atomicOr(a.status, FT_STATUS_DT_COLLAPSE);
+        nx.failed = 1;
+      }
     }
     nx.t_done = t_k;
     nx.dt_done = dt_next;
     nx.dt_guide = dtmin;
     if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(a.status, kStatusDtFromFp32);
     nx.fresh = a.mode == kStep;
-    nx.pad = 0;
     ic->slot[a.parity ^ 1] = nx;
   }
 }
@@ -2658,7 +2663,7 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   bool any = false;
   for (int i = 0; i < g->n_inst; ++i) {
     before[i] = s[i].step;
-    const bool runs = max_steps > 0 && s[i].t < g->h_tmax[i];
+    const bool runs = max_steps > 0 && s[i].t < g->h_tmax[i] && !s[i].failed;
     cap[i] = runs ? (int)std::min<long>((long)before[i] + max_steps, INT32_MAX - 1) : before[i];
     any = any || runs;
   }
@@ -2682,11 +2687,13 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   int st = 0;
   HIPCHK(hipMemcpy(&st, g->status, sizeof(int), hipMemcpyDeviceToHost));
   if (st & 8) return fail(FT_ERR_MATERIAL, "Unknown material type");
+  // A batch keeps going when one of its instances stops without a stable
+  // step (ft_get_instance reports it); a single run ends as the reference's
+  // StableTimeStep does (code 19), or with 53 for non-finite forces.
+  if (g->n_inst > 1) return 0;
   if (st & FT_STATUS_DT_COLLAPSE) {
-    for (int i = 0; i < g->n_inst; ++i)
-      if (s[i].dt_done / g->factor < g->failure_dt)
-        return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e (instance %d)", s[i].dt_done / g->factor, i);
-    return fail(FT_ERR_DT, "Timestep too small");
+    const double d = s[0].dt_done / g->factor;
+    return fail(FT_ERR_DT, "Timestep too small, dt = %15.9e", d < 1e308 ? d : 0.0);
   }
   if (st & FT_STATUS_NONFINITE)
     return fail(FT_ERR_NUMERIC, "non-finite element force (an element inverted or a law left its domain)");
@@ -2760,7 +2767,7 @@ int ft_get_instance(ft_engine *g, int i, double out8[8]) {
   out8[0] = s.t;
   out8[1] = g->run_active ? s.dt_done : 0.0;
   out8[2] = s.step;
-  out8[3] = (s.t >= g->h_tmax[i]) ? 1.0 : 0.0;
+  out8[3] = s.failed ? 2.0 : ((s.t >= g->h_tmax[i]) ? 1.0 : 0.0);
   return instance_energy(g, i, out8 + 4);
 }
 

@@ -147,9 +147,12 @@ int ft_get_energy(ft_engine *eng, double out4[4]);
  * instance; ft_run_steps(n) advances every unfinished instance by up to n
  * steps and reports the largest count.  ft_set_instance_end_times sets a
  * per-instance end time (otherwise the tmax of ft_run_init).
- * ft_get_instance: time, next dt, step, finished flag, W_int, W_ext, KE,
- * |balance| of instance i.  Per-step history and the injury criteria are
- * single-instance features. */
+ * ft_get_instance: time, next dt, step, state (0 running, 1 reached its end
+ * time, 2 stopped without a stable step: an inverted element or non-finite
+ * forces), W_int, W_ext, KE, |balance| of instance i.  A stopped instance
+ * does not stop the others, and ft_run_steps on a batch returns FT_OK for
+ * it (a single run returns 19 or 53 as described above).  Per-step history
+ * and the injury criteria are single-instance features. */
 int ft_set_instances(ft_engine *eng, int n_inst, const int *node_offsets,
                      const int *elem_offsets);
 int ft_set_instance_end_times(ft_engine *eng, const double *tmax);

@@ -453,3 +453,52 @@ def test_single_instance_declared_equals_default(gpu_ready):
             e.run_steps(50)
             outs.append(e.state()[0])
     assert np.array_equal(outs[0], outs[1])
+
+
+def run_batch_or_single(insts, tmax, precision, steps):
+    from femtech_amd import workloads as W
+    m, no, eo = W.concatenate(insts)
+    with ft.Engine(m["X"], m["conn"], m["pid"], m["material_id"], m["props"], precision) as e:
+        if len(insts) > 1:
+            e.set_instances(no, eo)
+        if m.get("fibres") is not None:
+            e.set_fibres(m["fibres"])
+        e.set_boundary(m["dof_motion"], m["motions"])
+        e.set_state(None, m["v0"])
+        e.lumped_mass()
+        if len(insts) > 1:
+            e.set_instance_end_times(np.asarray(tmax))
+        e.run_init(1e9 if len(insts) > 1 else float(tmax[0]), 0.9, 1e-11)
+        e.run_steps(steps)
+        u = e.state()[0]
+        info = np.array([e.instance(i) for i in range(len(insts))]) if len(insts) > 1 else None
+        t = e.time()
+    return u, no, info, t
+
+
+def test_batched_fp32_equals_separate_fp32_runs(gpu_ready):
+    """FP32 engine: each instance of a batch follows exactly (bitwise) the
+    run of the same instance alone: its tiles, step slot and refinement of
+    the step's minimiser are the same."""
+    insts = batch_instances()
+    tmax = [4e-4, 3e-4, 2e-4, 5e-4]
+    u, no, info, _ = run_batch_or_single(insts, tmax, ft.FT_FP32, 600)
+    for i, w in enumerate(insts):
+        ui, _, _, ti = run_batch_or_single([w], [tmax[i]], ft.FT_FP32, 600)
+        assert np.array_equal(u[3 * no[i]:3 * no[i + 1]], ui), i
+        assert info[i][0] == ti[0] and int(info[i][2]) == ti[2], (i, info[i], ti)
+
+
+def test_batch_instance_failure_is_isolated(gpu_ready):
+    """An instance whose elements invert stops (state 2) while the others run
+    on to their end times, bitwise as when run alone; the batch call returns
+    normally."""
+    from femtech_amd import workloads as W
+    bad = W.ellipsoid_head(n=10, peak=5e5, duration=2e-3)
+    good = W.impact_block(n=4, slab=1, seed=11)
+    tmax = [1e-2, 3e-4]
+    u, no, info, _ = run_batch_or_single([bad, good], tmax, ft.FT_FP64, 2000)
+    assert info[0][3] == 2.0 and info[1][3] == 1.0, info
+    ug, _, _, tg = run_batch_or_single([good], [tmax[1]], ft.FT_FP64, 2000)
+    assert np.array_equal(u[3 * no[1]:3 * no[2]], ug)
+    assert int(info[1][2]) == tg[2]
