@@ -295,6 +295,8 @@ def main():
         "data": "synthetic",
         "config": config,
         "roofline": {"bound": "hbm", "kernel": "k_element (all material segments of a step)",
+                     "limited_by": "FP32 vector issue (non-packed), not HBM bandwidth; see nodal_kernel "
+                                   "and step_roofline_frac for the memory-bound part of the step",
                      "achieved": elem_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": frac(elem_gbs), "traffic": measured[0] if measured else None,
                      "algorithmic_bytes_per_launch": traffic[0],
