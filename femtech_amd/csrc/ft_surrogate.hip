@@ -106,29 +106,51 @@ __global__ void k_refresh(const float *P, __bf16 *Pb, __bf16 *Wi) {
   }
 }
 
-// Bias + SiLU on the four output tiles of one 32-sample column tile, then
-// the hand-over to the next layer's eight B operands.
-__device__ inline void act_chain(const f32x16 (&acc)[4], const float *b, int h, bf16x8 (&B)[8]) {
+// SiLU for inference: hardware exponential and reciprocal (1 ulp), the
+// BF16 rounding of the result dominates the error.
+__device__ inline float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// Accumulators of the four output tiles of a layer started at the bias (the
+// lane's rows 8 j + 4 h + r hold 4 consecutive biases: one 16-byte read).
+__device__ inline void bias_init(f32x16 (&acc)[4], const float *b, int h) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float4 v = *reinterpret_cast<const float4 *>(b + 32 * t + 8 * j + 4 * h);
+      acc[t][4 * j] = v.x;
+      acc[t][4 * j + 1] = v.y;
+      acc[t][4 * j + 2] = v.z;
+      acc[t][4 * j + 3] = v.w;
+    }
+}
+
+// SiLU on the four output tiles of one 32-sample column tile (bias already
+// in the accumulators), then the hand-over to the next layer's eight B
+// operands.
+__device__ inline void act_chain(const f32x16 (&acc)[4], bf16x8 (&B)[8]) {
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float z = acc[t][4 * j + r] + b[32 * t + 8 * j + 4 * h + r];
-        B[2 * t + (j >> 1)][(j & 1) * 4 + r] = (__bf16)silu(z);
-      }
+      for (int r = 0; r < 4; ++r) B[2 * t + (j >> 1)][(j & 1) * 4 + r] = (__bf16)silu_fast(acc[t][4 * j + r]);
 }
 
 // y[n][8] = network(x[n][16]).  Persistent blocks of 4 wavefronts: the
 // block stages all weights (BF16, 108 KB) and biases in LDS once, then each
 // wavefront walks 64-sample slabs (two column tiles), the A operands coming
 // from LDS.
+// NT column tiles of 32 samples per wavefront, THREADS per block.  Two
+// tiles at one wavefront per SIMD measured fastest: one tile at two or four
+// wavefronts per SIMD reads every weight fragment from LDS once per tile and
+// ran 25 % slower.
+constexpr int kInferTiles = 2;
 constexpr int kInferThreads = 256;
-constexpr int kBiases = b_off(kMlpLayers - 1) + kMlpOut;  // span of P holding every bias
 
-__global__ __launch_bounds__(kInferThreads) void k_mlp_infer(const __bf16 *x, long n, const __bf16 *wi,
-                                                             const float *P, float *y) {
+template <int NT, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_mlp_infer(const __bf16 *x, long n, const __bf16 *wi, const float *P,
+                                                       float *y) {
   __shared__ __attribute__((aligned(16))) __bf16 sw[kWi];
   __shared__ float sb[kMlpLayers * kMlpHid];
   for (int i = threadIdx.x; i < kWi / 8; i += blockDim.x)
@@ -139,59 +161,67 @@ __global__ __launch_bounds__(kInferThreads) void k_mlp_infer(const __bf16 *x, lo
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, h = lane >> 5, c32 = lane & 31;
-  const int waves = kInferThreads / 64;
-  const long slabs = (n + 63) / 64;
+  constexpr int waves = THREADS / 64, slab_n = 32 * NT;
+  const long slabs = (n + slab_n - 1) / slab_n;
   const bf16x8 zero8 = {};
   const f32x16 zero16 = {};
   for (long slab = (long)blockIdx.x * waves + (threadIdx.x >> 6); slab < slabs; slab += (long)gridDim.x * waves) {
-    const long base = slab * 64;
-    bf16x8 B[2][8];
+    const long base = slab * slab_n;
+    bf16x8 B[NT][8];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < NT; ++c) {
       const long s = base + 32 * c + c32;
       B[c][0] = s < n ? *reinterpret_cast<const bf16x8 *>(x + s * kMlpIn + 8 * h) : zero8;
     }
-    f32x16 acc[2][4];
+    f32x16 acc[NT][4];
+    bias_init(acc[0], sb, h);
+#pragma unroll
+    for (int c = 1; c < NT; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[c][t] = acc[0][t];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const bf16x8 a = *reinterpret_cast<const bf16x8 *>(sw + (32 * t + c32) * kMlpIn + 8 * h);
 #pragma unroll
-      for (int c = 0; c < 2; ++c) acc[c][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[c][0], zero16, 0, 0, 0);
+      for (int c = 0; c < NT; ++c) acc[c][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[c][0], acc[c][t], 0, 0, 0);
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) act_chain(acc[c], sb, h, B[c]);
+    for (int c = 0; c < NT; ++c) act_chain(acc[c], B[c]);
 #pragma unroll 1
     for (int l = 1; l < kMlpLayers - 1; ++l) {
       const __bf16 *w = sw + wi_off(l);
+      bias_init(acc[0], sb + l * kMlpHid, h);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        acc[0][t] = zero16;
-        acc[1][t] = zero16;
-      }
-      // depth-step outer, tiles inner: eight independent accumulation
-      // chains in flight behind each other on the matrix core
+      for (int c = 1; c < NT; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[c][t] = acc[0][t];
+      // depth-step outer, tiles inner: independent accumulation chains in
+      // flight behind each other on the matrix core
 #pragma unroll
       for (int s = 0; s < 8; ++s)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + (32 * t + c32) * kMlpHid + 16 * s + 8 * h);
-          acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[0][s], acc[0][t], 0, 0, 0);
-          acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[1][s], acc[1][t], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < NT; ++c)
+            acc[c][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[c][s], acc[c][t], 0, 0, 0);
         }
 #pragma unroll
-      for (int c = 0; c < 2; ++c) act_chain(acc[c], sb + l * kMlpHid, h, B[c]);
+      for (int c = 0; c < NT; ++c) act_chain(acc[c], B[c]);
     }
     const __bf16 *w = sw + wi_off(kMlpLayers - 1);
-    f32x16 o[2] = {zero16, zero16};
+    f32x16 o[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c) o[c] = zero16;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + c32 * kMlpHid + 16 * s + 8 * h);
-      o[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[0][s], o[0], 0, 0, 0);
-      o[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[1][s], o[1], 0, 0, 0);
+#pragma unroll
+      for (int c = 0; c < NT; ++c) o[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, B[c][s], o[c], 0, 0, 0);
     }
     const float *b4 = sb + (kMlpLayers - 1) * kMlpHid;
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < NT; ++c) {
       const long s = base + 32 * c + c32;
       if (s < n) {  // rows 4 h + r of the first row group are outputs 4 h + r
         float4 v;
@@ -506,10 +536,11 @@ int mlp_infer(MlpNet *net, const __bf16 *x, long n, float *y, hipStream_t st) {
     MCHK(hipGetDevice(&dev));
     MCHK(hipDeviceGetAttribute(&net->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
   }
-  const long slabs = (n + 63) / 64, per_block = kInferThreads / 64;
+  const long slabs = (n + 32 * kInferTiles - 1) / (32 * kInferTiles), per_block = kInferThreads / 64;
   const long blocks_needed = (slabs + per_block - 1) / per_block;
   const unsigned grid = (unsigned)std::min<long>(blocks_needed, (long)net->n_cu);
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kInferThreads), 0, st, x, n, net->Wi, net->P, y);
+  hipLaunchKernelGGL((k_mlp_infer<kInferTiles, kInferThreads>), dim3(grid), dim3(kInferThreads), 0, st, x, n,
+                     net->Wi, net->P, y);
   MCHK(hipGetLastError());
   return 0;
 }
