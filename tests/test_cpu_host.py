@@ -46,6 +46,42 @@ def test_engine_create_without_device_fails_loudly():
     assert ei.value.code == ft.ERR_HIP
 
 
+def test_engine_arguments_checked_before_the_device():
+    """Bad meshes are rejected with the reference's input code (3) or the
+    argument code (51) on any machine, before a device is touched."""
+    X, conn = ft.mesh_block(2, 1, 1, 1e-3)
+    pid = np.zeros(2, np.int32)
+    mid = np.array([1], np.int32)
+    props = np.zeros(9)
+    lib = ft.library()
+    for bad, code in (
+            (dict(conn=np.where(conn == conn.max(), 999, conn).astype(np.int32)), ft.FT_ERR_INPUT),
+            (dict(pid=np.array([0, 3], np.int32)), ft.FT_ERR_INPUT),
+            (dict(precision=7), ft.FT_ERR_ARG)):
+        a = dict(conn=conn, pid=pid, precision=ft.FT_FP64)
+        a.update(bad)
+        with pytest.raises(ft.FemTechError) as ei:
+            ft.Engine(X, a["conn"], a["pid"], mid, props, a["precision"])
+        assert ei.value.code == code, (bad, ei.value)
+    # an empty mesh
+    h = ctypes.c_void_p()
+    d = np.zeros(3)
+    rc = lib.ft_engine_create(0, 0, d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              np.zeros(8, np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              np.zeros(1, np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1,
+                              mid.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              props.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ft.FT_FP64, ctypes.byref(h))
+    assert rc == ft.FT_ERR_ARG and not h.value
+
+
+def test_network_without_device_fails_loudly():
+    if ft.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(ft.FemTechError) as ei:
+        ft.Mlp()
+    assert ei.value.code == ft.ERR_HIP
+
+
 def test_block_generator_geometry():
     X, conn = ft.mesh_block(4, 3, 2, (1e-3, 2e-3, 3e-3))
     X = X.reshape(-1, 3)
