@@ -137,6 +137,29 @@ __device__ inline T block_min(T v, T *sh) {
   return r;
 }
 
+// Fixed-order block sums of N values with one barrier: each wavefront
+// reduces by the butterfly of block_sum, then the wavefront totals are added
+// in wavefront order (the same tree as N calls of block_sum).  The result is
+// valid in thread 0.
+template <int N>
+__device__ inline void block_sum_n(double (&v)[N], double (*sh)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) sh[w][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double r = 0.0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k][i];
+      v[i] = r;
+    }
+}
+
 // Fixed-order block sum (same tree every launch -> bitwise reproducible).
 __device__ inline double block_sum(double v, double *sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -806,11 +829,10 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
     }
   }
   if (a.energy && (a.mode == kStep || go)) {  // uniform over the grid
-    const double w = block_sum(sWint, sh);
-    const double x = block_sum(sWext, sh);
-    const double k = block_sum(KE, sh);
-    const double pw = block_sum(pInt, sh);
-    const double px = block_sum(pExt, sh);
+    __shared__ double sh5[kBlock / 64][5];
+    double t5[5] = {sWint, sWext, KE, pInt, pExt};
+    block_sum_n<5>(t5, sh5);
+    const double w = t5[0], x = t5[1], k = t5[2], pw = t5[3], px = t5[4];
     if (threadIdx.x == 0) {
       double *acc = a.eacc + kEaccStride * L;
       if (a.mode == kStep) {
@@ -3178,7 +3200,16 @@ int ft_autotune(ft_engine *g, int repeats, double *report, int *n_segments) {
     for (int tile : tiles) {
       sg.tile = tile;
       if ((rc = build_element_tiles(g))) break;
-      if (tile == kBlock && (rc = time_one(si, 0, t_def))) break;
+      if (tile == kBlock) {
+        // The default shape (launch bounds of the compiler's choosing) is a
+        // candidate like the others.
+        if ((rc = time_one(si, 0, t_def))) break;
+        if (t_def < best) {
+          best = t_def;
+          bw = 0;
+          bt = tile;
+        }
+      }
       for (int w = fp32 ? 1 : 0; w <= (fp32 ? kMaxTuneWaves : 0) && !rc; ++w) {
         double us = 0;
         if ((rc = time_one(si, w, us))) break;

@@ -112,8 +112,9 @@ def test_autotune_keeps_the_run(gpu_ready):
         before = snapshot(e1, n)
         rep = e1.autotune(2)
         assert rep.shape == (len(e1.segments()), 6)
-        assert np.all(np.isin(rep[:, 3], [64, 128, 256])) and np.all((rep[:, 2] >= 1) & (rep[:, 2] <= 5))
-        assert np.all(rep[:, 4] > 0) and np.all(rep[:, 4] <= rep[:, 5] * 1.5)
+        assert np.all(np.isin(rep[:, 3], [64, 128, 256])) and np.all((rep[:, 2] >= 0) & (rep[:, 2] <= 5))
+        assert np.all(rep[:, 4] > 0) and np.all(rep[:, 4] <= rep[:, 5])
+        assert np.all((rep[:, 2] > 0) | (rep[:, 3] == 256))  # waves 0 only at the default tile
         seg = e1.segments()
         assert np.array_equal(seg[:, 3], rep[:, 2].astype(int)) and np.array_equal(seg[:, 4], rep[:, 3].astype(int))
         for x, y in zip(before, snapshot(e1, n)):
