@@ -97,6 +97,18 @@ __host__ __device__ inline void wht8(R v[8]) {
   }
 }
 
+// Unnormalised Walsh-Hadamard coefficients of component d of a nodal field
+// (v: the 8 nodal values in cube-position order, FP64, overwritten), rounded
+// to R once: c[m][d] for m = xi, eta, xi.eta, zeta, xi.zeta, eta.zeta,
+// xi.eta.zeta (the sum, index 0, is not needed).  Taken in FP64, the
+// differences of nearby node positions lose nothing to their magnitude.
+template <typename R>
+__host__ __device__ inline void field_coeffs(double v[8], R c[7][3], int d) {
+  wht8(v);
+#pragma unroll
+  for (int m = 0; m < 7; ++m) c[m][d] = R(v[m + 1]);
+}
+
 // Inverse direction used for forces: coefficients -> nodal values with sign
 // convention (bit set -> +1, clear -> -1).
 template <typename R>
@@ -571,9 +583,10 @@ __host__ __device__ inline R norm3(const R x[3]) {
 // The 2x2 rule sums |v1 x v2| with v1 = q_j cD + c1, v2 = q_i cD + c2; since
 // cD x cD = 0 that is |c1 x c2 + q_i (c1 x cD) + q_j (cD x c2)|, so three cross
 // products serve all four points.
+// `tol` is the reference's 1e-6, times the scale of the vectors (see
+// char_length).
 template <typename R>
-__host__ __device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3]) {
-  const R tol = R(1e-6);
+__host__ __device__ inline R face_area(const R cD[3], const R c1[3], const R c2[3], const R tol = R(1e-6)) {
   R w0[3];
   cross3(c1, c2, w0);
   if ((cD[0] < tol) && (cD[1] < tol) && (cD[2] < tol)) return R(4) * norm3(w0);
@@ -599,25 +612,21 @@ __host__ __device__ inline R face_area(const R cD[3], const R c1[3], const R c2[
 }
 
 // Characteristic length V / max face area (CalculateCharacteristicLength_
-// C3D8.cpp:3-30) from the normalised Walsh-Hadamard coefficients of the
-// deformed position field: d[m][comp] for m = xi, eta, xi.eta, zeta,
-// xi.zeta, eta.zeta, xi.eta.zeta.  The reference's q0..q5 of
-// volumeHexahedron are 8 x (xi.eta, xi.zeta, xi, eta.zeta, eta, zeta).
+// C3D8.cpp:3-30) from the unnormalised Walsh-Hadamard coefficients of the
+// deformed position field (8 x the coefficients of its monomial expansion):
+// d[m][comp] for m = xi, eta, xi.eta, zeta, xi.zeta, eta.zeta, xi.eta.zeta.
+// These are the reference's q0..q5 of volumeHexahedron as they stand (q0..q5
+// = xi.eta, xi.zeta, xi, eta.zeta, eta, zeta).  The face vectors are 8 x the
+// normalised ones, so the areas come out 64 x too large and the tolerance
+// 8 x; both factors are powers of two and exact.
 template <typename R>
 __host__ __device__ inline R char_length(const R d[7][3]) {
   const R *c1 = d[0], *c2 = d[1], *c3 = d[2], *c4 = d[3], *c5 = d[4], *c6 = d[5], *c7 = d[6];
-  R q0[3], q1[3], q2[3], q3[3], q4[3], q5[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    q0[k] = R(8) * c3[k];
-    q1[k] = R(8) * c5[k];
-    q2[k] = R(8) * c1[k];
-    q3[k] = R(8) * c6[k];
-    q4[k] = R(8) * c2[k];
-    q5[k] = R(8) * c4[k];
-  }
-  R vol = (triple(q0, q4, q3) + triple(q2, q0, q1) + triple(q1, q3, q5)) / R(192) +
-          triple(q2, q4, q5) / R(64);
+  const R *q0 = c3, *q1 = c5, *q2 = c1, *q3 = c6, *q4 = c2, *q5 = c4;
+  // FP64 keeps the reference's divisions; FP32 multiplies by reciprocals
+  // (its minimum element is recomputed in FP64 where it matters).
+  const R t123 = triple(q0, q4, q3) + triple(q2, q0, q1) + triple(q1, q3, q5), t4 = triple(q2, q4, q5);
+  const R vol = sizeof(R) == 8 ? t123 / R(192) + t4 / R(64) : t123 * R(1.0 / 192) + t4 * R(1.0 / 64);
   // Faces in the reference's order (CalculateCharacteristicLength_C3D8.cpp:12-13):
   // zeta-, zeta+, xi-, xi+, eta-, eta+.
   R amax = R(0);
@@ -641,10 +650,10 @@ __host__ __device__ inline R char_length(const R d[7][3]) {
         e2[k] = c4[k] + s * c6[k];
       }
     }
-    R fa = face_area(cD, e1, e2);
+    R fa = face_area(cD, e1, e2, R(8) * R(1e-6));
     if (fa > amax) amax = fa;
   }
-  return vol / amax;
+  return sizeof(R) == 8 ? vol / amax * R(64) : vol * Num<R>::rcp_(amax) * R(64);
 }
 
 // One piece of the characteristic length of a deformed hex from its
@@ -688,22 +697,42 @@ __host__ __device__ inline R hex_part(const R d[7][3], int k) {
 
 // f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k accumulated in the monomial basis of
 // the element (1, xi, eta, xi.eta, zeta, xi.zeta, eta.zeta, xi.eta.zeta; the
-// constant term is zero).  The signs are +-1, so each product is exact and
-// chaining the updates as FMAs changes only the order of the additions.
+// constant term is zero).  w holds this Gauss point's weights of the seven
+// monomials: the signs of its coordinates times the scale of the monomial
+// class (linear, bilinear, trilinear), so h needs no scaling afterwards.
+template <typename R> struct GpWeights { R l, x, y, z, yz, xz, xy; };
 template <typename R>
-__host__ __device__ inline void accumulate_force_basis(R h[7], R q0, R q1, R q2, R sx, R sy, R sz) {
-  h[0] += q0;
-  h[1] += q1;
-  h[3] += q2;
-  h[2] = Num<R>::fma_(sy, q0, h[2]);
-  h[2] = Num<R>::fma_(sx, q1, h[2]);
-  h[4] = Num<R>::fma_(sz, q0, h[4]);
-  h[4] = Num<R>::fma_(sx, q2, h[4]);
-  h[5] = Num<R>::fma_(sz, q1, h[5]);
-  h[5] = Num<R>::fma_(sy, q2, h[5]);
-  h[6] = Num<R>::fma_(sy * sz, q0, h[6]);
-  h[6] = Num<R>::fma_(sx * sz, q1, h[6]);
-  h[6] = Num<R>::fma_(sx * sy, q2, h[6]);
+__host__ __device__ inline void accumulate_force_basis(R h[7], R q0, R q1, R q2, const GpWeights<R> &w) {
+  h[0] = Num<R>::fma_(w.l, q0, h[0]);
+  h[1] = Num<R>::fma_(w.l, q1, h[1]);
+  h[3] = Num<R>::fma_(w.l, q2, h[3]);
+  h[2] = Num<R>::fma_(w.y, q0, h[2]);
+  h[2] = Num<R>::fma_(w.x, q1, h[2]);
+  h[4] = Num<R>::fma_(w.z, q0, h[4]);
+  h[4] = Num<R>::fma_(w.x, q2, h[4]);
+  h[5] = Num<R>::fma_(w.z, q1, h[5]);
+  h[5] = Num<R>::fma_(w.y, q2, h[5]);
+  h[6] = Num<R>::fma_(w.yz, q0, h[6]);
+  h[6] = Num<R>::fma_(w.xz, q1, h[6]);
+  h[6] = Num<R>::fma_(w.xy, q2, h[6]);
+}
+
+// c with its sign flipped when neg (0 or 1).  Integer arithmetic on a
+// Gauss-point index that is uniform over the wavefront stays on the scalar
+// unit, and the result is a scalar operand of the vector instructions.
+template <typename R> struct SignBit;
+template <> struct SignBit<float> {
+  using U = unsigned;
+  static constexpr int kShift = 31;
+};
+template <> struct SignBit<double> {
+  using U = unsigned long long;
+  static constexpr int kShift = 63;
+};
+template <typename R>
+__host__ __device__ inline R flip_sign(R c, unsigned neg) {
+  using U = typename SignBit<R>::U;
+  return __builtin_bit_cast(R, __builtin_bit_cast(U, c) ^ ((U)neg << SignBit<R>::kShift));
 }
 
 // ---------------------------------------------------------------------------
@@ -965,33 +994,36 @@ constexpr int kFlagNonFinite = 1;
 constexpr int kFlagInverted = 2;
 constexpr int kFlagMaterial = 8;
 
-// One element: xv/uv hold the reference coordinates and displacements of
-// the 8 nodes relative to node 0, in cube-position order p (see p2n).  Writes
-// the 24 internal-force components in C3D8 node order (node-major, xyz) and
-// lowers dt_e when DT is set and the element is not skipped.  `hist` is an
-// accessor with load(gp, k) / store(gp, k, v) for the 18 Prony values of a
-// Gauss point (material 5 only).
+// One element from the unnormalised Walsh-Hadamard coefficients of its
+// reference coordinates (cx) and displacements (cu): c[m][d] = sum_p
+// sign_m(p) v_d(p) over the 8 nodes in cube-position order p (see p2n), m =
+// xi, eta, xi.eta, zeta, xi.zeta, eta.zeta, xi.eta.zeta -- 8 x the
+// coefficients of the field's monomial expansion.  The factor 8 is never
+// divided out: it cancels in H = Ju J0^-1, the force accumulators come out
+// 64 x too large and the final constants take that out, and char_length
+// takes it out itself (powers of two: every scaled result is exact).  cx and
+// cu are overwritten.  Writes the 24 internal-force components in C3D8 node
+// order (node-major, xyz) and lowers dt_e when DT is set and the element is
+// not skipped.  `hist` is an accessor with load(gp, k) / store(gp, k, v) for
+// the 18 Prony values of a Gauss point (materials 5 and 7).
 template <typename R, bool FORCE, bool DT, int MAT, class Hist, bool SURR = false>
-__host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartDev &pp,
+__host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartDev &pp,
                                              bool skip_dt, double dt_step, Hist &hist,
                                              bool update_hist, R outv[24], double &my_dt,
                                              int &flags, const R fib[3] = nullptr,
                                              const SurrIn *sin = nullptr, SurrTally *tally = nullptr) {
+  if (DT && !skip_dt) {
+    // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
+    R dcf[7][3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    wht8(xv[d]);
-    wht8(uv[d]);
+    for (int m = 0; m < 7; ++m)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
+    const R le = char_length(dcf);
+    // FP64 keeps the reference's division; FP32 multiplies by 1 / ce
+    const double dte = sizeof(R) == 8 ? (double)le / pp.ce : (double)(le * R(pp.inv_ce));
+    if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
   }
-  // Normalised coefficients c[m][d], m = xi, eta, xi.eta, zeta, xi.zeta,
-  // eta.zeta, xi.eta.zeta (Walsh-Hadamard index m + 1).
-  R cx[7][3], cu[7][3];
-#pragma unroll
-  for (int m = 0; m < 7; ++m)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      cx[m][d] = xv[d][m + 1] * R(0.125);
-      cu[m][d] = uv[d][m + 1] * R(0.125);
-    }
   const int mat = MAT >= 0 ? MAT : pp.mat;
   if (FORCE && mat == 0) {  // rigid: no stress, no internal force
 #pragma unroll
@@ -1011,24 +1043,57 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
     for (int d = 0; d < 3; ++d)
 #pragma unroll
       for (int m = 0; m < 7; ++m) h[d][m] = R(0);
+    // Half sums: on the Gauss-point plane zeta = sz the columns of
+    // J = d(field)/d(xi) are
+    //   J[.][0] = a0 + sy b0,  a0 = c0 + sz g c4,  b0 = g c2 + sz g^2 c6,
+    //   J[.][1] = a1 + sx b0,  a1 = c1 + sz g c5,
+    //   J[.][2] = c3 + sx g c4 + sy g c5 + sx sy g^2 c6.
+    // a0, a1, b0 replace c0, c1, c2 in place (plane zeta = +) and move to the
+    // plane zeta = - once, so the loop holds no more values than before.
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R a0 = Num<R>::fma_(g, cx[4][d], cx[0][d]), a1 = Num<R>::fma_(g, cx[5][d], cx[1][d]);
+      R b0 = Num<R>::fma_(g2, cx[6][d], g * cx[2][d]);
+      cx[0][d] = a0, cx[1][d] = a1, cx[2][d] = b0;
+      a0 = Num<R>::fma_(g, cu[4][d], cu[0][d]), a1 = Num<R>::fma_(g, cu[5][d], cu[1][d]);
+      b0 = Num<R>::fma_(g2, cu[6][d], g * cu[2][d]);
+      cu[0][d] = a0, cu[1][d] = a1, cu[2][d] = b0;
+    }
     // Gauss points in the reference's order (GaussQuadrature3D.cpp:19-49):
     // (-,-,+), (+,-,+), (+,+,+), (-,+,+), then the same at zeta = -.  The loop
     // is kept rolled so one Gauss point's temporaries are live at a time.
 #pragma unroll 1
     for (int gp = 0; gp < 8; ++gp) {
-      const R sx = ((gp & 3) == 1 || (gp & 3) == 2) ? R(1) : R(-1);
-      const R sy = (gp & 2) ? R(1) : R(-1);
-      const R sz = gp < 4 ? R(1) : R(-1);
-      // J[d][k] = d(field_d)/d(xi_k) at this Gauss point.
+      if (gp == 4) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          cx[0][d] = Num<R>::fma_(R(-2) * g, cx[4][d], cx[0][d]);
+          cx[1][d] = Num<R>::fma_(R(-2) * g, cx[5][d], cx[1][d]);
+          cx[2][d] = Num<R>::fma_(R(-2) * g2, cx[6][d], cx[2][d]);
+          cu[0][d] = Num<R>::fma_(R(-2) * g, cu[4][d], cu[0][d]);
+          cu[1][d] = Num<R>::fma_(R(-2) * g, cu[5][d], cu[1][d]);
+          cu[2][d] = Num<R>::fma_(R(-2) * g2, cu[6][d], cu[2][d]);
+        }
+      }
+      // sign bits of xi, eta, zeta at this Gauss point
+      const unsigned nx = ((gp ^ (gp >> 1)) & 1) ^ 1, ny = ((gp >> 1) & 1) ^ 1, nz = (gp >> 2) & 1;
+      const R sx = flip_sign(R(1), nx), sy = flip_sign(R(1), ny);
+      const R gx = flip_sign(g, nx), gy = flip_sign(g, ny), gxy = flip_sign(g2, nx ^ ny);
+      // monomial weights: 1/8 (linear), g/8 (bilinear), g^2/8 (trilinear),
+      // and 1/64 for the factor 8 of the coefficients (adj(8 J) = 64 adj(J))
+      const R w1 = R(0.125 / 64), wg = g * R(0.125 / 64), wg2 = g2 * R(0.125 / 64);
+      const GpWeights<R> gw = {w1, flip_sign(wg, nx), flip_sign(wg, ny), flip_sign(wg, nz),
+                               flip_sign(wg2, ny ^ nz), flip_sign(wg2, nx ^ nz), flip_sign(wg2, nx ^ ny)};
+      // J[d][k] = d(field_d)/d(xi_k) at this Gauss point (8 x).
       R J0[3][3], Ju[3][3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        J0[d][0] = cx[0][d] + (sy * g) * cx[2][d] + (sz * g) * cx[4][d] + (sy * sz * g2) * cx[6][d];
-        J0[d][1] = cx[1][d] + (sx * g) * cx[2][d] + (sz * g) * cx[5][d] + (sx * sz * g2) * cx[6][d];
-        J0[d][2] = cx[3][d] + (sx * g) * cx[4][d] + (sy * g) * cx[5][d] + (sx * sy * g2) * cx[6][d];
-        Ju[d][0] = cu[0][d] + (sy * g) * cu[2][d] + (sz * g) * cu[4][d] + (sy * sz * g2) * cu[6][d];
-        Ju[d][1] = cu[1][d] + (sx * g) * cu[2][d] + (sz * g) * cu[5][d] + (sx * sz * g2) * cu[6][d];
-        Ju[d][2] = cu[3][d] + (sx * g) * cu[4][d] + (sy * g) * cu[5][d] + (sx * sy * g2) * cu[6][d];
+        J0[d][0] = Num<R>::fma_(sy, cx[2][d], cx[0][d]);
+        J0[d][1] = Num<R>::fma_(sx, cx[2][d], cx[1][d]);
+        J0[d][2] = Num<R>::fma_(gxy, cx[6][d], Num<R>::fma_(gy, cx[5][d], Num<R>::fma_(gx, cx[4][d], cx[3][d])));
+        Ju[d][0] = Num<R>::fma_(sy, cu[2][d], cu[0][d]);
+        Ju[d][1] = Num<R>::fma_(sx, cu[2][d], cu[1][d]);
+        Ju[d][2] = Num<R>::fma_(gxy, cu[6][d], Num<R>::fma_(gy, cu[5][d], Num<R>::fma_(gx, cu[4][d], cu[3][d])));
       }
       // adj(J0): J0^-1 = A / det0 (ShapeFunction_C3D8.cpp:95-123)
       R A[3][3];
@@ -1057,11 +1122,23 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
         // tau adj(Jx)^T / J.
         const R Jm1 = det_ip1_m1(H);
         if (!(Jm1 > R(-1))) flags |= kFlagInverted;
-        Sym<R> tau;
-        kirchhoff_stress<R, MAT>(pp, H, Jm1, fib, tau);
+        Sym<R> tau;  // tau / J
         const R iJ = Num<R>::rcp_(R(1) + Jm1);
+        if constexpr (MAT == 1) {
+          // kirchhoff_stress's neo-Hookean tau = mu (b - I) + lambda ln J I,
+          // with 1 / J applied to the two coefficients
+          const Sym<R> bm = b_minus_identity(H);
+          const R m = R(pp.mu) * iJ, l = R(pp.lambda) * Num<R>::log1p_(Jm1) * iJ;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) tau.v[k] *= iJ;
+          for (int k = 0; k < 6; ++k) tau.v[k] = m * bm.v[k];
+          tau.v[0] += l;
+          tau.v[1] += l;
+          tau.v[2] += l;
+        } else {
+          kirchhoff_stress<R, MAT>(pp, H, Jm1, fib, tau);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) tau.v[k] *= iJ;
+        }
         R X[3][3], B[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -1082,7 +1159,7 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
           const R q0 = t0 * B[0][0] + t1 * B[0][1] + t2 * B[0][2];
           const R q1 = t0 * B[1][0] + t1 * B[1][1] + t2 * B[1][2];
           const R q2 = t0 * B[2][0] + t1 * B[2][1] + t2 * B[2][2];
-          accumulate_force_basis(h[i], q0, q1, q2, sx, sy, sz);
+          accumulate_force_basis(h[i], q0, q1, q2, gw);
         }
         continue;
       }
@@ -1118,22 +1195,21 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
         const R q1 = P[i][0] * A[1][0] + P[i][1] * A[1][1] + P[i][2] * A[1][2];
         const R q2 = P[i][0] * A[2][0] + P[i][1] * A[2][1] + P[i][2] * A[2][2];
         // f_a,i = sum_gp sum_k Q_ik dN_a/dxi_k in the monomial basis
-        accumulate_force_basis(h[i], q0, q1, q2, sx, sy, sz);
+        accumulate_force_basis(h[i], q0, q1, q2, gw);
       }
     }
-    // Back to the nodes: f_p = sum_q c_q prod(signs), c = h scaled by
-    // 1/8 (linear), g/8 (bilinear), g^2/8 (trilinear).
+    // Back to the nodes: f_p = sum_q c_q prod(signs) (h carries the scale).
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       R c[8];
       c[0] = R(0);
-      c[1] = h[d][0] * R(0.125);
-      c[2] = h[d][1] * R(0.125);
-      c[4] = h[d][3] * R(0.125);
-      c[3] = h[d][2] * (g * R(0.125));
-      c[5] = h[d][4] * (g * R(0.125));
-      c[6] = h[d][5] * (g * R(0.125));
-      c[7] = h[d][6] * (g2 * R(0.125));
+      c[1] = h[d][0];
+      c[2] = h[d][1];
+      c[4] = h[d][3];
+      c[3] = h[d][2];
+      c[5] = h[d][4];
+      c[6] = h[d][5];
+      c[7] = h[d][6];
       iwht8(c);
 #pragma unroll
       for (int p = 0; p < 8; ++p) outv[3 * p2n(p) + d] = c[p];
@@ -1142,18 +1218,6 @@ __host__ __device__ inline void element_eval(R xv[3][8], R uv[3][8], const PartD
 #pragma unroll
     for (int k = 0; k < 24; ++k) finite = finite && isfinite(outv[k]);
     if (!finite) flags |= kFlagNonFinite;
-  }
-  if (DT && !skip_dt) {
-    // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
-    R dcf[7][3];
-#pragma unroll
-    for (int m = 0; m < 7; ++m)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
-    const R le = char_length(dcf);
-    // FP64 keeps the reference's division; FP32 multiplies by 1 / ce
-    const double dte = sizeof(R) == 8 ? (double)le / pp.ce : (double)(le * R(pp.inv_ce));
-    if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
   }
 }
 

@@ -260,6 +260,16 @@ __host__ __device__ inline void motion_eval(const double *mo, const double *X, d
 // ---------------------------------------------------------------------------
 // Element kernel
 // ---------------------------------------------------------------------------
+// Element of an array at a 32-bit byte offset from a (uniform) base.
+template <typename T>
+__device__ __forceinline__ T &at_byte(T *base, unsigned off) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ const T &at_byte(const T *base, unsigned off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
 // Prony history, component-major in HBM: value k of Gauss point gp of
 // element e at [(gp * 18 + k) * ne + e] (coalesced across a wavefront).
 template <typename R>
@@ -360,28 +370,27 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
     const int nd[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
     const int info = a.einfo[e];
     const PartDev pp = a.parts[info & kPidMask];
-    // Node-0-relative coordinates and displacements, subtracted in FP64 and
-    // then rounded to R (keeps the relative precision of the element's own
-    // geometry, not of its absolute position).
-    // uniform per-component bases + 32-bit node offsets (scalar-base
-    // addressing, no per-load 64-bit address arithmetic)
+    // Monomial coefficients of the reference coordinates and displacements
+    // (field_coeffs).  Uniform per-component bases and one 32-bit byte offset
+    // per node (the loads take a scalar base and a vector offset: no
+    // per-load 64-bit address arithmetic; the mesh size limit keeps 24 nn
+    // below 2^31).
     const double *Xd[3] = {a.X, a.X + a.nn, a.X + 2 * (size_t)a.nn};
     const double *Ud[3] = {a.u, a.u + a.nn, a.u + 2 * (size_t)a.nn};
-    double X0[3], U0[3];
+    unsigned off[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) off[p] = 8u * (unsigned)nd[p2n(p)];
+    R cx[7][3], cu[7][3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      X0[d] = Xd[d][(unsigned)nd[0]];
-      U0[d] = Ud[d][(unsigned)nd[0]];
-    }
-    R xv[3][8], uv[3][8];
+      double vx[8], vu[8];
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      const unsigned n = (unsigned)nd[p2n(p)];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        xv[d][p] = p == 0 ? R(0) : R(Xd[d][n] - X0[d]);
-        uv[d][p] = p == 0 ? R(0) : R(Ud[d][n] - U0[d]);
+      for (int p = 0; p < 8; ++p) {
+        vx[p] = at_byte(Xd[d], off[p]);
+        vu[p] = at_byte(Ud[d], off[p]);
       }
+      field_coeffs(vx, cx, d);
+      field_coeffs(vu, cu, d);
     }
     HistSoA<R> hist{a.hist, a.ne, e};
     R outv[24];
@@ -394,10 +403,10 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
     if constexpr (SURR) {
       const SurrIn sin{a.surr_y + (size_t)(e - a.surr_e0) * kSurrOut, a.surr_stride, a.surr_norm,
                        a.surr_audit != 0 && audited(e, a.surr_permille)};
-      element_eval<R, FORCE, DT, MAT, HistSoA<R>, true>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
+      element_eval<R, FORCE, DT, MAT, HistSoA<R>, true>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
                                                        a.update_hist != 0, outv, my_dt, flags, fib, &sin, &tally);
     } else {
-      element_eval<R, FORCE, DT, MAT>(xv, uv, pp, (info & kSkipBit) != 0, dt_step, hist,
+      element_eval<R, FORCE, DT, MAT>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
                                  a.update_hist != 0, outv, my_dt, flags, fib);
     }
     if (FORCE) {
@@ -408,10 +417,10 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);  // uniform
-        const unsigned o = 3u * (unsigned)(a.corner ? nd[k] : e);
-        dst[o] = outv[3 * k];
-        dst[o + 1] = outv[3 * k + 1];
-        dst[o + 2] = outv[3 * k + 2];
+        const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
+        at_byte(dst, o) = outv[3 * k];
+        at_byte(dst, o + (unsigned)sizeof(R)) = outv[3 * k + 1];
+        at_byte(dst, o + 2u * (unsigned)sizeof(R)) = outv[3 * k + 2];
       }
     }
   }
@@ -2190,7 +2199,9 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       !props)
     return fail(FT_ERR_ARG, "invalid mesh arguments");
   if (precision != FT_FP64 && precision != FT_FP32) return fail(FT_ERR_ARG, "bad precision");
-  if ((long)n_elems * 24 > (long)INT32_MAX / 1 || n_parts > kPidMask)
+  // 24 n stays below 2^31: element and node arrays are addressed with 32-bit
+  // byte offsets from scalar bases in the kernels.
+  if ((long)n_elems * 24 > (long)INT32_MAX || (long)n_nodes * 24 > (long)INT32_MAX || n_parts > kPidMask)
     return fail(FT_ERR_ARG, "mesh too large");
   for (long i = 0; i < 8L * n_elems; ++i)
     if (conn[i] < 0 || conn[i] >= n_nodes) return fail(FT_ERR_INPUT, "connectivity index out of range at %ld", i);

@@ -72,7 +72,8 @@ int ft_device_count(int *count);
  * alpha), 7 1-term Ogden + 2-term Prony (rho, mu, lambda, alpha, -, g1, tau1,
  * g2, tau2), 8 anisotropic HGO with fibre dispersion (rho, mu, lambda, k1,
  * k2, kappa; fibres from ft_set_fibres).  K = lambda + 2 mu / 3 is the bulk
- * modulus of the volumetric term K/4 (J^2 - 1 - 2 ln J) of 4-8. */
+ * modulus of the volumetric term K/4 (J^2 - 1 - 2 ln J) of 4-8.  At most
+ * 89,478,485 nodes and as many elements (FT_ERR_ARG beyond). */
 int ft_engine_create(int n_nodes, int n_elems, const double *coords,
                      const int *conn, const int *pid, int n_parts,
                      const int *material_id, const double *props,

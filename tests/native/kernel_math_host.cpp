@@ -35,13 +35,16 @@ int run(const double *X24, const double *U24, int mat, const double *p9, double 
   double nu = 0.5 * pp.lambda / (pp.lambda + pp.mu);
   pp.ce = sqrt(pp.lambda * (1.0 / nu - 1.0) / pp.rho);
   pp.inv_ce = 1.0 / pp.ce;
-  R xv[3][8], uv[3][8];
-  for (int p = 0; p < 8; ++p) {
-    int a = ft::p2n(p);
-    for (int d = 0; d < 3; ++d) {
-      xv[d][p] = p == 0 ? R(0) : R(X24[3 * a + d] - X24[d]);
-      uv[d][p] = p == 0 ? R(0) : R(U24[3 * a + d] - U24[d]);
+  // the element kernel's gather: FP64 nodal values in cube-position order
+  R xv[7][3], uv[7][3];
+  for (int d = 0; d < 3; ++d) {
+    double vx[8], vu[8];
+    for (int p = 0; p < 8; ++p) {
+      vx[p] = X24[3 * ft::p2n(p) + d];
+      vu[p] = U24[3 * ft::p2n(p) + d];
     }
+    ft::field_coeffs(vx, xv, d);
+    ft::field_coeffs(vu, uv, d);
   }
   R hl[144];
   for (int k = 0; k < 144; ++k) hl[k] = hist144 ? R(hist144[k]) : R(0);
