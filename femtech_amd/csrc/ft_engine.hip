@@ -1784,6 +1784,39 @@ int force_segment_mat(ft_engine *g, const ft_engine::Seg &sg, int waves, bool in
   return launch_element<R, true, DT, MAT>(g, t0, t1, in_loop, dt_fixed, dtbits, update_hist, b);
 }
 
+// Scratch (private) bytes per thread of the FP32 run-loop element kernel of
+// a material at an occupancy variant (0 when unknown).
+template <int MAT>
+size_t elem_scratch_mat(int waves) {
+  const void *f = nullptr;
+  switch (waves) {
+    case 1: f = reinterpret_cast<const void *>(&k_element<float, true, true, MAT, 1>); break;
+    case 2: f = reinterpret_cast<const void *>(&k_element<float, true, true, MAT, 2>); break;
+    case 3: f = reinterpret_cast<const void *>(&k_element<float, true, true, MAT, 3>); break;
+    case 4: f = reinterpret_cast<const void *>(&k_element<float, true, true, MAT, 4>); break;
+    case 5: f = reinterpret_cast<const void *>(&k_element<float, true, true, MAT, 5>); break;
+    default: f = reinterpret_cast<const void *>(&k_element<float, true, true, MAT, 0>); break;
+  }
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, f) != hipSuccess) return 0;
+  return fa.localSizeBytes;
+}
+
+size_t elem_scratch(int mat, int waves) {
+  switch (mat) {
+    case 0: return elem_scratch_mat<0>(waves);
+    case 1: return elem_scratch_mat<1>(waves);
+    case 2: return elem_scratch_mat<2>(waves);
+    case 3: return elem_scratch_mat<3>(waves);
+    case 4: return elem_scratch_mat<4>(waves);
+    case 5: return elem_scratch_mat<5>(waves);
+    case 6: return elem_scratch_mat<6>(waves);
+    case 7: return elem_scratch_mat<7>(waves);
+    case 8: return elem_scratch_mat<8>(waves);
+  }
+  return 0;
+}
+
 template <typename R, bool DT>
 int force_segment(ft_engine *g, const ft_engine::Seg &sg, int waves, bool in_loop, double dt_fixed,
                   unsigned long long *dtbits, int update_hist) {
@@ -3222,6 +3255,8 @@ int ft_autotune(ft_engine *g, int repeats, double *report, int *n_segments) {
         }
       }
       for (int w = fp32 ? 1 : 0; w <= (fp32 ? kMaxTuneWaves : 0) && !rc; ++w) {
+        // a variant that spills registers to scratch memory is not a candidate
+        if (fp32 && elem_scratch(sg.mat, w) > 0) continue;
         double us = 0;
         if ((rc = time_one(si, w, us))) break;
         if (us < best) {
