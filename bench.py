@@ -175,6 +175,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=8000, help="Adam steps of the surrogate training")
     ap.add_argument("--graph", type=int, default=0,
                     help="steps per captured HIP graph (0: one launch per kernel)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="run the timed steps in one cooperative launch (grid-wide barriers between the "
+                         "phases of every step); implies --kernel-timing 0")
     ap.add_argument("--autotune", type=int, default=5,
                     help="timed launches per element-kernel shape in the autotuner (0: default shapes)")
     ap.add_argument("--kernel-timing", type=int, default=50,
@@ -187,6 +190,8 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     args = ap.parse_args()
+    if args.persistent:
+        args.kernel_timing = 0  # per-launch events need launches per step
     if args.gpus != 1:
         print("bench.py: the engine holds the whole mesh on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
@@ -215,6 +220,7 @@ def main():
     e.run_init(float(np.max(w["end_times"])) if batch else 1e9, 0.9, 1e-11)
     tune = e.autotune(args.autotune) if args.autotune > 0 else None
     e.set_graph(args.graph)
+    e.set_persistent(args.persistent)
     if args.warmup:
         e.run_steps(args.warmup)
     n_inst = len(w["info"]) if batch else 1
@@ -304,7 +310,8 @@ def main():
         "nodal_kernel": {"achieved": nodal_gbs, "frac": frac(nodal_gbs),
                          "traffic": measured[1] if measured else None,
                          "algorithmic_bytes_per_launch": traffic[1], "avg_launch_ms": nodal_ms},
-        "launch": {"graph_steps": args.graph, "kernel_timing_every": args.kernel_timing,
+        "launch": {"graph_steps": args.graph, "persistent": bool(args.persistent),
+                   "kernel_timing_every": args.kernel_timing,
                    "sampled_steps": sampled if timed else 0,
                    "autotune": None if tune is None else
                    [{"material": int(r[0]), "elements": int(r[1]), "waves": int(r[2]), "tile": int(r[3]),

@@ -71,6 +71,7 @@ _SIGS = {
     "ft_get_visco_history": [_VP, _D],
     "ft_set_timing": [_VP, ctypes.c_int],
     "ft_set_graph": [_VP, ctypes.c_int],
+    "ft_set_persistent": [_VP, ctypes.c_int],
     "ft_autotune": [_VP, ctypes.c_int, _D, _I],
     "ft_get_segments": [_VP, _I, _I],
     "ft_set_segment_shape": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
@@ -403,6 +404,11 @@ class Engine:
     def set_graph(self, steps):
         """Steps per captured HIP graph in run_steps (0: separate launches)."""
         _check(self._lib.ft_set_graph(self._h, int(steps)))
+
+    def set_persistent(self, enable=True):
+        """run_steps in one cooperative launch (grid-wide barriers between
+        the phases of every step) instead of launches per step."""
+        _check(self._lib.ft_set_persistent(self._h, 1 if enable else 0))
 
     def segments(self):
         """Material segments: rows (material, begin, end, waves, tile)."""

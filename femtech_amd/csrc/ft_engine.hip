@@ -720,7 +720,18 @@ struct NodalArgs {
   const int *cap;         // per instance: last step this call may complete
   double factor, failure_dt;
   int *status;            // engine status word (FT_STATUS_* bits)
+  int inline_motions;     // motion states evaluated by the tile itself (mstate unused)
+  int *running;           // nullable: the first tile of each instance that goes on adds 1
 };
+
+// Time of motion state w (0: t_{k-1}, 1: t_k, 2: t_{k+1} = t_k + factor *
+// min Le/c) for the nodal update of step k, rounded step by step (the same
+// bits wherever it is evaluated).
+__device__ inline double motion_state_time(const StepSlot &cur, double factor, int w) {
+  if (w == 0) return cur.t_prev;
+  if (w == 1) return cur.t;
+  return __dadd_rn(cur.t, __dmul_rn(factor, bits_to_double(cur.dtmin_bits)));
+}
 
 // End of step k (gather, a = f/m, v = v_half + dt/2 a, energy) fused with the
 // start of step k+1 (v_half, u += dt v_half, boundary motions at t_{k+1}),
@@ -754,8 +765,13 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   // this instance's motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
   __shared__ MotionT ms[3][kMaxMotions];
   const int mb = a.mrange[2 * inst], mc = a.mrange[2 * inst + 1];
-  for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x)
-    ms[i / mc][i % mc] = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
+  for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x) {
+    if (a.inline_motions)
+      motion_time(a.motions + (size_t)FT_MOTION_STRIDE * (mb + i % mc), motion_state_time(cur, a.factor, i / mc),
+                  ms[i / mc][i % mc]);
+    else
+      ms[i / mc][i % mc] = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
+  }
   __syncthreads();
   const double *motions = a.motions + (size_t)FT_MOTION_STRIDE * mb;
   // Energy (CheckEnergy.cpp:15-52): the terms of step k pair each DOF's
@@ -879,6 +895,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
     if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(a.status, kStatusDtFromFp32);
     nx.fresh = a.mode == kStep;
     ic->slot[a.parity ^ 1] = nx;
+    if (go && a.running) atomicAdd(a.running, 1);
   }
 }
 
@@ -895,16 +912,13 @@ __global__ void k_motion_states(const Ctl *ctl, int parity, double factor, const
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nm; i += gridDim.x * blockDim.x) {
     const int w = i / nm, k = i % nm;
     const StepSlot cur = ctl[motion_inst[k]].slot[parity];
-    const double t_next = cur.t + factor * bits_to_double(cur.dtmin_bits);
-    const double tt = w == 0 ? cur.t_prev : (w == 1 ? cur.t : t_next);
-    motion_time(motions + FT_MOTION_STRIDE * k, tt, out[i]);
+    motion_time(motions + FT_MOTION_STRIDE * k, motion_state_time(cur, factor, w), out[i]);
   }
 }
 
 // Per-step history row (time, dt, W_int, W_ext, KE, |KE + W_int - W_ext|):
 // sums the per-block energy accumulators in block order.
-__global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, double *rows,
-                          int cap) {
+__device__ inline void history_row(const double *eacc, int nblk, Ctl *ctl, int parity, double *rows, int cap) {
   __shared__ double sh[kBlock / 64];
   double w = 0, x = 0, k = 0;
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
@@ -928,6 +942,158 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
       row[5] = fabs(k + w - x);
     }
     ctl->hist_count = i + 1;
+  }
+}
+
+__global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, double *rows, int cap) {
+  history_row(eacc, nblk, ctl, parity, rows, cap);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent run loop: one cooperative launch runs up to `steps` steps of the
+// loop (element phase, node phase, history row) for every instance, the
+// phases separated by grid-wide barriers.  All blocks are co-resident (the
+// launch is cooperative), every block passes the same barriers, and the loop
+// ends for all of them together: after a step in which no instance went on.
+// ---------------------------------------------------------------------------
+constexpr int kStatusBarrier = 1 << 20;  // a barrier wait hit its bound
+
+template <typename R>
+struct PersistArgs {
+  ElemArgs<R> ea;        // all element tiles
+  NodalArgs<R> na;       // all node tiles, kStep
+  int steps;
+  int parity0;
+  int n_inst;
+  unsigned long long spins;    // bound of a barrier wait, in polls
+  unsigned long long *arrive;  // barrier arrivals (zero at launch)
+  int *running;                // 3 counters, the first zero at launch
+  const double *eacc;          // history row (hist_cap > 0)
+  int nblk;
+  double *rows;
+  int hist_cap;
+};
+
+// Grid-wide barrier number `b` (1, 2, ...) of the launch.  Every wave waits
+// for its own memory operations to complete (its stores are then in its
+// XCD's L2); one thread per block makes the block's writes visible at device
+// scope (release), arrives on a 64-bit counter that only grows, polls it
+// without cache maintenance until b * gridDim.x blocks have arrived, and
+// then acquires once (the block's waves share the CU's L1 and the XCD's L2).
+// The wait is bounded: past `max_spins` polls, or once another block has
+// given up, the block raises kStatusBarrier and goes on, so no wave waits
+// for ever (the run's results are then void and the host reports the error).
+__device__ inline void grid_barrier(unsigned long long *arrive, unsigned long long b, int *status,
+                                    unsigned long long max_spins) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long target = b * (unsigned long long)gridDim.x;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long spins = 0;
+    while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(16);
+      if ((++spins & 255) == 0 &&
+          (spins > max_spins ||
+           (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kStatusBarrier))) {
+        atomicOr(status, kStatusBarrier);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Tile i of this block in a phase of n tiles: each XCD (blocks b with the
+// same b mod 8) takes a contiguous eighth of the tiles, as xcd_block does for
+// the one-tile-per-block launches.
+__device__ inline int persist_tile(int n, int i) {
+  const int per_xcd = gridDim.x >> 3;
+  const int chunk = (n + 7) >> 3;
+  const int x = blockIdx.x & 7;
+  const int L = x * chunk + (blockIdx.x >> 3) + i * per_xcd;
+  return L < min(n, (x + 1) * chunk) ? L : -1;
+}
+
+// Occupancy target of the persistent kernel (waves per SIMD): the FP32
+// neo-Hookean body is held to 168 VGPRs (3 waves per SIMD, no spills).
+template <typename R, int MAT>
+constexpr int persist_waves() {
+  return (sizeof(R) == 4 && MAT == 1) ? 3 : 1;
+}
+
+// Element tile L with the kernel body of its material (tiles never mix
+// materials; the switch is uniform over the block).
+template <typename R>
+__device__ __forceinline__ void element_tile_any(const ElemArgs<R> &a, int L) {
+  const int4 tile = a.tiles[L];
+  switch (a.parts[a.einfo[tile.x] & kPidMask].mat) {
+    case 0: element_tile<R, true, true, 0>(a, L); break;
+    case 1: element_tile<R, true, true, 1>(a, L); break;
+    case 2: element_tile<R, true, true, 2>(a, L); break;
+    case 3: element_tile<R, true, true, 3>(a, L); break;
+    case 4: element_tile<R, true, true, 4>(a, L); break;
+    case 5: element_tile<R, true, true, 5>(a, L); break;
+    case 6: element_tile<R, true, true, 6>(a, L); break;
+    case 7: element_tile<R, true, true, 7>(a, L); break;
+    case 8: element_tile<R, true, true, 8>(a, L); break;
+    default: element_tile<R, true, true, -1>(a, L); break;  // raises the unknown-material bit
+  }
+}
+
+// MAT = 1: a mesh of neo-Hookean elements only; MAT = -1: each tile runs its
+// material's body (element_tile_any).
+template <typename R, int MAT, int W>
+__global__ __launch_bounds__(kBlock, (persist_waves<R, MAT>())) void k_persistent(PersistArgs<R> p) {
+  ElemArgs<R> ea = p.ea;
+  NodalArgs<R> na = p.na;
+  int parity = p.parity0;
+  unsigned long long b = 0;
+  for (int k = 0; k < p.steps; ++k) {
+    ea.parity = parity;
+    // a block's consecutive tiles reuse the same LDS: each starts after a
+    // block barrier
+    for (int i = 0;; ++i) {
+      const int L = persist_tile(ea.n_tiles, i);
+      if (L < 0) break;
+      __syncthreads();
+      if constexpr (MAT >= 0)
+        element_tile<R, true, true, MAT>(ea, L);
+      else
+        element_tile_any<R>(ea, L);
+    }
+    grid_barrier(p.arrive, ++b, na.status, p.spins);
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.running[(k + 1) % 3] = 0;
+    na.parity = parity;
+    na.running = p.running + k % 3;
+    for (int i = 0;; ++i) {
+      const int L = persist_tile(na.n_tiles, i);
+      if (L < 0) break;
+      __syncthreads();
+      nodal_tile<R, W>(na, L);
+    }
+    grid_barrier(p.arrive, ++b, na.status, p.spins);
+    parity ^= 1;
+    // the history row of this step (before any block reaches the next node
+    // phase, which rewrites the accumulators)
+    if (p.hist_cap > 0 && blockIdx.x == 0) history_row(p.eacc, p.nblk, na.ctl, parity, p.rows, p.hist_cap);
+    const int going = __hip_atomic_load(p.running + k % 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int abort = __hip_atomic_load(na.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kStatusBarrier;
+    if (going == 0 || abort) {
+      // every instance has stopped: both ping-pong slots of each get its
+      // final state (as the no-op steps of the launch-per-step loop leave
+      // them), so the caller's parity bookkeeping holds for any `steps`
+      if (going == 0 && blockIdx.x == 0) {
+        for (int inst = threadIdx.x; inst < p.n_inst; inst += blockDim.x) {
+          StepSlot nx = na.ctl[inst].slot[parity];
+          nx.fresh = 0;
+          na.ctl[inst].slot[parity ^ 1] = nx;
+        }
+      }
+      break;
+    }
   }
 }
 
@@ -1586,6 +1752,11 @@ struct ft_engine {
   // graph_steps == 0 launches every step eagerly.
   int graph_steps = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // Persistent run loop (ft_set_persistent): barrier counter, 3 running
+  // counters, blocks of the cooperative launch.
+  bool persistent = false;
+  unsigned long long *p_arrive = nullptr;
+  int *p_running = nullptr;
   // stream of the next launch (the engine's stream)
   hipStream_t lstream = nullptr;
   // Constitutive surrogate of materials 7 and 8 (FP32 run loop): network,
@@ -1952,6 +2123,8 @@ NodalArgs<R> nodal_args(ft_engine *g, int mode, int t0, int t1) {
   a.factor = g->factor;
   a.failure_dt = g->failure_dt;
   a.status = g->status;
+  a.inline_motions = 0;
+  a.running = nullptr;
   return a;
 }
 
@@ -1984,6 +2157,63 @@ int launch_nodal(ft_engine *g, int mode) {
   if (rc) return rc;
   g->parity ^= 1;
   return 0;
+}
+
+// One cooperative launch of k_persistent for `steps` steps.  The element
+// phase runs the neo-Hookean body alone when the whole mesh is material 1,
+// otherwise every material's body behind a per-tile switch (the kernel's
+// registers are then those of the largest); the per-segment launch shapes of
+// the autotuner do not apply.
+template <typename R, int MAT, int W>
+int launch_persistent_k(ft_engine *g, int steps) {
+  PersistArgs<R> p;
+  p.ea = elem_args<R>(g, 0, g->n_etiles, true, 0.0, nullptr, 1);
+  p.ea.want_force = 1;
+  p.ea.want_dt = 1;
+  p.na = nodal_args<R>(g, kStep, 0, g->n_ntiles);
+  p.na.inline_motions = g->timed_motions ? 1 : 0;
+  p.steps = steps;
+  p.parity0 = g->parity;
+  p.n_inst = g->n_inst;
+  // A phase takes well under 1 ns per element; a poll takes at least 400 ns
+  // (the sleep alone): ne / 10 polls leave a margin of 40x or more.
+  p.spins = (1ull << 16) + (unsigned long long)g->ne / 10;
+  p.arrive = g->p_arrive;
+  p.running = g->p_running;
+  p.eacc = g->eacc;
+  p.nblk = g->ntile_of_inst[1];
+  p.rows = g->hist_rows;
+  p.hist_cap = g->hist_cap;
+  const void *f = reinterpret_cast<const void *>(&k_persistent<R, MAT, W>);
+  int per_cu = 0, n_cu = 0, coop = 0;
+  HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, g->device));
+  if (!coop) return fail(FT_ERR_HIP, "the device does not take cooperative launches");
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, 0));
+  HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, g->device));
+  const long need = std::max(g->n_etiles, g->n_ntiles);
+  long grid = std::min<long>((long)per_cu * n_cu, (need + 7) & ~7L) & ~7L;
+  if (grid < 8) return fail(FT_ERR_HIP, "persistent launch: %d blocks per CU do not fill 8 XCDs", per_cu);
+  HIPCHK(hipMemsetAsync(g->p_arrive, 0, sizeof(unsigned long long), g->stream));
+  HIPCHK(hipMemsetAsync(g->p_running, 0, 3 * sizeof(int), g->stream));
+  void *args[] = {&p};
+  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)grid), dim3(kBlock), args, 0, g->stream));
+  return 0;
+}
+
+int launch_persistent(ft_engine *g, int steps) {
+  if (!g->p_arrive) {
+    int rc;
+    if ((rc = dalloc(&g->p_arrive, 1)) || (rc = dalloc(&g->p_running, 3)))
+      return rc;
+  }
+  bool all_nh = !g->segs.empty();
+  for (const auto &sg : g->segs) all_nh = all_nh && sg.mat == 1;
+  if (g->precision == FT_FP32) {
+    if (g->corner) return all_nh ? launch_persistent_k<float, 1, -1>(g, steps) : launch_persistent_k<float, -1, -1>(g, steps);
+    return launch_persistent_k<float, -1, 0>(g, steps);
+  }
+  if (g->corner) return launch_persistent_k<double, -1, -1>(g, steps);
+  return launch_persistent_k<double, -1, 0>(g, steps);
 }
 
 InjuryArgs injury_args(ft_engine *g, const StepSlot *slot, double time, double dt) {
@@ -2382,7 +2612,7 @@ int ft_engine_destroy(ft_engine *g) {
   drop_graphs(g);
   if (g->mlp) ft::mlp_destroy(g->mlp);
   for (void *p : {(void *)g->surr_norm, (void *)g->surr_feat, (void *)g->surr_y, (void *)g->surr_tally,
-                  (void *)g->ds_x, (void *)g->ds_t})
+                  (void *)g->ds_x, (void *)g->ds_t, (void *)g->p_arrive, (void *)g->p_running})
     if (p) (void)hipFree(p);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
@@ -2737,6 +2967,11 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   if ((rc = upload_caps(g, cap))) return rc;
   if ((rc = launch_nodal(g, kResume))) return rc;  // start step before + 1
   int k = 0;
+  if (g->persistent && !g->timing && !g->surr_on && !g->inj_on) {
+    if ((rc = launch_persistent(g, max_steps))) return rc;
+    g->parity ^= max_steps & 1;
+    k = max_steps;
+  }
   if (g->graph_steps > 0 && !g->timing && !g->surr_on) {
     for (; k + g->graph_steps <= max_steps; k += g->graph_steps) {
       if (!g->gexec[g->parity] && (rc = capture_chunk(g))) return rc;
@@ -2752,6 +2987,7 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   if (steps_done) *steps_done = done;
   int st = 0;
   HIPCHK(hipMemcpy(&st, g->status, sizeof(int), hipMemcpyDeviceToHost));
+  if (st & kStatusBarrier) return fail(FT_ERR_HIP, "persistent launch: a grid barrier wait ran out; the run is void");
   if (st & 8) return fail(FT_ERR_MATERIAL, "Unknown material type");
   // A batch keeps going when one of its instances stops without a stable
   // step (ft_get_instance reports it); a single run ends as the reference's
@@ -3169,6 +3405,13 @@ int ft_surrogate_stats(ft_engine *g, double out[6]) {
   out[3] = v[5];                                // audited Gauss-point evaluations
   out[4] = v[4];                                // surrogate-segment Gauss-point evaluations
   out[5] = v[3];                                // evaluations that fell back to the analytic law
+  return 0;
+}
+
+int ft_set_persistent(ft_engine *g, int enable) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  drop_graphs(g);
+  g->persistent = enable != 0;
   return 0;
 }
 

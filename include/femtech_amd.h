@@ -235,6 +235,20 @@ int ft_injury_results(ft_engine *eng, double out[21], int *flags);
  * first and end element (internal, grouped order), waves (0 = default), tile.
  * ft_set_segment_shape sets one segment's shape by hand. */
 int ft_set_graph(ft_engine *eng, int steps);
+/* ft_set_persistent (no counterpart in the reference): with enable != 0,
+ * ft_run_steps runs all of its steps in one cooperative launch, the element
+ * phase, the node phase and the history row of every step separated by
+ * grid-wide barriers, and ends early once no instance goes on.  Results
+ * agree with the per-step launches to floating-point round-off (same bodies,
+ * tiles and reduction order; the compiler may fuse multiply-adds differently
+ * in the one kernel).  On MI355X it is slower than the per-step launches for
+ * every mesh size measured.  The element phase runs the neo-Hookean
+ * body when the whole mesh is material 1 and the run-time dispatched body of
+ * all laws otherwise; autotuned segment shapes do not apply.  Runs with
+ * kernel timing, the surrogate or the injury criteria keep the per-step
+ * launches.  A barrier wait is bounded (about 0.3 s); if one runs out the
+ * call fails with FT_ERR_HIP and the run is void. */
+int ft_set_persistent(ft_engine *eng, int enable);
 int ft_autotune(ft_engine *eng, int repeats, double *report, int *n_segments);
 int ft_get_segments(ft_engine *eng, int *n_segments, int *desc);
 int ft_set_segment_shape(ft_engine *eng, int segment, int waves, int tile);
