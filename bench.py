@@ -57,6 +57,35 @@ def calibrated_traffic(ne, precision):
     return float(m.group(1)), float(m.group(2))
 
 
+def fp64_accuracy(w, steps, u32, energy32, principal32):
+    """Error of the FP32-element run against an FP64 run of the same workload
+    and step count on the GPU (the FP64 engine, whose displacements match the
+    FP64 oracle to 1e-10 relative in the parity tests): relative L2 nodal
+    displacement, energy-balance error relative to the largest energy term,
+    relative error of the peak (over elements) maximum principal strain."""
+    import femtech_amd as ft
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP64) as r:
+        r.set_boundary(w["dof_motion"], w["motions"])
+        r.set_state(None, w["v0"])
+        r.lumped_mass()
+        r.run_init(1e9, 0.9, 1e-11)
+        t0 = time.perf_counter()
+        r.run_steps(steps)
+        u64 = r.state()[0]
+        e64 = r.energy()
+        p64 = r.element_strain()[1].reshape(-1, 3)[:, 0]
+        seconds = time.perf_counter() - t0
+    scale = float(np.max(np.abs(e64[:3])))
+    p32 = principal32.reshape(-1, 3)[:, 0]
+    return {"reference": "FP64 engine on the GPU, same mesh, loading and step count",
+            "steps": int(steps),
+            "rel_l2_displacement": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            "energy_balance_err": float(abs(energy32[3] - e64[3]) / scale),
+            "peak_max_principal_strain_rel": float(abs(p32.max() - p64.max()) / abs(p64.max())),
+            "bars": {"rel_l2_displacement": 1e-5, "energy_balance_err": 1e-4},
+            "fp64_run_s": seconds}
+
+
 def cpu_baseline(w, steps, threads):
     """The oracle (FP64 C++ restatement, OpenMP over elements with the same
     deterministic gather) on the host cores, on a bounded sample of the same
@@ -187,6 +216,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="impact workload, FP32: skip the FP64 run that states the FP32 result's error")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     args = ap.parse_args()
@@ -236,6 +267,8 @@ def main():
     e.set_timing(0)
     E = e.energy()
     status = e.status()
+    final_u = e.state()[0] if args.workload == "impact" and prec == ft.FT_FP32 else None
+    final_p = e.element_strain()[1] if final_u is not None else None
     elapsed = t1 - t0
     if batch:
         # every instance advances by its own dt; instances that reached their
@@ -322,6 +355,9 @@ def main():
         "energy": {"W_int": E[0], "W_ext": E[1], "KE": E[2], "balance": E[3]},
         "status": status,
     }
+    if final_u is not None and not args.no_accuracy:
+        e.close()
+        out["accuracy_vs_fp64"] = fp64_accuracy(w, args.warmup + args.steps, final_u, E, final_p)
     if args.copy_probe > 0:
         out["copy_probe_gbs"] = ft.copy_probe(args.copy_probe, 5)
     if not args.no_cpu and not batch:
