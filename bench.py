@@ -65,6 +65,8 @@ def fp64_accuracy(w, steps, u32, energy32, principal32):
     relative error of the peak (over elements) maximum principal strain."""
     import femtech_amd as ft
     with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP64) as r:
+        if w.get("fibres") is not None:
+            r.set_fibres(w["fibres"])
         r.set_boundary(w["dof_motion"], w["motions"])
         r.set_state(None, w["v0"])
         r.lumped_mass()
@@ -217,7 +219,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
-                    help="impact workload, FP32: skip the FP64 run that states the FP32 result's error")
+                    help="single-mesh FP32 runs: skip the FP64 run that states the FP32 result's error")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     args = ap.parse_args()
@@ -267,7 +269,7 @@ def main():
     e.set_timing(0)
     E = e.energy()
     status = e.status()
-    final_u = e.state()[0] if args.workload == "impact" and prec == ft.FT_FP32 else None
+    final_u = e.state()[0] if args.workload != "batch" and prec == ft.FT_FP32 else None
     final_p = e.element_strain()[1] if final_u is not None else None
     elapsed = t1 - t0
     if batch:
