@@ -107,8 +107,9 @@ def cpu_baseline(w, steps, threads):
     ne = w["conn"].size // 8
     o.close()
     return dict(value=ne * steps / loop, unit="element-steps/s", cores=threads, kind="port",
-                sample="%d explicit steps of the same %d-element mesh (FP64 oracle, %d OpenMP threads), "
-                       "loop time %.2f s" % (steps, ne, threads, loop))
+                sample="%d explicit step%s of the same %d-element mesh (FP64 oracle, %d OpenMP thread%s), "
+                       "loop time %.2f s" % (steps, "" if steps == 1 else "s", ne, threads,
+                                             "" if threads == 1 else "s", loop))
 
 
 def surrogate_bench(args):
@@ -365,6 +366,8 @@ def main():
     if not args.no_cpu and not batch:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, threads)
+        # the same restatement on one core (one step: a few seconds)
+        out["cpu_baseline_1core"] = cpu_baseline(w, 1, 1)
     print(json.dumps(out), flush=True)
 
 
