@@ -74,6 +74,36 @@ def test_engine_arguments_checked_before_the_device():
     assert rc == ft.FT_ERR_ARG and not h.value
 
 
+def test_mesh_size_limit():
+    """Node counts past the 32-bit byte offsets of the kernels' gathers are
+    rejected up front (the arrays are not read)."""
+    lib = ft.library()
+    h = ctypes.c_void_p()
+    d = np.zeros(3)
+    one = np.zeros(8, np.int32)
+    mid = np.array([1], np.int32)
+    props = np.zeros(9)
+    rc = lib.ft_engine_create(100_000_000, 1, d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              one.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              np.zeros(1, np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1,
+                              mid.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              props.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ft.FT_FP64, ctypes.byref(h))
+    assert rc == ft.FT_ERR_ARG and not h.value
+    assert b"too large" in lib.ft_last_error()
+
+
+def test_null_engine_is_an_argument_error():
+    """Engine entry points reject a null handle with FT_ERR_ARG."""
+    lib = ft.library()
+    out = ctypes.c_int(0)
+    d = ctypes.c_double(0.0)
+    assert lib.ft_set_persistent(None, 1) == ft.FT_ERR_ARG
+    assert lib.ft_set_graph(None, 4) == ft.FT_ERR_ARG
+    assert lib.ft_run_steps(None, 1, ctypes.byref(out)) == ft.FT_ERR_ARG
+    assert lib.ft_get_time(None, ctypes.byref(d), ctypes.byref(d), ctypes.byref(out)) == ft.FT_ERR_ARG
+    assert lib.ft_autotune(None, 1, None, ctypes.byref(out)) == ft.FT_ERR_ARG
+
+
 def test_network_without_device_fails_loudly():
     if ft.device_count() > 0:
         pytest.skip("a GPU is present")
