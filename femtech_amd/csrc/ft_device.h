@@ -66,6 +66,16 @@ template <> struct Num<float> {
     return 1.0f / x;
 #endif
   }
+  // hardware reciprocal square root (1 ulp) in the FP32 element path
+  __host__ __device__ static float rsqrt_(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+  }
+  // x / c for a constant c: FP32 multiplies by the rounded reciprocal
+  __host__ __device__ static float div_c(float x, double c) { return x * float(1.0 / c); }
 };
 template <> struct Num<double> {
   __host__ __device__ static double log1p_(double x) { return log1p(x); }
@@ -74,6 +84,8 @@ template <> struct Num<double> {
   __host__ __device__ static double exp_(double x) { return exp(x); }
   __host__ __device__ static double sqrt_(double x) { return sqrt(x); }
   __host__ __device__ static double rcp_(double x) { return 1.0 / x; }
+  __host__ __device__ static double rsqrt_(double x) { return 1.0 / sqrt(x); }
+  __host__ __device__ static double div_c(double x, double c) { return x / c; }
 };
 
 // Position p = i + 2j + 4k of the 2x2x2 node cube (i: xi, j: eta, k: zeta
@@ -212,13 +224,23 @@ __host__ __device__ inline void sym_eig3(const Sym<R> &m, R w[3], R V[3][3]) {
     for (int r = 0; r < 3; ++r) {
       const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2, o = 3 - p - q;
       const R apq = a[p][q];
-      if (apq == R(0)) continue;
-      const R theta = (a[q][q] - a[p][p]) / (R(2) * apq);
+      if constexpr (sizeof(R) == 4) {
+        // The FP32 reciprocal of a vanishing (denormal) 2 apq is infinite,
+        // and 0 * inf would poison the rotation: skip rotations of an
+        // off-diagonal term below 1e-18 of the diagonal pair (far below
+        // round-off) or below 1e-30 (strains of no consequence).
+        if (!(fabs(apq) > R(1e-30) && fabs(apq) > R(1e-18) * (fabs(a[p][p]) + fabs(a[q][q])))) continue;
+      } else {
+        if (apq == R(0)) continue;
+      }
+      // (FP64 divides; FP32 uses the 1-ulp reciprocal and reciprocal square
+      // root: the rotation stays orthogonal to round-off)
+      const R theta = (a[q][q] - a[p][p]) * Num<R>::rcp_(R(2) * apq);
       // t = tan of the rotation angle, the smaller root (|t| <= 1)
-      R t = R(1) / (fabs(theta) + Num<R>::sqrt_(theta * theta + R(1)));
+      R t = Num<R>::rcp_(fabs(theta) + Num<R>::sqrt_(theta * theta + R(1)));
       if (theta < R(0)) t = -t;
-      if (!(fabs(theta) < R(1e18))) t = R(0.5) / theta;  // theta^2 would overflow
-      const R c = R(1) / Num<R>::sqrt_(t * t + R(1)), sn = t * c;
+      if (!(fabs(theta) < R(1e18))) t = R(0.5) * Num<R>::rcp_(theta);  // theta^2 would overflow
+      const R c = Num<R>::rsqrt_(t * t + R(1)), sn = t * c;
       a[p][p] -= t * apq;
       a[q][q] += t * apq;
       a[p][q] = a[q][p] = R(0);
@@ -262,15 +284,15 @@ __host__ __device__ inline Sym<R> b_minus_identity(const R H[3][3]) {
 template <typename R>
 __host__ __device__ inline void ogden_tau(const PartDev &pp, const Sym<R> &bm, R Jm1, Sym<R> &tau, R &p) {
   const R mu = R(pp.mu), alpha = R(pp.alpha);
-  const R K = R(pp.lambda) + R(2) * mu / R(3);
-  const R lnJ3 = Num<R>::log1p_(Jm1) / R(3);
+  const R K = R(pp.lambda) + Num<R>::div_c(R(2) * mu, 3.0);
+  const R lnJ3 = Num<R>::div_c(Num<R>::log1p_(Jm1), 3.0);
   R w[3], V[3][3];
   sym_eig3<R, sizeof(R) == 4 ? 4 : 6>(bm, w, V);
   R e[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) e[a] = Num<R>::expm1_(alpha * (R(0.5) * Num<R>::log1p_(w[a]) - lnJ3));
-  const R mean = (e[0] + e[1] + e[2]) / R(3);
-  const R f = R(2) * mu / alpha;
+  const R mean = Num<R>::div_c(e[0] + e[1] + e[2], 3.0);
+  const R f = R(2) * mu * Num<R>::rcp_(alpha);
   R t[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) t[a] = f * (e[a] - mean);
@@ -424,14 +446,14 @@ __host__ __device__ inline bool pk2_stress(const PartDev &pp, const R H[3][3], R
     // HGOIsotropic.cpp:37-90 with kappa = 1/3:
     //   S = J F^-1 sigma F^-T = C^-1 [J^-2/3 (mu + fib) dev(2E) + K/2 (J^2-1) I]
     const R kappa = R(1.0 / 3.0);
-    const R K = lambda + R(2) * mu / R(3);
+    const R K = lambda + Num<R>::div_c(R(2) * mu, 3.0);
     const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * lnJ);
     const R Jm23 = R(1) + Jm23m1;
     const R Ea = kappa * (R(3) * Jm23m1 + R(2) * Jm23 * trE);
     R fib = R(0);
     if (Ea > R(0)) fib = R(2) * R(pp.k1) * Num<R>::exp_(R(pp.k2) * Ea * Ea) * Ea * kappa;
     const R a = Jm23 * (mu + fib);
-    const R tr3 = R(2) * trE / R(3);
+    const R tr3 = Num<R>::div_c(R(2) * trE, 3.0);
     Sym<R> Tdev;
 #pragma unroll
     for (int k = 0; k < 6; ++k) Tdev.v[k] = a * R(2) * E.v[k];
@@ -496,7 +518,7 @@ __host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3]
   const R lnJ = Num<R>::log1p_(Jm1);
   if (MAT == 8) {
     const R kappa = R(pp.kappa);
-    const R K = lambda + R(2) * mu / R(3);
+    const R K = lambda + Num<R>::div_c(R(2) * mu, 3.0);
     const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * lnJ);
     const R Jm23 = R(1) + Jm23m1;
     const R trb = bm.v[0] + bm.v[1] + bm.v[2];
@@ -516,7 +538,7 @@ __host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3]
     if (E > R(0)) psi = R(pp.k1) * E * Num<R>::exp_(R(pp.k2) * E * E);
     const R c1 = Jm23 * (mu + R(2) * psi * kappa);
     const R c4 = Jm23 * R(2) * psi * (R(1) - R(3) * kappa);
-    const R p = R(0.5) * K * Jm1 * (Jm1 + R(2)) - c1 * trb / R(3) - c4 * (R(1) + s4) / R(3);
+    const R p = R(0.5) * K * Jm1 * (Jm1 + R(2)) - Num<R>::div_c(c1 * trb, 3.0) - Num<R>::div_c(c4 * (R(1) + s4), 3.0);
     const int I[6] = {0, 1, 2, 1, 0, 0}, J[6] = {0, 1, 2, 2, 2, 1};
 #pragma unroll
     for (int k = 0; k < 6; ++k) tau.v[k] = c1 * bm.v[k] + c4 * fa[I[k]] * fa[J[k]];
@@ -533,7 +555,7 @@ __host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3]
     tau.v[2] += lambda * lnJ;
   } else {
     const R kappa = R(1.0 / 3.0);
-    const R K = lambda + R(2) * mu / R(3);
+    const R K = lambda + Num<R>::div_c(R(2) * mu, 3.0);
     const R Jm23m1 = Num<R>::expm1_(R(-2.0 / 3.0) * lnJ);
     const R Jm23 = R(1) + Jm23m1;
     const R trb = bm.v[0] + bm.v[1] + bm.v[2];  // tr(b - I) = tr(C - I)
@@ -541,7 +563,7 @@ __host__ __device__ inline void kirchhoff_stress(const PartDev &pp, const R H[3]
     R fib = R(0);
     if (Ea > R(0)) fib = R(2) * R(pp.k1) * Num<R>::exp_(R(pp.k2) * Ea * Ea) * Ea * kappa;
     const R a = Jm23 * (mu + fib);
-    const R p = R(0.5) * K * Jm1 * (Jm1 + R(2)) - a * trb / R(3);
+    const R p = R(0.5) * K * Jm1 * (Jm1 + R(2)) - Num<R>::div_c(a * trb, 3.0);
 #pragma unroll
     for (int k = 0; k < 6; ++k) tau.v[k] = a * bm.v[k];
     tau.v[0] += p;
@@ -829,7 +851,7 @@ __host__ __device__ inline void iso_pk2(const PartDev &pp, const R H[3][3], R Jm
     ogden_tau(pp, b_minus_identity(H), Jm1, tau, p);
   } else {
     kirchhoff_stress<R, 8>(pp, H, Jm1, fib, tau);
-    const R K = R(pp.lambda) + R(2) * R(pp.mu) / R(3);
+    const R K = R(pp.lambda) + Num<R>::div_c(R(2) * R(pp.mu), 3.0);
     p = R(0.5) * K * Jm1 * (Jm1 + R(2));
     tau.v[0] -= p;
     tau.v[1] -= p;
@@ -927,7 +949,7 @@ __host__ __device__ inline void surr_pk2(const PartDev &pp, const R H[3][3], con
   bool inside = true;
 #pragma unroll
   for (int k = 0; k < 12; ++k) inside = inside && f[k] >= R(in.norm[32 + k]) && f[k] <= R(in.norm[48 + k]);
-  const R K = R(pp.lambda) + R(2) * R(pp.mu) / R(3);
+  const R K = R(pp.lambda) + Num<R>::div_c(R(2) * R(pp.mu), 3.0);
   R p = R(0.5) * K * Jm1 * (Jm1 + R(2));
   Sym<R> Siso;
   if (inside) {

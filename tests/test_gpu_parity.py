@@ -372,6 +372,27 @@ def test_ellipsoid_three_regions_fp32_split(gpu_ready):
     assert np.abs((en[:, 2] + en[:, 0] - en[:, 1]) - (ref[:, 2] + ref[:, 0] - ref[:, 1])).max() / scale < 1e-4
 
 
+def test_ellipsoid_fp32_through_vanishing_strain(gpu_ready):
+    """The start of the rotational pulse (strains rising from zero through
+    the FP32 denormal range) stays finite in FP32 and follows the FP64 run:
+    the Ogden eigen-solver's reciprocals never meet a vanishing off-diagonal
+    term."""
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=20)
+    outs = []
+    for prec in (ft.FT_FP32, ft.FT_FP64):
+        with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], prec) as e:
+            e.set_boundary(w["dof_motion"], w["motions"])
+            e.set_fibres(w["fibres"])
+            e.lumped_mass()
+            e.run_init(1e9, 0.9, 1e-11)
+            assert e.run_steps(200) == 200
+            outs.append((e.state()[0], e.status()))
+    (u32, st32), (u64, _) = outs
+    assert np.all(np.isfinite(u32)) and not st32 & (ft.STATUS_NONFINITE | ft.STATUS_DT_COLLAPSE)
+    assert rel(u32, u64) < 1e-5
+
+
 def test_new_laws_element_forces_fp64(gpu_ready):
     """Materials 6-8 in the device kernels against the oracle."""
     from test_cpu_kernel_math import new_law_block, field
