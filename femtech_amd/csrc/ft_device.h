@@ -48,7 +48,21 @@ template <> struct Num<float> {
     return log1pf(x);
   }
   __host__ __device__ static float fma_(float a, float b, float c) { return fmaf(a, b, c); }
-  __host__ __device__ static float expm1_(float x) { return expm1f(x); }
+  // exp(x) - 1: 8-term series for |x| < 1/16 (truncation < 1e-13
+  // relative), the library routine otherwise
+  __host__ __device__ static float expm1_(float x) {
+    if (fabsf(x) < 0.0625f) {
+      float p = 1.0f / 5040.0f;
+      p = p * x + 1.0f / 720.0f;
+      p = p * x + 1.0f / 120.0f;
+      p = p * x + 1.0f / 24.0f;
+      p = p * x + 1.0f / 6.0f;
+      p = p * x + 0.5f;
+      p = p * x + 1.0f;
+      return p * x;
+    }
+    return expm1f(x);
+  }
   __host__ __device__ static float exp_(float x) { return expf(x); }
   // hardware square root (1 ulp) in the FP32 element path
   __host__ __device__ static float sqrt_(float x) {
@@ -205,6 +219,12 @@ __host__ __device__ inline Sym<R> sym_mul(const Sym<R> &A, const Sym<R> &T, R s)
   return S;
 }
 
+// Whether an FP32 Jacobi rotation of the off-diagonal term apq of the
+// diagonal pair (app, aqq) changes anything at FP32 round-off.
+__host__ __device__ inline bool jacobi_active(float apq, float app, float aqq) {
+  return fabsf(apq) > 1e-30f && fabsf(apq) > 1e-12f * (fabsf(app) + fabsf(aqq));
+}
+
 // Eigenvalues w and eigenvectors (columns of V) of a symmetric 3x3 matrix by
 // cyclic Jacobi rotations, a fixed number of sweeps (converged to round-off
 // for strain-sized matrices; repeated eigenvalues need no special case).
@@ -227,9 +247,9 @@ __host__ __device__ inline void sym_eig3(const Sym<R> &m, R w[3], R V[3][3]) {
       if constexpr (sizeof(R) == 4) {
         // The FP32 reciprocal of a vanishing (denormal) 2 apq is infinite,
         // and 0 * inf would poison the rotation: skip rotations of an
-        // off-diagonal term below 1e-18 of the diagonal pair (far below
+        // off-diagonal term below 1e-12 of the diagonal pair (far below FP32
         // round-off) or below 1e-30 (strains of no consequence).
-        if (!(fabs(apq) > R(1e-30) && fabs(apq) > R(1e-18) * (fabs(a[p][p]) + fabs(a[q][q])))) continue;
+        if (!jacobi_active(apq, a[p][p], a[q][q])) continue;
       } else {
         if (apq == R(0)) continue;
       }
@@ -253,6 +273,12 @@ __host__ __device__ inline void sym_eig3(const Sym<R> &m, R w[3], R V[3][3]) {
         V[k][p] = c * vp - sn * vq;
         V[k][q] = sn * vp + c * vq;
       }
+    }
+    if constexpr (sizeof(R) == 4) {
+      // converged to FP32 round-off: the remaining sweeps would not rotate
+      if (!jacobi_active(a[0][1], a[0][0], a[1][1]) && !jacobi_active(a[0][2], a[0][0], a[2][2]) &&
+          !jacobi_active(a[1][2], a[1][1], a[2][2]))
+        break;
     }
   }
   w[0] = a[0][0];
