@@ -737,7 +737,10 @@ __device__ inline double motion_state_time(const StepSlot &cur, double factor, i
 // start of step k+1 (v_half, u += dt v_half, boundary motions at t_{k+1}),
 // ex9.cpp:107-175.  vs holds v_half(k) for free DOFs and the prescribed v
 // for constrained ones while a run is in progress, v(k) once it stops.
-template <typename R, int W>
+// ROT: some motion of the run is a rotation; its parameters are then staged
+// in LDS (a rotation DOF evaluates them three times per step), which costs
+// the other runs a little occupancy.
+template <typename R, int W, bool ROT = true>
 __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   __shared__ double sh[kBlock / 64];
   if (L >= a.n_tiles) return;  // grid padding
@@ -771,6 +774,13 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
                   ms[i / mc][i % mc]);
     else
       ms[i / mc][i % mc] = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
+  }
+  // the parameters motion_apply reads (type, axis, centre), in LDS when
+  // rotations are present
+  __shared__ double mp[ROT ? kMaxMotions : 1][8];
+  if constexpr (ROT) {
+    for (int i = threadIdx.x; i < 7 * mc; i += blockDim.x)
+      mp[i / 7][i % 7] = a.motions[(size_t)FT_MOTION_STRIDE * (mb + i / 7) + i % 7];
   }
   __syncthreads();
   const double *motions = a.motions + (size_t)FT_MOTION_STRIDE * mb;
@@ -808,7 +818,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
       const double vso = a.vs[id];
       double acc, v;
       double ub = 0.0, vb = 0.0, ab = 0.0;
-      const double *mo = motions + FT_MOTION_STRIDE * (mi < 0 ? 0 : mi);
+      const double *mo = ROT ? mp[mi < 0 ? 0 : mi] : motions + FT_MOTION_STRIDE * (mi < 0 ? 0 : mi);
       if (mi < 0) {
         acc = (0.0 - f[d]) / m;
         v = (a.mode == kStep) ? vso + (t_k - t_khalf) * acc : vso;
@@ -899,9 +909,9 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   }
 }
 
-template <typename R, int W>
+template <typename R, int W, bool ROT>
 __global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
-  nodal_tile<R, W>(a, xcd_block());
+  nodal_tile<R, W, ROT>(a, xcd_block());
 }
 
 // Scalar state of every motion at t_{k-1}, t_k and t_{k+1} = t_k + dt_{k+1}
@@ -1721,6 +1731,7 @@ struct ft_engine {
   double *motions = nullptr;
   int n_motions = 0;
   bool timed_motions = false;  // any motion other than FIXED
+  bool rot_motions = false;    // any rotation motion
   MotionT *mstate = nullptr;
   std::vector<int> h_dof_motion;
   std::vector<double> h_motions;
@@ -2134,12 +2145,21 @@ int launch_nodal_t(ft_engine *g, int mode, int t0, int t1) {
   const NodalArgs<R> a = nodal_args<R>(g, mode, t0, t1);
   Timed t(g, 1);
   const dim3 grid(grid_for_tiles(t1 - t0));
-  if (g->corner)
-    hipLaunchKernelGGL((k_nodal<R, -1>), grid, dim3(kBlock), 0, g->lstream, a);
-  else if (g->width == 8)
-    hipLaunchKernelGGL((k_nodal<R, 8>), grid, dim3(kBlock), 0, g->lstream, a);
-  else
-    hipLaunchKernelGGL((k_nodal<R, 0>), grid, dim3(kBlock), 0, g->lstream, a);
+  if (g->rot_motions) {
+    if (g->corner)
+      hipLaunchKernelGGL((k_nodal<R, -1, true>), grid, dim3(kBlock), 0, g->lstream, a);
+    else if (g->width == 8)
+      hipLaunchKernelGGL((k_nodal<R, 8, true>), grid, dim3(kBlock), 0, g->lstream, a);
+    else
+      hipLaunchKernelGGL((k_nodal<R, 0, true>), grid, dim3(kBlock), 0, g->lstream, a);
+  } else {
+    if (g->corner)
+      hipLaunchKernelGGL((k_nodal<R, -1, false>), grid, dim3(kBlock), 0, g->lstream, a);
+    else if (g->width == 8)
+      hipLaunchKernelGGL((k_nodal<R, 8, false>), grid, dim3(kBlock), 0, g->lstream, a);
+    else
+      hipLaunchKernelGGL((k_nodal<R, 0, false>), grid, dim3(kBlock), 0, g->lstream, a);
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2365,8 +2385,12 @@ int rebuild_boundary(ft_engine *g) {
   }
   g->n_motions = (int)minst.size();
   g->timed_motions = false;
-  for (int k = 0; k < g->n_motions; ++k)
-    if ((int)mexp[FT_MOTION_STRIDE * (size_t)k] != FT_MOTION_FIXED) g->timed_motions = true;
+  g->rot_motions = false;
+  for (int k = 0; k < g->n_motions; ++k) {
+    const int type = (int)mexp[FT_MOTION_STRIDE * (size_t)k];
+    if (type != FT_MOTION_FIXED) g->timed_motions = true;
+    if (type == FT_MOTION_ROT_HALFSINE) g->rot_motions = true;
+  }
   for (void *p : {(void *)g->motions, (void *)g->motion_inst, (void *)g->mstate})
     if (p) (void)hipFree(p);
   g->motions = nullptr;
