@@ -134,6 +134,23 @@ def test_surrogate_run_close_to_plain(gpu_ready, small_surrogate):
     assert abs(e1[3]) <= 1e-2 * max(abs(e1[:3]).max(), 1e-30) + abs(e0[3])
 
 
+def test_training_gains_over_the_untrained_network(gpu_ready, small_surrogate):
+    """What training gains: the same run with the network's initial weights
+    (same normalisation and training box) has an audited error of the
+    isochoric stress of order one, the trained network a few percent at
+    most; the analytic law is the judge of both (the audit) and the fallback
+    of both.  (The speed of the surrogate against the plain loop is measured
+    by bench.py --workload ellipsoid --surrogate.)"""
+    params, norm, _ = small_surrogate
+    with ft.Mlp(seed=3) as m:
+        initial = m.params()
+    _, _, trained = run_ellipsoid(12, (7.0e4, 2.0e-3), 300, (params, norm))
+    _, _, untrained = run_ellipsoid(12, (7.0e4, 2.0e-3), 300, (initial, norm))
+    assert trained[0] < 0.1, trained
+    assert untrained[0] > 0.3, untrained
+    assert untrained[0] > 10 * trained[0]
+
+
 def test_fallback_outside_training_box(gpu_ready, small_surrogate):
     """With an empty training box every Gauss point falls back to the
     analytic law: the run then follows the plain run to FP32 round-off."""
