@@ -126,3 +126,24 @@ def test_persistent_ends_with_the_run(gpu_ready):
     assert outs[0][0] == outs[1][0] and outs[0][0] < 500
     assert outs[0][1][2] == outs[1][1][2] and close(outs[1][1][:2], outs[0][1][:2], 1e-13)
     assert close(outs[1][2], outs[0][2], 1e-9)
+
+
+def test_persistent_falls_back_with_injury_criteria(gpu_ready):
+    """Runs with the per-step injury update keep the per-step launches: the
+    persistent switch changes nothing there, results included."""
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+    outs = []
+    for persistent in (False, True):
+        with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP32) as e:
+            e.set_boundary(w["dof_motion"], w["motions"])
+            e.set_fibres(w["fibres"])
+            e.lumped_mass()
+            e.injury_enable()
+            e.run_init(1e9, 0.9, 1e-11)
+            e.set_persistent(persistent)
+            e.run_steps(60)
+            res, fl = e.injury_results()
+            outs.append((e.state()[0], res, fl))
+    for x, y in zip(*outs):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
