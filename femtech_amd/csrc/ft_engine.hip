@@ -910,7 +910,10 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
 }
 
 template <typename R, int W, bool ROT>
-__global__ __launch_bounds__(kBlock) void k_nodal(NodalArgs<R> a) {
+// 7 waves per SIMD: the node update is bound by memory latency, and 71
+// VGPRs (no spills) keep 7 of the 8 wave slots in flight (5 at the
+// compiler's unconstrained 85: 5 % slower on the 1M block).
+__global__ __launch_bounds__(kBlock, 7) void k_nodal(NodalArgs<R> a) {
   nodal_tile<R, W, ROT>(a, xcd_block());
 }
 
