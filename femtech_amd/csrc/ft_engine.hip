@@ -722,7 +722,16 @@ struct NodalArgs {
   int *status;            // engine status word (FT_STATUS_* bits)
   int inline_motions;     // motion states evaluated by the tile itself (mstate unused)
   int *running;           // nullable: the first tile of each instance that goes on adds 1
+  int mcap;               // largest motion count of any instance (>= 1): dynamic LDS rows
 };
+
+// Dynamic LDS of a node tile of a run with rotations: the motion states of
+// its instance at the three instants, [3][mcap] MotionT, then the motion
+// parameters, [mcap][8] (runs without rotations use a fixed-size table).
+template <bool ROT>
+size_t nodal_lds_bytes(int mcap) {
+  return ROT ? 3 * (size_t)mcap * sizeof(MotionT) + (size_t)mcap * 8 * sizeof(double) : 0;
+}
 
 // Time of motion state w (0: t_{k-1}, 1: t_k, 2: t_{k+1} = t_k + factor *
 // min Le/c) for the nodal update of step k, rounded step by step (the same
@@ -766,18 +775,29 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   const bool collapse = !(dtmin >= a.failure_dt) || !(dtmin < 1e308);
   const bool go = (t_k < a.tmax[inst]) && (cur.step < a.cap[inst]) && !collapse;
   // this instance's motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
-  __shared__ MotionT ms[3][kMaxMotions];
+  // and, when rotations are present, the parameters motion_apply reads (type,
+  // axis, centre), in LDS sized for the run's largest motion count
+  // (runs without rotations keep the fixed-size table: its constant strides
+  // cost no registers)
+  extern __shared__ __align__(16) unsigned char ft_nodal_lds[];
+  __shared__ MotionT ms_fixed[3][ROT ? 1 : kMaxMotions];
+  MotionT *ms_dyn = reinterpret_cast<MotionT *>(ft_nodal_lds);
+  double(*mp)[8] = reinterpret_cast<double(*)[8]>(ms_dyn + 3 * a.mcap);
+  // motion state w (0: t_{k-1}, 1: t_k, 2: t_{k+1}) of the instance's motion j
+  auto ms = [&](int w, int j) -> MotionT & {
+    if constexpr (ROT)
+      return ms_dyn[w * a.mcap + j];
+    else
+      return ms_fixed[w][j];
+  };
   const int mb = a.mrange[2 * inst], mc = a.mrange[2 * inst + 1];
   for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x) {
     if (a.inline_motions)
       motion_time(a.motions + (size_t)FT_MOTION_STRIDE * (mb + i % mc), motion_state_time(cur, a.factor, i / mc),
-                  ms[i / mc][i % mc]);
+                  ms(i / mc, i % mc));
     else
-      ms[i / mc][i % mc] = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
+      ms(i / mc, i % mc) = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
   }
-  // the parameters motion_apply reads (type, axis, centre), in LDS when
-  // rotations are present
-  __shared__ double mp[ROT ? kMaxMotions : 1][8];
   if constexpr (ROT) {
     for (int i = threadIdx.x; i < 7 * mc; i += blockDim.x)
       mp[i / 7][i % 7] = a.motions[(size_t)FT_MOTION_STRIDE * (mb + i / 7) + i % 7];
@@ -823,7 +843,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
         acc = (0.0 - f[d]) / m;
         v = (a.mode == kStep) ? vso + (t_k - t_khalf) * acc : vso;
       } else {
-        motion_apply(mo, ms[1][mi], Xn, d, ub, vb, ab);
+        motion_apply(mo, ms(1, mi), Xn, d, ub, vb, ab);
         acc = ab;
         v = vb;
       }
@@ -835,7 +855,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
           dd = dt_k * vso;
         } else {
           double up, vp, ap;
-          motion_apply(mo, ms[0][mi], Xn, d, up, vp, ap);
+          motion_apply(mo, ms(0, mi), Xn, d, up, vp, ap);
           dd = ub - up;
           sWext += dd * (f[d] + m * acc);
         }
@@ -850,7 +870,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
           pInt += (dt_next * vh) * f[d];
         } else {
           double un, vn, an;
-          motion_apply(mo, ms[2][mi], Xn, d, un, vn, an);
+          motion_apply(mo, ms(2, mi), Xn, d, un, vn, an);
           a.u[id] = un;
           a.vs[id] = vn;
           const double dd1 = un - ub;
@@ -912,8 +932,9 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
 template <typename R, int W, bool ROT>
 // 7 waves per SIMD: the node update is bound by memory latency, and 71
 // VGPRs (no spills) keep 7 of the 8 wave slots in flight (5 at the
-// compiler's unconstrained 85: 5 % slower on the 1M block).
-__global__ __launch_bounds__(kBlock, 7) void k_nodal(NodalArgs<R> a) {
+// compiler's unconstrained 85: 5 % slower on the 1M block).  The rotation
+// path needs a few more registers: 6 waves.
+__global__ __launch_bounds__(kBlock, ROT ? 6 : 7) void k_nodal(NodalArgs<R> a) {
   nodal_tile<R, W, ROT>(a, xcd_block());
 }
 
@@ -1735,6 +1756,7 @@ struct ft_engine {
   int n_motions = 0;
   bool timed_motions = false;  // any motion other than FIXED
   bool rot_motions = false;    // any rotation motion
+  int mcap = 1;                // largest motion count of any instance (at least 1)
   MotionT *mstate = nullptr;
   std::vector<int> h_dof_motion;
   std::vector<double> h_motions;
@@ -2139,6 +2161,7 @@ NodalArgs<R> nodal_args(ft_engine *g, int mode, int t0, int t1) {
   a.status = g->status;
   a.inline_motions = 0;
   a.running = nullptr;
+  a.mcap = g->mcap;
   return a;
 }
 
@@ -2149,19 +2172,21 @@ int launch_nodal_t(ft_engine *g, int mode, int t0, int t1) {
   Timed t(g, 1);
   const dim3 grid(grid_for_tiles(t1 - t0));
   if (g->rot_motions) {
+    const size_t lds = nodal_lds_bytes<true>(g->mcap);
     if (g->corner)
-      hipLaunchKernelGGL((k_nodal<R, -1, true>), grid, dim3(kBlock), 0, g->lstream, a);
+      hipLaunchKernelGGL((k_nodal<R, -1, true>), grid, dim3(kBlock), lds, g->lstream, a);
     else if (g->width == 8)
-      hipLaunchKernelGGL((k_nodal<R, 8, true>), grid, dim3(kBlock), 0, g->lstream, a);
+      hipLaunchKernelGGL((k_nodal<R, 8, true>), grid, dim3(kBlock), lds, g->lstream, a);
     else
-      hipLaunchKernelGGL((k_nodal<R, 0, true>), grid, dim3(kBlock), 0, g->lstream, a);
+      hipLaunchKernelGGL((k_nodal<R, 0, true>), grid, dim3(kBlock), lds, g->lstream, a);
   } else {
+    const size_t lds = nodal_lds_bytes<false>(g->mcap);
     if (g->corner)
-      hipLaunchKernelGGL((k_nodal<R, -1, false>), grid, dim3(kBlock), 0, g->lstream, a);
+      hipLaunchKernelGGL((k_nodal<R, -1, false>), grid, dim3(kBlock), lds, g->lstream, a);
     else if (g->width == 8)
-      hipLaunchKernelGGL((k_nodal<R, 8, false>), grid, dim3(kBlock), 0, g->lstream, a);
+      hipLaunchKernelGGL((k_nodal<R, 8, false>), grid, dim3(kBlock), lds, g->lstream, a);
     else
-      hipLaunchKernelGGL((k_nodal<R, 0, false>), grid, dim3(kBlock), 0, g->lstream, a);
+      hipLaunchKernelGGL((k_nodal<R, 0, false>), grid, dim3(kBlock), lds, g->lstream, a);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -2211,7 +2236,8 @@ int launch_persistent_k(ft_engine *g, int steps) {
   int per_cu = 0, n_cu = 0, coop = 0;
   HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, g->device));
   if (!coop) return fail(FT_ERR_HIP, "the device does not take cooperative launches");
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, 0));
+  const size_t lds = nodal_lds_bytes<true>(g->mcap);  // the persistent node phase stages rotations
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kBlock, lds));
   HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, g->device));
   const long need = std::max(g->n_etiles, g->n_ntiles);
   long grid = std::min<long>((long)per_cu * n_cu, (need + 7) & ~7L) & ~7L;
@@ -2219,7 +2245,7 @@ int launch_persistent_k(ft_engine *g, int steps) {
   HIPCHK(hipMemsetAsync(g->p_arrive, 0, sizeof(unsigned long long), g->stream));
   HIPCHK(hipMemsetAsync(g->p_running, 0, 3 * sizeof(int), g->stream));
   void *args[] = {&p};
-  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)grid), dim3(kBlock), args, 0, g->stream));
+  HIPCHK(hipLaunchCooperativeKernel(f, dim3((unsigned)grid), dim3(kBlock), args, lds, g->stream));
   return 0;
 }
 
@@ -2387,6 +2413,8 @@ int rebuild_boundary(ft_engine *g) {
       }
   }
   g->n_motions = (int)minst.size();
+  g->mcap = 1;
+  for (int i = 0; i < g->n_inst; ++i) g->mcap = std::max(g->mcap, mr[2 * i + 1]);
   g->timed_motions = false;
   g->rot_motions = false;
   for (int k = 0; k < g->n_motions; ++k) {
