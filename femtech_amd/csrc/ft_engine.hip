@@ -270,15 +270,24 @@ __device__ __forceinline__ const T &at_byte(const T *base, unsigned off) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
 
-// Prony history, component-major in HBM: value k of Gauss point gp of
-// element e at [(gp * 18 + k) * ne + e] (coalesced across a wavefront).
+// Prony history in HBM, blocked by 64 elements: component c = 18 gp + k of
+// element e at [((e / 64) * 144 + c) * 64 + e % 64].  A wavefront's 64
+// elements read one contiguous 256-byte run per component (coalesced), and
+// all 144 runs of the wavefront lie in one contiguous 36 KB block (FP32).
+constexpr int kHistBlock = 64;
+__host__ __device__ inline size_t hist_index(int c, int e) {
+  return ((size_t)(e / kHistBlock) * 144 + c) * kHistBlock + e % kHistBlock;
+}
+// values in the history array of ne elements (whole blocks)
+inline size_t hist_len(int ne) { return 144 * (((size_t)ne + kHistBlock - 1) / kHistBlock * kHistBlock); }
+
 template <typename R>
 struct HistSoA {
   R *base;
   int ne, e;
   __device__ bool valid() const { return base != nullptr; }
-  __device__ R load(int gp, int k) const { return base[(size_t)(gp * 18 + k) * ne + e]; }
-  __device__ void store(int gp, int k, R v) { base[(size_t)(gp * 18 + k) * ne + e] = v; }
+  __device__ R load(int gp, int k) const { return base[hist_index(gp * 18 + k, e)]; }
+  __device__ void store(int gp, int k, R v) { base[hist_index(gp * 18 + k, e)] = v; }
 };
 
 // A tile is the work of one block: a range of at most kBlock consecutive
@@ -299,7 +308,7 @@ struct ElemArgs {
   const PartDev *parts;
   R *fe;                 // element forces (nullable when only dt is wanted)
   int corner;            // fe[corner][node][xyz] instead of fe[slot][element][xyz]
-  R *hist;               // 144 per element, component-major [c * ne + e]
+  R *hist;               // 144 per element, in 64-element blocks (hist_index)
   const R *fib;          // fibre directions [3][ne] (material 8)
   Ctl *ctl;              // run loop: per-instance control, slot `parity` gives dt_k and done
   int parity;
@@ -2592,7 +2601,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   // unwritten corner slots must read as zero for the whole run
   if (hipMemset(g->fe, 0, fe_len * esz) != hipSuccess) return bail(fail(FT_ERR_HIP, "memset failed"));
   g->h_conn.assign(conn, conn + 8 * (size_t)n_elems);
-  if (g->has_visco && hipMalloc(&g->hist, 144 * (size_t)n_elems * esz) != hipSuccess)
+  if (g->has_visco && hipMalloc(&g->hist, hist_len(n_elems) * esz) != hipSuccess)
     return bail(fail(FT_ERR_ALLOC, "history allocation failed"));
   g->nodal_grid = grid_for(n_nodes);
   // Group elements by material (stable counting sort on the material id).
@@ -2649,7 +2658,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   HIPCHK(hipMemset(g->node_code, 0, (size_t)n_nodes * sizeof(int)));
   HIPCHK(hipMemset(g->node_mask, 0, (size_t)n_nodes));
   HIPCHK(hipMemset(g->status, 0, sizeof(int)));
-  if (g->hist) HIPCHK(hipMemset(g->hist, 0, 144 * (size_t)n_elems * esz));
+  if (g->hist) HIPCHK(hipMemset(g->hist, 0, hist_len(n_elems) * esz));
   g->h_dof_motion.assign(nd, -1);
   // one instance covering the whole mesh
   g->n_inst = 1;
@@ -2755,7 +2764,7 @@ int ft_reset_history(ft_engine *g) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   if (g->hist) {
     const size_t esz = g->precision == FT_FP32 ? 4 : 8;
-    HIPCHK(hipMemsetAsync(g->hist, 0, 144 * (size_t)g->ne * esz, g->stream));
+    HIPCHK(hipMemsetAsync(g->hist, 0, hist_len(g->ne) * esz, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
   }
   return 0;
@@ -3175,23 +3184,23 @@ int ft_get_status(ft_engine *g, int *status) {
 
 int ft_get_visco_history(ft_engine *g, double *hist) {
   if (!g || !hist) return fail(FT_ERR_ARG, "null argument");
-  const size_t n = 144 * (size_t)g->ne;
+  const size_t n = 144 * (size_t)g->ne, len = hist_len(g->ne);
   if (!g->hist) {
     memset(hist, 0, n * sizeof(double));
     return 0;
   }
   HIPCHK(hipStreamSynchronize(g->stream));
-  std::vector<double> cm(n);
+  std::vector<double> cm(len);
   if (g->precision == FT_FP32) {
-    std::vector<float> t(n);
-    HIPCHK(hipMemcpy(t.data(), g->hist, n * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n; ++i) cm[i] = t[i];
+    std::vector<float> t(len);
+    HIPCHK(hipMemcpy(t.data(), g->hist, len * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < len; ++i) cm[i] = t[i];
   } else {
-    HIPCHK(hipMemcpy(cm.data(), g->hist, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cm.data(), g->hist, len * sizeof(double), hipMemcpyDeviceToHost));
   }
-  // component-major [c * ne + internal e] -> element-major original order
-  for (size_t c = 0; c < 144; ++c)
-    for (size_t e = 0; e < (size_t)g->ne; ++e) hist[e * 144 + c] = cm[c * g->ne + g->inv[e]];
+  // 64-element blocks, internal order -> element-major original order
+  for (size_t e = 0; e < (size_t)g->ne; ++e)
+    for (int c = 0; c < 144; ++c) hist[e * 144 + c] = cm[hist_index(c, g->inv[e])];
   return 0;
 }
 
@@ -3500,7 +3509,7 @@ int ft_autotune(ft_engine *g, int repeats, double *report, int *n_segments) {
   // the history the timed launches update: a copy of the engine's
   void *hist_run = g->hist, *hist_copy = nullptr;
   if (hist_run) {
-    const size_t bytes = 144 * (size_t)g->ne * (fp32 ? 4 : 8);
+    const size_t bytes = hist_len(g->ne) * (fp32 ? 4 : 8);
     if ((rc = dalloc((char **)&hist_copy, bytes))) {
       g->timing = timing;
       return rc;
