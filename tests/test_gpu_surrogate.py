@@ -138,8 +138,8 @@ def test_training_gains_over_the_untrained_network(gpu_ready, small_surrogate):
     """What training gains: the same run with the network's initial weights
     (same normalisation and training box) has an audited error of the
     isochoric stress of order one, the trained network a few percent at
-    most; the analytic law is the judge of both (the audit) and the fallback
-    of both.  (The speed of the surrogate against the plain loop is measured
+    most; the analytic law is the reference of both (the audit) and the
+    fallback of both.  (The speed of the surrogate against the plain loop is measured
     by bench.py --workload ellipsoid --surrogate.)"""
     params, norm, _ = small_surrogate
     with ft.Mlp(seed=3) as m:
