@@ -102,6 +102,17 @@ template <> struct Num<double> {
   __host__ __device__ static double div_c(double x, double c) { return x / c; }
 };
 
+// Prony history in HBM, blocked by 64 elements: component c = 18 gp + k of
+// element e at [((e / 64) * 144 + c) * 64 + e % 64].  A wavefront's 64
+// elements read one contiguous 256-byte run per component (coalesced), and
+// all 144 runs of the wavefront lie in one contiguous 36 KB block (FP32).
+constexpr int kHistBlock = 64;
+__host__ __device__ inline size_t hist_index(int c, int e) {
+  return ((size_t)(e / kHistBlock) * 144 + c) * kHistBlock + e % kHistBlock;
+}
+// values in the history array of ne elements (whole blocks)
+inline size_t hist_len(int ne) { return 144 * (((size_t)ne + kHistBlock - 1) / kHistBlock * kHistBlock); }
+
 // Position p = i + 2j + 4k of the 2x2x2 node cube (i: xi, j: eta, k: zeta
 // sign bit) -> C3D8 local node number.
 __host__ __device__ constexpr int p2n(int p) {

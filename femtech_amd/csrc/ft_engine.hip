@@ -270,16 +270,7 @@ __device__ __forceinline__ const T &at_byte(const T *base, unsigned off) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
 
-// Prony history in HBM, blocked by 64 elements: component c = 18 gp + k of
-// element e at [((e / 64) * 144 + c) * 64 + e % 64].  A wavefront's 64
-// elements read one contiguous 256-byte run per component (coalesced), and
-// all 144 runs of the wavefront lie in one contiguous 36 KB block (FP32).
-constexpr int kHistBlock = 64;
-__host__ __device__ inline size_t hist_index(int c, int e) {
-  return ((size_t)(e / kHistBlock) * 144 + c) * kHistBlock + e % kHistBlock;
-}
-// values in the history array of ne elements (whole blocks)
-inline size_t hist_len(int ne) { return 144 * (((size_t)ne + kHistBlock - 1) / kHistBlock * kHistBlock); }
+// Prony history in HBM: 64-element blocks (hist_index, hist_len of ft_device.h).
 
 template <typename R>
 struct HistSoA {

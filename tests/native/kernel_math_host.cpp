@@ -90,3 +90,7 @@ extern "C" int km_element(int fp32, const double *X24, const double *U24, int ma
   return fp32 ? run<float>(X24, U24, mat, p9, dt, hist144, update, f24, dte, fib3)
               : run<double>(X24, U24, mat, p9, dt, hist144, update, f24, dte, fib3);
 }
+
+// The Prony history layout of the element kernel (64-element blocks).
+extern "C" long long km_hist_index(int c, int e) { return (long long)ft::hist_index(c, e); }
+extern "C" long long km_hist_len(int ne) { return (long long)ft::hist_len(ne); }

@@ -153,3 +153,22 @@ def test_new_laws_element_forces(km, fp32):
             den += np.sum(fo[e] ** 2)
         err = np.sqrt(num / den)
         assert err < (1e-6 if fp32 else 1e-11), (scale, err)
+
+
+@pytest.mark.parametrize("ne", [1, 63, 64, 65, 200, 1728])
+def test_history_layout_blocks(km, ne):
+    """The Prony history layout the element kernel loads and stores: every
+    (component, element) pair has its own slot inside the allocation; the 64
+    elements of a block read one contiguous run per component; a block's 144
+    runs fill one contiguous range of 144 x 64 slots."""
+    km.km_hist_index.restype = ctypes.c_longlong
+    km.km_hist_len.restype = ctypes.c_longlong
+    n = km.km_hist_len(ne)
+    assert n == 144 * (-(-ne // 64) * 64)
+    idx = np.array([[km.km_hist_index(c, e) for c in range(144)] for e in range(ne)])
+    assert idx.min() == 0 and idx.max() < n
+    assert len(np.unique(idx)) == idx.size
+    for b in range(0, ne, 64):
+        blk = idx[b:b + 64]
+        assert np.all(np.diff(blk, axis=0) == 1)  # consecutive elements, one run per component
+        assert blk.min() == 144 * b and blk.max() < 144 * (b + 64)
