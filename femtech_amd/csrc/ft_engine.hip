@@ -2009,6 +2009,10 @@ size_t elem_scratch_mat(int waves) {
   return fa.localSizeBytes;
 }
 
+// Largest register spill (bytes of scratch per lane) of a variant the
+// autotuner still times.
+constexpr size_t kTuneSpillMax = 128;
+
 size_t elem_scratch(int mat, int waves) {
   switch (mat) {
     case 0: return elem_scratch_mat<0>(waves);
@@ -3553,8 +3557,10 @@ int ft_autotune(ft_engine *g, int repeats, double *report, int *n_segments) {
         }
       }
       for (int w = fp32 ? 1 : 0; w <= (fp32 ? kMaxTuneWaves : 0) && !rc; ++w) {
-        // a variant that spills registers to scratch memory is not a candidate
-        if (fp32 && elem_scratch(sg.mat, w) > 0) continue;
+        // a variant that spills more than a few registers to scratch memory
+        // is not a candidate (kilobyte spills ran 20x slower; a few dozen
+        // bytes can still win through the higher occupancy)
+        if (fp32 && elem_scratch(sg.mat, w) > kTuneSpillMax) continue;
         double us = 0;
         if ((rc = time_one(si, w, us))) break;
         if (us < best) {

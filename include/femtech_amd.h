@@ -227,8 +227,8 @@ int ft_injury_results(ft_engine *eng, double out[21], int *flags);
  * every tile size (64, 128, 256 elements per block) and, in FP32, every
  * occupancy variant (1..5 waves per SIMD), `repeats` launches each, plus the
  * default shape (256 elements per block, compiler-chosen occupancy), and
- * keeps the fastest; occupancy variants that spill registers to scratch
- * memory are skipped.  report (may be NULL): 6 doubles per segment = material,
+ * keeps the fastest; occupancy variants that spill more than 128 bytes of
+ * registers per lane to scratch memory are skipped.  report (may be NULL): 6 doubles per segment = material,
  * elements, waves, tile, best us per launch, default-shape us per launch.
  * The run state is unchanged (the Prony history is not advanced).
  * ft_get_segments: desc (may be NULL) gets 5 ints per segment = material,
