@@ -53,6 +53,7 @@ _SIGS = {
                          ctypes.c_int, ctypes.POINTER(_VP)],
     "ft_engine_destroy": [_VP],
     "ft_engine_info": [_VP, _I, _I, _I],
+    "ft_engine_layout": [_VP, _I, _I],
     "ft_set_boundary": [_VP, _I, ctypes.c_int, _D],
     "ft_set_state": [_VP, _D, _D],
     "ft_reset_history": [_VP],
@@ -265,6 +266,13 @@ class Engine:
 
     def __exit__(self, *exc):
         self.close()
+
+    def layout(self):
+        """(corner, ell_width): 1 when element forces go to per-node corner
+        slots, 0 when they are gathered through the node-to-element table."""
+        c, w = ctypes.c_int(0), ctypes.c_int(0)
+        _check(self._lib.ft_engine_layout(self._h, ctypes.byref(c), ctypes.byref(w)))
+        return c.value, w.value
 
     # --- set-up -----------------------------------------------------------
     def set_boundary(self, dof_motion, motions):

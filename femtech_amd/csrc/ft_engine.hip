@@ -2700,6 +2700,13 @@ int ft_engine_info(ft_engine *g, int *n_nodes, int *n_elems, int *precision) {
   return 0;
 }
 
+int ft_engine_layout(ft_engine *g, int *corner, int *ell_width) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (corner) *corner = g->corner ? 1 : 0;
+  if (ell_width) *ell_width = g->width;
+  return 0;
+}
+
 int ft_set_boundary(ft_engine *g, const int *dof_motion, int n_motions, const double *motions) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   drop_graphs(g);

@@ -80,6 +80,13 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords,
                      int precision, ft_engine **out);
 int ft_engine_destroy(ft_engine *eng);
 int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
+/* Force-gather layout chosen at creation (no reference counterpart; the
+ * reference scatter-adds, GetForce_3D.cpp:39-44).  corner = 1: every node is
+ * local corner k of at most one element for each k, so element forces go to
+ * per-node corner slots; 0: slot-major element forces summed through the
+ * node-to-element table.  ell_width: that table's width (the largest node
+ * valence, at least 8). */
+int ft_engine_layout(ft_engine *eng, int *corner, int *ell_width);
 
 /* ---- set-up --------------------------------------------------------------
  * Boundary conditions as data: dof_motion[3 n + d] = -1 for a free DOF or the

@@ -167,3 +167,117 @@ def case_injury(steps=150):
     G, K = 500.0, 2.0e4
     c["props"] = props_row(1040.0, G, K - 2.0 * G / 3.0)[None, :]
     return c
+
+
+# Reference-cube coordinates of the C3D8 local nodes (ShapeFunction_C3D8.cpp:22-29).
+C3D8_SIGNS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                       [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], np.int64)
+
+
+def cube_rotations():
+    """The 24 proper rotations of the reference cube as node renumberings:
+    row r maps new local node a to old local node perm[r, a], the node that
+    sits at R s_a (s_a: reference coordinates of node a).  The renumbered
+    element x'(xi) = x(R xi) is the same hexahedron with det J of the same
+    sign, and the 2x2x2 Gauss points map onto each other."""
+    import itertools
+    out = []
+    for p in itertools.permutations(range(3)):
+        for sg in itertools.product((-1, 1), repeat=3):
+            R = np.zeros((3, 3), np.int64)
+            for i in range(3):
+                R[i, p[i]] = sg[i]
+            if round(np.linalg.det(R)) != 1:
+                continue
+            img = C3D8_SIGNS @ R.T
+            perm = [int(np.nonzero(np.all(C3D8_SIGNS == img[a], axis=1))[0][0]) for a in range(8)]
+            out.append(perm)
+    out = np.array(out, np.int64)
+    assert out.shape == (24, 8) and len({tuple(r) for r in out}) == 24
+    return out
+
+
+def permute_elements(conn, seed):
+    """Each element's local node order renumbered by a random proper cube
+    rotation (the identity excluded), so that nodes are the same local corner
+    of several elements and the engine's general force gather is used."""
+    rots = cube_rotations()
+    ident = np.nonzero(np.all(rots == np.arange(8), axis=1))[0][0]
+    choices = np.delete(np.arange(24), ident)
+    rng = np.random.default_rng(seed)
+    c = np.asarray(conn, np.int64).reshape(-1, 8)
+    pick = rots[rng.choice(choices, size=c.shape[0])]
+    return np.take_along_axis(c, pick, axis=1).reshape(-1).astype(np.int32)
+
+
+def corner_eligible(conn, nn):
+    """True when every node is local corner k of at most one element for each
+    k (the engine then writes forces to per-node corner slots)."""
+    c = np.asarray(conn).reshape(-1, 8)
+    return all(np.unique(c[:, k]).size == c.shape[0] for k in range(8))
+
+
+def fan_mesh(nz=3, r0=1e-3, h=0.5e-3, jitter=0.08, seed=5):
+    """Hexagonal prism meshed as an unstructured fan: per z layer, 6 kite
+    quads around the axis (centre, edge midpoint, vertex, edge midpoint of a
+    regular hexagon of radius r0) and a ring of 12 quads out to radius 2 r0,
+    extruded over nz layers of height h.  Axis nodes of interior layers
+    belong to 12 elements (valence above 8); interior nodes are jittered by
+    +-jitter r0 in x and y.  Returns X (3 per node), conn, pid (0 inner fan,
+    1 outer ring) and the per-layer node count."""
+    ang = np.pi / 3.0 * np.arange(6)
+    V = r0 * np.stack([np.cos(ang), np.sin(ang)], 1)
+    M = 0.5 * (V + np.roll(V, -1, axis=0))
+    W = 2.0 * V
+    N = 0.5 * (W + np.roll(W, -1, axis=0))
+    P = np.concatenate([[[0.0, 0.0]], V, M, W, N])  # 0 | 1-6 V | 7-12 M | 13-18 W | 19-24 N
+    npl = P.shape[0]
+    iv, im, iw, inn = 1, 7, 13, 19
+    quads, parts = [], []
+    for i in range(6):
+        quads.append([0, im + (i - 1) % 6, iv + i, im + i])
+        parts.append(0)
+        quads.append([iv + i, iw + i, inn + i, im + i])
+        parts.append(1)
+        quads.append([im + i, inn + i, iw + (i + 1) % 6, iv + (i + 1) % 6])
+        parts.append(1)
+    fixed = []
+    for q in quads:
+        a = P[q]
+        area = 0.5 * sum(a[k, 0] * a[(k + 1) % 4, 1] - a[(k + 1) % 4, 0] * a[k, 1] for k in range(4))
+        fixed.append(q if area > 0 else q[::-1])
+    rng = np.random.default_rng(seed)
+    X = []
+    for k in range(nz + 1):
+        Pk = P.copy()
+        if 0 < k < nz:
+            Pk[:iw] += jitter * r0 * rng.uniform(-1, 1, (iw, 2))
+        X.append(np.concatenate([Pk, np.full((npl, 1), k * h)], 1))
+    X = np.concatenate(X).reshape(-1)
+    conn, pid = [], []
+    for k in range(nz):
+        for q, p in zip(fixed, parts):
+            conn.append([k * npl + n for n in q] + [(k + 1) * npl + n for n in q])
+            pid.append(p)
+    return X, np.array(conn, np.int32).reshape(-1), np.array(pid, np.int32), npl
+
+
+def case_fan(steps=300):
+    """fan_mesh under impact loading: base layer fixed, top layer starting at
+    v_z = -5 m/s; neo-Hookean fan (G = 1 MPa, K = 10 MPa) inside an isotropic
+    HGO + Prony ring."""
+    X, conn, pid, npl = fan_mesh()
+    nn = X.size // 3
+    props = np.stack([props_row(1000.0, 1.0e6, 1.0e7 - 2.0e6 / 3.0),
+                      props_row(1000.0, 4.0e5, 4.0e6, 3.0e5, 6.0, 0.4, 2e-4, 0.3, 2e-3)])
+    Xr = X.reshape(-1, 3)
+    zmax = Xr[:, 2].max()
+    dm = -np.ones(3 * nn, np.int32)
+    base = np.nonzero(Xr[:, 2] < 1e-12)[0]
+    for d in range(3):
+        dm[3 * base + d] = 0
+    v0 = np.zeros(3 * nn)
+    v0[3 * np.nonzero(Xr[:, 2] > zmax - 1e-12)[0] + 2] = -5.0
+    return dict(X=X, conn=conn, pid=pid, material_id=np.array([1, 5], np.int32), props=props.reshape(-1),
+                dof_motion=dm, motions=ft.motion(ft.MOTION_FIXED)[None, :], u0=None, v0=v0,
+                max_steps=steps, tmax=1e9, dt_factor=0.9, failure_dt=1e-11)

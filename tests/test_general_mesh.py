@@ -1,0 +1,223 @@
+"""Meshes that take the general force gather.
+
+The engine writes element forces to per-node corner slots only when every
+node is local corner k of at most one element for each k (every structured
+block and mapped ellipsoid qualifies).  Any other mesh takes the general
+path: slot-major element forces summed per node through the node-to-element
+table in the reference's scatter-add order (GetForce_3D.cpp:39-44), with the
+8-wide table when no node has more than 8 elements and the variable-width
+loop otherwise.  These tests build such meshes by renumbering each element's
+local nodes with a random proper rotation of the reference cube (the same
+hexahedra, the same Gauss-point set) and from an unstructured fan whose axis
+nodes belong to 12 elements, and check the engine against the oracle run on
+the same renumbered mesh (the face-area shortcut of Geometry.cpp:46 depends
+on the corner order, so the unpermuted run is not the reference for it).
+
+Bars: FP64 engine u <= 1e-10, v <= 1e-9, energies <= 1e-9, element forces
+<= 1e-11, dt <= 1e-13 relative; FP32 elements with FP64 nodal state
+u <= 1e-5 and energy balance <= 1e-4 of the largest energy term.
+"""
+import numpy as np
+import pytest
+
+import femtech_amd as ft
+from cases import case_fan, corner_eligible, cube_rotations, permute_elements
+from conftest import golden, rel
+from oracle import oracle as O
+
+
+def general_case(name):
+    if name in ("impact", "materials"):
+        z = golden("run_" + name)
+        c = {k: z[k] for k in ("X", "conn", "pid", "material_id", "props", "dof_motion", "motions")}
+        c["v0"] = z["v0"] if "v0" in z else None
+        c.update(tmax=float(z["tmax"]), dt_factor=float(z["dt_factor"]), failure_dt=float(z["failure_dt"]),
+                 steps=int(z["ref_steps"]), fibres=None)
+        c["conn"] = permute_elements(c["conn"], seed=1 if name == "impact" else 2)
+        return c
+    if name == "ellipsoid":
+        from femtech_amd import workloads as W
+        w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+        c = {k: w[k] for k in ("X", "conn", "pid", "material_id", "props", "dof_motion", "motions",
+                               "v0", "fibres")}
+        c["conn"] = permute_elements(c["conn"], seed=3)
+        c.update(tmax=1e9, dt_factor=0.9, failure_dt=1e-11, steps=300)
+        return c
+    if name in ("fan", "fan_permuted"):
+        f = case_fan()
+        c = {k: f[k] for k in ("X", "conn", "pid", "material_id", "props", "dof_motion", "motions", "v0")}
+        if name == "fan_permuted":
+            c["conn"] = permute_elements(c["conn"], seed=4)
+        c.update(tmax=f["tmax"], dt_factor=f["dt_factor"], failure_dt=f["failure_dt"], steps=f["max_steps"],
+                 fibres=None)
+        return c
+    raise KeyError(name)
+
+
+CASES = ["impact", "materials", "ellipsoid", "fan", "fan_permuted"]
+
+
+def oracle_for(c):
+    o = O.Model(c["X"], c["conn"], c["pid"], c["material_id"], c["props"])
+    if c["fibres"] is not None:
+        o.set_fibres(c["fibres"])
+    return o
+
+
+def engine_for(c, precision):
+    e = ft.Engine(c["X"], c["conn"], c["pid"], c["material_id"], c["props"], precision)
+    e.set_boundary(c["dof_motion"], c["motions"])
+    if c["fibres"] is not None:
+        e.set_fibres(c["fibres"])
+    return e
+
+
+def displacement(c, scale, seed):
+    rng = np.random.default_rng(seed)
+    X = c["X"].reshape(-1, 3)
+    h = 1e-3
+    return (scale * h * (np.sin(X / (3 * h)) + 0.3 * rng.uniform(-1, 1, X.shape))).reshape(-1)
+
+
+# ---- CPU: the renumbering itself (no GPU) --------------------------------
+
+def test_cube_rotations_are_the_rotation_group():
+    rots = cube_rotations()
+    # closed under composition, and each one keeps a node's three edge
+    # neighbours as edge neighbours (it is a symmetry of the cube graph)
+    edges = {(0, 1), (1, 2), (2, 3), (0, 3), (4, 5), (5, 6), (6, 7), (4, 7), (0, 4), (1, 5), (2, 6), (3, 7)}
+    edges |= {(b, a) for a, b in edges}
+    keys = {tuple(r) for r in rots}
+    for r in rots:
+        assert all((int(r[a]), int(r[b])) in edges for a, b in edges)
+        for q in rots[::5]:
+            assert tuple(r[q]) in keys
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_cases_leave_the_corner_layout(name):
+    c = general_case(name)
+    assert not corner_eligible(c["conn"], c["X"].size // 3)
+
+
+@pytest.mark.parametrize("name", ["impact", "materials"])
+def test_oracle_is_invariant_under_renumbering(name):
+    """The oracle on a renumbered mesh gives the same lumped mass and nodal
+    internal force as on the original (the Gauss points map onto each other;
+    only the summation order changes), and the same stable step where no
+    face takes the parallelogram shortcut."""
+    z = golden("run_" + name)
+    c = general_case(name)
+    a = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    b = oracle_for(c)
+    assert rel(b.lumped_mass(), a.lumped_mass()) < 1e-14
+    u = displacement(c, 2e-2, 5)
+    assert rel(b.get_force(u, 1e-5), a.get_force(u, 1e-5)) < 1e-12
+    bnd = (c["dof_motion"] >= 0).astype(np.int32)
+    assert b.stable_dt(u, bnd) == pytest.approx(a.stable_dt(u, bnd), rel=1e-12)
+
+
+# ---- GPU: engine against the oracle on the same meshes --------------------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_general_layout_chosen(gpu_ready, name):
+    c = general_case(name)
+    width = int(np.bincount(c["conn"]).max())
+    with engine_for(c, ft.FT_FP64) as e:
+        assert e.layout() == (0, max(8, width))
+    assert (width > 8) == name.startswith("fan")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_general_forces_mass_dt_fp64(gpu_ready, name):
+    c = general_case(name)
+    o = oracle_for(c)
+    bnd = (c["dof_motion"] >= 0).astype(np.int32)
+    with engine_for(c, ft.FT_FP64) as e:
+        assert rel(e.lumped_mass(), o.lumped_mass()) < 1e-13
+        for seed, scale in ((1, 1e-3), (2, 5e-2)):
+            u = displacement(c, scale, seed)
+            assert rel(e.element_forces(u, dt=1e-5), o.element_forces(u, 1e-5)) < 1e-11, scale
+            assert rel(e.get_force(u, dt=1e-5), o.get_force(u, 1e-5)) < 1e-11, scale
+            assert e.stable_time_step(u, bnd) == pytest.approx(o.stable_dt(u, bnd), rel=1e-13)
+
+
+def run_engine(c, precision, history=True, graph=0, persistent=False, chunks=None):
+    with engine_for(c, precision) as e:
+        e.set_state(None, c["v0"])
+        e.lumped_mass()
+        if history:
+            e.set_history(c["steps"] + 4)
+        e.run_init(c["tmax"], c["dt_factor"], c["failure_dt"])
+        if graph:
+            e.set_graph(graph)
+        if persistent:
+            e.set_persistent(True)
+        done = sum(e.run_steps(k) for k in (chunks or [c["steps"]]))
+        out = dict(done=done, state=e.state(), time=e.time(), energy=e.energy(), status=e.status())
+        if history:
+            out["history"] = e.history()
+    return out
+
+
+def run_oracle(c):
+    o = oracle_for(c)
+    o.set_bc(c["dof_motion"], c["motions"])
+    o.set_initial(None, c["v0"])
+    return o.run(c["steps"], c["tmax"], c["dt_factor"], c["failure_dt"]), o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_general_run_fp64_matches_oracle(gpu_ready, name):
+    c = general_case(name)
+    r, o = run_oracle(c)
+    g = run_engine(c, ft.FT_FP64)
+    assert g["done"] == r["steps"] == c["steps"]
+    u, v, a, fi = g["state"]
+    th, dth, en = g["history"]
+    assert rel(u, r["u"]) < 1e-10
+    assert rel(v, r["v"]) < 1e-9
+    assert rel(fi, r["fi"]) < 1e-9
+    assert g["time"][0] == pytest.approx(r["time"][-1], rel=1e-13)
+    assert rel(th, r["time"][1:]) < 1e-13
+    assert rel(dth, r["dt"][1:]) < 1e-11
+    assert rel(en[:, :3], r["energy"][1:, :3]) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_general_run_fp32_split(gpu_ready, name):
+    c = general_case(name)
+    r, _ = run_oracle(c)
+    g = run_engine(c, ft.FT_FP32)
+    assert g["done"] == r["steps"]
+    th, dth, en = g["history"]
+    assert rel(g["state"][0], r["u"]) < 1e-5
+    ref = r["energy"][1:]
+    scale = np.abs(ref[:, :3]).max()
+    bal = en[:, 2] + en[:, 0] - en[:, 1]
+    assert np.abs(bal - (ref[:, 2] + ref[:, 0] - ref[:, 1])).max() / scale < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["impact", "fan"])
+@pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
+def test_general_graph_and_persistent_equal_launches(gpu_ready, name, precision):
+    """Captured chunks equal the per-step launches bitwise; the persistent
+    loop agrees to round-off (relative 1e-9 FP32 engine, 1e-12 FP64)."""
+    c = general_case(name)
+    chunks = [16, 7, 40, c["steps"] - 63]
+    base = run_engine(c, precision, history=False, chunks=chunks)
+    graph = run_engine(c, precision, history=False, graph=8, chunks=chunks)
+    pers = run_engine(c, precision, history=False, persistent=True, chunks=chunks)
+    assert base["done"] == graph["done"] == pers["done"] == c["steps"]
+    for k in range(4):
+        assert np.array_equal(graph["state"][k], base["state"][k]), k
+    tol = 1e-9 if precision == ft.FT_FP32 else 1e-12
+    for k in (0, 1):
+        b = base["state"][k]
+        assert np.abs(pers["state"][k] - b).max() <= tol * np.abs(b).max(), k
+    assert pers["time"][2] == base["time"][2]
