@@ -14,14 +14,11 @@
  *   geom <dir>    volumeHexahedron / areaHexahedronFace /
  *                 CalculateCharacteristicLength_C3D8 on given element corners.
  *
- * The two library units that cannot be compiled here are replaced by what a
- * one-rank run of them amounts to:
- *   InitFinalizeFemTech.cpp (needs a CMake-generated header): MPI_Init,
- *     world_rank = 0, world_size = 1, uid, and TerminateFemTech -> exit(code).
- *   PartitionMesh() (needs the ParMETIS library): on one rank it keeps every
- *     element and calls updateConnectivityGlobalToLocal(), which this driver
- *     calls directly (the unit is compiled; its ParMETIS calls are never
- *     reached), then sets nDOF = 3 nNodes (PartitionMesh.cpp:433-435).
+ * Every FemTech routine it calls is the reference's own, compiled unmodified
+ * (InitFemTechWoInput, ReadInputFile, ReadMaterials, PartitionMesh with the
+ * vendored ParMETIS, AllocateArrays, ShapeFunctions, ..., FinalizeFemTech);
+ * the driver defines only what ex9.cpp defines itself (ex9.cpp:9-17) and
+ * its own boundary-condition and output code.
  */
 #include "FemTech.h"
 
@@ -41,19 +38,6 @@ double FailureTimeStep = 1e-11;
 bool ImplicitStatic = false;
 bool ImplicitDynamic = false;
 bool ExplicitDynamic = true;
-
-/* One-rank stand-ins for InitFinalizeFemTech.cpp's definitions. */
-int world_rank;
-int world_size;
-std::string uid;
-void TerminateFemTech(int errorCode) {
-  fprintf(stderr, "TerminateFemTech(%d)\n", errorCode);
-  finaliseLog();
-  MPI_Finalize();
-  exit(100 + errorCode);
-}
-
-void updateConnectivityGlobalToLocal(void);  // PartitionMesh.cpp:485
 
 namespace {
 
@@ -82,19 +66,6 @@ void write_bin(const std::string &path, const T *p, size_t n) {
   FILE *f = fopen(path.c_str(), "wb");
   fwrite(p, sizeof(T), n, f);
   fclose(f);
-}
-
-void one_rank_partition() {
-  sendProcessCount = 0;
-  sendNeighbourCountCum = (int *)calloc(1, sizeof(int));
-  sendNeighbourCount = (int *)calloc(1, sizeof(int));
-  sendProcessID = (int *)calloc(1, sizeof(int));
-  sendNodeIndex = (int *)calloc(1, sizeof(int));
-  sendNodeDisplacement = (double *)calloc(1, sizeof(double));
-  recvNodeDisplacement = (double *)calloc(1, sizeof(double));
-  nNodes = eptr[nelements];
-  updateConnectivityGlobalToLocal();
-  nDOF = nNodes * ndim;
 }
 
 /* Boundary motions as data: same table layout as the oracle. */
@@ -139,11 +110,11 @@ void apply_bc(const Motions &m) {
     }
 }
 
-int mode_run(const std::string &dir) {
-  if (chdir(dir.c_str()) != 0) return 2;
+// ex9.cpp:19-27 then the explicit branch, in the current directory.
+int mode_run() {
   ReadInputFile(access("mesh.k", F_OK) == 0 ? "mesh.k" : "mesh.inp");
   ReadMaterials();
-  one_rank_partition();
+  PartitionMesh();
   AllocateArrays();
   Time = 0.0;
 
@@ -277,15 +248,13 @@ int mode_run(const std::string &dir) {
   write_bin("out_dt.f64", hist_dt.data(), hist_dt.size());
   write_bin("out_energy.f64", hist_energy.data(), hist_energy.size());
   write_bin("out_snap_u.f64", snap_u.data(), snap_u.size());
-  fclose(energyFile);
-  printf("steps %d time %.17g\n", step, Time);
+  printf("steps %d time %.17g uid %s\n", step, Time, uid.c_str());
   return 0;
 }
 
 /* Sets up n one-Gauss-point "elements" so the reference's material routines
  * can be called on given deformation gradients. */
-int mode_stress(const std::string &dir) {
-  if (chdir(dir.c_str()) != 0) return 2;
+int mode_stress() {
   std::vector<double> in = read_bin<double>("stress_in.f64");
   // record: mat, props[9], dt, T, F[T][9]  (T fixed for the file: first record)
   size_t pos = 0;
@@ -336,8 +305,7 @@ int mode_stress(const std::string &dir) {
   return 0;
 }
 
-int mode_geom(const std::string &dir) {
-  if (chdir(dir.c_str()) != 0) return 2;
+int mode_geom() {
   std::vector<double> in = read_bin<double>("geom_in.f64");
   int n = (int)(in.size() / 24);
   ndim = 3;
@@ -369,18 +337,20 @@ int main(int argc, char **argv) {
     fprintf(stderr, "usage: %s run|stress|geom <dir>\n", argv[0]);
     return 2;
   }
-  MPI_Init(NULL, NULL);
-  MPI_Comm_size(MPI_COMM_WORLD, &world_size);
-  MPI_Comm_rank(MPI_COMM_WORLD, &world_rank);
-  uid = "golden";
-  std::string dir = argv[2];
-  std::string logFile = dir + "/femtech_golden.log";
-  initLog(logFile.c_str());
+  const std::string mode = argv[1];
+  // The log and energy files are written to the working directory.
+  if (chdir(argv[2]) != 0) return 2;
+  InitFemTechWoInput(argc, argv);
   int rc = 2;
-  std::string mode = argv[1];
-  if (mode == "run") rc = mode_run(dir);
-  if (mode == "stress") rc = mode_stress(dir);
-  if (mode == "geom") rc = mode_geom(dir);
+  if (mode == "run") {
+    rc = mode_run();
+    FinalizeFemTech();  // frees the arrays and closes energy_<uid>.dat
+    return rc;
+  }
+  if (mode == "stress") rc = mode_stress();
+  if (mode == "geom") rc = mode_geom();
+  // The unit modes set up no mesh and no energy file (FinalizeFemTech
+  // would close it): the log and MPI are closed directly.
   finaliseLog();
   MPI_Finalize();
   return rc;

@@ -4,7 +4,7 @@ Run in a container that has the FemTech source tree (default
 /root/reference) and the image's MPICH/MKL:
 
     make -C oracle/ref REF=/root/reference      # builds oracle/_ref/ref_golden
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [out_dir]   # default: tests/golden
 
 Each fixture holds the inputs (mesh, materials, boundary motions as data, run
 parameters) and the reference's outputs for them.  Outputs are produced by
@@ -70,8 +70,9 @@ def golden_run(name, c, mesh_file=None):
             f.write("%d %.17g %.17g %.17g %d %s\n" % (c["max_steps"], c["tmax"], c["dt_factor"],
                                                       c["failure_dt"], len(c["snaps"]),
                                                       " ".join(str(s) for s in c["snaps"])))
-        out = run_driver("run", d)
-        steps = int(out.split()[1])
+        out = run_driver("run", d).split()
+        steps = int(out[1])
+        uid = out[out.index("uid") + 1]
         r = lambda f, t=np.float64: np.fromfile(os.path.join(d, f), dtype=t)  # noqa: E731
         res = dict(ref_coords=r("out_coords.f64"), ref_conn=r("out_conn.i32", np.int32),
                    ref_dshp=r("out_dshp.f64"), ref_detJ=r("out_detJ.f64"), ref_mass=r("out_mass.f64"),
@@ -82,7 +83,7 @@ def golden_run(name, c, mesh_file=None):
                    ref_snap_u=r("out_snap_u.f64").reshape(-1, 3 * nn), ref_steps=np.int64(steps),
                    ref_eavg=r("out_eavg.f64"), ref_princ=r("out_princ.f64"))
         rows = []
-        with open(os.path.join(d, "energy_golden.dat")) as f:
+        with open(os.path.join(d, "energy_%s.dat" % uid)) as f:
             for line in f:
                 if not line.startswith("#") and line.strip():
                     rows.append([float(x) for x in line.split()])
@@ -157,13 +158,20 @@ def geom_kats(rng):
     return dict(xe=xe, ref_volume=out[:, 0], ref_area=out[:, 1:7], ref_char_length=out[:, 7])
 
 
+OUT_DIR = HERE
+
+
 def save(name, **arrays):
-    path = os.path.join(HERE, name + ".npz")
+    path = os.path.join(OUT_DIR, name + ".npz")
     np.savez_compressed(path, **arrays)
     print("wrote %s (%d bytes)" % (path, os.path.getsize(path)))
 
 
 def main():
+    global OUT_DIR
+    if len(sys.argv) > 1:  # write the fixtures elsewhere (e.g. to compare them)
+        OUT_DIR = sys.argv[1]
+        os.makedirs(OUT_DIR, exist_ok=True)
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference first: make -C oracle/ref REF=%s" % REF)
     rng = np.random.default_rng(20261202)
