@@ -103,6 +103,11 @@ struct StepSlot {
   int done;        // 1 when the run has stopped at the end of step k
   int fresh;       // 1 if the last nodal launch completed a step
   int failed;      // 1 when it stopped because no stable step was left
+  // FP32 run loop: best unrefined block minimum of the step (dt_candidate),
+  // and 1 once k_motion_states has merged it into dtmin_bits
+  unsigned long long cand_bits;
+  int settled;
+  int pad;
 };
 
 struct Ctl {
@@ -112,7 +117,6 @@ struct Ctl {
   int pad[2];
 };
 
-constexpr int kStatusDtFromFp32 = FT_STATUS_DT_FP32;  // no element refined in FP64
 
 __device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 
@@ -124,6 +128,14 @@ __device__ inline double bits_to_double(unsigned long long b) { return __longlon
 __device__ inline unsigned long long dt_key(double d) {
   if (d > 0.0) return (unsigned long long)__double_as_longlong(d);
   return d <= 0.0 ? 0ull : kInfBits;
+}
+// Candidate key of a block minimum that was not refined (FP32 run loop):
+// the FP32 bits of the positive step (ordered like the values) above the
+// element tile's index, so the unsigned minimum gives the smallest step and
+// the first tile on ties.
+constexpr unsigned long long kNoCandidate = ~0ull;
+__device__ inline unsigned long long dt_candidate(double d, int tile) {
+  return ((unsigned long long)__float_as_uint((float)d) << 32) | (unsigned)tile;
 }
 
 template <typename T>
@@ -304,6 +316,7 @@ struct ElemArgs {
   Ctl *ctl;              // run loop: per-instance control, slot `parity` gives dt_k and done
   int parity;
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
+  double *cand_geom;       // FP32 run loop: kCandStride per element tile (candidate geometry)
   double dt_fixed;       // used outside the run loop (ctl == nullptr)
   unsigned long long *dtbits;  // outside the run loop: per-instance minimum (nullable)
   int *status;
@@ -343,6 +356,49 @@ constexpr int kMaxTuneWaves = 5;
 __device__ inline bool audited(int e, unsigned permille) {
   return ((unsigned)e * 2654435761u >> 8) % 1000u < permille;
 }
+
+// FP64 normalised Walsh-Hadamard coefficients c[m][d] (m = xi, eta, xi.eta,
+// zeta, xi.zeta, eta.zeta, xi.eta.zeta) of the deformed position x = X + u
+// of element es, written by threads 0..2 (one component each) to c.
+__device__ inline void deformed_coeffs(const int *conn, const double *X, const double *u, int nn, int es,
+                                       double *c) {
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    const double *Xc = X + (size_t)d * nn, *Uc = u + (size_t)d * nn;
+    const int n0 = conn[8 * (size_t)es];
+    double v[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int n = conn[8 * (size_t)es + p2n(p)];
+      v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
+    }
+    wht8(v);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) c[3 * m + d] = v[m + 1] * 0.125;
+  }
+}
+
+// Le / ce (StableTimeStep.cpp:22-27, CalculateCharacteristicLength_C3D8.cpp:
+// 3-30) in FP64 from coefficients sh_c (filled by deformed_coeffs, LDS),
+// returned to every thread.  Called by every thread of the block; the six
+// face areas and the volume go to threads 0..6, so it adds no registers to
+// the calling kernel.
+__device__ inline double refined_dt(const double (*sh_c)[3], double ce) {
+  __shared__ double sh_a[7];
+  __syncthreads();
+  if (threadIdx.x < 7) sh_a[threadIdx.x] = hex_part(sh_c, threadIdx.x);
+  __syncthreads();
+  double amax = 0.0;
+  for (int f = 0; f < 6; ++f)
+    if (sh_a[f] > amax) amax = sh_a[f];
+  const double dte = sh_a[6] / amax / ce;
+  __syncthreads();  // sh_a may be rewritten by the next call
+  return dte;
+}
+
+// Geometry of a block's unrefined minimum (FP32 run loop), one slot per
+// element tile: the 21 coefficients of deformed_coeffs and the wave speed.
+constexpr int kCandStride = 22;
 
 template <typename R, bool FORCE, bool DT, int MAT, bool SURR = false>
 __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) {
@@ -426,43 +482,43 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
   }
   if (DT) {
     const double bmin = block_min(my_dt, sh_min);
-    if (sizeof(R) == 4 && a.refine && bmin <= dt_guide * 1.001) {
-      // The step size sets the time grid and is sensitive to round-off: the
-      // block's minimising element (smallest id on ties) recomputes Le / ce
-      // from FP64 geometry (StableTimeStep.cpp:22-27).  Rare (blocks near the
-      // last minimum only) and spread over a few threads through LDS, so it
-      // adds no registers to the kernel.
+    // FP32 run loop: the step size sets the time grid and is sensitive to
+    // round-off, so it always comes from FP64 geometry.  A block whose
+    // minimum lies within 0.1 % of the last step's global minimum refines
+    // its minimising element (smallest id on ties) in FP64 at once; every
+    // other block posts (FP32 minimum, tile) to the instance's candidate key
+    // and the FP64 coefficients of its minimising element to its tile's
+    // slot, and the nodal kernel refines the best candidate where it could
+    // undercut the refined minimum (the minimum moved up by more than the
+    // window in one step).
+    const bool exact = sizeof(R) == 4 && a.refine && a.ctl;
+    const bool refine_blk = sizeof(R) == 4 && a.refine && bmin < 1e300 && bmin <= dt_guide * 1.001;
+    if (refine_blk || (exact && bmin < 1e300)) {
       __shared__ int sh_arg;
       __shared__ double sh_c[7][3];
-      __shared__ double sh_a[7];
       if (threadIdx.x == 0) sh_arg = INT32_MAX;
       __syncthreads();
       if (e < end && my_dt == bmin) atomicMin(&sh_arg, e);
       __syncthreads();
       const int es = sh_arg;
-      if (threadIdx.x < 3) {
-        const int d = threadIdx.x;
-        const double *Xc = a.X + (size_t)d * a.nn, *Uc = a.u + (size_t)d * a.nn;
-        const int n0 = a.conn[8 * (size_t)es];
-        double v[8];
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-          const int n = a.conn[8 * (size_t)es + p2n(p)];
-          v[p] = p == 0 ? 0.0 : (Xc[n] - Xc[n0]) + (Uc[n] - Uc[n0]);
+      const double ce = a.parts[a.einfo[es] & kPidMask].ce;
+      if (refine_blk) {
+        deformed_coeffs(a.conn, a.X, a.u, a.nn, es, &sh_c[0][0]);
+        const double dte = refined_dt(sh_c, ce);
+        if (threadIdx.x == 0) atomicMin(dtmin, dt_key(dte));
+      } else if (bmin > 0.0) {
+        // the tile's candidate: its FP32 minimum (ordered key) and geometry;
+        // the nodal kernel refines the instance's best one when it could
+        // undercut the refined minimum
+        const int gt = a.tile0 + L;
+        double *slot = a.cand_geom + (size_t)kCandStride * gt;
+        deformed_coeffs(a.conn, a.X, a.u, a.nn, es, slot);
+        if (threadIdx.x == 0) {
+          slot[21] = ce;
+          atomicMin(&a.ctl[tile.z].slot[a.parity].cand_bits, dt_candidate(bmin, gt));
         }
-        wht8(v);
-#pragma unroll
-        for (int m = 0; m < 7; ++m) sh_c[m][d] = v[m + 1] * 0.125;
-      }
-      __syncthreads();
-      if (threadIdx.x < 7) sh_a[threadIdx.x] = hex_part(sh_c, threadIdx.x);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double amax = 0.0;
-        for (int f = 0; f < 6; ++f)
-          if (sh_a[f] > amax) amax = sh_a[f];
-        const double dte = sh_a[6] / amax / a.parts[a.einfo[es] & kPidMask].ce;
-        atomicMin(dtmin, dt_key(dte));
+      } else if (threadIdx.x == 0) {
+        atomicMin(dtmin, dt_key(bmin));  // an inverted element: the step collapses
       }
     } else if (threadIdx.x == 0 && bmin < 1e300 && dtmin) {
       atomicMin(dtmin, dt_key(bmin));
@@ -723,6 +779,7 @@ struct NodalArgs {
   int inline_motions;     // motion states evaluated by the tile itself (mstate unused)
   int *running;           // nullable: the first tile of each instance that goes on adds 1
   int mcap;               // largest motion count of any instance (>= 1): dynamic LDS rows
+  const double *cand_geom;  // FP32 engine: candidate geometry per element tile (element kernel)
 };
 
 // Dynamic LDS of a node tile of a run with rotations: the motion states of
@@ -736,10 +793,35 @@ size_t nodal_lds_bytes(int mcap) {
 // Time of motion state w (0: t_{k-1}, 1: t_k, 2: t_{k+1} = t_k + factor *
 // min Le/c) for the nodal update of step k, rounded step by step (the same
 // bits wherever it is evaluated).
-__device__ inline double motion_state_time(const StepSlot &cur, double factor, int w) {
+__device__ inline double motion_state_time(const StepSlot &cur, double factor, int w, double dtmin) {
   if (w == 0) return cur.t_prev;
   if (w == 1) return cur.t;
-  return __dadd_rn(cur.t, __dmul_rn(factor, bits_to_double(cur.dtmin_bits)));
+  return __dadd_rn(cur.t, __dmul_rn(factor, dtmin));
+}
+
+// Whether the step's best unrefined candidate (FP32 run loop) could undercut
+// the refined minimum dtmin.
+__device__ inline bool candidate_pending(const StepSlot &cur, double dtmin) {
+  return cur.cand_bits != kNoCandidate &&
+         (double)__uint_as_float((unsigned)(cur.cand_bits >> 32)) <= dtmin * 1.001;
+}
+
+// The candidate's FP64 step (the geometry its tile left), by one thread:
+// the same parts as refined_dt, evaluated in turn.
+__device__ inline double candidate_dt(const double *cg) {
+  double c[7][3];
+  for (int i = 0; i < 21; ++i) c[i / 3][i % 3] = cg[i];
+  double amax = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    const double af = hex_part(c, f);
+    if (af > amax) amax = af;
+  }
+  return hex_part(c, 6) / amax / cg[21];
+}
+
+__device__ inline double merge_candidate(double dtmin, double dte) {
+  if (dte <= 0.0) return 0.0;  // inverted in FP64: the step collapses
+  return dte < dtmin ? dte : dtmin;
 }
 
 // End of step k (gather, a = f/m, v = v_half + dt/2 a, energy) fused with the
@@ -768,7 +850,18 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   }
   const int n = tile.x + threadIdx.x;
   const double t_k = cur.t, t_km1 = cur.t_prev, dt_k = cur.dt;
-  const double dtmin = bits_to_double(cur.dtmin_bits);
+  double dtmin = bits_to_double(cur.dtmin_bits);
+  if (sizeof(R) == 4 && a.cand_geom && !cur.settled && candidate_pending(cur, dtmin)) {
+    // FP32 engine: the step's best unrefined element could undercut the
+    // refined minimum (it moved up by more than the element kernel's window):
+    // its FP64 step from the geometry its tile left (uniform over the block;
+    // every block of the instance computes the same value), unless
+    // k_motion_states has settled it already
+    __shared__ double sh_c[7][3];
+    const double *cg = a.cand_geom + (size_t)kCandStride * (unsigned)(cur.cand_bits & 0xffffffffu);
+    if (threadIdx.x < 21) (&sh_c[0][0])[threadIdx.x] = cg[threadIdx.x];
+    dtmin = merge_candidate(dtmin, refined_dt(sh_c, cg[21]));
+  }
   const double dt_next = a.factor * dtmin;
   // no stable step left: below the failure step (an inverted element's is
   // negative and counts as 0), or none finite at all (non-finite forces)
@@ -793,8 +886,8 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   const int mb = a.mrange[2 * inst], mc = a.mrange[2 * inst + 1];
   for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x) {
     if (a.inline_motions)
-      motion_time(a.motions + (size_t)FT_MOTION_STRIDE * (mb + i % mc), motion_state_time(cur, a.factor, i / mc),
-                  ms(i / mc, i % mc));
+      motion_time(a.motions + (size_t)FT_MOTION_STRIDE * (mb + i % mc),
+                  motion_state_time(cur, a.factor, i / mc, dtmin), ms(i / mc, i % mc));
     else
       ms(i / mc, i % mc) = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
   }
@@ -908,6 +1001,8 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
       nx.t_prev = t_k;
       nx.dt = dt_next;
       nx.dtmin_bits = kInfBits;
+      nx.cand_bits = kNoCandidate;
+      nx.settled = 0;
       nx.step = cur.step + 1;
       nx.done = 0;
       nx.failed = 0;
@@ -922,7 +1017,6 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
     nx.t_done = t_k;
     nx.dt_done = dt_next;
     nx.dt_guide = dtmin;
-    if (a.mode == kStep && dtmin > cur.dt_guide * 1.001) atomicOr(a.status, kStatusDtFromFp32);
     nx.fresh = a.mode == kStep;
     ic->slot[a.parity ^ 1] = nx;
     if (go && a.running) atomicAdd(a.running, 1);
@@ -941,12 +1035,27 @@ __global__ __launch_bounds__(kBlock, ROT ? 6 : 7) void k_nodal(NodalArgs<R> a) {
 // Scalar state of every motion at t_{k-1}, t_k and t_{k+1} = t_k + dt_{k+1}
 // for the nodal kernel of step k (one small block; the trigonometry of the
 // rotation motions is kept out of the per-node kernel).
-__global__ void k_motion_states(const Ctl *ctl, int parity, double factor, const double *motions,
-                                const int *motion_inst, int nm, MotionT *out) {
+// Motion states of every motion at t_{k-1}, t_k and t_{k+1}.  In the FP32
+// engine t_{k+1} needs the step's final minimum: each thread settles a
+// pending candidate itself (the same value), and the first state of each
+// instance's first motion writes it back to the slot and marks the slot
+// settled, so the nodal kernel takes it as final.
+__global__ void k_motion_states(Ctl *ctl, int parity, double factor, const double *motions,
+                                const int *motion_inst, int nm, MotionT *out, const double *cand_geom) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nm; i += gridDim.x * blockDim.x) {
     const int w = i / nm, k = i % nm;
-    const StepSlot cur = ctl[motion_inst[k]].slot[parity];
-    motion_time(motions + FT_MOTION_STRIDE * k, motion_state_time(cur, factor, w), out[i]);
+    const int inst = motion_inst[k];
+    const StepSlot cur = ctl[inst].slot[parity];
+    double dtmin = bits_to_double(cur.dtmin_bits);
+    if (cand_geom && candidate_pending(cur, dtmin)) {
+      dtmin = merge_candidate(dtmin, candidate_dt(cand_geom + (size_t)kCandStride *
+                                                                (unsigned)(cur.cand_bits & 0xffffffffu)));
+      if (w == 0 && (k == 0 || motion_inst[k - 1] != inst)) {
+        ctl[inst].slot[parity].dtmin_bits = dt_key(dtmin);
+        ctl[inst].slot[parity].settled = 1;  // read only by the nodal kernel
+      }
+    }
+    motion_time(motions + FT_MOTION_STRIDE * k, motion_state_time(cur, factor, w, dtmin), out[i]);
   }
 }
 
@@ -1737,6 +1846,10 @@ struct ft_engine {
   int n_inst = 1;
   std::vector<int> inst_node, inst_elem;   // n_inst + 1 offsets each
   int4 *etiles = nullptr, *ntiles = nullptr;
+  double *cand_geom = nullptr;  // kCandStride per element tile (FP32 exact step size)
+  // DT-only passes outside the run loop (StableTimeStep, the run's first dt):
+  // the FP32 engine's blocks take their minimum from FP64 geometry
+  bool dt_exact_pass = false;
   int n_etiles = 0, n_ntiles = 0;
   std::vector<int> ntile_of_inst;          // n_inst + 1: node tiles per instance
   int *mrange = nullptr;                   // per instance: first motion, count
@@ -1939,7 +2052,8 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.fib = reinterpret_cast<const R *>(g->fib);
   a.ctl = in_loop ? g->ctl : nullptr;
   a.parity = g->parity;
-  a.refine = (sizeof(R) == 4 && in_loop) ? 1 : 0;
+  a.refine = (sizeof(R) == 4 && (in_loop || g->dt_exact_pass)) ? 1 : 0;
+  a.cand_geom = g->cand_geom;
   a.dt_fixed = dt_fixed;
   a.dtbits = in_loop ? nullptr : dtbits;
   a.status = g->status;
@@ -2166,6 +2280,7 @@ NodalArgs<R> nodal_args(ft_engine *g, int mode, int t0, int t1) {
   a.inline_motions = 0;
   a.running = nullptr;
   a.mcap = g->mcap;
+  a.cand_geom = g->precision == FT_FP32 ? g->cand_geom : nullptr;
   return a;
 }
 
@@ -2201,7 +2316,8 @@ int launch_nodal(ft_engine *g, int mode) {
     Timed t(g, 2);
     const int nb = std::max(1, std::min(64, (3 * g->n_motions + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_motion_states, dim3(nb), dim3(kBlock), 0, g->lstream, g->ctl, g->parity, g->factor,
-                       g->motions, g->motion_inst, g->n_motions, g->mstate);
+                       g->motions, g->motion_inst, g->n_motions, g->mstate,
+                       g->precision == FT_FP32 ? (const double *)g->cand_geom : nullptr);
     HIPCHK(hipGetLastError());
   }
   const int nt = g->n_ntiles;
@@ -2224,6 +2340,7 @@ int launch_persistent_k(ft_engine *g, int steps) {
   p.ea.want_dt = 1;
   p.na = nodal_args<R>(g, kStep, 0, g->n_ntiles);
   p.na.inline_motions = g->timed_motions ? 1 : 0;
+  p.na.cand_geom = g->precision == FT_FP32 ? g->cand_geom : nullptr;
   p.steps = steps;
   p.parity0 = g->parity;
   p.n_inst = g->n_inst;
@@ -2328,11 +2445,14 @@ int build_element_tiles(ft_engine *g) {
     }
     sg.tile_end = (int)et.size();
   }
-  if (g->etiles) (void)hipFree(g->etiles);
+  for (void *p : {(void *)g->etiles, (void *)g->cand_geom})
+    if (p) (void)hipFree(p);
   g->etiles = nullptr;
+  g->cand_geom = nullptr;
   g->n_etiles = (int)et.size();
   int rc;
-  if ((rc = dalloc(&g->etiles, et.size()))) return rc;
+  if ((rc = dalloc(&g->etiles, et.size())) || (rc = dalloc(&g->cand_geom, kCandStride * et.size())))
+    return rc;
   HIPCHK(hipMemcpy(g->etiles, et.data(), et.size() * sizeof(int4), hipMemcpyHostToDevice));
   return 0;
 }
@@ -2676,7 +2796,7 @@ int ft_engine_destroy(ft_engine *g) {
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
-                  g->etiles, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
+                  g->etiles, g->cand_geom, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -2830,7 +2950,10 @@ int ft_stable_time_step(ft_engine *g, const double *u, const int *boundary, doub
   }
   std::vector<unsigned long long> bits(g->n_inst, kInfBits);
   HIPCHK(hipMemcpyAsync(g->dtbits, bits.data(), bits.size() * sizeof(bits[0]), hipMemcpyHostToDevice, g->stream));
-  if ((rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0))) return rc;
+  g->dt_exact_pass = true;
+  rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0);
+  g->dt_exact_pass = false;
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(bits.data(), g->dtbits, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
   const unsigned long long bmin = *std::min_element(bits.begin(), bits.end());
@@ -2942,7 +3065,10 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
   int rc;
   std::vector<unsigned long long> bits(g->n_inst, kInfBits);
   HIPCHK(hipMemcpy(g->dtbits, bits.data(), bits.size() * sizeof(bits[0]), hipMemcpyHostToDevice));
-  if ((rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0))) return rc;
+  g->dt_exact_pass = true;
+  rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0);
+  g->dt_exact_pass = false;
+  if (rc) return rc;
   HIPCHK(hipMemcpy(bits.data(), g->dtbits, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost));
   std::vector<Ctl> c(g->n_inst);
   for (int i = 0; i < g->n_inst; ++i) {
@@ -2958,6 +3084,10 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
     memcpy(&s0.dtmin_bits, &dtmin, sizeof(dtmin));
     s0.dt_guide = dtmin;
     c[i].slot[1].dtmin_bits = kInfBits;
+    for (StepSlot &sl : c[i].slot) {
+      sl.cand_bits = kNoCandidate;
+      sl.settled = 0;
+    }
   }
   HIPCHK(hipMemcpy(g->ctl, c.data(), c.size() * sizeof(Ctl), hipMemcpyHostToDevice));
   // Initial GetForce + CalculateAccelerations (ex9.cpp:89-92); the Prony

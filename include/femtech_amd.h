@@ -48,9 +48,8 @@ extern "C" {
 #define FT_STATUS_NONFINITE 1
 #define FT_STATUS_INVERTED 2
 #define FT_STATUS_DT_COLLAPSE 4
-/* Informational (FP32 engine): in some step the minimum element time step
- * moved by more than 0.1 % and came from FP32 geometry instead of the FP64
- * refinement of near-minimum elements. */
+/* Kept for compatibility, never raised: in the FP32 engine's run loop the
+ * step size always comes from FP64 geometry of the minimising element. */
 #define FT_STATUS_DT_FP32 16
 
 typedef struct ft_engine ft_engine;

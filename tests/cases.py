@@ -265,11 +265,11 @@ def fan_mesh(nz=3, r0=1e-3, h=0.5e-3, jitter=0.08, seed=5):
 def case_fan(steps=300):
     """fan_mesh under impact loading: base layer fixed, top layer starting at
     v_z = -5 m/s; neo-Hookean fan (G = 1 MPa, K = 10 MPa) inside an isotropic
-    HGO + Prony ring."""
+    HGO + Prony ring (Prony time constants 2 and 20 ms, as case_materials)."""
     X, conn, pid, npl = fan_mesh()
     nn = X.size // 3
     props = np.stack([props_row(1000.0, 1.0e6, 1.0e7 - 2.0e6 / 3.0),
-                      props_row(1000.0, 4.0e5, 4.0e6, 3.0e5, 6.0, 0.4, 2e-4, 0.3, 2e-3)])
+                      props_row(1000.0, 4.0e5, 4.0e6, 3.0e5, 6.0, 0.4, 2e-3, 0.3, 2e-2)])
     Xr = X.reshape(-1, 3)
     zmax = Xr[:, 2].max()
     dm = -np.ones(3 * nn, np.int32)

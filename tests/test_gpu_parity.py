@@ -523,3 +523,33 @@ def test_batch_instance_failure_is_isolated(gpu_ready):
     ug, _, _, tg = run_batch_or_single([good], [tmax[1]], ft.FT_FP64, 2000)
     assert np.array_equal(u[3 * no[1]:3 * no[2]], ug)
     assert int(info[1][2]) == tg[2]
+
+
+@pytest.mark.parametrize("name", ["impact", "materials"])
+def test_fp32_step_sizes_come_from_fp64_geometry(gpu_ready, name):
+    """FP32 engine, one step per call: each next step size equals dt_factor x
+    the FP64 stable step (the oracle's StableTimeStep) at the engine's own
+    displacement, including the steps where the minimum rises by more than
+    0.1 % (half of the impact run's steps): the element kernel's last block
+    refines the best unrefined candidate in FP64."""
+    z = golden("run_" + name)
+    o = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"])
+    bnd = (z["dof_motion"] >= 0).astype(np.int32)
+    factor = float(z["dt_factor"])
+    worst, rises, prev = 0.0, 0, None
+    with engine_for(z, ft.FT_FP32) as e:
+        e.set_state(None, z["v0"] if "v0" in z else None)
+        e.lumped_mass()
+        e.run_init(float(z["tmax"]), factor, float(z["failure_dt"]))
+        for k in range(120):
+            assert e.run_steps(1) == 1
+            u = e.state()[0]
+            _, dt_next, _ = e.time()
+            want = factor * o.stable_dt(u, bnd)
+            worst = max(worst, abs(dt_next - want) / want)
+            if prev is not None and dt_next > prev * 1.001:
+                rises += 1
+            prev = dt_next
+        assert e.status() == 0
+    assert rises > 0 or name == "materials"
+    assert worst < 1e-12, worst
