@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1650,6 +1651,33 @@ __global__ void k_sum3(int nblk, const double *part, double *out) {
   }
 }
 
+// Scratch of ft_energy_terms_noalloc: the kEnergyBlocks partial triples of
+// k_energy_terms and the three totals of k_sum3 (12.3 KB), one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is only a stand-in for the symbol): the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device at engine creation (scratch_address) and kept per device.
+constexpr size_t kScratchDoubles = 3 * (size_t)kEnergyBlocks + 3;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// Device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(FT_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FT_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
 // Plain streaming copy (calibration of the HBM traffic counters).
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const float4 *src, float4 *dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1897,6 +1925,7 @@ struct ft_engine {
   // staging for the entry points that take host arrays every call
   double *aux = nullptr;
   size_t aux_len = 0;
+  double *scratch = nullptr;  // this device's copy of `scratch` (scratch_address)
   // Run-loop chunks captured as HIP graphs, one per starting parity;
   // graph_steps == 0 launches every step eagerly.
   int graph_steps = 0;
@@ -2664,6 +2693,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
     return code;
   };
   if (hipStreamCreate(&g->stream) != hipSuccess) return bail(fail(FT_ERR_HIP, "stream creation failed"));
+  if ((rc = scratch_address(g->device, &g->scratch))) return bail(rc);
   g->lstream = g->stream;
   // parts (CalculateTimeStep.cpp:13-19 wave speed kept in FP64)
   g->h_parts.resize(n_parts);
@@ -3854,6 +3884,34 @@ int ft_energy_terms(ft_engine *g, const double *u, const double *u_prev, const d
   HIPCHK(hipMemcpyAsync(db, boundary, nd * sizeof(int), hipMemcpyHostToDevice, g->stream));
   EnergyIn ei{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], g->mass, db};
   hipLaunchKernelGGL(k_energy_terms, dim3(kEnergyBlocks), dim3(kBlock), 0, g->stream, (int)nd, ei, part);
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(64), 0, g->stream, kEnergyBlocks, part, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out3, res, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+// ft_energy_terms on device arrays, with no allocation: the partial sums go
+// to this GPU's copy of the static `scratch` (every call overwrites all of
+// it, so nothing needs clearing); calls that overlap on one GPU share it and
+// race.  ndof must be 3 * nodes of the engine.
+int ft_energy_terms_noalloc(ft_engine *g, long ndof, const double *u, const double *u_prev,
+                            const double *v, const double *a, const double *a_prev, const double *fi,
+                            const double *fi_prev, const double *fe, const double *fe_prev,
+                            const int *boundary, double out3[3]) {
+  const double *in[] = {u, u_prev, v, a, a_prev, fi, fi_prev, fe, fe_prev};
+  if (!g || !boundary || !out3) return fail(FT_ERR_ARG, "null argument");
+  for (const double *p : in)
+    if (!p) return fail(FT_ERR_ARG, "null argument");
+  if (ndof != 3L * g->nn) return fail(FT_ERR_ARG, "ndof %ld is not 3 x %d nodes", ndof, g->nn);
+  if (!g->mass_ready) return fail(FT_ERR_STATE, "lumped mass not assembled");
+  if (!g->scratch) return fail(FT_ERR_STATE, "scratch address not set");
+  // The partials of the fixed kEnergyBlocks-block grid and the totals fill
+  // the scratch exactly, whatever the mesh size (the kernel strides).
+  static_assert(3 * (size_t)kEnergyBlocks + 3 <= kScratchDoubles, "scratch too small");
+  double *part = g->scratch, *res = g->scratch + 3 * kEnergyBlocks;
+  EnergyIn ei{u, u_prev, v, a, a_prev, fi, fi_prev, fe, fe_prev, g->mass, boundary};
+  hipLaunchKernelGGL(k_energy_terms, dim3(kEnergyBlocks), dim3(kBlock), 0, g->stream, (int)ndof, ei, part);
   hipLaunchKernelGGL(k_sum3, dim3(1), dim3(64), 0, g->stream, kEnergyBlocks, part, res);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out3, res, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));

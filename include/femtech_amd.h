@@ -199,6 +199,21 @@ int ft_energy_terms(ft_engine *eng, const double *u, const double *u_prev,
                     const double *v, const double *a, const double *a_prev,
                     const double *fi, const double *fi_prev, const double *fe,
                     const double *fe_prev, const int *boundary, double out3[3]);
+/* ft_energy_terms on DEVICE arrays, allocating nothing (no counterpart in the
+ * reference: for callers that cannot allocate device memory, e.g. code under
+ * HIP graph capture or a tight serving loop).  u ... fe_prev, boundary: device
+ * pointers, node-major (3 per node), ndof = 3 x the engine's nodes; out3: host
+ * pointer, as ft_energy_terms.  The partial sums live in a static 12.3 KB
+ * device array, one copy per GPU held until the process ends; any mesh fits
+ * (the kernel strides over the DOFs with a fixed grid), so no size is
+ * rejected for scratch.  Calls that overlap on one GPU share that array and
+ * race.  Needs the lumped mass. */
+int ft_energy_terms_noalloc(ft_engine *eng, long ndof, const double *u,
+                            const double *u_prev, const double *v, const double *a,
+                            const double *a_prev, const double *fi,
+                            const double *fi_prev, const double *fe,
+                            const double *fe_prev, const int *boundary,
+                            double out3[3]);
 
 /* ---- injury criteria ------------------------------------------------------
  * CalculateInjuryCriterions of FemTech's ex5 (ex5.cpp:1311-1430) with its
