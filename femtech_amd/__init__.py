@@ -79,6 +79,9 @@ _SIGS = {
     "ft_get_timing": [_VP, _D, _I],
     "ft_traffic_model": [_VP, _D],
     "ft_copy_probe": [ctypes.c_double, ctypes.c_int, _D],
+    "ft_point_eval": [ctypes.c_int, ctypes.c_int, _D, _D],
+    "ft_get_force_fields": [_VP, _D, ctypes.c_double, _D, _D, _D, _D],
+    "ft_set_visco_history": [_VP, _D],
     "ft_mesh_ellipsoid": [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _I, _D],
     "ft_set_instances": [_VP, ctypes.c_int, _I, _I],
     "ft_set_instance_end_times": [_VP, _D],
@@ -163,6 +166,25 @@ def device_count():
     c = ctypes.c_int(0)
     _check(library().ft_device_count(ctypes.byref(c)))
     return c.value
+
+
+# ft_point_eval operations (include/femtech_amd.h) and their record sizes
+PT_SHAPE, PT_DEFGRAD, PT_DETF, PT_INVF, PT_STRESS, PT_BMAT, PT_FINT, PT_CHARLEN, PT_TIMESTEP, PT_INV3 = range(10)
+PT_SIZES = {PT_SHAPE: (27, 33), PT_DEFGRAD: (48, 9), PT_DETF: (9, 1), PT_INVF: (10, 9), PT_STRESS: (20, 34),
+            PT_BMAT: (12, 18), PT_FINT: (64, 192), PT_CHARLEN: (24, 1), PT_TIMESTEP: (27, 1), PT_INV3: (9, 10)}
+
+
+def point_eval(op, records, out=None):
+    """FemTech's per-Gauss-point routines on the device: `records` is an
+    (n, in_size) array of input records; returns the (n, out_size) outputs
+    (`out` supplies their initial contents, e.g. the Prony history of the
+    stress operation)."""
+    ni, no = PT_SIZES[op]
+    rec = _f64(np.asarray(records, np.float64).reshape(-1, ni))
+    n = rec.shape[0]
+    res = np.zeros((n, no)) if out is None else _f64(np.array(out, np.float64).reshape(n, no))
+    _check(library().ft_point_eval(op, n, _dp(rec), _dp(res)))
+    return res
 
 
 def copy_probe(megabytes=256.0, repeats=10):
@@ -304,6 +326,21 @@ class Engine:
         u = None if u is None else _f64(u)
         _check(self._lib.ft_get_force(self._h, _dp(u), dt, _dp(fi)))
         return fi
+
+    def force_fields(self, u=None, dt=0.0):
+        """GetForce with the per-Gauss-point fields: (fi, F [ne, 8, 9],
+        detF [ne, 8], pk2 [ne, 8, 6])."""
+        fi = np.empty(3 * self.nn)
+        F = np.empty(72 * self.ne)
+        dF = np.empty(8 * self.ne)
+        S = np.empty(48 * self.ne)
+        u = None if u is None else _f64(u)
+        _check(self._lib.ft_get_force_fields(self._h, _dp(u), dt, _dp(fi), _dp(F), _dp(dF), _dp(S)))
+        return fi, F.reshape(-1, 8, 9), dF.reshape(-1, 8), S.reshape(-1, 8, 6)
+
+    def set_visco_history(self, hist):
+        h = _f64(hist)
+        _check(self._lib.ft_set_visco_history(self._h, _dp(h)))
 
     def element_forces(self, u=None, dt=0.0, update_history=False):
         fe = np.empty(24 * self.ne, np.float64)

@@ -18,7 +18,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libfemtech_amd.so")
 COMPAT_PATH = os.path.join(LIB_DIR, "libFemTech.so")
 DRIVER_PATH = os.path.join(LIB_DIR, "explicit_tension")
 INCLUDE = os.path.join(ROOT, "include")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("ft_engine.hip", "ft_surrogate.hip", "ft_mesh.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("ft_engine.hip", "ft_surrogate.hip", "ft_point.hip",
+                                                    "ft_mesh.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are switched off for every HIP translation unit.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

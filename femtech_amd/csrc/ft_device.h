@@ -1065,12 +1065,20 @@ constexpr int kFlagMaterial = 8;
 // order (node-major, xyz) and lowers dt_e when DT is set and the element is
 // not skipped.  `hist` is an accessor with load(gp, k) / store(gp, k, v) for
 // the 18 Prony values of a Gauss point (materials 5 and 7).
-template <typename R, bool FORCE, bool DT, int MAT, class Hist, bool SURR = false>
+// Per-Gauss-point fields of FemTech's element layer for one element, in the
+// reference's layout (ShapeFunctions.cpp:174-205 with 8 Gauss points): F
+// (9 per point, column-major), det F (1) and the PK2 stress (6, Voigt).
+struct GpSink {
+  double *F, *detF, *pk2;
+};
+
+template <typename R, bool FORCE, bool DT, int MAT, class Hist, bool SURR = false, bool GPOUT = false>
 __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartDev &pp,
                                              bool skip_dt, double dt_step, Hist &hist,
                                              bool update_hist, R outv[24], double &my_dt,
                                              int &flags, const R fib[3] = nullptr,
-                                             const SurrIn *sin = nullptr, SurrTally *tally = nullptr) {
+                                             const SurrIn *sin = nullptr, SurrTally *tally = nullptr,
+                                             const GpSink *gps = nullptr) {
   if (DT && !skip_dt) {
     // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
     R dcf[7][3];
@@ -1084,7 +1092,7 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
     if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
   }
   const int mat = MAT >= 0 ? MAT : pp.mat;
-  if (FORCE && mat == 0) {  // rigid: no stress, no internal force
+  if (FORCE && mat == 0 && !GPOUT) {  // rigid: no stress, no internal force
 #pragma unroll
     for (int k = 0; k < 24; ++k) outv[k] = R(0);
   } else if (FORCE) {
@@ -1175,7 +1183,7 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
 #pragma unroll
         for (int j = 0; j < 3; ++j)
           H[i][j] = (Ju[i][0] * A[0][j] + Ju[i][1] * A[1][j] + Ju[i][2] * A[2][j]) * idet0;
-      if constexpr ((MAT == 1 || MAT == 4 || MAT == 6 || MAT == 8) && !SURR) {
+      if constexpr ((MAT == 1 || MAT == 4 || MAT == 6 || MAT == 8) && !SURR && !GPOUT) {
         // Spatial form: f_a = sum_gp det0 P J0^-T dN_a/dxi with P = tau F^-T
         // and F J0 = Jx (the deformed Jacobian), so Q = det0 tau Jx^-T =
         // tau adj(Jx)^T / J.
@@ -1239,6 +1247,13 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
       if (use_hist && update_hist) {
 #pragma unroll
         for (int k = 0; k < 18; ++k) hist.store(gp, k, hloc[k]);
+      }
+      if constexpr (GPOUT) {
+        // F = I + H (CalculateDeformationGradient.cpp:14 layout), det F, S
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) gps->F[9 * gp + 3 * j + i] = (double)H[i][j] + (i == j ? 1.0 : 0.0);
+        gps->detF[gp] = 1.0 + (double)Jm1;
+        for (int k = 0; k < 6; ++k) gps->pk2[6 * gp + k] = (double)S.v[k];
       }
       // P = F S (first Piola), Q = w detJ0 P J0^-T = P A^T  (w = 1)
       R P[3][3];

@@ -318,6 +318,10 @@ struct ElemArgs {
   int parity;
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
   double *cand_geom;       // FP32 run loop: kCandStride per element tile (candidate geometry)
+  // per-Gauss-point fields (k_element_gp): 72 / 8 / 48 doubles per element in
+  // the caller's element order (perm: internal -> caller)
+  double *gp_F, *gp_detF, *gp_pk2;
+  const int *perm;
   double dt_fixed;       // used outside the run loop (ctl == nullptr)
   unsigned long long *dtbits;  // outside the run loop: per-instance minimum (nullable)
   int *status;
@@ -401,7 +405,7 @@ __device__ inline double refined_dt(const double (*sh_c)[3], double ce) {
 // element tile: the 21 coefficients of deformed_coeffs and the wave speed.
 constexpr int kCandStride = 22;
 
-template <typename R, bool FORCE, bool DT, int MAT, bool SURR = false>
+template <typename R, bool FORCE, bool DT, int MAT, bool SURR = false, bool GPOUT = false>
 __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) {
   __shared__ double sh_min[kBlock / 64];
   if (L >= a.n_tiles) return;  // grid padding (uniform over the block)
@@ -462,6 +466,12 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
                        a.surr_audit != 0 && audited(e, a.surr_permille)};
       element_eval<R, FORCE, DT, MAT, HistSoA<R>, true>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
                                                        a.update_hist != 0, outv, my_dt, flags, fib, &sin, &tally);
+    } else if constexpr (GPOUT) {
+      const size_t ce = (size_t)a.perm[e];
+      const GpSink gs{a.gp_F + 72 * ce, a.gp_detF + 8 * ce, a.gp_pk2 + 48 * ce};
+      element_eval<R, FORCE, DT, MAT, HistSoA<R>, false, true>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
+                                                              a.update_hist != 0, outv, my_dt, flags, fib,
+                                                              nullptr, nullptr, &gs);
     } else {
       element_eval<R, FORCE, DT, MAT>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
                                  a.update_hist != 0, outv, my_dt, flags, fib);
@@ -539,6 +549,13 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
 template <typename R, bool FORCE, bool DT, int MAT, int WAVES>
 __global__ __launch_bounds__(kBlock, (elem_waves<R, MAT, WAVES>())) void k_element(ElemArgs<R> a) {
   element_tile<R, FORCE, DT, MAT>(a, xcd_block());
+}
+
+// Forces plus FemTech's per-Gauss-point F, det F and PK2 (every law through
+// its PK2 form; the compatibility layer's GetForce_3D).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_element_gp(ElemArgs<R> a) {
+  element_tile<R, true, false, -1, false, true>(a, xcd_block());
 }
 
 // Run-loop element kernel of a surrogate segment (FP32, force + dt).
@@ -1875,9 +1892,8 @@ struct ft_engine {
   std::vector<int> inst_node, inst_elem;   // n_inst + 1 offsets each
   int4 *etiles = nullptr, *ntiles = nullptr;
   double *cand_geom = nullptr;  // kCandStride per element tile (FP32 exact step size)
-  // DT-only passes outside the run loop (StableTimeStep, the run's first dt):
-  // the FP32 engine's blocks take their minimum from FP64 geometry
-  bool dt_exact_pass = false;
+  double *gp_fields = nullptr;  // per-Gauss-point F, det F, PK2 (128 per element), on demand
+  int *perm_dev = nullptr;      // internal -> caller element index, on demand
   int n_etiles = 0, n_ntiles = 0;
   std::vector<int> ntile_of_inst;          // n_inst + 1: node tiles per instance
   int *mrange = nullptr;                   // per instance: first motion, count
@@ -2081,8 +2097,10 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.fib = reinterpret_cast<const R *>(g->fib);
   a.ctl = in_loop ? g->ctl : nullptr;
   a.parity = g->parity;
-  a.refine = (sizeof(R) == 4 && (in_loop || g->dt_exact_pass)) ? 1 : 0;
+  a.refine = (sizeof(R) == 4 && in_loop) ? 1 : 0;
   a.cand_geom = g->cand_geom;
+  a.gp_F = a.gp_detF = a.gp_pk2 = nullptr;
+  a.perm = nullptr;
   a.dt_fixed = dt_fixed;
   a.dtbits = in_loop ? nullptr : dtbits;
   a.status = g->status;
@@ -2826,7 +2844,7 @@ int ft_engine_destroy(ft_engine *g) {
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
-                  g->etiles, g->cand_geom, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
+                  g->etiles, g->cand_geom, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -2980,10 +2998,7 @@ int ft_stable_time_step(ft_engine *g, const double *u, const int *boundary, doub
   }
   std::vector<unsigned long long> bits(g->n_inst, kInfBits);
   HIPCHK(hipMemcpyAsync(g->dtbits, bits.data(), bits.size() * sizeof(bits[0]), hipMemcpyHostToDevice, g->stream));
-  g->dt_exact_pass = true;
-  rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0);
-  g->dt_exact_pass = false;
-  if (rc) return rc;
+  if ((rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0))) return rc;
   HIPCHK(hipMemcpyAsync(bits.data(), g->dtbits, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
   const unsigned long long bmin = *std::min_element(bits.begin(), bits.end());
@@ -2995,6 +3010,37 @@ int ft_stable_time_step(ft_engine *g, const double *u, const int *boundary, doub
   return 0;
 }
 
+// fi = node sums of the element forces (GetForce_3D.cpp:39-44), into g->fi.
+static int gather_forces(ft_engine *g) {
+  {
+    Timed t(g, 1);
+    const bool w8 = g->width == 8;
+    if (g->corner) {
+      if (g->precision == FT_FP32)
+        hipLaunchKernelGGL((k_gather<float, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const float *)g->fe, g->fi);
+      else
+        hipLaunchKernelGGL((k_gather<double, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const double *)g->fe, g->fi);
+    } else if (g->precision == FT_FP32) {
+      if (w8)
+        hipLaunchKernelGGL((k_gather<float, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const float *)g->fe, g->fi);
+      else
+        hipLaunchKernelGGL((k_gather<float, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const float *)g->fe, g->fi);
+    } else {
+      if (w8)
+        hipLaunchKernelGGL((k_gather<double, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const double *)g->fe, g->fi);
+      else
+        hipLaunchKernelGGL((k_gather<double, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
+                           g->ell, g->width, (const double *)g->fe, g->fi);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
 int ft_element_forces(ft_engine *g, const double *u, double dt, int update_history, double *fe_out) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = upload_u(g, u);
@@ -3026,34 +3072,42 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
   int rc = upload_u(g, u);
   if (rc) return rc;
   if ((rc = element_pass(g, true, false, false, dt, nullptr, 1))) return rc;
-  {
-    Timed t(g, 1);
-    const bool w8 = g->width == 8;
-    if (g->corner) {
-      if (g->precision == FT_FP32)
-        hipLaunchKernelGGL((k_gather<float, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const float *)g->fe, g->fi);
-      else
-        hipLaunchKernelGGL((k_gather<double, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const double *)g->fe, g->fi);
-    } else if (g->precision == FT_FP32) {
-      if (w8)
-        hipLaunchKernelGGL((k_gather<float, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const float *)g->fe, g->fi);
-      else
-        hipLaunchKernelGGL((k_gather<float, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const float *)g->fe, g->fi);
-    } else {
-      if (w8)
-        hipLaunchKernelGGL((k_gather<double, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const double *)g->fe, g->fi);
-      else
-        hipLaunchKernelGGL((k_gather<double, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const double *)g->fe, g->fi);
-    }
-    HIPCHK(hipGetLastError());
-  }
+  if ((rc = gather_forces(g))) return rc;
   if ((rc = check_status(g))) return rc;
+  if (fi_out) return d2h_nodal(g, fi_out, g->fi);
+  return 0;
+}
+
+int ft_get_force_fields(ft_engine *g, const double *u, double dt, double *fi_out, double *F_out,
+                        double *detF_out, double *pk2_out) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  const size_t ne = (size_t)g->ne;
+  int rc;
+  if (!g->gp_fields) {
+    if ((rc = dalloc(&g->gp_fields, 128 * ne)) || (rc = dalloc(&g->perm_dev, ne))) return rc;
+    HIPCHK(hipMemcpy(g->perm_dev, g->perm.data(), ne * sizeof(int), hipMemcpyHostToDevice));
+  }
+  if ((rc = upload_u(g, u))) return rc;
+  {
+    Timed t(g, 0);
+    auto launch = [&](auto zero) -> int {
+      using R = decltype(zero);
+      ElemArgs<R> a = elem_args<R>(g, 0, g->n_etiles, false, dt, nullptr, 1);
+      a.gp_F = g->gp_fields;
+      a.gp_detF = g->gp_fields + 72 * ne;
+      a.gp_pk2 = g->gp_fields + 80 * ne;
+      a.perm = g->perm_dev;
+      hipLaunchKernelGGL((k_element_gp<R>), dim3(grid_for_tiles(g->n_etiles)), dim3(kBlock), 0, g->stream, a);
+      HIPCHK(hipGetLastError());
+      return 0;
+    };
+    if ((rc = g->precision == FT_FP32 ? launch(0.0f) : launch(0.0))) return rc;
+  }
+  if ((rc = gather_forces(g))) return rc;
+  if ((rc = check_status(g))) return rc;
+  if (F_out) HIPCHK(hipMemcpy(F_out, g->gp_fields, 72 * ne * sizeof(double), hipMemcpyDeviceToHost));
+  if (detF_out) HIPCHK(hipMemcpy(detF_out, g->gp_fields + 72 * ne, 8 * ne * sizeof(double), hipMemcpyDeviceToHost));
+  if (pk2_out) HIPCHK(hipMemcpy(pk2_out, g->gp_fields + 80 * ne, 48 * ne * sizeof(double), hipMemcpyDeviceToHost));
   if (fi_out) return d2h_nodal(g, fi_out, g->fi);
   return 0;
 }
@@ -3095,10 +3149,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
   int rc;
   std::vector<unsigned long long> bits(g->n_inst, kInfBits);
   HIPCHK(hipMemcpy(g->dtbits, bits.data(), bits.size() * sizeof(bits[0]), hipMemcpyHostToDevice));
-  g->dt_exact_pass = true;
-  rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0);
-  g->dt_exact_pass = false;
-  if (rc) return rc;
+  if ((rc = element_pass(g, false, true, false, 0.0, g->dtbits, 0))) return rc;
   HIPCHK(hipMemcpy(bits.data(), g->dtbits, bits.size() * sizeof(bits[0]), hipMemcpyDeviceToHost));
   std::vector<Ctl> c(g->n_inst);
   for (int i = 0; i < g->n_inst; ++i) {
@@ -3363,6 +3414,24 @@ int ft_get_visco_history(ft_engine *g, double *hist) {
   // 64-element blocks, internal order -> element-major original order
   for (size_t e = 0; e < (size_t)g->ne; ++e)
     for (int c = 0; c < 144; ++c) hist[e * 144 + c] = cm[hist_index(c, g->inv[e])];
+  return 0;
+}
+
+int ft_set_visco_history(ft_engine *g, const double *hist) {
+  if (!g || !hist) return fail(FT_ERR_ARG, "null argument");
+  if (!g->hist) return 0;  // no Prony law in the mesh
+  drop_graphs(g);
+  const size_t len = hist_len(g->ne);
+  std::vector<double> cm(len, 0.0);
+  for (size_t e = 0; e < (size_t)g->ne; ++e)
+    for (int c = 0; c < 144; ++c) cm[hist_index(c, g->inv[e])] = hist[e * 144 + c];
+  HIPCHK(hipStreamSynchronize(g->stream));
+  if (g->precision == FT_FP32) {
+    std::vector<float> t(cm.begin(), cm.end());
+    HIPCHK(hipMemcpy(g->hist, t.data(), len * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(g->hist, cm.data(), len * sizeof(double), hipMemcpyHostToDevice));
+  }
   return 0;
 }
 

@@ -62,6 +62,7 @@ namespace {
 
 ft_engine *g_engine = nullptr;
 std::vector<double> g_principal;  // 3 per element, from the last CalculateStrain
+bool g_hist_host_newer = false;   // Hn_1 / Hn_2 / S0n changed by a host-side StressUpdate
 double g_wint = 0.0, g_wext = 0.0;  // CheckEnergy.cpp:4-5 accumulators
 FILE *g_log = nullptr;
 
@@ -380,9 +381,13 @@ void FreeArrays() {
 }
 
 // ---- the explicit path ---------------------------------------------------------
-// ShapeFunctions.cpp:32: the engine uploads the mesh and materials once; shape
-// function gradients are recomputed in registers every step, so shp / dshp /
-// detJacobian are not materialised.
+// ShapeFunctions.cpp:32-255: the engine uploads the mesh and materials once
+// (its element kernel recomputes shape-function gradients in registers every
+// step), and FemTech's per-Gauss-point arrays are set up as the reference
+// leaves them: pointer tables, N, dN/dX and det J at the 8 Gauss points
+// (evaluated on the device), unit weights, F = invF = I, det F = 1, zero PK2
+// and internals, and the Prony history arrays when a part uses material 5
+// (or this engine's Ogden + Prony law, 7).
 void ShapeFunctions() {
   if (g_engine) ft_engine_destroy(g_engine);
   g_engine = nullptr;
@@ -393,6 +398,75 @@ void ShapeFunctions() {
   check(ft_engine_create(nNodes, nelements, coordinates, connectivity, pid, nPIDglobal, materialID, properties,
                          precision_from_env(), &g_engine),
         "ShapeFunctions");
+  const size_t ne = (size_t)nelements;
+  release(GaussPoints), release(nShapeFunctions), release(gptr), release(dsptr), release(gpPtr);
+  release(fptr), release(pk2ptr), release(detFptr), release(InternalsPtr), release(shp), release(dshp);
+  release(detJacobian), release(gaussWeights), release(F), release(detF), release(invF), release(internals);
+  release(pk2), release(fintGQ), release(B), release(Hn_1), release(Hn_2), release(S0n);
+  GaussPoints = zalloc<int>(ne, "GaussPoints");
+  nShapeFunctions = zalloc<int>(ne, "nShapeFunctions");
+  gptr = zalloc<int>(ne + 1, "gptr");
+  dsptr = zalloc<int>(ne + 1, "dsptr");
+  gpPtr = zalloc<int>(ne + 1, "gpPtr");
+  fptr = zalloc<int>(ne + 1, "fptr");
+  pk2ptr = zalloc<int>(ne + 1, "pk2ptr");
+  detFptr = zalloc<int>(ne + 1, "detFptr");
+  InternalsPtr = zalloc<int>(ne + 1, "InternalsPtr");
+  for (size_t e = 0; e < ne; ++e) {  // C3D8 only: 8 Gauss points, 8 shape functions
+    GaussPoints[e] = 8;
+    nShapeFunctions[e] = 8;
+    gptr[e + 1] = gptr[e] + 64;
+    dsptr[e + 1] = dsptr[e] + 192;
+    gpPtr[e + 1] = gpPtr[e] + 8;
+    fptr[e + 1] = fptr[e] + 72;
+    pk2ptr[e + 1] = pk2ptr[e] + 48;
+    detFptr[e + 1] = detFptr[e] + 8;
+    InternalsPtr[e + 1] = InternalsPtr[e] + MAXINTERNALVARS * 8;
+  }
+  shp = zalloc<double>(64 * ne, "shp");
+  dshp = zalloc<double>(192 * ne, "dshp");
+  detJacobian = zalloc<double>(8 * ne, "detJacobian");
+  gaussWeights = zalloc<double>(8 * ne, "gaussWeights");
+  F = zalloc<double>(72 * ne, "F");
+  detF = zalloc<double>(8 * ne, "detF");
+  invF = zalloc<double>(72 * ne, "invF");
+  for (size_t q = 0; q < 8 * ne; ++q) {
+    F[9 * q] = F[9 * q + 4] = F[9 * q + 8] = 1.0;
+    invF[9 * q] = invF[9 * q + 4] = invF[9 * q + 8] = 1.0;
+    detF[q] = 1.0;
+  }
+  internals = zalloc<double>((size_t)MAXINTERNALVARS * 8 * ne, "internals");
+  pk2 = zalloc<double>(48 * ne, "pk2");
+  fintGQ = zalloc<double>(24, "fintGQ");
+  B = zalloc<double>(144, "B");
+  for (int p = 0; p < nPIDglobal; ++p)
+    if (materialID[p] == 5 || materialID[p] == 7) {  // Prony laws
+      Hn_1 = zalloc<double>(72 * ne, "Hn_1");
+      Hn_2 = zalloc<double>(72 * ne, "Hn_2");
+      S0n = zalloc<double>(72 * ne, "S0n");
+      break;
+    }
+  // N, dN/dX, det J at the Gauss points of every element, one device pass
+  std::vector<double> in(27 * 8 * ne), out(33 * 8 * ne);
+  double chi[24], w[8];
+  GaussQuadrature3D(0, 8, chi, w);
+  for (size_t e = 0; e < ne; ++e) {
+    for (int gp = 0; gp < 8; ++gp) {
+      double *r = &in[27 * (8 * e + gp)];
+      for (int k = 0; k < 3; ++k) r[k] = chi[3 * gp + k];
+      for (int a = 0; a < 8; ++a)
+        for (int k = 0; k < 3; ++k) r[3 + 3 * a + k] = coordinates[3 * (size_t)connectivity[8 * e + a] + k];
+      gaussWeights[8 * e + gp] = w[gp];
+    }
+  }
+  check(ft_point_eval(FT_PT_SHAPE, (int)(8 * ne), in.data(), out.data()), "ShapeFunctions");
+  for (size_t q = 0; q < 8 * ne; ++q) {
+    const double *r = &out[33 * q];
+    for (int a = 0; a < 8; ++a) shp[8 * q + a] = r[a];
+    for (int k = 0; k < 24; ++k) dshp[24 * q + k] = r[8 + k];
+    detJacobian[q] = r[32];
+  }
+  g_hist_host_newer = false;
 }
 
 // GaussQuadrature3D.cpp:3-60 (C3D8, 8 points): the 2x2x2 Gauss table in the
@@ -440,10 +514,214 @@ void GetForce() {
 }
 
 // GetForce_3D.cpp:5-53: fi from the element kernel and the node gather (the
-// Prony history advances with the driver's dt); f_net = fe - fi.
+// Prony history advances with the driver's dt); f_net = fe - fi.  The
+// per-Gauss-point F, det F and PK2 the reference's loop leaves in its arrays
+// come back from the same pass, and so does the Prony history (Hn_1, Hn_2,
+// S0n as full 3 x 3 matrices); a history changed on the host by the
+// per-point StressUpdate goes to the device first.
 void GetForce_3D() {
-  check(ft_get_force(g_engine, displacements, dt, fi), "GetForce_3D");
+  const size_t ne = (size_t)nelements;
+  if (Hn_1 && g_hist_host_newer) {
+    std::vector<double> h(144 * ne);
+    static const int vi[6] = {0, 4, 8, 7, 6, 3};
+    for (size_t e = 0; e < ne; ++e)
+      for (int gp = 0; gp < 8; ++gp)
+        for (int t = 0; t < 3; ++t) {
+          const double *m = (t == 0 ? Hn_1 : t == 1 ? Hn_2 : S0n) + fptr[e] + 9 * gp;
+          for (int k = 0; k < 6; ++k) h[144 * e + 18 * gp + 6 * t + k] = m[vi[k]];
+        }
+    check(ft_set_visco_history(g_engine, h.data()), "GetForce_3D");
+    g_hist_host_newer = false;
+  }
+  std::vector<double> stress(F ? 48 * ne : 0);
+  check(ft_get_force_fields(g_engine, displacements, dt, fi, F, detF, F ? stress.data() : nullptr), "GetForce_3D");
+  if (F)  // rigid parts: StressUpdate leaves their PK2 untouched (StressUpdate.cpp:8-9)
+    for (size_t e = 0; e < ne; ++e)
+      if (materialID[pid[e]] != 0) memcpy(pk2 + pk2ptr[e], &stress[48 * e], 48 * sizeof(double));
+  if (Hn_1) {
+    std::vector<double> h(144 * ne);
+    check(ft_get_visco_history(g_engine, h.data()), "GetForce_3D");
+    for (size_t e = 0; e < ne; ++e) {
+      if (materialID[pid[e]] != 5 && materialID[pid[e]] != 7) continue;
+      for (int gp = 0; gp < 8; ++gp)
+        for (int t = 0; t < 3; ++t) {
+          double *m = (t == 0 ? Hn_1 : t == 1 ? Hn_2 : S0n) + fptr[e] + 9 * gp;
+          const double *v = &h[144 * e + 18 * gp + 6 * t];
+          m[0] = v[0], m[4] = v[1], m[8] = v[2];
+          m[5] = m[7] = v[3];
+          m[2] = m[6] = v[4];
+          m[1] = m[3] = v[5];
+        }
+    }
+  }
   for (int i = 0; i < nDOF; ++i) f_net[i] = fe[i] - fi[i];
+}
+
+// ---- per-element, per-Gauss-point routines ------------------------------------
+// Each gathers its inputs from the global arrays the reference routine reads,
+// evaluates on the device (ft_point_eval) and writes the outputs where the
+// reference writes them.  Calls need ShapeFunctions first (it sets up the
+// arrays), as in the reference.
+namespace {
+
+void need_arrays(const char *who) {
+  if (!dshp || !F) {
+    FILE_LOG_SINGLE(ERROR, "%s before ShapeFunctions", who);
+    TerminateFemTech(3);
+  }
+}
+
+void point(int op, const double *in, double *out, const char *who) {
+  check(ft_point_eval(op, 1, in, out), who);
+}
+
+// X + u of element e's nodes in local order (CalculateDeformationGradient.cpp:20,
+// CalculateCharacteristicLength_C3D8.cpp:6-11).
+void deformed_nodes(int e, double *x) {
+  for (int a = 0; a < 8; ++a)
+    for (int k = 0; k < 3; ++k) {
+      const size_t i = 3 * (size_t)connectivity[eptr[e] + a] + k;
+      x[3 * a + k] = coordinates[i] + displacements[i];
+    }
+}
+
+// The material routines with the material fixed (StressUpdate.cpp:7-23).
+void material_update(int e, int gp, int mat, const char *who) {
+  need_arrays(who);
+  double in[20], out[34] = {0};
+  memcpy(in, F + fptr[e] + 9 * gp, 9 * sizeof(double));
+  in[9] = mat;
+  memcpy(in + 10, properties + MAXMATPARAMS * pid[e], 9 * sizeof(double));
+  in[19] = dt;
+  const bool visco = mat == 5 || mat == 7;
+  if (visco) {
+    if (!Hn_1) {
+      FILE_LOG_SINGLE(ERROR, "%s: no viscoelastic part was set up", who);
+      TerminateFemTech(3);
+    }
+    memcpy(out + 6, Hn_1 + fptr[e] + 9 * gp, 9 * sizeof(double));
+    memcpy(out + 15, Hn_2 + fptr[e] + 9 * gp, 9 * sizeof(double));
+    memcpy(out + 24, S0n + fptr[e] + 9 * gp, 9 * sizeof(double));
+  }
+  point(FT_PT_STRESS, in, out, who);
+  memcpy(pk2 + pk2ptr[e] + 6 * gp, out, 6 * sizeof(double));
+  if (visco) {
+    memcpy(Hn_1 + fptr[e] + 9 * gp, out + 6, 9 * sizeof(double));
+    memcpy(Hn_2 + fptr[e] + 9 * gp, out + 15, 9 * sizeof(double));
+    memcpy(S0n + fptr[e] + 9 * gp, out + 24, 9 * sizeof(double));
+    g_hist_host_newer = true;
+  }
+}
+
+}  // namespace
+
+// ShapeFunction_C3D8.cpp:4-128
+void ShapeFunction_C3D8(int e, int gp, double *Chi, double *detJ) {
+  need_arrays("ShapeFunction_C3D8");
+  double in[27], out[33];
+  for (int k = 0; k < 3; ++k) in[k] = Chi[3 * gp + k];
+  for (int a = 0; a < 8; ++a)
+    for (int k = 0; k < 3; ++k) in[3 + 3 * a + k] = coordinates[3 * (size_t)connectivity[eptr[e] + a] + k];
+  point(FT_PT_SHAPE, in, out, "ShapeFunction_C3D8");
+  memcpy(shp + gptr[e] + 8 * gp, out, 8 * sizeof(double));
+  memcpy(dshp + dsptr[e] + 24 * gp, out + 8, 24 * sizeof(double));
+  detJ[gp] = out[32];
+}
+
+// CalculateDeformationGradient.cpp:11-25
+void CalculateDeformationGradient(int e, int gp) {
+  need_arrays("CalculateDeformationGradient");
+  double in[48];
+  deformed_nodes(e, in);
+  memcpy(in + 24, dshp + dsptr[e] + 24 * gp, 24 * sizeof(double));
+  point(FT_PT_DEFGRAD, in, F + fptr[e] + 9 * gp, "CalculateDeformationGradient");
+}
+
+// DeterminateF.cpp:38-57 (writes at gpPtr, as the reference does)
+void DeterminateF(int e, int gp) {
+  need_arrays("DeterminateF");
+  point(FT_PT_DETF, F + fptr[e] + 9 * gp, detF + gpPtr[e] + gp, "DeterminateF");
+}
+
+// InverseF.cpp (3-D branch): F and det F from the global arrays
+void InverseF(int e, int gp, double *fInv) {
+  need_arrays("InverseF");
+  double in[10];
+  memcpy(in, F + fptr[e] + 9 * gp, 9 * sizeof(double));
+  in[9] = detF[detFptr[e] + gp];
+  point(FT_PT_INVF, in, fInv, "InverseF");
+}
+
+// StressUpdate.cpp:5-29
+void StressUpdate(int e, int gp) {
+  const int mat = materialID[pid[e]];
+  if (mat == 0) return;  // rigid-body motion
+  if (mat < 0 || mat > 8) {
+    FILE_LOG_SINGLE(ERROR, "Unknown material type");
+    TerminateFemTech(1);
+  }
+  material_update(e, gp, mat, "StressUpdate");
+}
+
+void CompressibleNeoHookean(int e, int gp) { material_update(e, gp, 1, "CompressibleNeoHookean"); }
+void StVenantKirchhoff(int e, int gp) { material_update(e, gp, 2, "StVenantKirchhoff"); }
+void LinearElastic(int e, int gp) { material_update(e, gp, 3, "LinearElastic"); }
+void HGOIsotropic(int e, int gp) { material_update(e, gp, 4, "HGOIsotropic"); }
+void HGOIsotropicViscoelastic(int e, int gp) { material_update(e, gp, 5, "HGOIsotropicViscoelastic"); }
+
+// StrainDisplacementMatrix.cpp:25-77
+void StrainDisplacementMatrix(int e, int gp, int nI, double *Bmat) {
+  need_arrays("StrainDisplacementMatrix");
+  double in[12];
+  memcpy(in, dshp + dsptr[e] + 24 * gp + 3 * nI, 3 * sizeof(double));
+  memcpy(in + 3, F + fptr[e] + 9 * gp, 9 * sizeof(double));
+  point(FT_PT_BMAT, in, Bmat, "StrainDisplacementMatrix");
+}
+
+// InternalForceUpdate.cpp:4-28 (also leaves B and fintGQ as the reference does)
+void InternalForceUpdate(int e, int gp, double *force) {
+  need_arrays("InternalForceUpdate");
+  double in[64], out[192];
+  memcpy(in, dshp + dsptr[e] + 24 * gp, 24 * sizeof(double));
+  memcpy(in + 24, F + fptr[e] + 9 * gp, 9 * sizeof(double));
+  memcpy(in + 33, pk2 + pk2ptr[e] + 6 * gp, 6 * sizeof(double));
+  in[39] = gaussWeights[gpPtr[e] + gp] * detJacobian[gpPtr[e] + gp];
+  memcpy(in + 40, force, 24 * sizeof(double));
+  point(FT_PT_FINT, in, out, "InternalForceUpdate");
+  memcpy(force, out, 24 * sizeof(double));
+  memcpy(fintGQ, out + 24, 24 * sizeof(double));
+  memcpy(B, out + 48, 144 * sizeof(double));
+}
+
+// CalculateCharacteristicLength_C3D8.cpp:3-30 at the current displacements
+double CalculateCharacteristicLength_C3D8(int e) {
+  double in[24], out = 0.0;
+  deformed_nodes(e, in);
+  point(FT_PT_CHARLEN, in, &out, "CalculateCharacteristicLength_C3D8");
+  return out;
+}
+
+// CalculateCharacteristicLength.cpp (C3D8 is the only element type here)
+double CalculateCharacteristicLength(int e) { return CalculateCharacteristicLength_C3D8(e); }
+
+// CalculateTimeStep.cpp:7-21
+double CalculateTimeStep(int e) {
+  double in[27], out = 0.0;
+  deformed_nodes(e, in);
+  const double *pr = properties + MAXMATPARAMS * pid[e];
+  in[24] = pr[0];
+  in[25] = pr[1];
+  in[26] = pr[2];
+  point(FT_PT_TIMESTEP, in, &out, "CalculateTimeStep");
+  return out;
+}
+
+// math.cpp:8-28
+void inverse3x3Matrix(double *mat, double *invMat, double *det) {
+  double out[10];
+  point(FT_PT_INV3, mat, out, "inverse3x3Matrix");
+  memcpy(invMat, out, 9 * sizeof(double));
+  *det = out[9];
 }
 
 void CalculateAccelerations() {

@@ -21,12 +21,15 @@
  * (the driver then includes a real mpi.h first), the few MPI names drivers use
  * are given one-rank definitions here.
  *
- * Per-Gauss-point fields (F, detF, invF, pk2, shp, dshp, ...) are never
- * materialised: the fused element kernel keeps them in registers.  The
- * corresponding globals are declared for link compatibility and stay NULL;
- * the per-element, per-Gauss-point routines that fill them in the reference
- * (CalculateDeformationGradient, StressUpdate, InternalForceUpdate, ...) are
- * not part of this surface.  Implicit solvers (Assembly, SolveSteadyImplicit,
+ * Per-Gauss-point fields (F, detF, invF, pk2, shp, dshp, ...): the fused
+ * element kernel keeps them in registers, but ShapeFunctions sets the arrays
+ * up as the reference does, GetForce_3D fills F, detF and pk2 (and the Prony
+ * history) from the same device pass, and the per-element, per-Gauss-point
+ * routines (ShapeFunction_C3D8, CalculateDeformationGradient, DeterminateF,
+ * InverseF, StressUpdate and the material routines, StrainDisplacementMatrix,
+ * InternalForceUpdate, CalculateTimeStep, CalculateCharacteristicLength,
+ * inverse3x3Matrix) read and write them like the reference, each call
+ * evaluated on the GPU.  Implicit solvers (Assembly, SolveSteadyImplicit,
  * SolveUnsteadyNewmarkImplicit, ...) are outside the engine's scope: they are
  * defined so that drivers link, and terminate with code 3 if called.
  */
@@ -189,6 +192,24 @@ void WriteVTU(const char *FileName, int step, int **intCellData = NULL, int cell
               const char **cellDataNames = NULL); /* WriteVTU.cpp:3 */
 void WritePVD(const char *FileName, int step);    /* WritePVD.cpp:6 */
 void FreeArrays();                                /* FreeArrays.cpp:4 */
+
+/* ---- per-element, per-Gauss-point routines (FemTech.h:29-104) ------------- */
+void ShapeFunction_C3D8(int e, int gp, double *Chi, double *detJ);  /* ShapeFunction_C3D8.cpp:4 */
+double CalculateTimeStep(int e);                      /* CalculateTimeStep.cpp:7 */
+void CalculateDeformationGradient(int e, int gp);     /* CalculateDeformationGradient.cpp:11 */
+void StrainDisplacementMatrix(int e, int gp, int nI, double *B);  /* StrainDisplacementMatrix.cpp:3 */
+void StressUpdate(int e, int gp);                     /* StressUpdate.cpp:5 */
+void DeterminateF(int e, int gp);                     /* DeterminateF.cpp:38 */
+void InverseF(int e, int gp, double *fInv);           /* InverseF.cpp:4 */
+void InternalForceUpdate(int e, int gp, double *force);  /* InternalForceUpdate.cpp:4 */
+void StVenantKirchhoff(int e, int gp);                /* StVenantKirchhoff.cpp:14 */
+void CompressibleNeoHookean(int e, int gp);           /* CompressibleNeoHookean.cpp:14 */
+void LinearElastic(int e, int gp);                    /* LinearElastic.cpp:14 */
+void HGOIsotropic(int e, int gp);                     /* HGOIsotropic.cpp:21 */
+void HGOIsotropicViscoelastic(int e, int gp);         /* HGOIsotropicViscoelastic.cpp:27 */
+void inverse3x3Matrix(double *mat, double *invMat, double *det);  /* math.cpp:8 */
+double CalculateCharacteristicLength(int e);          /* CalculateCharacteristicLength.cpp:3 */
+double CalculateCharacteristicLength_C3D8(int e);     /* CalculateCharacteristicLength_C3D8.cpp:3 */
 
 /* ---- outside the engine (link compatibility only; terminate with 3) ------ */
 void CreateLinearElasticityCMatrix();

@@ -118,6 +118,16 @@ int ft_stable_time_step(ft_engine *eng, const double *u, const int *boundary,
  * displacement u (NULL = device displacement).  dt is the step size used by
  * the Prony update of material 5 (the reference reads the global dt). */
 int ft_get_force(ft_engine *eng, const double *u, double dt, double *fi_out);
+/* GetForce_3D with the per-Gauss-point fields the reference's loop leaves in
+ * its global arrays (CalculateDeformationGradient, DeterminateF and
+ * StressUpdate inside GetForce_3D.cpp:21-34): F (72 per element: 8 Gauss
+ * points x 9, column-major), det F (8 per element) and the PK2 stress (48 per
+ * element, Voigt), in the caller's element order and the reference's Gauss
+ * point order; any output may be NULL.  Rigid elements (material 0) get F
+ * and det F and a zero stress.  The Prony history advances as in
+ * ft_get_force. */
+int ft_get_force_fields(ft_engine *eng, const double *u, double dt, double *fi_out,
+                        double *F_out, double *detF_out, double *pk2_out);
 /* Per-element forces (24 per element) at u; history is advanced only when
  * update_history != 0.  Exposes InternalForceUpdate.cpp:4-28 summed over the
  * Gauss points of each element. */
@@ -173,6 +183,9 @@ int ft_get_status(ft_engine *eng, int *status);
 /* Prony history of material-5 elements (8 Gauss points x 3 tensors x 6
  * Voigt components per element, element order), as FP64. */
 int ft_get_visco_history(ft_engine *eng, double *hist);
+/* Sets that history (same layout; a host-side StressUpdate of the
+ * compatibility layer hands its updates back this way). */
+int ft_set_visco_history(ft_engine *eng, const double *hist);
 
 /* Element-averaged Green-Lagrange strain (CalculateStrain, CalculateStrain.cpp:
  * 77-97; 9 column-major values per element) and the principal-strain summary
@@ -214,6 +227,49 @@ int ft_energy_terms_noalloc(ft_engine *eng, long ndof, const double *u,
                             const double *fi_prev, const double *fe,
                             const double *fe_prev, const int *boundary,
                             double out3[3]);
+
+/* ---- per-Gauss-point routines ----------------------------------------------
+ * FemTech's element layer evaluates one element and one Gauss point per call
+ * through free functions over global arrays.  ft_point_eval runs n such calls
+ * on the device, one thread each, in FP64: `in` holds n input records and
+ * `out` n output records of the sizes below (doubles), back to back.  Node
+ * data are in C3D8 local order (ShapeFunction_C3D8.cpp:22-29), 3 per node;
+ * 3x3 matrices are column-major (CalculateDeformationGradient.cpp:14).
+ *   FT_PT_SHAPE     ShapeFunction_C3D8.cpp:4-128.  in 27: chi[3], node
+ *                   coordinates[24]; out 33: N[8], dN/dX[24] (node-major),
+ *                   det J.
+ *   FT_PT_DEFGRAD   CalculateDeformationGradient.cpp:11-25.  in 48: x = X + u
+ *                   per node [24], dN/dX[24]; out 9: F.
+ *   FT_PT_DETF      DeterminateF.cpp:38-57.  in 9: F; out 1: det F.
+ *   FT_PT_INVF      InverseF.cpp (3-D branch).  in 10: F, det F; out 9.
+ *   FT_PT_STRESS    StressUpdate.cpp:5-29 and the material routines.  in 20:
+ *                   F[9], material id, props[9] (ReadMaterials order), dt;
+ *                   out 34: PK2[6] (Voigt 11, 22, 33, 23, 13, 12), Hn_1,
+ *                   Hn_2, S0n [3 x 9] (read and updated in place for the
+ *                   Prony laws 5 and 7: the caller fills them before the
+ *                   call), flag.  FT_ERR_MATERIAL for an unknown id.
+ *   FT_PT_BMAT      StrainDisplacementMatrix.cpp:25-77.  in 12: dN_I/dX[3],
+ *                   F[9]; out 18: B_I (6 x 3, column-major).
+ *   FT_PT_FINT      InternalForceUpdate.cpp:4-28.  in 64: dN/dX[24], F[9],
+ *                   PK2[6], w det J, force[24]; out 192: force + w det J
+ *                   B^T S [24], B^T S [24], B [6 x 24].
+ *   FT_PT_CHARLEN   CalculateCharacteristicLength_C3D8.cpp:3-30.  in 24:
+ *                   x = X + u per node; out 1: volume / largest face area.
+ *   FT_PT_TIMESTEP  CalculateTimeStep.cpp:7-21.  in 27: x[24], rho, mu,
+ *                   lambda; out 1: Le / ce.
+ *   FT_PT_INV3      inverse3x3Matrix (math.cpp:8-28).  in 9; out 10:
+ *                   inverse[9], determinant. */
+#define FT_PT_SHAPE 0
+#define FT_PT_DEFGRAD 1
+#define FT_PT_DETF 2
+#define FT_PT_INVF 3
+#define FT_PT_STRESS 4
+#define FT_PT_BMAT 5
+#define FT_PT_FINT 6
+#define FT_PT_CHARLEN 7
+#define FT_PT_TIMESTEP 8
+#define FT_PT_INV3 9
+int ft_point_eval(int op, int n, const double *in, double *out);
 
 /* ---- injury criteria ------------------------------------------------------
  * CalculateInjuryCriterions of FemTech's ex5 (ex5.cpp:1311-1430) with its

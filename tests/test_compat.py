@@ -35,8 +35,18 @@ PATH_FUNCTIONS = [
     "fileLogSingle(logLevel, char const*, ...)",
     "fileLogMatrixRM(logLevel, double const*, int, int, char const*, ...)",
     "initLog(char const*)", "finaliseLog()",
+    # per-element, per-Gauss-point routines (FemTech.h:29-104)
+    "ShapeFunction_C3D8(int, int, double*, double*)", "CalculateTimeStep(int)",
+    "CalculateDeformationGradient(int, int)", "StrainDisplacementMatrix(int, int, int, double*)",
+    "StressUpdate(int, int)", "DeterminateF(int, int)", "InverseF(int, int, double*)",
+    "InternalForceUpdate(int, int, double*)", "StVenantKirchhoff(int, int)",
+    "CompressibleNeoHookean(int, int)", "LinearElastic(int, int)", "HGOIsotropic(int, int)",
+    "HGOIsotropicViscoelastic(int, int)", "inverse3x3Matrix(double*, double*, double*)",
+    "CalculateCharacteristicLength(int)", "CalculateCharacteristicLength_C3D8(int)",
 ]
 GLOBALS = ["coordinates", "connectivity", "displacements", "velocities", "velocities_half",
+           "shp", "dshp", "detJacobian", "gaussWeights", "F", "detF", "invF", "pk2", "gptr", "dsptr",
+           "gpPtr", "fptr", "pk2ptr", "detFptr", "B", "fintGQ", "Hn_1", "Hn_2", "S0n",
            "accelerations", "boundary", "mass", "fi", "fe", "f_net", "fi_prev", "fe_prev",
            "displacements_prev", "accelerations_prev", "Eavg", "stepTime", "nNodes", "nelements",
            "ndim", "nDOF", "nPIDglobal", "materialID", "properties", "pid", "eptr", "energyFile",
@@ -73,7 +83,14 @@ def test_every_compute_routine_goes_through_the_engine():
     for routine, call in [("ShapeFunctions", "ft_engine_create"), ("AssembleLumpedMass", "ft_assemble_lumped_mass"),
                           ("StableTimeStep", "ft_stable_time_step"), ("GetForce_3D", "ft_get_force"),
                           ("CalculateAccelerations", "ft_accelerations"), ("CheckEnergy", "ft_energy_terms"),
-                          ("CalculateStrain", "ft_element_strain")]:
+                          ("CalculateStrain", "ft_element_strain"),
+                          ("ShapeFunction_C3D8", "point(FT_PT_SHAPE"),
+                          ("CalculateDeformationGradient", "point(FT_PT_DEFGRAD"),
+                          ("DeterminateF", "point(FT_PT_DETF"), ("InverseF", "point(FT_PT_INVF"),
+                          ("StrainDisplacementMatrix", "point(FT_PT_BMAT"),
+                          ("InternalForceUpdate", "point(FT_PT_FINT"),
+                          ("CalculateCharacteristicLength_C3D8", "point(FT_PT_CHARLEN"),
+                          ("CalculateTimeStep", "point(FT_PT_TIMESTEP")]:
         m = re.search(r"^(?:void|double) %s\([^)]*\) \{(.*?)^\}" % routine, src, re.M | re.S)
         assert m, routine
         assert call in m.group(1), routine
