@@ -22,6 +22,7 @@ STATUS_NONFINITE = 1
 STATUS_INVERTED = 2
 STATUS_DT_COLLAPSE = 4
 STATUS_DT_FP32 = 16
+STATUS_DT_BOUND = 32
 # error codes (include/femtech_amd.h): FemTech's own, then the engine's
 FT_ERR_MATERIAL = 1
 FT_ERR_INPUT = 3

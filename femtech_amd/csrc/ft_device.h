@@ -715,6 +715,48 @@ __host__ __device__ inline R char_length(const R d[7][3]) {
   return sizeof(R) == 8 ? vol / amax * R(64) : vol * Num<R>::rcp_(amax) * R(64);
 }
 
+// A lower bound of char_length(d) that needs one square root instead of 24:
+// the same volume over an upper bound of the largest face area.  Each face's
+// 2x2 rule sums |v1 x v2| <= |v1| |v2| over its four points, and by
+// Cauchy-Schwarz sum |v1| |v2| <= sqrt(sum |v1|^2 sum |v2|^2), where (the
+// odd terms cancel between the points q = +-1/sqrt(3)) sum |v1|^2 = 4 (|c1|^2
+// + |cD|^2 / 3) and sum |v2|^2 = 4 (|c2|^2 + |cD|^2 / 3); the parallelogram
+// branch, 4 |c1 x c2|, is below the same value.  Within 1 % of the exact
+// length on near-cubic elements (0.992 median, 0.89 worst over the 1M-hex
+// jittered block).
+template <typename R>
+__host__ __device__ inline R char_length_lower(const R d[7][3]) {
+  const R *c1 = d[0], *c2 = d[1], *c3 = d[2], *c4 = d[3], *c5 = d[4], *c6 = d[5], *c7 = d[6];
+  const R *q0 = c3, *q1 = c5, *q2 = c1, *q3 = c6, *q4 = c2, *q5 = c4;
+  const R t123 = triple(q0, q4, q3) + triple(q2, q0, q1) + triple(q1, q3, q5), t4 = triple(q2, q4, q5);
+  const R vol = t123 * R(1.0 / 192) + t4 * R(1.0 / 64);
+  // The two faces of a pair share their vectors up to the sign of one term,
+  // |a +- b|^2 = |a|^2 +- 2 a.b + |b|^2: seven squared norms and nine dot
+  // products serve all six faces (pairs zeta-/+, xi-/+, eta-/+).
+  auto dot = [](const R *a, const R *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  const R n1 = dot(c1, c1), n2 = dot(c2, c2), n3 = dot(c3, c3), n4 = dot(c4, c4), n5 = dot(c5, c5),
+          n6 = dot(c6, c6), n7 = dot(c7, c7);
+  const R pairs[3][9] = {{n3, dot(c3, c7), n1, dot(c1, c5), n5, n2, dot(c2, c6), n6, n7},
+                         {n6, dot(c6, c7), n2, dot(c2, c3), n3, n4, dot(c4, c5), n5, n7},
+                         {n5, dot(c5, c7), n1, dot(c1, c3), n3, n4, dot(c4, c6), n6, n7}};
+  R bmax = R(0);
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const R *p = pairs[g];
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg) {
+      const R s2 = sg ? R(2) : R(-2);
+      const R nd = p[0] + s2 * p[1] + p[8];  // |cD|^2
+      const R ne1 = p[2] + s2 * p[3] + p[4];  // |e1|^2
+      const R ne2 = p[5] + s2 * p[6] + p[7];  // |e2|^2
+      const R b = (ne1 + nd * R(1.0 / 3)) * (ne2 + nd * R(1.0 / 3));
+      if (b > bmax) bmax = b;
+    }
+  }
+  // area <= 4 sqrt(bmax) in the 8x units of char_length (x 64 below)
+  return vol * Num<R>::rsqrt_(bmax) * R(16);
+}
+
 // One piece of the characteristic length of a deformed hex from its
 // normalised coefficients d[m][comp]: face area k (k < 6, faces in the order
 // of CalculateCharacteristicLength_C3D8.cpp:12-13) or the volume (k == 6).
@@ -1078,7 +1120,7 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
                                              bool update_hist, R outv[24], double &my_dt,
                                              int &flags, const R fib[3] = nullptr,
                                              const SurrIn *sin = nullptr, SurrTally *tally = nullptr,
-                                             const GpSink *gps = nullptr) {
+                                             const GpSink *gps = nullptr, double dt_exact_below = 1e300) {
   if (DT && !skip_dt) {
     // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
     R dcf[7][3];
@@ -1086,10 +1128,19 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
     for (int m = 0; m < 7; ++m)
 #pragma unroll
       for (int d = 0; d < 3; ++d) dcf[m][d] = cx[m][d] + cu[m][d];
-    const R le = char_length(dcf);
-    // FP64 keeps the reference's division; FP32 multiplies by 1 / ce
-    const double dte = sizeof(R) == 8 ? (double)le / pp.ce : (double)(le * R(pp.inv_ce));
-    if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
+    // FP32 run loop: an element whose lower bound is above dt_exact_below
+    // (a margin over the last minimum) cannot set the step; it reports the
+    // bound instead of its exact length (the bound is below it).
+    double dlb = 0.0;
+    if (sizeof(R) == 4 && dt_exact_below < 1e300) dlb = (double)(char_length_lower(dcf) * R(pp.inv_ce));
+    if (dlb > dt_exact_below) {
+      if (dlb < my_dt) my_dt = dlb;
+    } else {
+      const R le = char_length(dcf);
+      // FP64 keeps the reference's division; FP32 multiplies by 1 / ce
+      const double dte = sizeof(R) == 8 ? (double)le / pp.ce : (double)(le * R(pp.inv_ce));
+      if (dte < my_dt) my_dt = dte;  // NaN never lowers the minimum
+    }
   }
   const int mat = MAT >= 0 ? MAT : pp.mat;
   if (FORCE && mat == 0 && !GPOUT) {  // rigid: no stress, no internal force

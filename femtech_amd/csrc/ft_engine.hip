@@ -105,10 +105,11 @@ struct StepSlot {
   int fresh;       // 1 if the last nodal launch completed a step
   int failed;      // 1 when it stopped because no stable step was left
   // FP32 run loop: best unrefined block minimum of the step (dt_candidate),
-  // and 1 once k_motion_states has merged it into dtmin_bits
+  // 1 once k_motion_states has merged it into dtmin_bits, and the FP32 bits
+  // of the smallest block minimum beyond the skip margin (far_bits)
   unsigned long long cand_bits;
   int settled;
-  int pad;
+  unsigned far_bits;
 };
 
 struct Ctl {
@@ -135,6 +136,9 @@ __device__ inline unsigned long long dt_key(double d) {
 // element tile's index, so the unsigned minimum gives the smallest step and
 // the first tile on ties.
 constexpr unsigned long long kNoCandidate = ~0ull;
+// FP32 run loop: elements whose time-step lower bound (char_length_lower)
+// exceeds the last minimum by this factor skip the exact geometry.
+constexpr double kSkipMargin = 1.10;
 __device__ inline unsigned long long dt_candidate(double d, int tile) {
   return ((unsigned long long)__float_as_uint((float)d) << 32) | (unsigned)tile;
 }
@@ -473,8 +477,11 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
                                                               a.update_hist != 0, outv, my_dt, flags, fib,
                                                               nullptr, nullptr, &gs);
     } else {
+      // FP32 run loop: exact lengths only within kSkipMargin of the last minimum
+      const double exact_below = sizeof(R) == 4 && a.refine ? dt_guide * kSkipMargin : 1e300;
       element_eval<R, FORCE, DT, MAT>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
-                                 a.update_hist != 0, outv, my_dt, flags, fib);
+                                 a.update_hist != 0, outv, my_dt, flags, fib, nullptr, nullptr, nullptr,
+                                 exact_below);
     }
     if (FORCE) {
       // Slot-major layouts: fe[slot][element][xyz] or fe[corner][node][xyz].
@@ -504,7 +511,13 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
     // window in one step).
     const bool exact = sizeof(R) == 4 && a.refine && a.ctl;
     const bool refine_blk = sizeof(R) == 4 && a.refine && bmin < 1e300 && bmin <= dt_guide * 1.001;
-    if (refine_blk || (exact && bmin < 1e300)) {
+    // A block whose minimum is beyond the skip margin holds no element that
+    // can set the step unless the minimum rises past the margin in one step;
+    // it posts its value only, for the conservative rule of merge_candidate.
+    const bool far = exact && bmin < 1e300 && bmin > 0.0 && bmin > dt_guide * kSkipMargin;
+    if (far) {
+      if (threadIdx.x == 0) atomicMin(&a.ctl[tile.z].slot[a.parity].far_bits, __float_as_uint((float)bmin));
+    } else if (refine_blk || (exact && bmin < 1e300)) {
       __shared__ int sh_arg;
       __shared__ double sh_c[7][3];
       if (threadIdx.x == 0) sh_arg = INT32_MAX;
@@ -837,9 +850,26 @@ __device__ inline double candidate_dt(const double *cg) {
   return hex_part(c, 6) / amax / cg[21];
 }
 
+// The step's minimum with the best unrefined candidate's FP64 step dte.
 __device__ inline double merge_candidate(double dtmin, double dte) {
   if (dte <= 0.0) return 0.0;  // inverted in FP64: the step collapses
   return dte < dtmin ? dte : dtmin;
+}
+
+// If the step's minimum d rose past the skip margin over the last minimum,
+// elements that reported only a lower bound (their own, or their block's
+// beyond the margin) could undercut it: the smallest such value, a lower
+// bound of every one of their steps, is taken instead (conservative; *bounded
+// set).
+__device__ inline double bound_rule(const StepSlot &cur, double d, bool *bounded) {
+  if (!(d > cur.dt_guide * kSkipMargin)) return d;
+  double cb = cur.cand_bits != kNoCandidate ? (double)__uint_as_float((unsigned)(cur.cand_bits >> 32)) : 1e300;
+  if (cur.far_bits < 0x7f800000u) cb = fmin(cb, (double)__uint_as_float(cur.far_bits));
+  if (cb < d) {
+    *bounded = true;
+    return cb;
+  }
+  return d;
 }
 
 // End of step k (gather, a = f/m, v = v_half + dt/2 a, energy) fused with the
@@ -869,16 +899,21 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   const int n = tile.x + threadIdx.x;
   const double t_k = cur.t, t_km1 = cur.t_prev, dt_k = cur.dt;
   double dtmin = bits_to_double(cur.dtmin_bits);
-  if (sizeof(R) == 4 && a.cand_geom && !cur.settled && candidate_pending(cur, dtmin)) {
-    // FP32 engine: the step's best unrefined element could undercut the
-    // refined minimum (it moved up by more than the element kernel's window):
-    // its FP64 step from the geometry its tile left (uniform over the block;
-    // every block of the instance computes the same value), unless
-    // k_motion_states has settled it already
-    __shared__ double sh_c[7][3];
-    const double *cg = a.cand_geom + (size_t)kCandStride * (unsigned)(cur.cand_bits & 0xffffffffu);
-    if (threadIdx.x < 21) (&sh_c[0][0])[threadIdx.x] = cg[threadIdx.x];
-    dtmin = merge_candidate(dtmin, refined_dt(sh_c, cg[21]));
+  if (sizeof(R) == 4 && a.cand_geom && !cur.settled) {
+    // FP32 engine, unless k_motion_states has settled the step already: the
+    // step's best unrefined element could undercut the refined minimum (it
+    // moved up by more than the element kernel's window): its FP64 step from
+    // the geometry its tile left (uniform over the block; every block of the
+    // instance computes the same value)
+    if (candidate_pending(cur, dtmin)) {
+      __shared__ double sh_c[7][3];
+      const double *cg = a.cand_geom + (size_t)kCandStride * (unsigned)(cur.cand_bits & 0xffffffffu);
+      if (threadIdx.x < 21) (&sh_c[0][0])[threadIdx.x] = cg[threadIdx.x];
+      dtmin = merge_candidate(dtmin, refined_dt(sh_c, cg[21]));
+    }
+    bool bounded = false;
+    dtmin = bound_rule(cur, dtmin, &bounded);
+    if (bounded && (tile.w & kTileFirst) && threadIdx.x == 0) atomicOr(a.status, FT_STATUS_DT_BOUND);
   }
   const double dt_next = a.factor * dtmin;
   // no stable step left: below the failure step (an inverted element's is
@@ -1021,6 +1056,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
       nx.dtmin_bits = kInfBits;
       nx.cand_bits = kNoCandidate;
       nx.settled = 0;
+      nx.far_bits = ~0u;
       nx.step = cur.step + 1;
       nx.done = 0;
       nx.failed = 0;
@@ -1059,18 +1095,23 @@ __global__ __launch_bounds__(kBlock, ROT ? 6 : 7) void k_nodal(NodalArgs<R> a) {
 // instance's first motion writes it back to the slot and marks the slot
 // settled, so the nodal kernel takes it as final.
 __global__ void k_motion_states(Ctl *ctl, int parity, double factor, const double *motions,
-                                const int *motion_inst, int nm, MotionT *out, const double *cand_geom) {
+                                const int *motion_inst, int nm, MotionT *out, const double *cand_geom,
+                                int *status) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nm; i += gridDim.x * blockDim.x) {
     const int w = i / nm, k = i % nm;
     const int inst = motion_inst[k];
     const StepSlot cur = ctl[inst].slot[parity];
     double dtmin = bits_to_double(cur.dtmin_bits);
-    if (cand_geom && candidate_pending(cur, dtmin)) {
-      dtmin = merge_candidate(dtmin, candidate_dt(cand_geom + (size_t)kCandStride *
-                                                                (unsigned)(cur.cand_bits & 0xffffffffu)));
+    if (cand_geom) {
+      if (candidate_pending(cur, dtmin))
+        dtmin = merge_candidate(
+            dtmin, candidate_dt(cand_geom + (size_t)kCandStride * (unsigned)(cur.cand_bits & 0xffffffffu)));
+      bool bounded = false;
+      dtmin = bound_rule(cur, dtmin, &bounded);
       if (w == 0 && (k == 0 || motion_inst[k - 1] != inst)) {
         ctl[inst].slot[parity].dtmin_bits = dt_key(dtmin);
         ctl[inst].slot[parity].settled = 1;  // read only by the nodal kernel
+        if (bounded) atomicOr(status, FT_STATUS_DT_BOUND);
       }
     }
     motion_time(motions + FT_MOTION_STRIDE * k, motion_state_time(cur, factor, w, dtmin), out[i]);
@@ -2364,7 +2405,7 @@ int launch_nodal(ft_engine *g, int mode) {
     const int nb = std::max(1, std::min(64, (3 * g->n_motions + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_motion_states, dim3(nb), dim3(kBlock), 0, g->lstream, g->ctl, g->parity, g->factor,
                        g->motions, g->motion_inst, g->n_motions, g->mstate,
-                       g->precision == FT_FP32 ? (const double *)g->cand_geom : nullptr);
+                       g->precision == FT_FP32 ? (const double *)g->cand_geom : nullptr, g->status);
     HIPCHK(hipGetLastError());
   }
   const int nt = g->n_ntiles;
@@ -3168,6 +3209,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
     for (StepSlot &sl : c[i].slot) {
       sl.cand_bits = kNoCandidate;
       sl.settled = 0;
+      sl.far_bits = ~0u;
     }
   }
   HIPCHK(hipMemcpy(g->ctl, c.data(), c.size() * sizeof(Ctl), hipMemcpyHostToDevice));

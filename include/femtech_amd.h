@@ -51,6 +51,12 @@ extern "C" {
 /* Kept for compatibility, never raised: in the FP32 engine's run loop the
  * step size always comes from FP64 geometry of the minimising element. */
 #define FT_STATUS_DT_FP32 16
+/* Informational (FP32 engine): in some step the minimum element time step
+ * rose by more than 5 % in one step, so elements that had skipped their
+ * exact geometry (their lower bound was 5 % above the last minimum) could
+ * set it; that step used the best lower bound (smaller than the exact
+ * minimum, so the step stays stable). */
+#define FT_STATUS_DT_BOUND 32
 
 typedef struct ft_engine ft_engine;
 
