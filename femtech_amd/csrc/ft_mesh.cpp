@@ -85,15 +85,23 @@ int read_inp(FILE *f, RawMesh &m) {
       sec = NONE;
       auto toks = split_tokens(hdr.c_str(), ", \t=");
       if (toks.empty()) continue;
-      if (toks[0] == "*NODE") {
+      // The keyword is the whole word before the first comma: "*Node" and
+      // "*Element, type=..." open sections, "*Node Output" and "*Element
+      // Output" do not (ReadAbaqus.cpp:359-412).
+      const std::string kw = toks[0];
+      const size_t after = hdr.find_first_not_of(" \t", kw.size());
+      const bool bare = after == std::string::npos || hdr[after] == '\n' || hdr[after] == '\r';
+      const bool comma = after != std::string::npos && hdr[after] == ',';
+      if (kw == "*NODE" && (bare || comma)) {
         // only the first *Node section is read (ReadAbaqus.cpp:41-49)
         if (!node_done) sec = NODE;
-      } else if (toks[0] == "*ELEMENT") {
+      } else if (kw == "*ELEMENT" && comma) {
         std::string type, elset;
         for (size_t i = 1; i + 1 < toks.size(); ++i) {
           if (toks[i] == "TYPE") type = toks[i + 1];
           if (toks[i] == "ELSET") elset = toks[i + 1];
         }
+        if (type.empty()) continue;  // not an element section without type=
         if (type != "C3D8")
           return mfail(FT_ERR_INPUT, "element type %s not supported (C3D8 only)", type.c_str());
         // pid = order of first appearance of the ELSET name (ReadAbaqus.cpp:172-179)

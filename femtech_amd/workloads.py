@@ -166,3 +166,44 @@ def batch_instances(count=512, ne_range=(2000, 80000), t_end=0.020, seed=BATCH_S
         inst.append(w)
     merged, node_off, elem_off = concatenate(inst)
     return merged, node_off, elem_off, np.full(count, t_end), info
+
+
+# Reference-cube coordinates of the C3D8 local nodes (ShapeFunction_C3D8.cpp:22-29).
+C3D8_SIGNS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                       [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], np.int64)
+
+
+def cube_rotations():
+    """The 24 proper rotations of the reference cube as node renumberings:
+    row r maps new local node a to old local node perm[r, a], the node that
+    sits at R s_a (s_a: reference coordinates of node a).  The renumbered
+    element x'(xi) = x(R xi) is the same hexahedron with det J of the same
+    sign, and the 2x2x2 Gauss points map onto each other."""
+    import itertools
+    out = []
+    for p in itertools.permutations(range(3)):
+        for sg in itertools.product((-1, 1), repeat=3):
+            R = np.zeros((3, 3), np.int64)
+            for i in range(3):
+                R[i, p[i]] = sg[i]
+            if round(np.linalg.det(R)) != 1:
+                continue
+            img = C3D8_SIGNS @ R.T
+            perm = [int(np.nonzero(np.all(C3D8_SIGNS == img[a], axis=1))[0][0]) for a in range(8)]
+            out.append(perm)
+    out = np.array(out, np.int64)
+    assert out.shape == (24, 8) and len({tuple(r) for r in out}) == 24
+    return out
+
+
+def permute_elements(conn, seed):
+    """Each element's local node order renumbered by a random proper cube
+    rotation (the identity excluded), so that nodes are the same local corner
+    of several elements and the engine's general force gather is used."""
+    rots = cube_rotations()
+    ident = np.nonzero(np.all(rots == np.arange(8), axis=1))[0][0]
+    choices = np.delete(np.arange(24), ident)
+    rng = np.random.default_rng(seed)
+    c = np.asarray(conn, np.int64).reshape(-1, 8)
+    pick = rots[rng.choice(choices, size=c.shape[0])]
+    return np.take_along_axis(c, pick, axis=1).reshape(-1).astype(np.int32)

@@ -13,6 +13,8 @@
  *   stress <dir>  StressUpdate (StressUpdate.cpp:5-29) on given F sequences.
  *   geom <dir>    volumeHexahedron / areaHexahedronFace /
  *                 CalculateCharacteristicLength_C3D8 on given element corners.
+ *   time <dir>    the run's loop timed alone, no output, on 1..N MPI ranks
+ *                 (CPU-baseline calibration; <dir>/coords.bin added).
  *
  * Every FemTech routine it calls is the reference's own, compiled unmodified
  * (InitFemTechWoInput, ReadInputFile, ReadMaterials, PartitionMesh with the
@@ -28,6 +30,8 @@
 #include <string.h>
 #include <unistd.h>
 
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -252,6 +256,86 @@ int mode_run() {
   return 0;
 }
 
+/* Wall time of the ex9 loop alone (ex9.cpp:107-175), with no fixture output,
+ * on any number of MPI ranks: the boundary motions of bc.bin are assigned to
+ * the rank's nodes by their coordinates (coords.bin: the global mesh's nodes
+ * in bc.bin's order, as read from the same mesh file).  Rank 0 prints the
+ * step count and the seconds between barriers around the loop. */
+int mode_time() {
+  ReadInputFile(access("mesh.k", F_OK) == 0 ? "mesh.k" : "mesh.inp");
+  ReadMaterials();
+  PartitionMesh();
+  AllocateArrays();
+  Time = 0.0;
+  std::vector<int> bc = read_bin<int>("bc.bin");
+  std::vector<double> gx = read_bin<double>("coords.bin");
+  const size_t ng = gx.size() / 3;
+  Motions mot;
+  mot.table = read_bin<double>("motions.bin");
+  if ((int)mot.table.size() != 16 * bc[3 * ng]) return 3;
+  std::map<std::array<double, 3>, size_t> where;
+  for (size_t n = 0; n < ng; ++n) where[{gx[3 * n], gx[3 * n + 1], gx[3 * n + 2]}] = n;
+  mot.dof_motion.assign(3 * (size_t)nNodes, -1);
+  for (int n = 0; n < nNodes; ++n) {
+    auto it = where.find({coordinates[3 * n], coordinates[3 * n + 1], coordinates[3 * n + 2]});
+    if (it == where.end()) return 5;
+    for (int d = 0; d < 3; ++d) mot.dof_motion[3 * n + d] = bc[3 * it->second + d];
+  }
+  int max_steps = 0, nsnap = 0;
+  double tmax = 0, factor = 0;
+  {
+    FILE *f = fopen("params.txt", "r");
+    if (fscanf(f, "%d %lf %lf %lf %d", &max_steps, &tmax, &factor, &FailureTimeStep,
+               &nsnap) != 5)
+      return 4;
+    fclose(f);
+  }
+  ExplicitTimeStepReduction = factor;
+  ShapeFunctions();
+  AssembleLumpedMass();
+  apply_bc(mot);
+  dt = ExplicitTimeStepReduction * StableTimeStep();
+  GetForce();
+  CalculateAccelerations();
+  MPI_Barrier(MPI_COMM_WORLD);
+  const double t0 = MPI_Wtime();
+  int step = 0;
+  while (Time < tmax && step < max_steps) {
+    double t_n = Time;
+    double t_np1 = Time + dt;
+    Time = t_np1;
+    double dt_nphalf = dt;
+    double t_nphalf = 0.5 * (t_np1 + t_n);
+    for (int i = 0; i < nDOF; i++) {
+      if (boundary[i])
+        velocities_half[i] = velocities[i];
+      else
+        velocities_half[i] = velocities[i] + (t_nphalf - t_n) * accelerations[i];
+    }
+    memcpy(displacements_prev, displacements, nDOF * sizeof(double));
+    memcpy(accelerations_prev, accelerations, nDOF * sizeof(double));
+    memcpy(fi_prev, fi, nDOF * sizeof(double));
+    memcpy(fe_prev, fe, nDOF * sizeof(double));
+    for (int i = 0; i < nDOF; i++)
+      if (!boundary[i]) displacements[i] = displacements[i] + dt_nphalf * velocities_half[i];
+    apply_bc(mot);
+    GetForce();
+    CalculateAccelerations();
+    for (int i = 0; i < nDOF; i++)
+      if (!boundary[i])
+        velocities[i] = velocities_half[i] + (t_np1 - t_nphalf) * accelerations[i];
+    CheckEnergy(Time, 0);
+    ++step;
+    dt = ExplicitTimeStepReduction * StableTimeStep();
+  }
+  MPI_Barrier(MPI_COMM_WORLD);
+  const double t1 = MPI_Wtime();
+  if (world_rank == 0)
+    printf("steps %d time %.17g loop_seconds %.6f ranks %d elements_rank0 %d\n", step, Time,
+           t1 - t0, world_size, nelements);
+  return 0;
+}
+
 /* Sets up n one-Gauss-point "elements" so the reference's material routines
  * can be called on given deformation gradients. */
 int mode_stress() {
@@ -334,7 +418,7 @@ int mode_geom() {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: %s run|stress|geom <dir>\n", argv[0]);
+    fprintf(stderr, "usage: %s run|time|stress|geom <dir>\n", argv[0]);
     return 2;
   }
   const std::string mode = argv[1];
@@ -342,8 +426,8 @@ int main(int argc, char **argv) {
   if (chdir(argv[2]) != 0) return 2;
   InitFemTechWoInput(argc, argv);
   int rc = 2;
-  if (mode == "run") {
-    rc = mode_run();
+  if (mode == "run" || mode == "time") {
+    rc = mode == "run" ? mode_run() : mode_time();
     FinalizeFemTech();  // frees the arrays and closes energy_<uid>.dat
     return rc;
   }

@@ -164,6 +164,27 @@ def test_reader_renumbers_like_partition_mesh(tmp_path):
     assert np.allclose(X.reshape(-1, 3)[used.index(40)], coords[40])
 
 
+def test_reader_skips_output_requests_and_assembly_nodes(tmp_path):
+    """Abaqus/CAE decks (the 27-element benchmark deck is one) end with
+    assembly reference nodes and "*Node Output" / "*Element Output"
+    requests: only "*Node" / "*Node," and "*Element, type=..." open sections,
+    and only the first node section is read (ReadAbaqus.cpp:41-49, 359-412)."""
+    X, conn = ft.mesh_block(2, 1, 1, 1e-3)
+    nn = X.size // 3
+    Xr = X.reshape(-1, 3)
+    lines = ["*Heading", "*Part, name=P", "*Node"]
+    lines += ["%d, %r, %r, %r" % ((i + 1,) + tuple(float(x) for x in Xr[i])) for i in range(nn)]
+    lines += ["*Element, type=C3D8"]
+    lines += [", ".join(str(v) for v in [e + 1] + list(conn.reshape(-1, 8)[e] + 1)) for e in range(2)]
+    lines += ["*End Part", "*Assembly, name=A", "*Node", "      1, 0., 0.5, 0.25", "*Nset, nset=x",
+              " 1, 2", "*End Assembly", "*Step, name=S", "*Element Output, directions=YES",
+              "LE, S", "*Node Output", "RF, U", "*End Step"]
+    p = tmp_path / "cae.inp"
+    p.write_text("\n".join(lines) + "\n")
+    X2, conn2, pid2, gid = ft.read_mesh(str(p))
+    assert np.array_equal(X2, X) and np.array_equal(conn2, conn) and list(pid2) == [0, 0]
+
+
 def test_reader_matches_reference_on_its_meshes():
     for name in ("ex9", "bench10"):
         z = golden("run_" + name)
