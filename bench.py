@@ -27,6 +27,11 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
+def note(msg):
+    """Progress of the phases (stderr; the JSON line stays alone on stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def workload(kind, n, count=512):
     """Engine input of the workload; a batch also carries node_off, elem_off,
     end_times and info (per-instance kind, n, elements, peak, duration)."""
@@ -88,10 +93,27 @@ def fp64_accuracy(w, steps, u32, energy32, principal32):
             "fp64_run_s": seconds}
 
 
+def host_cpu():
+    """(logical CPUs, CPU model) of this host, from nproc and lscpu."""
+    import subprocess
+    model = None
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=30).stdout)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = os.cpu_count()
+    return nproc, model
+
+
 def cpu_baseline(w, steps, threads):
     """The oracle (FP64 C++ restatement, OpenMP over elements with the same
     deterministic gather) on the host cores, on a bounded sample of the same
-    workload: `steps` explicit steps of the full 1M-hex mesh."""
+    workload: `steps` explicit steps of the full mesh."""
     from oracle import oracle as O
     o = O.Model(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], threads=threads)
     if w.get("fibres") is not None:
@@ -106,7 +128,9 @@ def cpu_baseline(w, steps, threads):
     loop = (t2 - t1) - (t1 - t0)
     ne = w["conn"].size // 8
     o.close()
+    nproc, model = host_cpu()
     return dict(value=ne * steps / loop, unit="element-steps/s", cores=threads, kind="port",
+                nproc=nproc, cpu_model=model,
                 sample="%d explicit step%s of the same %d-element mesh (FP64 oracle, %d OpenMP thread%s), "
                        "loop time %.2f s" % (steps, "" if steps == 1 else "s", ne, threads,
                                              "" if threads == 1 else "s", loop))
@@ -193,7 +217,10 @@ def surrogate_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4900)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: impact 4900; ellipsoid 19900, which with the warmup ends "
+                         "at 4.0 ms, inside the 4.79 ms the model survives; batch: until every instance "
+                         "reached its end time or stopped)")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", choices=["impact", "ellipsoid", "batch"], default="impact",
                     help="impact: 1M-hex neo-Hookean block (the headline); ellipsoid: 2.0M-hex "
@@ -216,8 +243,15 @@ def main():
                     help="N > 0: HIP events around the launches of every N-th step of the timed region "
                          "(per-kernel averages; launches are not captured in graphs meanwhile); 0: none")
     ap.add_argument("--n", type=int, default=0, help="elements per edge (default 100 / 126)")
+    ap.add_argument("--permute", type=int, default=None, metavar="SEED",
+                    help="impact / ellipsoid: renumber every element's local nodes by a seeded random "
+                         "cube rotation (same mesh and physics), so that no node is the same local "
+                         "corner of one element only and the engine takes its general force gather")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--layout", choices=["auto", "corner", "slots", "ell"], default="auto",
+                    help="force-gather layout (auto: corner slots, else node slots, else the ELL)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the multi-threaded CPU sample (default: 50 impact, 10 ellipsoid)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="single-mesh FP32 runs: skip the FP64 run that states the FP32 result's error")
@@ -232,17 +266,32 @@ def main():
         sys.exit(2)
 
     import femtech_amd as ft
+    batch = args.workload == "batch"
+    if args.steps is None:
+        args.steps = {"impact": 4900, "ellipsoid": 19900, "batch": 0}[args.workload]
+        if args.surrogate:
+            args.steps = 4900
+    if args.cpu_steps is None:
+        args.cpu_steps = 50 if args.workload == "impact" else 10
     if args.surrogate:
         if args.workload != "ellipsoid":
             print("bench.py: --surrogate applies to the ellipsoid workload", file=sys.stderr)
             sys.exit(2)
         surrogate_bench(args)
         return
+    if batch and args.permute is not None:
+        print("bench.py: --permute applies to the impact and ellipsoid workloads", file=sys.stderr)
+        sys.exit(2)
     w = workload(args.workload, args.n, args.count)
+    note("workload built: %d elements" % (w["conn"].size // 8))
+    if args.permute is not None:
+        from femtech_amd.workloads import permute_elements
+        w["conn"] = permute_elements(w["conn"], args.permute)
     ne, nn = w["conn"].size // 8, w["X"].size // 3
     prec = ft.FT_FP32 if args.precision == "fp32" else ft.FT_FP64
     e = ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], prec)
-    batch = args.workload == "batch"
+    if args.layout != "auto":
+        e.set_layout({"corner": ft.LAYOUT_CORNER, "slots": ft.LAYOUT_NODE_SLOTS, "ell": ft.LAYOUT_ELL}[args.layout])
     if batch:
         e.set_instances(w["node_off"], w["elem_off"])
         e.set_instance_end_times(w["end_times"])
@@ -255,6 +304,7 @@ def main():
     tune = e.autotune(args.autotune) if args.autotune > 0 else None
     e.set_graph(args.graph)
     e.set_persistent(args.persistent)
+    note("set-up done")
     if args.warmup:
         e.run_steps(args.warmup)
     n_inst = len(w["info"]) if batch else 1
@@ -263,20 +313,36 @@ def main():
     e.set_timing(args.kernel_timing)
     e.time()  # synchronises the stream
     t0 = time.perf_counter()
-    done = e.run_steps(args.steps)
+    if batch and args.steps == 0:
+        # the whole window: chunks of steps until no instance is running
+        # (each instance stops at its end time or when it loses its stable
+        # step; the state query between chunks synchronises)
+        done = 0
+        while True:
+            done += e.run_steps(2000)
+            note("batch: %d steps" % done)
+            if not any(e.instance(i)[3] == 0 for i in range(n_inst)) or done >= 1_000_000:
+                break
+        args.steps = done
+    else:
+        done = e.run_steps(args.steps)
     t_sim, _, k = e.time()  # synchronises
     t1 = time.perf_counter()
+    note("timed loop done: %d steps in %.2f s" % (args.steps, t1 - t0))
     ms, launches = e.timing()
     e.set_timing(0)
     E = e.energy()
     status = e.status()
     final_u = e.state()[0] if args.workload != "batch" and prec == ft.FT_FP32 else None
     final_p = e.element_strain()[1] if final_u is not None else None
+    layout = e.layout()
     elapsed = t1 - t0
     if batch:
         # every instance advances by its own dt; instances that reached their
-        # end time stop, so count the element-steps actually taken
-        steps1 = np.array([e.instance(i)[2] for i in range(n_inst)])
+        # end time or stopped take no more steps, so count the element-steps
+        # actually taken
+        rows = np.array([e.instance(i) for i in range(n_inst)])
+        steps1 = rows[:, 2]
         elem_steps = float(np.sum(ne_inst * (steps1 - steps0)))
         running = int(np.sum(steps1 - steps0 == args.steps))
     else:
@@ -304,24 +370,49 @@ def main():
     # v read/write 48 + mass 8 + BC flag 1 = 145 B, plus fi_prev read/write 48
     # for the FP64 energy balance.
     fixed_bytes = 260.0 * ne + (145.0 + 48.0) * nn
+    gather = {"layout": {1: "corner slots", 2: "node slots", 3: "ELL"}.get(layout[0], str(layout[0])),
+              "ell_width": int(layout[1]),
+              "element_node_order": "permuted by seeded cube rotations (seed %d)" % args.permute
+              if args.permute is not None else "as generated"}
     if args.workload == "impact":
         config = {"workload": "1M-hex jittered neo-Hookean impact block (plain method)",
                   "elements": ne, "nodes": nn, "edge_elements": args.n or 100,
-                  "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k}
+                  "jitter": 0.15, "seed": SEED, "simulated_time_s": t_sim, "step_index": k, "gather": gather}
     elif args.workload == "ellipsoid":
         config = {"workload": "2.0M-hex ellipsoid 90x75x65 mm: linear-elastic shell, Ogden + Prony, "
                               "anisotropic HGO core; half-sine rotational surface loading (plain method)",
                   "elements": ne, "nodes": nn, "edge_elements": args.n or 126,
-                  "simulated_time_s": t_sim, "step_index": k}
+                  "simulated_time_s": t_sim, "step_index": k, "gather": gather,
+                  "window": "steps 0-%d of the 30 ms load case; the model inverts elements of the soft "
+                            "Ogden + Prony layer after 4.79 ms (FP32 engine, step 24,080, 2.0M hexes), and "
+                            "the FP64 oracle inverts the 64k-hex ellipsoid at step 9,853 where the FP64 "
+                            "engine stops at 9,831: the constitutive model, not the engine, ends the run"
+                            % (args.warmup + args.steps)}
     else:
-        kinds = [i[0] for i in w["info"]]
+        kinds = np.array([i[0] for i in w["info"]])
+        state, t_end = rows[:, 3].astype(int), rows[:, 0]
+        per_kind = {}
+        for kind in ("block", "ellipsoid"):
+            m = kinds == kind
+            if m.any():
+                per_kind[kind] = {"instances": int(m.sum()), "finished": int(np.sum(state[m] == 1)),
+                                  "stopped": int(np.sum(state[m] == 2)), "running": int(np.sum(state[m] == 0)),
+                                  "simulated_time_s_min": float(t_end[m].min()),
+                                  "simulated_time_s_median": float(np.median(t_end[m])),
+                                  "simulated_time_s_max": float(t_end[m].max())}
         config = {"workload": "%d independent impact simulations in one run (jittered neo-Hookean blocks and "
                               "three-region ellipsoids, 2k-80k elements each, moduli +-30%%, half-sine "
                               "rotational pulses 2-8 krad/s^2 over 5-20 ms, 20 ms each, own dt each)" % n_inst,
-                  "instances": n_inst, "blocks": kinds.count("block"), "ellipsoids": kinds.count("ellipsoid"),
+                  "instances": n_inst, "blocks": int(np.sum(kinds == "block")),
+                  "ellipsoids": int(np.sum(kinds == "ellipsoid")),
                   "elements": ne, "nodes": nn, "min_instance_elements": int(ne_inst.min()),
                   "max_instance_elements": int(ne_inst.max()), "running_whole_window": running,
-                  "element_steps": elem_steps}
+                  "element_steps": elem_steps, "finished": int(np.sum(state == 1)),
+                  "stopped": int(np.sum(state == 2)), "still_running": int(np.sum(state == 0)),
+                  "per_kind": per_kind,
+                  "stop_cause": "an instance stops when its stable step collapses or its forces turn "
+                                "non-finite (element inversion); the ellipsoids' soft Ogden + Prony layer "
+                                "inverts under the rotational pulse in the FP64 oracle as well"}
     out = {
         "metric": "element-steps/s on 1M-hex neo-Hookean; surrogate vs plain at <=1% stress err",
         "value": value,
@@ -361,13 +452,16 @@ def main():
     if final_u is not None and not args.no_accuracy:
         e.close()
         out["accuracy_vs_fp64"] = fp64_accuracy(w, args.warmup + args.steps, final_u, E, final_p)
+        note("FP64 comparison run done")
     if args.copy_probe > 0:
         out["copy_probe_gbs"] = ft.copy_probe(args.copy_probe, 5)
     if not args.no_cpu and not batch:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, threads)
-        # the same restatement on one core (one step: a few seconds)
-        out["cpu_baseline_1core"] = cpu_baseline(w, 1, 1)
+        note("CPU baseline (%d threads) done" % threads)
+        # the same restatement on one core (a fifth of the steps: about 10 s
+        # on the 1M-hex block)
+        out["cpu_baseline_1core"] = cpu_baseline(w, max(1, args.cpu_steps // 5), 1)
     print(json.dumps(out), flush=True)
 
 

@@ -23,6 +23,8 @@ STATUS_INVERTED = 2
 STATUS_DT_COLLAPSE = 4
 STATUS_DT_FP32 = 16
 STATUS_DT_BOUND = 32
+# force-gather layouts (ft_engine_layout / ft_engine_set_layout)
+LAYOUT_AUTO, LAYOUT_CORNER, LAYOUT_NODE_SLOTS, LAYOUT_ELL = 0, 1, 2, 3
 # error codes (include/femtech_amd.h): FemTech's own, then the engine's
 FT_ERR_MATERIAL = 1
 FT_ERR_INPUT = 3
@@ -55,6 +57,7 @@ _SIGS = {
     "ft_engine_destroy": [_VP],
     "ft_engine_info": [_VP, _I, _I, _I],
     "ft_engine_layout": [_VP, _I, _I],
+    "ft_engine_set_layout": [_VP, ctypes.c_int],
     "ft_set_boundary": [_VP, _I, ctypes.c_int, _D],
     "ft_set_state": [_VP, _D, _D],
     "ft_reset_history": [_VP],
@@ -291,11 +294,18 @@ class Engine:
         self.close()
 
     def layout(self):
-        """(corner, ell_width): 1 when element forces go to per-node corner
-        slots, 0 when they are gathered through the node-to-element table."""
+        """(layout, ell_width): LAYOUT_CORNER (per-node corner slots),
+        LAYOUT_NODE_SLOTS (per-node slots by element rank) or LAYOUT_ELL
+        (element-major forces gathered through the node-to-element table),
+        and that table's width."""
         c, w = ctypes.c_int(0), ctypes.c_int(0)
         _check(self._lib.ft_engine_layout(self._h, ctypes.byref(c), ctypes.byref(w)))
         return c.value, w.value
+
+    def set_layout(self, layout):
+        """Choose the force-gather layout before run_init (LAYOUT_AUTO picks
+        the first of corner slots, node slots, ELL that applies)."""
+        _check(self._lib.ft_engine_set_layout(self._h, int(layout)))
 
     # --- set-up -----------------------------------------------------------
     def set_boundary(self, dof_motion, motions):

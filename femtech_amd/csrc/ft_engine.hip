@@ -315,7 +315,8 @@ struct ElemArgs {
   const double *u;       // component-major [3][nn]
   const PartDev *parts;
   R *fe;                 // element forces (nullable when only dt is wanted)
-  int corner;            // fe[corner][node][xyz] instead of fe[slot][element][xyz]
+  int corner;            // fe[slot][node][xyz] instead of fe[slot][element][xyz]
+  const unsigned *eslot; // node-major layout: slot of local node k in bits 4k..4k+2 (nullptr: slot k)
   R *hist;               // 144 per element, in 64-element blocks (hist_index)
   const R *fib;          // fibre directions [3][ne] (material 8)
   Ctl *ctl;              // run loop: per-instance control, slot `parity` gives dt_k and done
@@ -488,13 +489,25 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       // Each of the 8 stores of a wavefront covers one contiguous run (for
       // the corner layout: neighbouring elements of a block have
       // neighbouring corner-k nodes), and so does each load of the node sum.
+      if (a.eslot) {
+        // node slots: the slot varies per lane (the element's rank at node k)
+        const unsigned sl = a.eslot[e];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);  // uniform
-        const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
-        at_byte(dst, o) = outv[3 * k];
-        at_byte(dst, o + (unsigned)sizeof(R)) = outv[3 * k + 1];
-        at_byte(dst, o + 2u * (unsigned)sizeof(R)) = outv[3 * k + 2];
+        for (int k = 0; k < 8; ++k) {
+          R *dst = a.fe + 3 * ((size_t)((sl >> (4 * k)) & 7u) * a.nn + (size_t)nd[k]);
+          dst[0] = outv[3 * k];
+          dst[1] = outv[3 * k + 1];
+          dst[2] = outv[3 * k + 2];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);  // uniform
+          const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
+          at_byte(dst, o) = outv[3 * k];
+          at_byte(dst, o + (unsigned)sizeof(R)) = outv[3 * k + 1];
+          at_byte(dst, o + 2u * (unsigned)sizeof(R)) = outv[3 * k + 2];
+        }
       }
     }
   }
@@ -1910,11 +1923,17 @@ struct ft_engine {
   double *X = nullptr;
   int *conn = nullptr, *einfo = nullptr, *ell = nullptr;
   int width = 8;  // ELL width: max node valence, at least 8
-  // Corner-slot force layout: when no two elements share a node at the same
-  // local corner (every structured block), element forces go straight to
-  // fe[corner][node][xyz] and the node sum reads 8 contiguous slots with no
-  // incidence list.  Otherwise fe[slot][element][xyz] + the ELL.
+  // Node-major force layouts: element forces go straight to
+  // fe[slot][node][xyz] and the node sum reads 8 contiguous slots with no
+  // incidence list.  Corner slots (slot = local corner k) when no two
+  // elements share a node at the same local corner (every structured
+  // block); node slots (slot = the element's rank among the node's elements
+  // in element order, eslot) when every node has at most 8 elements.
+  // Otherwise fe[slot][element][xyz] + the ELL.
   bool corner = false;
+  unsigned *eslot = nullptr;        // node slots: per internal element, 3 bits per local node
+  std::vector<unsigned> h_eslot;    // the same in the caller's element order
+  int layout = 0;                   // FT_LAYOUT_* in use
   std::vector<int> h_conn;  // caller's connectivity (element-force read-back)
   PartDev *parts = nullptr;
   std::vector<PartDev> h_parts;
@@ -2134,6 +2153,7 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.parts = g->parts;
   a.fe = reinterpret_cast<R *>(g->fe);
   a.corner = g->corner ? 1 : 0;
+  a.eslot = g->eslot;
   a.hist = reinterpret_cast<R *>(g->hist);
   a.fib = reinterpret_cast<const R *>(g->fib);
   a.ctl = in_loop ? g->ctl : nullptr;
@@ -2720,6 +2740,69 @@ int ft_device_count(int *count) {
   return 0;
 }
 
+// Element-force layout (FT_LAYOUT_*): corner slots when every node is local
+// corner k of at most one element for each k; node slots when every node
+// has at most 8 elements (slot = rank of the element among the node's
+// elements in the caller's element order, so the node sum adds them in the
+// reference's scatter-add order); otherwise element-major slots read through
+// the ELL.  AUTO takes the first that applies.  The force buffer is
+// (re)allocated and zeroed: unwritten node-major slots read as zero for the
+// whole run.
+static int set_layout(ft_engine *g, int mode) {
+  const int nn = g->nn, ne = g->ne;
+  const int *conn = g->h_conn.data();
+  bool corner_ok = true;
+  {
+    std::vector<char> seen(8 * (size_t)nn, 0);
+    for (long i = 0; i < 8L * ne && corner_ok; ++i) {
+      char &c = seen[(size_t)(i % 8) * nn + conn[i]];
+      if (c) corner_ok = false;
+      c = 1;
+    }
+  }
+  const bool slots_ok = g->width == 8;
+  if (mode == FT_LAYOUT_AUTO) mode = corner_ok ? FT_LAYOUT_CORNER : slots_ok ? FT_LAYOUT_NODE_SLOTS : FT_LAYOUT_ELL;
+  if (mode == FT_LAYOUT_CORNER && !corner_ok)
+    return fail(FT_ERR_ARG, "corner-slot layout: a node is the same local corner of two elements");
+  if (mode == FT_LAYOUT_NODE_SLOTS && !slots_ok)
+    return fail(FT_ERR_ARG, "node-slot layout: a node has %d elements (at most 8)", g->width);
+  if (mode != FT_LAYOUT_CORNER && mode != FT_LAYOUT_NODE_SLOTS && mode != FT_LAYOUT_ELL)
+    return fail(FT_ERR_ARG, "unknown layout %d", mode);
+  if (g->fe) (void)hipFree(g->fe);
+  if (g->eslot) (void)hipFree(g->eslot);
+  g->fe = nullptr;
+  g->eslot = nullptr;
+  g->h_eslot.clear();
+  g->corner = mode != FT_LAYOUT_ELL;
+  g->layout = mode;
+  const size_t esz = g->precision == FT_FP32 ? 4 : 8;
+  const size_t fe_len = 24 * (size_t)(g->corner ? nn : ne);
+  if (hipMalloc(&g->fe, fe_len * esz) != hipSuccess)
+    return fail(FT_ERR_ALLOC, "element force buffer allocation failed");
+  HIPCHK(hipMemset(g->fe, 0, fe_len * esz));
+  if (mode == FT_LAYOUT_NODE_SLOTS) {
+    g->h_eslot.assign(ne, 0u);
+    std::vector<int> rank(nn, 0);
+    for (int e = 0; e < ne; ++e)
+      for (int k = 0; k < 8; ++k) g->h_eslot[e] |= (unsigned)rank[conn[8 * (size_t)e + k]]++ << (4 * k);
+    std::vector<unsigned> dev(ne);
+    for (int i = 0; i < ne; ++i) dev[i] = g->h_eslot[g->perm[i]];
+    int rc = dalloc(&g->eslot, (size_t)ne);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(g->eslot, dev.data(), ne * sizeof(unsigned), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int ft_engine_set_layout(ft_engine *g, int mode) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (g->run_active) return fail(FT_ERR_STATE, "the layout is chosen before ft_run_init");
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  drop_graphs(g);  // captured launches hold the old force buffer
+  return set_layout(g, mode);
+}
+
 int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *conn,
                      const int *pid, int n_parts, const int *material_id, const double *props,
                      int precision, ft_engine **out) {
@@ -2790,20 +2873,6 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       (rc = dalloc(&g->status, 1)))
     return bail(rc);
   const size_t esz = precision == FT_FP32 ? 4 : 8;
-  {
-    std::vector<char> seen(8 * (size_t)n_nodes, 0);
-    g->corner = true;
-    for (long i = 0; i < 8L * n_elems && g->corner; ++i) {
-      char &c = seen[(size_t)(i % 8) * n_nodes + conn[i]];
-      if (c) g->corner = false;
-      c = 1;
-    }
-  }
-  const size_t fe_len = 24 * (size_t)(g->corner ? n_nodes : n_elems);
-  if (hipMalloc(&g->fe, fe_len * esz) != hipSuccess)
-    return bail(fail(FT_ERR_ALLOC, "element force buffer allocation failed"));
-  // unwritten corner slots must read as zero for the whole run
-  if (hipMemset(g->fe, 0, fe_len * esz) != hipSuccess) return bail(fail(FT_ERR_HIP, "memset failed"));
   g->h_conn.assign(conn, conn + 8 * (size_t)n_elems);
   if (g->has_visco && hipMalloc(&g->hist, hist_len(n_elems) * esz) != hipSuccess)
     return bail(fail(FT_ERR_ALLOC, "history allocation failed"));
@@ -2842,6 +2911,7 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
       }
   }
   if ((rc = dalloc(&g->ell, hell.size()))) return bail(rc);
+  if ((rc = set_layout(g, FT_LAYOUT_AUTO))) return bail(rc);
   std::vector<int> hinfo(n_elems), hconn(8 * (size_t)n_elems);
   for (int i = 0; i < n_elems; ++i) {
     const int e = g->perm[i];
@@ -2884,7 +2954,7 @@ int ft_engine_destroy(ft_engine *g) {
     if (p) (void)hipFree(p);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
-                  g->ell, g->fe, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
+                  g->ell, g->fe, g->eslot, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
                   g->etiles, g->cand_geom, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
@@ -2909,9 +2979,9 @@ int ft_engine_info(ft_engine *g, int *n_nodes, int *n_elems, int *precision) {
   return 0;
 }
 
-int ft_engine_layout(ft_engine *g, int *corner, int *ell_width) {
+int ft_engine_layout(ft_engine *g, int *layout, int *ell_width) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
-  if (corner) *corner = g->corner ? 1 : 0;
+  if (layout) *layout = g->layout;
   if (ell_width) *ell_width = g->width;
   return 0;
 }
@@ -3100,7 +3170,8 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
     }
     for (int e = 0; e < g->ne; ++e)
       for (int k = 0; k < 8; ++k) {
-        const size_t slot = g->corner ? (size_t)k * g->nn + g->h_conn[8 * (size_t)e + k]
+        const size_t sk = g->h_eslot.empty() ? (size_t)k : (g->h_eslot[e] >> (4 * k)) & 7u;
+        const size_t slot = g->corner ? sk * g->nn + g->h_conn[8 * (size_t)e + k]
                                       : (size_t)k * g->ne + g->inv[e];
         for (int d = 0; d < 3; ++d) fe_out[24 * (size_t)e + 3 * k + d] = dev[3 * slot + d];
       }
@@ -4174,13 +4245,14 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
   // X (3 doubles) + u (3 doubles) per node; Prony history read + write and
   // fibre directions for the elements that carry them.
   double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0);
+  if (g->eslot) elem += ne * 4.0;  // node-slot numbers
   for (const auto &sg : g->segs) {
     const double n = sg.end - sg.begin;
     if (sg.mat == 5 || sg.mat == 7) elem += n * 2.0 * 144.0 * es;  // Prony history read + write
     if (sg.mat == 8) elem += n * 3.0 * es;                         // fibre direction
   }
-  // nodal kernel: element forces, 8 corner slots of 3 per node (corner
-  // layout) or 24 es per element + the ELL incidences 4 width per node; per
+  // nodal kernel: element forces, 8 slots of 3 per node (corner and node
+  // slots) or 24 es per element + the ELL incidences 4 width per node; per
   // node mass 8, motion code 4, u and v_half read + write 2 x 48 (X only at
   // constrained nodes and fi only when a run stops: not counted).
   double nodal = (g->corner ? nn * (24.0 * es) : ne * (24.0 * es) + nn * 4.0 * g->width) +

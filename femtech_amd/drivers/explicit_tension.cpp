@@ -9,13 +9,24 @@
 // box, where the reference's own drivers are not available.
 //
 //   explicit_tension MESH [--tmax T] [--dmax D] [--length L] [--plots N]
-//                         [--max-steps K] [--resident] [--out PREFIX]
+//                         [--max-steps K] [--resident] [--loop ex9|ex7]
+//                         [--out PREFIX]
 //
 // Default mode (compatibility): host arrays are authoritative; boundary
 // conditions and the velocity/displacement updates run here, every library
 // call goes to the GPU.  --resident: the same problem with the boundary
 // conditions handed over as data and the loop run on the device through
 // ft_run_init / ft_run_steps.
+//
+// --loop ex7 (compatibility mode): the loop of ex7.cpp:56-125 instead of
+// ex9's: the initial force and accelerations come before any boundary
+// condition, the half-step velocity and the displacement update cover every
+// DOF, only displacements_prev is saved each step (fi_prev, fe_prev and
+// accelerations_prev keep their zero allocation, so CheckEnergy sums with
+// them as ex7's run does), and the boundary routine (ex7.cpp:127-170) sets
+// displacements and flags only.  PREFIX.corner.f64 then holds, as ex7's
+// CustomPlot records (ex7.cpp:172-205), time and displacement of the node at
+// (L, L, L) at step 0 and every plot step.
 //
 // Output: PREFIX.u.f64 (final displacement, 3 per node, FP64),
 // PREFIX.summary.txt (steps, time, dt, energies), the energy file
@@ -42,6 +53,7 @@ struct Options {
   int plots = 50;
   long max_steps = -1;
   bool resident = false;
+  bool ex7 = false;
   std::string out = "tension";
 };
 
@@ -68,6 +80,31 @@ void apply_tension_bc(const Options &o) {
   }
 }
 
+// ex7's boundary routine: planes fixed and the loaded face displaced;
+// velocities and accelerations are left as they are.
+void apply_ex7_bc(const Options &o) {
+  const double applied = Time * (o.dmax / o.tmax);
+  for (int n = 0; n < nNodes; ++n) {
+    for (int d = 0; d < 3; ++d)
+      if (fabs(coordinates[3 * n + d]) < kTol) {
+        boundary[3 * n + d] = 1;
+        displacements[3 * n + d] = 0.0;
+      }
+    if (fabs(coordinates[3 * n + 1] - o.length) < kTol) {
+      boundary[3 * n + 1] = 1;
+      displacements[3 * n + 1] = applied;
+    }
+  }
+}
+
+int corner_node(const Options &o) {
+  for (int n = 0; n < nNodes; ++n)
+    if (fabs(coordinates[3 * n] - o.length) < kTol && fabs(coordinates[3 * n + 1] - o.length) < kTol &&
+        fabs(coordinates[3 * n + 2] - o.length) < kTol)
+      return n;
+  return -1;
+}
+
 void write_outputs(const Options &o, long steps, double energy[4]) {
   FILE *f = fopen((o.out + ".u.f64").c_str(), "wb");
   if (f) {
@@ -80,6 +117,55 @@ void write_outputs(const Options &o, long steps, double energy[4]) {
             Time, dt, energy[0], energy[1], energy[2], energy[3]);
     fclose(f);
   }
+}
+
+long run_ex7(const Options &o, const std::string &base) {
+  const int corner = corner_node(o);
+  std::vector<double> plot = {0.0, 0.0, 0.0, 0.0};
+  dt = ExplicitTimeStepReduction * StableTimeStep();
+  GetForce();
+  CalculateAccelerations();
+  const long n_steps = (long)(o.tmax / dt);
+  const long plot_every = n_steps / o.plots > 0 ? n_steps / o.plots : 1;
+  FILE_LOG_MASTER(INFO, "initial dt = %3.3e, nSteps = %ld, plot every %ld", dt, n_steps, plot_every);
+  memcpy(displacements_prev, displacements, nDOF * sizeof(double));
+  int nplot = 0;
+  long step = 1, taken = 0;
+  while (Time < o.tmax && (o.max_steps < 0 || taken < o.max_steps)) {
+    const double t_n = Time, t_np1 = Time + dt, t_half = 0.5 * (t_n + t_np1);
+    Time = t_np1;
+    for (int i = 0; i < nDOF; ++i) velocities_half[i] = velocities[i] + (t_half - t_n) * accelerations[i];
+    memcpy(displacements_prev, displacements, nDOF * sizeof(double));
+    for (int i = 0; i < nDOF; ++i) displacements[i] += dt * velocities_half[i];
+    apply_ex7_bc(o);
+    GetForce();
+    CalculateAccelerations();
+    for (int i = 0; i < nDOF; ++i) velocities[i] = velocities_half[i] + (t_np1 - t_half) * accelerations[i];
+    const int write_flag = (int)(step % plot_every);
+    CheckEnergy(Time, write_flag);
+    if (write_flag == 0) {
+      ++nplot;
+      WriteVTU(base.c_str(), nplot);
+      if (nplot < MAXPLOTSTEPS) {
+        stepTime[nplot] = Time;
+        WritePVD(base.c_str(), nplot);
+      }
+      if (corner >= 0) {
+        plot.push_back(Time);
+        plot.insert(plot.end(), displacements + 3 * corner, displacements + 3 * corner + 3);
+      }
+    }
+    ++step;
+    ++taken;
+    dt = ExplicitTimeStepReduction * StableTimeStep();
+    MPI_Barrier(MPI_COMM_WORLD);
+  }
+  FILE *f = fopen((o.out + ".corner.f64").c_str(), "wb");
+  if (f) {
+    fwrite(plot.data(), sizeof(double), plot.size(), f);
+    fclose(f);
+  }
+  return taken;
 }
 
 long run_compatibility(const Options &o, const std::string &base, double energy[4]) {
@@ -192,6 +278,14 @@ int main(int argc, char **argv) {
     else if (s == "--plots") o.plots = atoi(next());
     else if (s == "--max-steps") o.max_steps = atol(next());
     else if (s == "--resident") o.resident = true;
+    else if (s == "--loop") {
+      const std::string l = next();
+      if (l != "ex7" && l != "ex9") {
+        fprintf(stderr, "--loop takes ex9 or ex7\n");
+        return 2;
+      }
+      o.ex7 = l == "ex7";
+    }
     else if (s == "--out") o.out = next();
     else if (!o.mesh) o.mesh = argv[a];
     else {
@@ -201,7 +295,11 @@ int main(int argc, char **argv) {
   }
   if (!o.mesh) {
     fprintf(stderr, "usage: %s MESH [--tmax T] [--dmax D] [--length L] [--plots N] "
-                    "[--max-steps K] [--resident] [--out PREFIX]\n", argv[0]);
+                    "[--max-steps K] [--resident] [--loop ex9|ex7] [--out PREFIX]\n", argv[0]);
+    return 2;
+  }
+  if (o.ex7 && o.resident) {
+    fprintf(stderr, "--loop ex7 runs in compatibility mode only\n");
     return 2;
   }
   InitFemTechWoInput(argc, argv);
@@ -223,7 +321,7 @@ int main(int argc, char **argv) {
     stepTime[0] = Time;
     ShapeFunctions();
     AssembleLumpedMass();
-    steps = run_compatibility(o, base, energy);
+    steps = o.ex7 ? run_ex7(o, base) : run_compatibility(o, base, energy);
   }
   FILE_LOG_MASTER(INFO, "End of Iterative Loop: %ld steps, time %.6e", steps, Time);
   write_outputs(o, steps, energy);

@@ -85,13 +85,23 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords,
                      int precision, ft_engine **out);
 int ft_engine_destroy(ft_engine *eng);
 int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
-/* Force-gather layout chosen at creation (no reference counterpart; the
- * reference scatter-adds, GetForce_3D.cpp:39-44).  corner = 1: every node is
- * local corner k of at most one element for each k, so element forces go to
- * per-node corner slots; 0: slot-major element forces summed through the
- * node-to-element table.  ell_width: that table's width (the largest node
+/* Force-gather layout (no reference counterpart; the reference scatter-adds,
+ * GetForce_3D.cpp:39-44).  FT_LAYOUT_CORNER: every node is local corner k of
+ * at most one element for each k, so element forces go to per-node corner
+ * slots; FT_LAYOUT_NODE_SLOTS: every node has at most 8 elements, and each
+ * element writes its corner forces to per-node slots numbered by its rank
+ * among the node's elements (element order); FT_LAYOUT_ELL: element-major
+ * forces summed through the node-to-element table.  Creation picks the first
+ * that applies (FT_LAYOUT_AUTO); ft_engine_set_layout chooses another that
+ * applies, before ft_run_init (FT_ERR_ARG when it does not apply,
+ * FT_ERR_STATE inside a run).  ell_width: the table's width (the largest node
  * valence, at least 8). */
-int ft_engine_layout(ft_engine *eng, int *corner, int *ell_width);
+#define FT_LAYOUT_AUTO 0
+#define FT_LAYOUT_CORNER 1
+#define FT_LAYOUT_NODE_SLOTS 2
+#define FT_LAYOUT_ELL 3
+int ft_engine_layout(ft_engine *eng, int *layout, int *ell_width);
+int ft_engine_set_layout(ft_engine *eng, int layout);
 
 /* ---- set-up --------------------------------------------------------------
  * Boundary conditions as data: dof_motion[3 n + d] = -1 for a free DOF or the

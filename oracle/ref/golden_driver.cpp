@@ -13,6 +13,7 @@
  *   stress <dir>  StressUpdate (StressUpdate.cpp:5-29) on given F sequences.
  *   geom <dir>    volumeHexahedron / areaHexahedronFace /
  *                 CalculateCharacteristicLength_C3D8 on given element corners.
+ *   ex7 <dir>     ex7's run (ex7.cpp:19-170): its own loop and boundary routine.
  *   time <dir>    the run's loop timed alone, no output, on 1..N MPI ranks
  *                 (CPU-baseline calibration; <dir>/coords.bin added).
  *
@@ -256,6 +257,116 @@ int mode_run() {
   return 0;
 }
 
+/* ex7's run (ex7.cpp:19-125) on <dir>/mesh.k and <dir>/materials.dat: the
+ * initial force and accelerations before any boundary condition, the
+ * half-step velocity and the displacement update on every DOF, only
+ * displacements_prev saved each step (fi_prev, fe_prev and
+ * accelerations_prev keep their zero allocation), and its boundary routine
+ * (ex7.cpp:127-170), which sets displacements and flags only.  Constants as
+ * in ex7.cpp: tMax 1.0 s, dMax 1 mm, loaded face y = 5 mm, 50 plot steps.
+ * Outputs: final u/v/a/fi, time and dt per step, the energies per step
+ * (full-precision copy of CheckEnergy's sums), and the corner node
+ * (5, 5, 5) mm at every plot step as ex7's CustomPlot records it
+ * (ex7.cpp:172-205) in full precision. */
+const double kEx7Tol = 1e-5, kEx7L = 0.005, kEx7TMax = 1.0, kEx7DMax = 0.001;
+const int kEx7Plots = 50;
+
+void ex7_bc() {
+  const double applied = Time * (kEx7DMax / kEx7TMax);
+  for (int i = 0; i < nNodes; i++) {
+    for (int d = 0; d < 3; ++d)
+      if (fabs(coordinates[ndim * i + d] - 0.0) < kEx7Tol) {
+        boundary[ndim * i + d] = 1;
+        displacements[ndim * i + d] = 0.0;
+      }
+    if (fabs(coordinates[ndim * i + 1] - kEx7L) < kEx7Tol) {
+      boundary[ndim * i + 1] = 1;
+      displacements[ndim * i + 1] = applied;
+    }
+  }
+}
+
+int mode_ex7() {
+  ReadInputFile(access("mesh.k", F_OK) == 0 ? "mesh.k" : "mesh.inp");
+  ReadMaterials();
+  PartitionMesh();
+  AllocateArrays();
+  Time = 0.0;
+  write_bin("out_coords.f64", coordinates, nDOF);
+  write_bin("out_conn.i32", connectivity, eptr[nelements]);
+  int corner = -1;
+  for (int i = 0; i < nNodes; i++)
+    if (fabs(coordinates[ndim * i] - kEx7L) < kEx7Tol && fabs(coordinates[ndim * i + 1] - kEx7L) < kEx7Tol &&
+        fabs(coordinates[ndim * i + 2] - kEx7L) < kEx7Tol)
+      corner = i;
+  if (corner < 0) return 6;
+  std::vector<double> plot = {0.0, 0.0, 0.0, 0.0};
+  ShapeFunctions();
+  AssembleLumpedMass();
+  dt = ExplicitTimeStepReduction * StableTimeStep();
+  GetForce();
+  CalculateAccelerations();
+  const int nSteps = (int)(kEx7TMax / dt);
+  const int nsteps_plot = (int)(nSteps / kEx7Plots);
+  memcpy(displacements_prev, displacements, nDOF * sizeof(double));
+  std::vector<double> hist_time(1, Time), hist_dt(1, dt), hist_energy(4, 0.0);
+  double Wint = 0.0, Wext = 0.0;
+  int counter = 1, step = 0;
+  while (Time < kEx7TMax) {
+    double t_n = Time;
+    double t_np1 = Time + dt;
+    Time = t_np1;
+    double dt_nphalf = dt;
+    double t_nphalf = 0.5 * (t_np1 + t_n);
+    for (int i = 0; i < nDOF; i++) velocities_half[i] = velocities[i] + (t_nphalf - t_n) * accelerations[i];
+    memcpy(displacements_prev, displacements, nDOF * sizeof(double));
+    for (int i = 0; i < nDOF; i++) displacements[i] = displacements[i] + dt_nphalf * velocities_half[i];
+    ex7_bc();
+    GetForce();
+    CalculateAccelerations();
+    for (int i = 0; i < nDOF; i++) velocities[i] = velocities_half[i] + (t_np1 - t_nphalf) * accelerations[i];
+    const int writeFlag = counter % nsteps_plot;
+    CheckEnergy(Time, writeFlag);
+    double sWint = 0.0, sWext = 0.0, WKE = 0.0;
+    for (int i = 0; i < nDOF; ++i) {
+      double dd = displacements[i] - displacements_prev[i];
+      WKE += mass[i] * velocities[i] * velocities[i];
+      if (boundary[i]) {
+        double reaction = fi_prev[i] + fi[i] + mass[i] * (accelerations[i] + accelerations_prev[i]);
+        sWext += dd * reaction;
+      }
+      sWint += dd * (fi_prev[i] + fi[i]);
+      sWext += dd * (fe_prev[i] + fe[i]);
+    }
+    Wint += 0.5 * sWint;
+    Wext += 0.5 * sWext;
+    WKE *= 0.5;
+    if (writeFlag == 0) {
+      plot.push_back(Time);
+      for (int d = 0; d < 3; ++d) plot.push_back(displacements[3 * corner + d]);
+    }
+    ++counter;
+    ++step;
+    dt = ExplicitTimeStepReduction * StableTimeStep();
+    hist_time.push_back(Time);
+    hist_dt.push_back(dt);
+    hist_energy.push_back(Wint);
+    hist_energy.push_back(Wext);
+    hist_energy.push_back(WKE);
+    hist_energy.push_back(fabs(WKE + Wint - Wext));
+  }
+  write_bin("out_u.f64", displacements, nDOF);
+  write_bin("out_v.f64", velocities, nDOF);
+  write_bin("out_a.f64", accelerations, nDOF);
+  write_bin("out_fi.f64", fi, nDOF);
+  write_bin("out_time.f64", hist_time.data(), hist_time.size());
+  write_bin("out_dt.f64", hist_dt.data(), hist_dt.size());
+  write_bin("out_energy.f64", hist_energy.data(), hist_energy.size());
+  write_bin("out_plot.f64", plot.data(), plot.size());
+  printf("steps %d time %.17g uid %s corner %d plot_every %d\n", step, Time, uid.c_str(), corner, nsteps_plot);
+  return 0;
+}
+
 /* Wall time of the ex9 loop alone (ex9.cpp:107-175), with no fixture output,
  * on any number of MPI ranks: the boundary motions of bc.bin are assigned to
  * the rank's nodes by their coordinates (coords.bin: the global mesh's nodes
@@ -418,7 +529,7 @@ int mode_geom() {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: %s run|time|stress|geom <dir>\n", argv[0]);
+    fprintf(stderr, "usage: %s run|ex7|time|stress|geom <dir>\n", argv[0]);
     return 2;
   }
   const std::string mode = argv[1];
@@ -426,8 +537,8 @@ int main(int argc, char **argv) {
   if (chdir(argv[2]) != 0) return 2;
   InitFemTechWoInput(argc, argv);
   int rc = 2;
-  if (mode == "run" || mode == "time") {
-    rc = mode == "run" ? mode_run() : mode_time();
+  if (mode == "run" || mode == "time" || mode == "ex7") {
+    rc = mode == "run" ? mode_run() : mode == "time" ? mode_time() : mode_ex7();
     FinalizeFemTech();  // frees the arrays and closes energy_<uid>.dat
     return rc;
   }

@@ -95,6 +95,32 @@ def golden_run(name, c, mesh_file=None):
     return inputs, res
 
 
+def golden_ex7(mesh_file, materials_file):
+    """ex7's own run (mesh, materials, loop and boundary routine of
+    examples/ex7) through the golden driver's ex7 mode."""
+    d = tempfile.mkdtemp(prefix="golden_ex7_")
+    try:
+        shutil.copy(mesh_file, os.path.join(d, "mesh.k"))
+        shutil.copy(materials_file, os.path.join(d, "materials.dat"))
+        out = run_driver("ex7", d).split()
+        uid = out[out.index("uid") + 1]
+        r = lambda f, t=np.float64: np.fromfile(os.path.join(d, f), dtype=t)  # noqa: E731
+        res = dict(ref_coords=r("out_coords.f64"), ref_conn=r("out_conn.i32", np.int32), ref_u=r("out_u.f64"),
+                   ref_v=r("out_v.f64"), ref_a=r("out_a.f64"), ref_fi=r("out_fi.f64"),
+                   ref_time=r("out_time.f64"), ref_dt=r("out_dt.f64"),
+                   ref_energy=r("out_energy.f64").reshape(-1, 4), ref_corner=r("out_plot.f64").reshape(-1, 4),
+                   ref_steps=np.int64(int(out[1])), ref_plot_every=np.int64(int(out[out.index("plot_every") + 1])))
+        rows = []
+        with open(os.path.join(d, "energy_%s.dat" % uid)) as f:
+            for line in f:
+                if not line.startswith("#") and line.strip():
+                    rows.append([float(x) for x in line.split()])
+        res["ref_energy_file"] = np.array(rows)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return res
+
+
 def abaqus_rpt(path):
     rows = []
     with open(path) as f:
@@ -198,6 +224,17 @@ def main():
                                                 "10elements.rpt"))
     res["global_node_id"] = gid
     save("run_bench10", **inp, **res)
+
+    # ex7: the reference's two-element column under its own loop, with the
+    # Abaqus corner-node history (TensionNoDamping.dat: time, U1, U2, U3)
+    ex7 = os.path.join(REF, "examples", "ex7")
+    X, conn, pid, gid = ft.read_mesh(os.path.join(ex7, "2elts-hex.k"))
+    res = golden_ex7(os.path.join(ex7, "2elts-hex.k"), os.path.join(ex7, "materials.dat"))
+    mat, props = ft.read_materials(os.path.join(ex7, "materials.dat"), 1)
+    res["abaqus_corner"] = np.loadtxt(os.path.join(ex7, "TensionNoDamping.dat"))
+    res["abaqus_rpt"] = abaqus_rpt(os.path.join(ex7, "abaqus", "abaqus.rpt"))
+    save("run_ex7", X=X, conn=conn, pid=pid, material_id=mat, props=props, tmax=np.float64(1.0),
+         dmax=np.float64(0.001), length=np.float64(0.005), plots=np.int64(50), **res)
 
     inp, res = golden_run("c1", cases.case_c1())
     save("run_c1", **inp, **res)

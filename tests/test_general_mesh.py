@@ -2,11 +2,13 @@
 
 The engine writes element forces to per-node corner slots only when every
 node is local corner k of at most one element for each k (every structured
-block and mapped ellipsoid qualifies).  Any other mesh takes the general
-path: slot-major element forces summed per node through the node-to-element
-table in the reference's scatter-add order (GetForce_3D.cpp:39-44), with the
-8-wide table when no node has more than 8 elements and the variable-width
-loop otherwise.  These tests build such meshes by renumbering each element's
+block and mapped ellipsoid qualifies).  Any other mesh whose nodes have at
+most 8 elements takes per-node slots numbered by the element's rank among
+the node's elements; the rest take slot-major element forces summed per node
+through the node-to-element table, with the 8-wide table when no node has
+more than 8 elements and the variable-width loop otherwise.  Both general
+paths sum in the reference's scatter-add order (GetForce_3D.cpp:39-44), so
+they agree bitwise.  These tests build such meshes by renumbering each element's
 local nodes with a random proper rotation of the reference cube (the same
 hexahedra, the same Gauss-point set) and from an unstructured fan whose axis
 nodes belong to 12 elements, and check the engine against the oracle run on
@@ -64,8 +66,10 @@ def oracle_for(c):
     return o
 
 
-def engine_for(c, precision):
+def engine_for(c, precision, layout=ft.LAYOUT_AUTO):
     e = ft.Engine(c["X"], c["conn"], c["pid"], c["material_id"], c["props"], precision)
+    if layout != ft.LAYOUT_AUTO:
+        e.set_layout(layout)
     e.set_boundary(c["dof_motion"], c["motions"])
     if c["fibres"] is not None:
         e.set_fibres(c["fibres"])
@@ -119,23 +123,41 @@ def test_oracle_is_invariant_under_renumbering(name):
 
 # ---- GPU: engine against the oracle on the same meshes --------------------
 
+# (case, layout): the node-slot layout where it applies (creation's choice)
+# and the ELL forced on the same mesh; the fan's axis nodes have 12 elements
+LAYOUT_CASES = [(n, ft.LAYOUT_AUTO) for n in CASES] + [(n, ft.LAYOUT_ELL) for n in CASES if not n.startswith("fan")]
+LAYOUT_IDS = ["%s-%s" % (n, "auto" if lay == ft.LAYOUT_AUTO else "ell") for n, lay in LAYOUT_CASES]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", CASES)
 def test_general_layout_chosen(gpu_ready, name):
     c = general_case(name)
     width = int(np.bincount(c["conn"]).max())
+    fan = name.startswith("fan")
+    assert (width > 8) == fan
     with engine_for(c, ft.FT_FP64) as e:
-        assert e.layout() == (0, max(8, width))
-    assert (width > 8) == name.startswith("fan")
+        assert e.layout() == ((ft.LAYOUT_ELL if fan else ft.LAYOUT_NODE_SLOTS), max(8, width))
+        with pytest.raises(ft.FemTechError) as ei:
+            e.set_layout(ft.LAYOUT_CORNER)
+        assert ei.value.code == ft.FT_ERR_ARG
+        if fan:
+            with pytest.raises(ft.FemTechError):
+                e.set_layout(ft.LAYOUT_NODE_SLOTS)
+        else:
+            e.set_layout(ft.LAYOUT_ELL)
+            assert e.layout() == (ft.LAYOUT_ELL, 8)
+            e.set_layout(ft.LAYOUT_AUTO)
+            assert e.layout() == (ft.LAYOUT_NODE_SLOTS, 8)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", CASES)
-def test_general_forces_mass_dt_fp64(gpu_ready, name):
+@pytest.mark.parametrize("name,layout", LAYOUT_CASES, ids=LAYOUT_IDS)
+def test_general_forces_mass_dt_fp64(gpu_ready, name, layout):
     c = general_case(name)
     o = oracle_for(c)
     bnd = (c["dof_motion"] >= 0).astype(np.int32)
-    with engine_for(c, ft.FT_FP64) as e:
+    with engine_for(c, ft.FT_FP64, layout) as e:
         assert rel(e.lumped_mass(), o.lumped_mass()) < 1e-13
         for seed, scale in ((1, 1e-3), (2, 5e-2)):
             u = displacement(c, scale, seed)
@@ -144,8 +166,8 @@ def test_general_forces_mass_dt_fp64(gpu_ready, name):
             assert e.stable_time_step(u, bnd) == pytest.approx(o.stable_dt(u, bnd), rel=1e-13)
 
 
-def run_engine(c, precision, history=True, graph=0, persistent=False, chunks=None):
-    with engine_for(c, precision) as e:
+def run_engine(c, precision, history=True, graph=0, persistent=False, chunks=None, layout=ft.LAYOUT_AUTO):
+    with engine_for(c, precision, layout) as e:
         e.set_state(None, c["v0"])
         e.lumped_mass()
         if history:
@@ -170,11 +192,11 @@ def run_oracle(c):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", CASES)
-def test_general_run_fp64_matches_oracle(gpu_ready, name):
+@pytest.mark.parametrize("name,layout", LAYOUT_CASES, ids=LAYOUT_IDS)
+def test_general_run_fp64_matches_oracle(gpu_ready, name, layout):
     c = general_case(name)
     r, o = run_oracle(c)
-    g = run_engine(c, ft.FT_FP64)
+    g = run_engine(c, ft.FT_FP64, layout=layout)
     assert g["done"] == r["steps"] == c["steps"]
     u, v, a, fi = g["state"]
     th, dth, en = g["history"]
@@ -188,11 +210,11 @@ def test_general_run_fp64_matches_oracle(gpu_ready, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", CASES)
-def test_general_run_fp32_split(gpu_ready, name):
+@pytest.mark.parametrize("name,layout", LAYOUT_CASES, ids=LAYOUT_IDS)
+def test_general_run_fp32_split(gpu_ready, name, layout):
     c = general_case(name)
     r, _ = run_oracle(c)
-    g = run_engine(c, ft.FT_FP32)
+    g = run_engine(c, ft.FT_FP32, layout=layout)
     assert g["done"] == r["steps"]
     th, dth, en = g["history"]
     assert rel(g["state"][0], r["u"]) < 1e-5
@@ -203,16 +225,17 @@ def test_general_run_fp32_split(gpu_ready, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["impact", "fan"])
+@pytest.mark.parametrize("name,layout", [("impact", ft.LAYOUT_AUTO), ("impact", ft.LAYOUT_ELL),
+                                         ("fan", ft.LAYOUT_AUTO)], ids=["impact-auto", "impact-ell", "fan-auto"])
 @pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
-def test_general_graph_and_persistent_equal_launches(gpu_ready, name, precision):
+def test_general_graph_and_persistent_equal_launches(gpu_ready, name, layout, precision):
     """Captured chunks equal the per-step launches bitwise; the persistent
     loop agrees to round-off (relative 1e-9 FP32 engine, 1e-12 FP64)."""
     c = general_case(name)
     chunks = [16, 7, 40, c["steps"] - 63]
-    base = run_engine(c, precision, history=False, chunks=chunks)
-    graph = run_engine(c, precision, history=False, graph=8, chunks=chunks)
-    pers = run_engine(c, precision, history=False, persistent=True, chunks=chunks)
+    base = run_engine(c, precision, history=False, chunks=chunks, layout=layout)
+    graph = run_engine(c, precision, history=False, graph=8, chunks=chunks, layout=layout)
+    pers = run_engine(c, precision, history=False, persistent=True, chunks=chunks, layout=layout)
     assert base["done"] == graph["done"] == pers["done"] == c["steps"]
     for k in range(4):
         assert np.array_equal(graph["state"][k], base["state"][k]), k
@@ -221,3 +244,18 @@ def test_general_graph_and_persistent_equal_launches(gpu_ready, name, precision)
         b = base["state"][k]
         assert np.abs(pers["state"][k] - b).max() <= tol * np.abs(b).max(), k
     assert pers["time"][2] == base["time"][2]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["impact", "materials", "ellipsoid"])
+@pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
+def test_node_slots_equal_ell_bitwise(gpu_ready, name, precision):
+    """Node slots and the ELL add the same element forces in the same order
+    (element order per node): identical runs, bit for bit."""
+    c = general_case(name)
+    a = run_engine(c, precision, history=False)
+    b = run_engine(c, precision, history=False, layout=ft.LAYOUT_ELL)
+    assert a["done"] == b["done"] == c["steps"]
+    for k in range(4):
+        assert np.array_equal(a["state"][k], b["state"][k]), k
+    assert a["time"] == b["time"] and np.array_equal(a["energy"], b["energy"])

@@ -85,3 +85,40 @@ def test_fp32_compatibility_mode_within_split_tolerance(gpu_ready, tmp_path):
     assert res.returncode == 0, res.stderr[-2000:]
     u = np.fromfile(str(tmp_path / "tension.u.f64"))
     assert rel(u, z["ref_u"]) < 1e-5
+
+
+def test_ex7_loop_matches_reference(gpu_ready, tmp_path):
+    """ex7's own run (tests/golden/run_ex7.npz: the reference's two-element
+    column, its loop and boundary routine, ex7.cpp:19-170) through the
+    driver's --loop ex7: same steps and times, final state to 1e-10, the
+    corner node's plot history to 1e-10, and CheckEnergy's rows (summed with
+    ex7's never-saved previous-step arrays) to their printed precision.  The
+    Abaqus corner history (TensionNoDamping.dat) then differs from this run
+    exactly as it differs from the reference's."""
+    z = golden("run_ex7")
+    write_case(str(tmp_path), z)
+    cmd = [DRIVER, "mesh.inp", "--loop", "ex7", "--tmax", repr(float(z["tmax"])), "--dmax", repr(float(z["dmax"])),
+           "--length", repr(float(z["length"])), "--plots", str(int(z["plots"]))]
+    print("driver: ex7 loop", flush=True)
+    res = subprocess.run(cmd, cwd=str(tmp_path), capture_output=True, text=True,
+                         env=dict(os.environ, FEMTECH_UID="t"), timeout=300)
+    assert res.returncode == 0, (res.returncode, res.stdout[-2000:], res.stderr[-2000:])
+    summary = dict(line.split() for line in open(tmp_path / "tension.summary.txt"))
+    assert int(summary["steps"]) == int(z["ref_steps"])
+    assert float(summary["time"]) == pytest.approx(float(z["ref_time"][-1]), rel=1e-13)
+    u = np.fromfile(str(tmp_path / "tension.u.f64"))
+    assert rel(u, z["ref_u"]) < 1e-10
+    corner = np.fromfile(str(tmp_path / "tension.corner.f64")).reshape(-1, 4)
+    ref = z["ref_corner"]
+    assert corner.shape == ref.shape
+    np.testing.assert_allclose(corner[:, 0], ref[:, 0], rtol=1e-13)
+    assert rel(corner[:, 1:], ref[:, 1:]) < 1e-10
+    rows = energy_rows(tmp_path / "energy_t.dat")
+    ef = z["ref_energy_file"]
+    assert rows.shape == ef.shape
+    np.testing.assert_allclose(rows[:, 0], ef[:, 0], rtol=1e-6)
+    assert np.abs(rows[:, 1:4] - ef[:, 1:4]).max() <= 2e-6 * np.abs(ef[:, 1:4]).max()
+    ab = z["abaqus_corner"]
+    ours = np.interp(ab[:, 0], corner[:, 0], corner[:, 1])
+    theirs = np.interp(ab[:, 0], ref[:, 0], ref[:, 1])
+    assert np.abs(ours - theirs).max() <= 1e-10 * np.abs(ab[:, 1]).max()
