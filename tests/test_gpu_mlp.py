@@ -58,12 +58,17 @@ def test_gradients_match_oracle(gpu_ready, n):
         p = net.params().astype(np.float64)
         loss, g = net.gradients(x, t)
     lref, gref = M.loss_and_grad(p, x, t)
-    assert abs(loss / lref - 1) < 1e-3
+    # The restatement rounds to BF16 where the kernels do; what is left is
+    # the FP32 accumulation order of the MFMA tiles and the device SiLU:
+    # measured 2.5e-8..1.8e-7 on the loss and at most 3.9e-4 on any layer's
+    # gradient over n = 256..4096 and two data seeds.  A systematic error of
+    # a tenth of a per cent in one layer fails these bars.
+    assert abs(loss / lref - 1) < 1e-5
     for l in range(M.LAYERS):
         w = slice(M.W_OFF[l], M.B_OFF[l])
         b = slice(M.B_OFF[l], M.B_OFF[l] + M.layer_out(l))
-        assert rel(g[w], gref[w]) < 1e-2, ("W", l, rel(g[w], gref[w]))
-        assert rel(g[b], gref[b]) < 1e-2, ("b", l, rel(g[b], gref[b]))
+        assert rel(g[w], gref[w]) < 1.5e-3, ("W", l, rel(g[w], gref[w]))
+        assert rel(g[b], gref[b]) < 1.5e-3, ("b", l, rel(g[b], gref[b]))
 
 
 def test_training_fits_a_smooth_map(gpu_ready):
