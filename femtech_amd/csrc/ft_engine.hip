@@ -873,12 +873,13 @@ __device__ inline double merge_candidate(double dtmin, double dte) {
 // elements that reported only a lower bound (their own, or their block's
 // beyond the margin) could undercut it: the smallest such value, a lower
 // bound of every one of their steps, is taken instead (conservative; *bounded
-// set).
+// set).  When no element reported anything finite (non-finite forces
+// everywhere), d stays infinite and the step collapses.
 __device__ inline double bound_rule(const StepSlot &cur, double d, bool *bounded) {
   if (!(d > cur.dt_guide * kSkipMargin)) return d;
   double cb = cur.cand_bits != kNoCandidate ? (double)__uint_as_float((unsigned)(cur.cand_bits >> 32)) : 1e300;
   if (cur.far_bits < 0x7f800000u) cb = fmin(cb, (double)__uint_as_float(cur.far_bits));
-  if (cb < d) {
+  if (cb < 1e300 && cb < d) {
     *bounded = true;
     return cb;
   }

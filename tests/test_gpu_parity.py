@@ -510,17 +510,26 @@ def test_batched_fp32_equals_separate_fp32_runs(gpu_ready):
         assert info[i][0] == ti[0] and int(info[i][2]) == ti[2], (i, info[i], ti)
 
 
-def test_batch_instance_failure_is_isolated(gpu_ready):
-    """An instance whose elements invert stops (state 2) while the others run
-    on to their end times, bitwise as when run alone; the batch call returns
-    normally."""
+@pytest.mark.parametrize("cause", ["inverted", "nonfinite"])
+@pytest.mark.parametrize("precision", [ft.FT_FP64, ft.FT_FP32])
+def test_batch_instance_failure_is_isolated(gpu_ready, cause, precision):
+    """An instance whose elements invert, or whose every element force turns
+    non-finite (NaN initial velocities), stops (state 2) at a finite time
+    while the others run on to their end times, bitwise as when run alone;
+    the batch call returns normally.  (No finite element step at all must
+    stop an instance, not send it past its end time.)"""
     from femtech_amd import workloads as W
-    bad = W.ellipsoid_head(n=10, peak=5e5, duration=2e-3)
+    if cause == "inverted":
+        bad = W.ellipsoid_head(n=10, peak=5e5, duration=2e-3)
+    else:
+        bad = W.impact_block(n=3, slab=1, seed=13)
+        bad["v0"] = np.full_like(bad["v0"], np.nan)
     good = W.impact_block(n=4, slab=1, seed=11)
     tmax = [1e-2, 3e-4]
-    u, no, info, _ = run_batch_or_single([bad, good], tmax, ft.FT_FP64, 2000)
+    u, no, info, _ = run_batch_or_single([bad, good], tmax, precision, 2000)
     assert info[0][3] == 2.0 and info[1][3] == 1.0, info
-    ug, _, _, tg = run_batch_or_single([good], [tmax[1]], ft.FT_FP64, 2000)
+    assert 0.0 <= info[0][0] < tmax[0], info[0]
+    ug, _, _, tg = run_batch_or_single([good], [tmax[1]], precision, 2000)
     assert np.array_equal(u[3 * no[1]:3 * no[2]], ug)
     assert int(info[1][2]) == tg[2]
 
