@@ -182,6 +182,54 @@ def test_prony_reduces_to_elastic_when_g_is_zero_and_relaxes_otherwise():
     np.testing.assert_allclose(S_late, S4, rtol=1e-6, atol=1e-6 * np.abs(S4).max())
 
 
+def test_prony_recurrence_converges_to_the_hereditary_integral():
+    """HGOIsotropicViscoelastic.cpp:126-159 integrates the two-term Prony
+    series H_i(t) = g_i int_0^t exp(-(t - s) / tau_i) dS_dev/ds ds step by
+    step (both coefficients of the recurrence enter).  Under a constant-rate
+    deformation F(t) = I + t A from rest, the viscous part of the stress at
+    T (material 5 minus its elastic law, material 4) converges to the
+    integral evaluated independently (by parts, with Gauss-Legendre
+    quadrature of exp(-(T - s) / tau) S_dev(s)) at second order in dt."""
+    rng = np.random.default_rng(8)
+    A = 20.0 * rng.uniform(-1, 1, (3, 3))        # strain rate 1/s: 20 % over T
+    T = 0.01
+    g, tau = (0.4, 0.3), (2e-3, 2e-2)
+    p4 = ft_props(4)
+    p5 = p4.copy()
+    p5[5:9] = [g[0], tau[0], g[1], tau[1]]
+
+    def s_dev(t):
+        F = np.eye(3) + t * A
+        S6, _ = O.stress(4, p4, col(F))
+        S = np.array([[S6[0], S6[5], S6[4]], [S6[5], S6[1], S6[3]], [S6[4], S6[3], S6[2]]])
+        C = F.T @ F
+        return S - np.sum(S * C) / 3.0 * np.linalg.inv(C)
+
+    # by parts: int_0^T e^{-(T-s)/tau} dSdev = Sdev(T) - (1/tau) int_0^T e^{-(T-s)/tau} Sdev(s) ds
+    xg, wg = np.polynomial.legendre.leggauss(24)
+    edges = np.linspace(0.0, T, 41)
+    exact = np.zeros((3, 3))
+    for gi, ti in zip(g, tau):
+        integral = np.zeros((3, 3))
+        for a, b in zip(edges[:-1], edges[1:]):
+            for x, w in zip(xg, wg):
+                s = 0.5 * (a + b) + 0.5 * (b - a) * x
+                integral += 0.5 * (b - a) * w * np.exp(-(T - s) / ti) * s_dev(s)
+        exact += gi * (s_dev(T) - integral / ti)
+    exact6 = voigt(exact)
+    S_el, _ = O.stress(4, p4, col(np.eye(3) + T * A))
+    errs = []
+    for n in (25, 50, 100, 200):
+        dt = T / n
+        h = np.zeros(27)
+        for k in range(1, n + 1):
+            S5, _ = O.stress(5, p5, col(np.eye(3) + (k * dt) * A), dt, h)
+        errs.append(np.abs((S5 - S_el) - exact6).max() / np.abs(exact6).max())
+    ratios = [errs[i] / errs[i + 1] for i in range(3)]
+    assert all(3.5 < r < 4.5 for r in ratios), (errs, ratios)
+    assert errs[-1] < 1e-4, errs
+
+
 def block_model(n=3, h=1e-3, mat=1):
     X, conn = ft.mesh_block(n, n, n, h)
     props = ft_props(mat)[None, :]
