@@ -2829,12 +2829,19 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   g->ne = n_elems;
   g->nparts = n_parts;
   g->precision = precision;
-  HIPCHK(hipGetDevice(&g->device));
-  int rc = 0;
   auto bail = [&](int code) {
     ft_engine_destroy(g);
     return code;
   };
+  // a HIP failure past this point destroys the partly built engine
+#define HIPCHK_BAIL(call)                                                                   \
+  do {                                                                                      \
+    const hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                                   \
+      return bail(fail(FT_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)));         \
+  } while (0)
+  HIPCHK_BAIL(hipGetDevice(&g->device));
+  int rc = 0;
   if (hipStreamCreate(&g->stream) != hipSuccess) return bail(fail(FT_ERR_HIP, "stream creation failed"));
   if ((rc = scratch_address(g->device, &g->scratch))) return bail(rc);
   g->lstream = g->stream;
@@ -2921,28 +2928,29 @@ int ft_engine_create(int n_nodes, int n_elems, const double *coords, const int *
   }
   {
     std::vector<double> soa = to_soa(coords, n_nodes);
-    HIPCHK(hipMemcpy(g->X, soa.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK_BAIL(hipMemcpy(g->X, soa.data(), nd * sizeof(double), hipMemcpyHostToDevice));
   }
-  HIPCHK(hipMemcpy(g->conn, hconn.data(), 8 * (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->einfo, hinfo.data(), (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->parts, g->h_parts.data(), n_parts * sizeof(PartDev), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g->ell, hell.data(), hell.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(g->u, 0, nd * sizeof(double)));
-  HIPCHK(hipMemset(g->vs, 0, nd * sizeof(double)));
-  HIPCHK(hipMemset(g->fi, 0, nd * sizeof(double)));
-  HIPCHK(hipMemset(g->node_code, 0, (size_t)n_nodes * sizeof(int)));
-  HIPCHK(hipMemset(g->node_mask, 0, (size_t)n_nodes));
-  HIPCHK(hipMemset(g->status, 0, sizeof(int)));
-  if (g->hist) HIPCHK(hipMemset(g->hist, 0, hist_len(n_elems) * esz));
+  HIPCHK_BAIL(hipMemcpy(g->conn, hconn.data(), 8 * (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK_BAIL(hipMemcpy(g->einfo, hinfo.data(), (size_t)n_elems * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK_BAIL(hipMemcpy(g->parts, g->h_parts.data(), n_parts * sizeof(PartDev), hipMemcpyHostToDevice));
+  HIPCHK_BAIL(hipMemcpy(g->ell, hell.data(), hell.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK_BAIL(hipMemset(g->u, 0, nd * sizeof(double)));
+  HIPCHK_BAIL(hipMemset(g->vs, 0, nd * sizeof(double)));
+  HIPCHK_BAIL(hipMemset(g->fi, 0, nd * sizeof(double)));
+  HIPCHK_BAIL(hipMemset(g->node_code, 0, (size_t)n_nodes * sizeof(int)));
+  HIPCHK_BAIL(hipMemset(g->node_mask, 0, (size_t)n_nodes));
+  HIPCHK_BAIL(hipMemset(g->status, 0, sizeof(int)));
+  if (g->hist) HIPCHK_BAIL(hipMemset(g->hist, 0, hist_len(n_elems) * esz));
   g->h_dof_motion.assign(nd, -1);
   // one instance covering the whole mesh
   g->n_inst = 1;
   g->inst_node = {0, n_nodes};
   g->inst_elem = {0, n_elems};
   if ((rc = alloc_instances(g)) || (rc = build_tiles(g)) || (rc = rebuild_boundary(g))) return bail(rc);
-  HIPCHK(hipDeviceSynchronize());
+  HIPCHK_BAIL(hipDeviceSynchronize());
   *out = g;
   return 0;
+#undef HIPCHK_BAIL
 }
 
 int ft_engine_destroy(ft_engine *g) {
@@ -3025,8 +3033,16 @@ int ft_set_instances(ft_engine *g, int n_inst, const int *node_offsets, const in
         if (n < node_offsets[i] || n >= node_offsets[i + 1])
           return fail(FT_ERR_ARG, "element %ld of instance %d uses node %d of another instance", e, i, n);
       }
+  // the step history and the injury criteria follow one instance's step
+  if (n_inst > 1 && g->hist_cap > 0)
+    return fail(FT_ERR_STATE, "the step history is a single-instance feature (ft_set_history 0 first)");
+  if (n_inst > 1 && g->inj_on)
+    return fail(FT_ERR_STATE, "the injury criteria are a single-instance feature (ft_injury_disable first)");
+  if (n_inst > 1 && g->surr_on)
+    return fail(FT_ERR_STATE, "the surrogate is a single-instance mode (ft_surrogate_enable 0 first)");
   HIPCHK(hipStreamSynchronize(g->stream));
   g->n_inst = n_inst;
+  g->tmax_per_instance = false;  // end times belong to the previous instances
   g->inst_node.assign(node_offsets, node_offsets + n_inst + 1);
   g->inst_elem.assign(elem_offsets, elem_offsets + n_inst + 1);
   g->run_active = false;

@@ -510,6 +510,38 @@ def test_batched_fp32_equals_separate_fp32_runs(gpu_ready):
         assert info[i][0] == ti[0] and int(info[i][2]) == ti[2], (i, info[i], ti)
 
 
+def test_set_instances_guards_single_instance_features(gpu_ready):
+    """ft_set_instances refuses several instances while the step history or
+    the injury criteria (single-instance features) are on, and forgets end
+    times declared for the previous instances: ft_run_init's end time then
+    applies to every instance."""
+    from femtech_amd import workloads as W
+    insts = [W.impact_block(n=3, slab=1, seed=21), W.impact_block(n=4, slab=1, seed=22)]
+    m, no, eo = W.concatenate(insts)
+    with ft.Engine(m["X"], m["conn"], m["pid"], m["material_id"], m["props"], ft.FT_FP64) as e:
+        e.set_history(10)
+        with pytest.raises(ft.FemTechError) as ei:
+            e.set_instances(no, eo)
+        assert ei.value.code == ft.FT_ERR_STATE
+        e.set_history(0)
+        e.injury_enable()
+        with pytest.raises(ft.FemTechError) as ei:
+            e.set_instances(no, eo)
+        assert ei.value.code == ft.FT_ERR_STATE
+        e.injury_disable()
+        e.set_instances(no, eo)
+        e.set_instance_end_times([1e-4, 2e-4])
+        e.set_instances(no, eo)  # new instances: the end times above no longer apply
+        e.set_boundary(m["dof_motion"], m["motions"])
+        e.set_state(None, m["v0"])
+        e.lumped_mass()
+        e.run_init(3e-4, 0.9, 1e-11)
+        e.run_steps(5000)
+        info = [e.instance(i) for i in range(2)]
+    for t, dt, step, state in (x[:4] for x in info):
+        assert state == 1.0 and 3e-4 <= t < 3e-4 + 2 * dt, info
+
+
 @pytest.mark.parametrize("cause", ["inverted", "nonfinite"])
 @pytest.mark.parametrize("precision", [ft.FT_FP64, ft.FT_FP32])
 def test_batch_instance_failure_is_isolated(gpu_ready, cause, precision):
