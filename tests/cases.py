@@ -245,3 +245,56 @@ def case_fan(steps=300):
     return dict(X=X, conn=conn, pid=pid, material_id=np.array([1, 5], np.int32), props=props.reshape(-1),
                 dof_motion=dm, motions=ft.motion(ft.MOTION_FIXED)[None, :], u0=None, v0=v0,
                 max_steps=steps, tmax=1e9, dt_factor=0.9, failure_dt=1e-11)
+
+
+def case_mesh_file(X, conn, pid, kind):
+    """Loading and materials for two of the reference's own unstructured
+    meshes (the fixture holds the arrays the reader made of them):
+    - "brain" (examples/ex5/brain.inp: 14,698 hexes in 10 parts, nodes of up
+      to 12 elements): soft tissue, the lowest 15 % of the height fixed, the
+      rest with an initial shearing velocity that grows with height;
+    - "spheres" (examples/ex14/sphere-in-sphere-hex.inp: 9,317 hexes in 3
+      parts, at most 8 elements per node): the nodes with x < -0.8 m fixed,
+      the rest with an initial sideways velocity growing with x.
+    The parts cycle through neo-Hookean, HGO, HGO + Prony, St Venant-
+    Kirchhoff and linear elastic laws with moduli varied per part."""
+    Xr = np.asarray(X).reshape(-1, 3)
+    nn = Xr.shape[0]
+    n_parts = int(np.max(pid)) + 1
+    cycle = [1, 4, 5, 2, 3]
+    mids, rows = [], []
+    if kind == "brain":
+        mu, lam, rho, k1, taus = 2.0e3, 2.0e4, 1040.0, 1.0e3, (2e-2, 2e-1)
+        lo, hi = Xr[:, 2].min(), Xr[:, 2].max()
+        s = (Xr[:, 2] - lo) / (hi - lo)
+        fixed = s < 0.15
+        v0 = np.zeros((nn, 3))
+        v0[:, 0] = 0.4 * s
+        v0[:, 2] = -0.3 * s
+        steps = 200
+    elif kind == "spheres":
+        mu, lam, rho, k1, taus = 1.0e4, 1.0e5, 1000.0, 5.0e3, (5e-2, 5e-1)
+        fixed = Xr[:, 0] < -0.8
+        v0 = np.zeros((nn, 3))
+        v0[:, 1] = 0.5 * (Xr[:, 0] + 1.0) / 2.0
+        steps = 200
+    else:
+        raise KeyError(kind)
+    for p in range(n_parts):
+        m = cycle[p % len(cycle)]
+        f = 1.0 + 0.1 * (p % 4)
+        if m in (1, 2, 3):
+            rows.append(props_row(rho, mu * f, lam * f))
+        elif m == 4:
+            rows.append(props_row(rho, mu * f, lam * f, k1, 5.0))
+        else:
+            rows.append(props_row(rho, mu * f, lam * f, k1, 5.0, 0.4, taus[0], 0.3, taus[1]))
+        mids.append(m)
+    dm = -np.ones(3 * nn, np.int32)
+    for d in range(3):
+        dm[3 * np.nonzero(fixed)[0] + d] = 0
+    v0[fixed] = 0.0
+    return dict(X=np.asarray(X, np.float64), conn=np.asarray(conn, np.int32), pid=np.asarray(pid, np.int32),
+                material_id=np.array(mids, np.int32), props=np.array(rows), dof_motion=dm,
+                motions=np.stack([ft.motion(ft.MOTION_FIXED)]), u0=None, v0=v0.reshape(-1), max_steps=steps,
+                tmax=1e9, dt_factor=0.8, failure_dt=1e-11, snaps=[])

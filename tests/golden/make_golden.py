@@ -121,6 +121,28 @@ def golden_ex7(mesh_file, materials_file):
     return res
 
 
+def golden_mesh_file(name, path, kind):
+    """A golden run on one of the reference's own unstructured meshes, kept
+    lean: inputs, mass, initial force, final state, dt and energy histories
+    and CheckEnergy's rows (no shape-function arrays or snapshots).  The
+    reader's arrays must equal the ones the reference read from the file
+    (their SHA-256 digests are kept)."""
+    import hashlib
+    X, conn, pid, gid = ft.read_mesh(path)
+    c = cases.case_mesh_file(X, conn, pid, kind)
+    inp, res = golden_run(name, c, mesh_file=path)
+    if not (np.array_equal(res["ref_coords"], X) and np.array_equal(res["ref_conn"], conn)):
+        raise SystemExit("%s: the reader's arrays differ from the reference's" % name)
+    keep = ("ref_mass", "ref_fi0", "ref_u", "ref_v", "ref_fi", "ref_time", "ref_dt", "ref_energy",
+            "ref_energy_file", "ref_steps")
+    out = {k: res[k] for k in keep}
+    out["ref_coords_sha256"] = np.array(hashlib.sha256(np.ascontiguousarray(res["ref_coords"]).tobytes()).hexdigest())
+    out["ref_conn_sha256"] = np.array(hashlib.sha256(np.ascontiguousarray(res["ref_conn"]).tobytes()).hexdigest())
+    inp.pop("snaps", None)
+    inp.pop("u0", None)
+    save("run_" + name, **inp, **out)
+
+
 def abaqus_rpt(path):
     rows = []
     with open(path) as f:
@@ -235,6 +257,10 @@ def main():
     res["abaqus_rpt"] = abaqus_rpt(os.path.join(ex7, "abaqus", "abaqus.rpt"))
     save("run_ex7", X=X, conn=conn, pid=pid, material_id=mat, props=props, tmax=np.float64(1.0),
          dmax=np.float64(0.001), length=np.float64(0.005), plots=np.int64(50), **res)
+
+    # two of the reference's own unstructured meshes
+    golden_mesh_file("brain", os.path.join(REF, "examples", "ex5", "brain.inp"), "brain")
+    golden_mesh_file("spheres", os.path.join(REF, "examples", "ex14", "sphere-in-sphere-hex.inp"), "spheres")
 
     inp, res = golden_run("c1", cases.case_c1())
     save("run_c1", **inp, **res)

@@ -1,0 +1,140 @@
+"""Two of the reference's own unstructured meshes, run by the reference build
+(tests/golden/run_brain.npz: examples/ex5/brain.inp, 14,698 hexes in 10
+parts, nodes of up to 12 elements; run_spheres.npz:
+examples/ex14/sphere-in-sphere-hex.inp, 9,317 hexes in 3 parts, at most 8
+elements per node), each part with one of the neo-Hookean, HGO, HGO + Prony,
+St Venant-Kirchhoff and linear elastic laws, part of the mesh fixed and the
+rest moving off with an initial velocity field, 200 steps of the ex9 loop
+(ex9.cpp:107-175).
+
+The fixtures hold the arrays this package's reader made of the files, and
+the SHA-256 digests of the arrays the reference made of them: equal digests
+mean the same nodes, numbering and connectivity (ReadAbaqus.cpp,
+PartitionMesh.cpp:485-536).
+
+CPU: the oracle against the reference (mass, initial force, the run).
+GPU: the FP64 engine against the reference (mass <= 1e-13, initial force
+<= 1e-11, u <= 1e-10, v and fi <= 1e-9, time and dt <= 1e-13 / 1e-11,
+energies <= 1e-9) on the layout creation picks (the ELL for the brain,
+node slots for the spheres) and on the ELL forced for the spheres; the FP32
+split within 1e-5 displacement and 1e-4 energy balance.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import femtech_amd as ft
+from conftest import golden, rel
+from oracle import oracle as O
+
+NAMES = ["brain", "spheres"]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_fixture_inputs_are_what_the_reference_read():
+    for name in NAMES:
+        z = golden("run_" + name)
+        assert sha(z["X"]) == str(z["ref_coords_sha256"]), name
+        assert sha(z["conn"]) == str(z["ref_conn_sha256"]), name
+
+
+def test_meshes_take_the_general_layouts():
+    """The brain has nodes of 12 elements (the ELL); no sphere node has
+    more than 8 elements (node slots); neither is corner-eligible."""
+    from cases import corner_eligible
+    for name, width in (("brain", 12), ("spheres", 8)):
+        z = golden("run_" + name)
+        assert int(np.bincount(z["conn"]).max()) == width
+        assert not corner_eligible(z["conn"], z["X"].size // 3)
+
+
+def oracle_of(z, threads=8):
+    o = O.Model(z["X"], z["conn"], z["pid"], z["material_id"], z["props"], threads=threads)
+    o.set_bc(z["dof_motion"], z["motions"])
+    o.set_initial(None, z["v0"])
+    return o
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_matches_reference(name):
+    z = golden("run_" + name)
+    o = oracle_of(z)
+    assert rel(o.lumped_mass(), z["ref_mass"]) < 1e-14
+    # the initial force (u = 0, the first step's dt for the Prony parts) is
+    # round-off: compared on the scale of the run's forces
+    f0 = o.get_force(np.zeros(z["X"].size), float(z["ref_dt"][0]))
+    assert np.abs(f0 - z["ref_fi0"]).max() <= 1e-12 * np.abs(z["ref_fi"]).max()
+    r = o.run(int(z["max_steps"]), float(z["tmax"]), float(z["dt_factor"]), float(z["failure_dt"]))
+    assert r["steps"] == int(z["ref_steps"])
+    assert rel(r["u"], z["ref_u"]) < 1e-12
+    assert rel(r["v"], z["ref_v"]) < 1e-10
+    assert rel(r["fi"], z["ref_fi"]) < 1e-10
+    assert rel(r["time"], z["ref_time"]) < 1e-14
+    assert rel(r["dt"], z["ref_dt"]) < 1e-13
+    assert rel(r["energy"], z["ref_energy"]) < 1e-12
+
+
+def engine_for(z, precision, layout=ft.LAYOUT_AUTO):
+    e = ft.Engine(z["X"], z["conn"], z["pid"], z["material_id"], z["props"], precision)
+    if layout != ft.LAYOUT_AUTO:
+        e.set_layout(layout)
+    e.set_boundary(z["dof_motion"], z["motions"])
+    return e
+
+
+def run(z, precision, layout=ft.LAYOUT_AUTO):
+    nsteps = int(z["ref_steps"])
+    with engine_for(z, precision, layout) as e:
+        e.set_state(None, z["v0"])
+        mass = e.lumped_mass()
+        e.set_history(nsteps + 4)
+        e.run_init(float(z["tmax"]), float(z["dt_factor"]), float(z["failure_dt"]))
+        done = e.run_steps(nsteps)
+        return dict(done=done, mass=mass, state=e.state(), time=e.time(), history=e.history(),
+                    layout=e.layout(), status=e.status())
+
+
+CASES = [("brain", ft.LAYOUT_AUTO), ("spheres", ft.LAYOUT_AUTO), ("spheres", ft.LAYOUT_ELL)]
+IDS = ["brain-auto", "spheres-auto", "spheres-ell"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,layout", CASES, ids=IDS)
+def test_engine_fp64_matches_reference(gpu_ready, name, layout):
+    z = golden("run_" + name)
+    print("reference mesh:", name, layout, flush=True)
+    with engine_for(z, ft.FT_FP64, layout) as e:
+        assert rel(e.lumped_mass(), z["ref_mass"]) < 1e-13
+        f0 = e.get_force(np.zeros(z["X"].size), dt=float(z["ref_dt"][0]))
+        assert np.abs(f0 - z["ref_fi0"]).max() <= 1e-11 * np.abs(z["ref_fi"]).max()
+    g = run(z, ft.FT_FP64, layout)
+    want = {"brain": ft.LAYOUT_ELL, "spheres": ft.LAYOUT_NODE_SLOTS}[name] if layout == ft.LAYOUT_AUTO else layout
+    assert g["layout"][0] == want
+    assert g["done"] == int(z["ref_steps"])
+    u, v, a, fi = g["state"]
+    th, dth, en = g["history"]
+    assert rel(u, z["ref_u"]) < 1e-10
+    assert rel(v, z["ref_v"]) < 1e-9
+    assert rel(fi, z["ref_fi"]) < 1e-9
+    assert g["time"][0] == pytest.approx(float(z["ref_time"][-1]), rel=1e-13)
+    assert rel(th, z["ref_time"][1:]) < 1e-13
+    assert rel(dth, z["ref_dt"][1:]) < 1e-11
+    assert rel(en[:, :3], z["ref_energy"][1:, :3]) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_engine_fp32_split_within_tolerance(gpu_ready, name):
+    z = golden("run_" + name)
+    g = run(z, ft.FT_FP32)
+    assert g["done"] == int(z["ref_steps"]) and g["status"] == 0
+    th, dth, en = g["history"]
+    assert rel(g["state"][0], z["ref_u"]) < 1e-5
+    ref = z["ref_energy"][1:]
+    scale = np.abs(ref[:, :3]).max()
+    bal = en[:, 2] + en[:, 0] - en[:, 1]
+    assert np.abs(bal - (ref[:, 2] + ref[:, 0] - ref[:, 1])).max() / scale < 1e-4
