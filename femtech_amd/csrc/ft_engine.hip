@@ -316,7 +316,10 @@ struct ElemArgs {
   const PartDev *parts;
   R *fe;                 // element forces (nullable when only dt is wanted)
   int corner;            // fe[slot][node][xyz] instead of fe[slot][element][xyz]
-  const unsigned *eslot; // node-major layout: slot of local node k in bits 4k..4k+2 (nullptr: slot k)
+  const unsigned *eslot; // node slots: rank of the element at local node k in bits 4k..4k+3 (nullptr: slot k)
+  R *fe_ovf;             // node slots: forces of ranks 8-15, [rank - 8][overflow node][xyz]
+  const int *ovf;        // per node: (overflow node index << 4) | ranks beyond 8, 0 for none
+  int n_ovf;             // overflow nodes
   R *hist;               // 144 per element, in 64-element blocks (hist_index)
   const R *fib;          // fibre directions [3][ne] (material 8)
   Ctl *ctl;              // run loop: per-instance control, slot `parity` gives dt_k and done
@@ -490,11 +493,15 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       // the corner layout: neighbouring elements of a block have
       // neighbouring corner-k nodes), and so does each load of the node sum.
       if (a.eslot) {
-        // node slots: the slot varies per lane (the element's rank at node k)
+        // node slots: the slot varies per lane (the element's rank at node
+        // k); ranks 8-15 of the few nodes with more than 8 elements go to the
+        // overflow slots
         const unsigned sl = a.eslot[e];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-          R *dst = a.fe + 3 * ((size_t)((sl >> (4 * k)) & 7u) * a.nn + (size_t)nd[k]);
+          const unsigned r = (sl >> (4 * k)) & 15u;
+          R *dst = r < 8u ? a.fe + 3 * ((size_t)r * a.nn + (size_t)nd[k])
+                          : a.fe_ovf + 3 * ((size_t)(r - 8u) * a.n_ovf + (size_t)(a.ovf[nd[k]] >> 4));
           dst[0] = outv[3 * k];
           dst[1] = outv[3 * k + 1];
           dst[2] = outv[3 * k + 2];
@@ -727,8 +734,18 @@ __global__ __launch_bounds__(kBlock) void k_surr_features(int nn, int ne, const 
 // element order, so the sum below adds the element forces in the order of the
 // reference's scatter-add (GetForce_3D.cpp:39-44).  W is the ELL width when
 // it is the common 8 (fully unrolled: 8 independent loads in flight).
+// Node slots beyond 8 (nodes with more than 8 elements): their forces, in
+// rank order after the first 8, so the sum keeps the element order.
+template <typename R>
+struct Overflow {
+  const R *fe;     // [rank - 8][overflow node][xyz]
+  const int *idx;  // per node: (overflow node index << 4) | ranks beyond 8; nullptr: none in the mesh
+  int n;           // overflow nodes
+};
+
 template <typename R, int W>
-__device__ inline void gather3(const int *ell, int nn, int width, const R *fe, int n, double f[3]) {
+__device__ inline void gather3(const int *ell, int nn, int width, const R *fe, int n, double f[3],
+                               const Overflow<R> &ov = Overflow<R>{nullptr, nullptr, 0}) {
   f[0] = f[1] = f[2] = 0.0;
   if (W < 0) {
     // Corner-slot layout fe[corner][node][xyz]: the element that has node n
@@ -747,6 +764,15 @@ __device__ inline void gather3(const int *ell, int nn, int width, const R *fe, i
       f[0] += (double)v[k][0];
       f[1] += (double)v[k][1];
       f[2] += (double)v[k][2];
+    }
+    if (ov.idx) {  // uniform: the mesh has nodes with more than 8 elements
+      const int o = ov.idx[n];
+      for (int r = 0; r < (o & 15); ++r) {
+        const R *src = ov.fe + 3 * ((size_t)r * ov.n + (size_t)(o >> 4));
+        f[0] += (double)src[0];
+        f[1] += (double)src[1];
+        f[2] += (double)src[2];
+      }
     }
   } else if (W > 0) {
     constexpr int WA = W > 0 ? W : 1;
@@ -782,11 +808,11 @@ __device__ inline void gather3(const int *ell, int nn, int width, const R *fe, i
 
 template <typename R, int W>
 __global__ __launch_bounds__(kBlock) void k_gather(int nn, const int *ell, int width, const R *fe,
-                                                   double *fi) {
+                                                   double *fi, Overflow<R> ov) {
   const int n = xcd_block() * kBlock + threadIdx.x;
   if (n >= nn) return;
   double f[3];
-  gather3<R, W>(ell, nn, width, fe, n, f);
+  gather3<R, W>(ell, nn, width, fe, n, f, ov);
   fi[n] = f[0];
   fi[(size_t)nn + n] = f[1];
   fi[2 * (size_t)nn + n] = f[2];
@@ -800,6 +826,7 @@ struct NodalArgs {
   const int *ell;
   int width;
   const R *fe;
+  Overflow<R> ov;         // node slots beyond 8
   const double *X;
   double *u, *vs, *fi;
   const double *mass;  // 1 per node
@@ -977,7 +1004,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
       f[1] = a.fi[(size_t)a.nn + n];
       f[2] = a.fi[2 * (size_t)a.nn + n];
     } else {
-      gather3<R, W>(a.ell, a.nn, a.width, a.fe, n, f);
+      gather3<R, W>(a.ell, a.nn, a.width, a.fe, n, f, a.ov);
     }
     const double m = a.mass[n];
     const int code = a.node_code[n];  // byte d: 0 free, else motion index + 1
@@ -1932,8 +1959,12 @@ struct ft_engine {
   // in element order, eslot) when every node has at most 8 elements.
   // Otherwise fe[slot][element][xyz] + the ELL.
   bool corner = false;
-  unsigned *eslot = nullptr;        // node slots: per internal element, 3 bits per local node
+  unsigned *eslot = nullptr;        // node slots: per internal element, 4 bits per local node (rank)
+  int *ovf = nullptr;               // node slots: per node (overflow index << 4) | ranks beyond 8
+  void *fe_ovf = nullptr;           // node slots: forces of ranks 8-15
+  int n_ovf = 0;                    // nodes with more than 8 elements
   std::vector<unsigned> h_eslot;    // the same in the caller's element order
+  std::vector<int> h_ovf;           // host copy of ovf
   int layout = 0;                   // FT_LAYOUT_* in use
   std::vector<int> h_conn;  // caller's connectivity (element-force read-back)
   PartDev *parts = nullptr;
@@ -2155,6 +2186,9 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.fe = reinterpret_cast<R *>(g->fe);
   a.corner = g->corner ? 1 : 0;
   a.eslot = g->eslot;
+  a.fe_ovf = reinterpret_cast<R *>(g->fe_ovf);
+  a.ovf = g->ovf;
+  a.n_ovf = g->n_ovf;
   a.hist = reinterpret_cast<R *>(g->hist);
   a.fib = reinterpret_cast<const R *>(g->fib);
   a.ctl = in_loop ? g->ctl : nullptr;
@@ -2364,6 +2398,7 @@ NodalArgs<R> nodal_args(ft_engine *g, int mode, int t0, int t1) {
   a.ell = g->ell;
   a.width = g->width;
   a.fe = reinterpret_cast<const R *>(g->fe);
+  a.ov = Overflow<R>{reinterpret_cast<const R *>(g->fe_ovf), g->ovf, g->n_ovf};
   a.X = g->X;
   a.u = g->u;
   a.vs = g->vs;
@@ -2761,19 +2796,23 @@ static int set_layout(ft_engine *g, int mode) {
       c = 1;
     }
   }
-  const bool slots_ok = g->width == 8;
+  const bool slots_ok = g->width <= 16;  // ranks 8-15 in overflow slots
   if (mode == FT_LAYOUT_AUTO) mode = corner_ok ? FT_LAYOUT_CORNER : slots_ok ? FT_LAYOUT_NODE_SLOTS : FT_LAYOUT_ELL;
   if (mode == FT_LAYOUT_CORNER && !corner_ok)
     return fail(FT_ERR_ARG, "corner-slot layout: a node is the same local corner of two elements");
   if (mode == FT_LAYOUT_NODE_SLOTS && !slots_ok)
-    return fail(FT_ERR_ARG, "node-slot layout: a node has %d elements (at most 8)", g->width);
+    return fail(FT_ERR_ARG, "node-slot layout: a node has %d elements (at most 16)", g->width);
   if (mode != FT_LAYOUT_CORNER && mode != FT_LAYOUT_NODE_SLOTS && mode != FT_LAYOUT_ELL)
     return fail(FT_ERR_ARG, "unknown layout %d", mode);
-  if (g->fe) (void)hipFree(g->fe);
-  if (g->eslot) (void)hipFree(g->eslot);
+  for (void *p : {(void *)g->fe, (void *)g->eslot, (void *)g->ovf, g->fe_ovf})
+    if (p) (void)hipFree(p);
   g->fe = nullptr;
   g->eslot = nullptr;
+  g->ovf = nullptr;
+  g->fe_ovf = nullptr;
+  g->n_ovf = 0;
   g->h_eslot.clear();
+  g->h_ovf.clear();
   g->corner = mode != FT_LAYOUT_ELL;
   g->layout = mode;
   const size_t esz = g->precision == FT_FP32 ? 4 : 8;
@@ -2791,6 +2830,17 @@ static int set_layout(ft_engine *g, int mode) {
     int rc = dalloc(&g->eslot, (size_t)ne);
     if (rc) return rc;
     HIPCHK(hipMemcpy(g->eslot, dev.data(), ne * sizeof(unsigned), hipMemcpyHostToDevice));
+    if (g->width > 8) {
+      // nodes with more than 8 elements: ranks 8.. in [rank - 8][overflow node][xyz]
+      g->h_ovf.assign(nn, 0);
+      for (int n = 0; n < nn; ++n)
+        if (rank[n] > 8) g->h_ovf[n] = (g->n_ovf++ << 4) | (rank[n] - 8);
+      if ((rc = dalloc(&g->ovf, (size_t)nn))) return rc;
+      HIPCHK(hipMemcpy(g->ovf, g->h_ovf.data(), nn * sizeof(int), hipMemcpyHostToDevice));
+      const size_t len = 3 * (size_t)(g->width - 8) * g->n_ovf * esz;
+      if (hipMalloc(&g->fe_ovf, len) != hipSuccess) return fail(FT_ERR_ALLOC, "overflow slot allocation failed");
+      HIPCHK(hipMemset(g->fe_ovf, 0, len));
+    }
   }
   return 0;
 }
@@ -2963,7 +3013,7 @@ int ft_engine_destroy(ft_engine *g) {
     if (p) (void)hipFree(p);
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
-                  g->ell, g->fe, g->eslot, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
+                  g->ell, g->fe, g->eslot, g->ovf, g->fe_ovf, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
                   g->etiles, g->cand_geom, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
@@ -3146,24 +3196,30 @@ static int gather_forces(ft_engine *g) {
     if (g->corner) {
       if (g->precision == FT_FP32)
         hipLaunchKernelGGL((k_gather<float, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const float *)g->fe, g->fi);
+                           g->ell, g->width, (const float *)g->fe, g->fi,
+                           Overflow<float>{(const float *)g->fe_ovf, g->ovf, g->n_ovf});
       else
         hipLaunchKernelGGL((k_gather<double, -1>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const double *)g->fe, g->fi);
+                           g->ell, g->width, (const double *)g->fe, g->fi,
+                           Overflow<double>{(const double *)g->fe_ovf, g->ovf, g->n_ovf});
     } else if (g->precision == FT_FP32) {
       if (w8)
         hipLaunchKernelGGL((k_gather<float, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const float *)g->fe, g->fi);
+                           g->ell, g->width, (const float *)g->fe, g->fi,
+                           Overflow<float>{(const float *)g->fe_ovf, g->ovf, g->n_ovf});
       else
         hipLaunchKernelGGL((k_gather<float, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const float *)g->fe, g->fi);
+                           g->ell, g->width, (const float *)g->fe, g->fi,
+                           Overflow<float>{(const float *)g->fe_ovf, g->ovf, g->n_ovf});
     } else {
       if (w8)
         hipLaunchKernelGGL((k_gather<double, 8>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const double *)g->fe, g->fi);
+                           g->ell, g->width, (const double *)g->fe, g->fi,
+                           Overflow<double>{(const double *)g->fe_ovf, g->ovf, g->n_ovf});
       else
         hipLaunchKernelGGL((k_gather<double, 0>), dim3(g->nodal_grid), dim3(kBlock), 0, g->stream, g->nn,
-                           g->ell, g->width, (const double *)g->fe, g->fi);
+                           g->ell, g->width, (const double *)g->fe, g->fi,
+                           Overflow<double>{(const double *)g->fe_ovf, g->ovf, g->n_ovf});
     }
     HIPCHK(hipGetLastError());
   }
@@ -3185,12 +3241,26 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
     } else {
       HIPCHK(hipMemcpy(dev.data(), g->fe, n * sizeof(double), hipMemcpyDeviceToHost));
     }
+    std::vector<double> ovf(3 * (size_t)std::max(g->width - 8, 0) * g->n_ovf);
+    if (!ovf.empty()) {
+      if (g->precision == FT_FP32) {
+        std::vector<float> tmp(ovf.size());
+        HIPCHK(hipMemcpy(tmp.data(), g->fe_ovf, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < ovf.size(); ++i) ovf[i] = tmp[i];
+      } else {
+        HIPCHK(hipMemcpy(ovf.data(), g->fe_ovf, ovf.size() * sizeof(double), hipMemcpyDeviceToHost));
+      }
+    }
     for (int e = 0; e < g->ne; ++e)
       for (int k = 0; k < 8; ++k) {
-        const size_t sk = g->h_eslot.empty() ? (size_t)k : (g->h_eslot[e] >> (4 * k)) & 7u;
-        const size_t slot = g->corner ? sk * g->nn + g->h_conn[8 * (size_t)e + k]
-                                      : (size_t)k * g->ne + g->inv[e];
-        for (int d = 0; d < 3; ++d) fe_out[24 * (size_t)e + 3 * k + d] = dev[3 * slot + d];
+        const int node = g->h_conn[8 * (size_t)e + k];
+        const size_t sk = g->h_eslot.empty() ? (size_t)k : (g->h_eslot[e] >> (4 * k)) & 15u;
+        const double *src;
+        if (sk >= 8)  // overflow slot of a node with more than 8 elements
+          src = &ovf[3 * ((sk - 8) * g->n_ovf + (size_t)(g->h_ovf[node] >> 4))];
+        else
+          src = &dev[3 * (g->corner ? sk * g->nn + node : (size_t)k * g->ne + g->inv[e])];
+        for (int d = 0; d < 3; ++d) fe_out[24 * (size_t)e + 3 * k + d] = src[d];
       }
   }
   return 0;
@@ -4269,10 +4339,12 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
     if (sg.mat == 8) elem += n * 3.0 * es;                         // fibre direction
   }
   // nodal kernel: element forces, 8 slots of 3 per node (corner and node
-  // slots) or 24 es per element + the ELL incidences 4 width per node; per
+  // slots; with overflow slots their index per node and the extra ranks of
+  // the nodes with more than 8 elements) or 24 es per element + the ELL incidences 4 width per node; per
   // node mass 8, motion code 4, u and v_half read + write 2 x 48 (X only at
   // constrained nodes and fi only when a run stops: not counted).
   double nodal = (g->corner ? nn * (24.0 * es) : ne * (24.0 * es) + nn * 4.0 * g->width) +
+                 (g->ovf ? nn * 4.0 + g->n_ovf * 3.0 * es * (g->width - 8) : 0.0) +
                  nn * (8.0 + 4.0 + 2.0 * 48.0);
   bytes[0] = elem;
   bytes[1] = nodal;

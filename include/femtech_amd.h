@@ -88,9 +88,10 @@ int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
 /* Force-gather layout (no reference counterpart; the reference scatter-adds,
  * GetForce_3D.cpp:39-44).  FT_LAYOUT_CORNER: every node is local corner k of
  * at most one element for each k, so element forces go to per-node corner
- * slots; FT_LAYOUT_NODE_SLOTS: every node has at most 8 elements, and each
+ * slots; FT_LAYOUT_NODE_SLOTS: every node has at most 16 elements, and each
  * element writes its corner forces to per-node slots numbered by its rank
- * among the node's elements (element order); FT_LAYOUT_ELL: element-major
+ * among the node's elements (element order; ranks 8-15 of the nodes with
+ * more than 8 elements in overflow slots); FT_LAYOUT_ELL: element-major
  * forces summed through the node-to-element table.  Creation picks the first
  * that applies (FT_LAYOUT_AUTO); ft_engine_set_layout chooses another that
  * applies, before ft_run_init (FT_ERR_ARG when it does not apply,

@@ -3,10 +3,11 @@
 The engine writes element forces to per-node corner slots only when every
 node is local corner k of at most one element for each k (every structured
 block and mapped ellipsoid qualifies).  Any other mesh whose nodes have at
-most 8 elements takes per-node slots numbered by the element's rank among
-the node's elements; the rest take slot-major element forces summed per node
-through the node-to-element table, with the 8-wide table when no node has
-more than 8 elements and the variable-width loop otherwise.  Both general
+most 16 elements takes per-node slots numbered by the element's rank among
+the node's elements (ranks 8-15 in overflow slots); the rest, or any mesh on
+request, take slot-major element forces summed per node through the
+node-to-element table, with the 8-wide table when no node has more than 8
+elements and the variable-width loop otherwise.  Both general
 paths sum in the reference's scatter-add order (GetForce_3D.cpp:39-44), so
 they agree bitwise.  These tests build such meshes by renumbering each element's
 local nodes with a random proper rotation of the reference cube (the same
@@ -123,9 +124,10 @@ def test_oracle_is_invariant_under_renumbering(name):
 
 # ---- GPU: engine against the oracle on the same meshes --------------------
 
-# (case, layout): the node-slot layout where it applies (creation's choice)
-# and the ELL forced on the same mesh; the fan's axis nodes have 12 elements
-LAYOUT_CASES = [(n, ft.LAYOUT_AUTO) for n in CASES] + [(n, ft.LAYOUT_ELL) for n in CASES if not n.startswith("fan")]
+# (case, layout): the node-slot layout (creation's choice; the fan's axis
+# nodes have 12 elements, so it takes overflow slots) and the ELL forced on
+# the same mesh
+LAYOUT_CASES = [(n, ft.LAYOUT_AUTO) for n in CASES] + [(n, ft.LAYOUT_ELL) for n in CASES]
 LAYOUT_IDS = ["%s-%s" % (n, "auto" if lay == ft.LAYOUT_AUTO else "ell") for n, lay in LAYOUT_CASES]
 
 
@@ -137,18 +139,14 @@ def test_general_layout_chosen(gpu_ready, name):
     fan = name.startswith("fan")
     assert (width > 8) == fan
     with engine_for(c, ft.FT_FP64) as e:
-        assert e.layout() == ((ft.LAYOUT_ELL if fan else ft.LAYOUT_NODE_SLOTS), max(8, width))
+        assert e.layout() == (ft.LAYOUT_NODE_SLOTS, max(8, width))
         with pytest.raises(ft.FemTechError) as ei:
             e.set_layout(ft.LAYOUT_CORNER)
         assert ei.value.code == ft.FT_ERR_ARG
-        if fan:
-            with pytest.raises(ft.FemTechError):
-                e.set_layout(ft.LAYOUT_NODE_SLOTS)
-        else:
-            e.set_layout(ft.LAYOUT_ELL)
-            assert e.layout() == (ft.LAYOUT_ELL, 8)
-            e.set_layout(ft.LAYOUT_AUTO)
-            assert e.layout() == (ft.LAYOUT_NODE_SLOTS, 8)
+        e.set_layout(ft.LAYOUT_ELL)
+        assert e.layout() == (ft.LAYOUT_ELL, max(8, width))
+        e.set_layout(ft.LAYOUT_AUTO)
+        assert e.layout() == (ft.LAYOUT_NODE_SLOTS, max(8, width))
 
 
 @pytest.mark.gpu
@@ -226,7 +224,8 @@ def test_general_run_fp32_split(gpu_ready, name, layout):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,layout", [("impact", ft.LAYOUT_AUTO), ("impact", ft.LAYOUT_ELL),
-                                         ("fan", ft.LAYOUT_AUTO)], ids=["impact-auto", "impact-ell", "fan-auto"])
+                                         ("fan", ft.LAYOUT_AUTO), ("fan", ft.LAYOUT_ELL)],
+                         ids=["impact-auto", "impact-ell", "fan-auto", "fan-ell"])
 @pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
 def test_general_graph_and_persistent_equal_launches(gpu_ready, name, layout, precision):
     """Captured chunks equal the per-step launches bitwise; the persistent
@@ -247,11 +246,12 @@ def test_general_graph_and_persistent_equal_launches(gpu_ready, name, layout, pr
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["impact", "materials", "ellipsoid"])
+@pytest.mark.parametrize("name", ["impact", "materials", "ellipsoid", "fan", "fan_permuted"])
 @pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
 def test_node_slots_equal_ell_bitwise(gpu_ready, name, precision):
-    """Node slots and the ELL add the same element forces in the same order
-    (element order per node): identical runs, bit for bit."""
+    """Node slots (with overflow slots for the fan's 12-element nodes) and
+    the ELL add the same element forces in the same order (element order per
+    node): identical runs, bit for bit."""
     c = general_case(name)
     a = run_engine(c, precision, history=False)
     b = run_engine(c, precision, history=False, layout=ft.LAYOUT_ELL)

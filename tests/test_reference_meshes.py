@@ -15,9 +15,9 @@ PartitionMesh.cpp:485-536).
 CPU: the oracle against the reference (mass, initial force, the run).
 GPU: the FP64 engine against the reference (mass <= 1e-13, initial force
 <= 1e-11, u <= 1e-10, v and fi <= 1e-9, time and dt <= 1e-13 / 1e-11,
-energies <= 1e-9) on the layout creation picks (the ELL for the brain,
-node slots for the spheres) and on the ELL forced for the spheres; the FP32
-split within 1e-5 displacement and 1e-4 energy balance.
+energies <= 1e-9) on the layout creation picks (node slots, with overflow
+slots for the brain's nodes of more than 8 elements) and on the ELL forced;
+the FP32 split within 1e-5 displacement and 1e-4 energy balance.
 """
 import hashlib
 
@@ -43,8 +43,8 @@ def test_fixture_inputs_are_what_the_reference_read():
 
 
 def test_meshes_take_the_general_layouts():
-    """The brain has nodes of 12 elements (the ELL); no sphere node has
-    more than 8 elements (node slots); neither is corner-eligible."""
+    """The brain has nodes of 12 elements (node slots with overflow slots);
+    no sphere node has more than 8 elements; neither is corner-eligible."""
     from cases import corner_eligible
     for name, width in (("brain", 12), ("spheres", 8)):
         z = golden("run_" + name)
@@ -98,8 +98,9 @@ def run(z, precision, layout=ft.LAYOUT_AUTO):
                     layout=e.layout(), status=e.status())
 
 
-CASES = [("brain", ft.LAYOUT_AUTO), ("spheres", ft.LAYOUT_AUTO), ("spheres", ft.LAYOUT_ELL)]
-IDS = ["brain-auto", "spheres-auto", "spheres-ell"]
+CASES = [("brain", ft.LAYOUT_AUTO), ("brain", ft.LAYOUT_ELL), ("spheres", ft.LAYOUT_AUTO),
+         ("spheres", ft.LAYOUT_ELL)]
+IDS = ["brain-auto", "brain-ell", "spheres-auto", "spheres-ell"]
 
 
 @pytest.mark.gpu
@@ -112,7 +113,7 @@ def test_engine_fp64_matches_reference(gpu_ready, name, layout):
         f0 = e.get_force(np.zeros(z["X"].size), dt=float(z["ref_dt"][0]))
         assert np.abs(f0 - z["ref_fi0"]).max() <= 1e-11 * np.abs(z["ref_fi"]).max()
     g = run(z, ft.FT_FP64, layout)
-    want = {"brain": ft.LAYOUT_ELL, "spheres": ft.LAYOUT_NODE_SLOTS}[name] if layout == ft.LAYOUT_AUTO else layout
+    want = ft.LAYOUT_NODE_SLOTS if layout == ft.LAYOUT_AUTO else layout
     assert g["layout"][0] == want
     assert g["done"] == int(z["ref_steps"])
     u, v, a, fi = g["state"]
@@ -124,6 +125,19 @@ def test_engine_fp64_matches_reference(gpu_ready, name, layout):
     assert rel(th, z["ref_time"][1:]) < 1e-13
     assert rel(dth, z["ref_dt"][1:]) < 1e-11
     assert rel(en[:, :3], z["ref_energy"][1:, :3]) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
+def test_node_slots_equal_ell_bitwise(gpu_ready, name, precision):
+    """Node slots (overflow slots included) and the ELL give identical runs."""
+    z = golden("run_" + name)
+    a = run(z, precision)
+    b = run(z, precision, ft.LAYOUT_ELL)
+    for k in range(4):
+        assert np.array_equal(a["state"][k], b["state"][k]), k
+    assert np.array_equal(a["history"][2], b["history"][2])
 
 
 @pytest.mark.gpu
