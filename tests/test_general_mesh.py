@@ -150,6 +150,25 @@ def test_general_layout_chosen(gpu_ready, name):
 
 
 @pytest.mark.gpu
+def test_layout_is_chosen_before_the_run(gpu_ready):
+    """ft_engine_set_layout inside a run is a call-order error and keeps the
+    layout; runs on either layout are the same, bit for bit."""
+    c = general_case("impact")
+    with engine_for(c, ft.FT_FP64) as e:
+        e.set_state(None, c["v0"])
+        e.lumped_mass()
+        e.run_init(c["tmax"], c["dt_factor"], c["failure_dt"])
+        e.run_steps(5)
+        with pytest.raises(ft.FemTechError) as ei:
+            e.set_layout(ft.LAYOUT_ELL)
+        assert ei.value.code == ft.FT_ERR_STATE
+        assert e.layout()[0] == ft.LAYOUT_NODE_SLOTS
+    a = run_engine(c, ft.FT_FP64, history=False, chunks=[30])
+    b = run_engine(c, ft.FT_FP64, history=False, chunks=[30], layout=ft.LAYOUT_ELL)
+    assert np.array_equal(a["state"][0], b["state"][0])
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name,layout", LAYOUT_CASES, ids=LAYOUT_IDS)
 def test_general_forces_mass_dt_fp64(gpu_ready, name, layout):
     c = general_case(name)
