@@ -2804,44 +2804,57 @@ static int set_layout(ft_engine *g, int mode) {
     return fail(FT_ERR_ARG, "node-slot layout: a node has %d elements (at most 16)", g->width);
   if (mode != FT_LAYOUT_CORNER && mode != FT_LAYOUT_NODE_SLOTS && mode != FT_LAYOUT_ELL)
     return fail(FT_ERR_ARG, "unknown layout %d", mode);
-  for (void *p : {(void *)g->fe, (void *)g->eslot, (void *)g->ovf, g->fe_ovf})
-    if (p) (void)hipFree(p);
-  g->fe = nullptr;
-  g->eslot = nullptr;
-  g->ovf = nullptr;
-  g->fe_ovf = nullptr;
-  g->n_ovf = 0;
-  g->h_eslot.clear();
-  g->h_ovf.clear();
-  g->corner = mode != FT_LAYOUT_ELL;
-  g->layout = mode;
+  // The new buffers are built first; the engine keeps its old layout if
+  // any step fails.
+  const bool corner = mode != FT_LAYOUT_ELL;
   const size_t esz = g->precision == FT_FP32 ? 4 : 8;
-  const size_t fe_len = 24 * (size_t)(g->corner ? nn : ne);
-  if (hipMalloc(&g->fe, fe_len * esz) != hipSuccess)
-    return fail(FT_ERR_ALLOC, "element force buffer allocation failed");
-  HIPCHK(hipMemset(g->fe, 0, fe_len * esz));
+  const size_t fe_len = 24 * (size_t)(corner ? nn : ne);
+  void *fe = nullptr, *fe_ovf = nullptr;
+  unsigned *eslot = nullptr;
+  int *ovf = nullptr, n_ovf = 0;
+  std::vector<unsigned> h_eslot;
+  std::vector<int> h_ovf;
+  auto undo = [&](int rc) {
+    for (void *p : {fe, (void *)eslot, (void *)ovf, fe_ovf})
+      if (p) (void)hipFree(p);
+    return rc;
+  };
+  if (hipMalloc(&fe, fe_len * esz) != hipSuccess || hipMemset(fe, 0, fe_len * esz) != hipSuccess)
+    return undo(fail(FT_ERR_ALLOC, "element force buffer allocation failed"));
   if (mode == FT_LAYOUT_NODE_SLOTS) {
-    g->h_eslot.assign(ne, 0u);
+    h_eslot.assign(ne, 0u);
     std::vector<int> rank(nn, 0);
     for (int e = 0; e < ne; ++e)
-      for (int k = 0; k < 8; ++k) g->h_eslot[e] |= (unsigned)rank[conn[8 * (size_t)e + k]]++ << (4 * k);
+      for (int k = 0; k < 8; ++k) h_eslot[e] |= (unsigned)rank[conn[8 * (size_t)e + k]]++ << (4 * k);
     std::vector<unsigned> dev(ne);
-    for (int i = 0; i < ne; ++i) dev[i] = g->h_eslot[g->perm[i]];
-    int rc = dalloc(&g->eslot, (size_t)ne);
-    if (rc) return rc;
-    HIPCHK(hipMemcpy(g->eslot, dev.data(), ne * sizeof(unsigned), hipMemcpyHostToDevice));
+    for (int i = 0; i < ne; ++i) dev[i] = h_eslot[g->perm[i]];
+    int rc = dalloc(&eslot, (size_t)ne);
+    if (rc) return undo(rc);
+    if (hipMemcpy(eslot, dev.data(), ne * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess)
+      return undo(fail(FT_ERR_HIP, "node-slot upload failed"));
     if (g->width > 8) {
       // nodes with more than 8 elements: ranks 8.. in [rank - 8][overflow node][xyz]
-      g->h_ovf.assign(nn, 0);
+      h_ovf.assign(nn, 0);
       for (int n = 0; n < nn; ++n)
-        if (rank[n] > 8) g->h_ovf[n] = (g->n_ovf++ << 4) | (rank[n] - 8);
-      if ((rc = dalloc(&g->ovf, (size_t)nn))) return rc;
-      HIPCHK(hipMemcpy(g->ovf, g->h_ovf.data(), nn * sizeof(int), hipMemcpyHostToDevice));
-      const size_t len = 3 * (size_t)(g->width - 8) * g->n_ovf * esz;
-      if (hipMalloc(&g->fe_ovf, len) != hipSuccess) return fail(FT_ERR_ALLOC, "overflow slot allocation failed");
-      HIPCHK(hipMemset(g->fe_ovf, 0, len));
+        if (rank[n] > 8) h_ovf[n] = (n_ovf++ << 4) | (rank[n] - 8);
+      if ((rc = dalloc(&ovf, (size_t)nn))) return undo(rc);
+      const size_t len = 3 * (size_t)(g->width - 8) * n_ovf * esz;
+      if (hipMemcpy(ovf, h_ovf.data(), nn * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMalloc(&fe_ovf, len) != hipSuccess || hipMemset(fe_ovf, 0, len) != hipSuccess)
+        return undo(fail(FT_ERR_ALLOC, "overflow slot set-up failed"));
     }
   }
+  for (void *p : {g->fe, (void *)g->eslot, (void *)g->ovf, g->fe_ovf})
+    if (p) (void)hipFree(p);
+  g->fe = fe;
+  g->eslot = eslot;
+  g->ovf = ovf;
+  g->fe_ovf = fe_ovf;
+  g->n_ovf = n_ovf;
+  g->h_eslot.swap(h_eslot);
+  g->h_ovf.swap(h_ovf);
+  g->corner = corner;
+  g->layout = mode;
   return 0;
 }
 
