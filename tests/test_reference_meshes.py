@@ -17,7 +17,10 @@ GPU: the FP64 engine against the reference (mass <= 1e-13, initial force
 <= 1e-11, u <= 1e-10, v and fi <= 1e-9, time and dt <= 1e-13 / 1e-11,
 energies <= 1e-9) on the layout creation picks (node slots, with overflow
 slots for the brain's nodes of more than 8 elements) and on the ELL forced;
-the FP32 split within 1e-5 displacement and 1e-4 energy balance.
+the FP32 split within 1e-5 displacement and 1e-4 energy balance.  And ex5's
+kind of load on its own brain mesh (its exterior driven by the half-sine
+rigid rotation) with the injury criteria updated after every step on the
+device, against the oracle.
 """
 import hashlib
 
@@ -152,3 +155,57 @@ def test_engine_fp32_split_within_tolerance(gpu_ready, name):
     scale = np.abs(ref[:, :3]).max()
     bal = en[:, 2] + en[:, 0] - en[:, 1]
     assert np.abs(bal - (ref[:, 2] + ref[:, 0] - ref[:, 1])).max() / scale < 1e-4
+
+
+def brain_rotation_case(steps=300):
+    """ex5's kind of load on its own brain mesh: the exterior nodes follow a
+    rigid rotation about the mesh's centroid (x axis) with a half-sine
+    angular acceleration of 2 krad/s^2 over 10 ms (ex5.cpp:339-389), the
+    interior follows by inertia; the fixture's five laws over the parts."""
+    from cases import case_mesh_file
+    z = golden("run_brain")
+    X, conn, pid = z["X"], z["conn"], z["pid"]
+    c = case_mesh_file(X, conn, pid, "brain")
+    faces = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]])
+    f = np.sort(conn.reshape(-1, 8)[:, faces].reshape(-1, 4), axis=1)
+    keys, counts = np.unique(f, axis=0, return_counts=True)
+    surface = np.unique(keys[counts == 1])
+    nn = X.size // 3
+    centre = X.reshape(-1, 3).mean(axis=0)
+    dm = -np.ones(3 * nn, np.int32)
+    for d in range(3):
+        dm[3 * surface + d] = 0
+    c.update(dof_motion=dm, motions=np.stack([ft.motion(ft.MOTION_ROT_HALFSINE, 1.0, 0.0, 0.0, *centre, 2000.0, 0.010)]),
+             v0=np.zeros(3 * nn), max_steps=steps)
+    return c, surface
+
+
+@pytest.mark.gpu
+def test_brain_rotation_with_injury_criteria_matches_oracle(gpu_ready):
+    """The brain under the rotational pulse with the injury criteria of
+    ex5.cpp:1311-1430 updated after every step on the device: FP64 engine
+    (node slots with overflow slots) against the oracle's loop."""
+    from test_gpu_parity import compare_injury
+    c, surface = brain_rotation_case()
+    steps = c["max_steps"]
+    print("brain rotation: %d surface nodes, %d steps" % (surface.size, steps), flush=True)
+    with ft.Engine(c["X"], c["conn"], c["pid"], c["material_id"], c["props"], ft.FT_FP64) as e:
+        e.set_boundary(c["dof_motion"], c["motions"])
+        e.lumped_mass()
+        e.run_init(c["tmax"], c["dt_factor"], c["failure_dt"])
+        e.injury_enable()
+        assert e.run_steps(steps) == steps
+        out, fl = e.injury_results()
+        u = e.state()[0]
+        t = e.time()[0]
+        assert e.layout()[0] == ft.LAYOUT_NODE_SLOTS
+    o = O.Model(c["X"], c["conn"], c["pid"], c["material_id"], c["props"], threads=8)
+    o.set_bc(c["dof_motion"], c["motions"])
+    o.injury_init()
+    r = o.run(steps, c["tmax"], c["dt_factor"], c["failure_dt"])
+    ro, rfl = o.injury_results()
+    assert r["steps"] == steps
+    assert t == pytest.approx(r["time"][-1], rel=1e-13)
+    assert rel(u, r["u"]) < 1e-10
+    assert ro[0] > 0.0  # the pulse strains the brain
+    compare_injury(out, fl, ro, rfl)
