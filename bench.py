@@ -136,6 +136,40 @@ def cpu_baseline(w, steps, threads):
                                              "" if threads == 1 else "s", loop))
 
 
+def reference_benchmark_block():
+    """The reference's published benchmark problem (27^3 Benchmarking-Parallel
+    cube, 445 steps to tMax = 0.1 s) run by the FP64 engine: loop wall time
+    beside the reference's published wall times (whole runs, Xeon E5-2650v4)
+    and the reference build's step loop measured on an 8-CPU Xeon container
+    (best of three, MKL on one thread)."""
+    import femtech_amd as ft
+    from femtech_amd import workloads as W
+    w = W.benchmark_cube()
+    ne = w["conn"].size // 8
+    with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP64) as e:
+        e.set_boundary(w["dof_motion"], w["motions"])
+        e.lumped_mass()
+        e.run_init(w["tmax"], w["dt_factor"], w["failure_dt"])
+        e.time()
+        t0 = time.perf_counter()
+        steps = e.run_steps(100000)  # ends at tMax
+        t_sim = e.time()[0]  # synchronises
+        t1 = time.perf_counter()
+    loop = t1 - t0
+    published = {"FEMTech GCC, 1/2/4/8/16 MPI processes": [93.0, 52.0, 27.5, 17.5, 15.0],
+                 "FEMTech Intel compiler, 1/2/4/8/16": [89.0, 46.5, 25.0, 14.5, 7.0],
+                 "Abaqus/Explicit, 1/2/4/8/16": [41.0, 21.0, 14.5, 12.0, 9.5]}
+    return {"case": "27^3 Benchmarking-Parallel cube (19,683 C3D8, neo-Hookean, stretched 7 mm to t = 0.1 s)",
+            "precision": "f64", "elements": ne, "steps": steps, "simulated_time_s": t_sim,
+            "gpu_loop_s": loop, "element_steps_per_s": ne * steps / loop,
+            "reference_published_wall_s": published,
+            "reference_build_loop_s": {"1 rank": 36.25, "8 ranks": 6.96,
+                                       "host": "Intel Xeon @ 2.70 GHz, 8 CPUs (not this box)"},
+            "speedup_vs_fastest_published": 7.0 / loop,
+            "note": "published figures are whole runs of the reference driver; the GPU figure is the step "
+                    "loop of the same problem (set-up excluded), FP64, the reference's results"}
+
+
 def surrogate_block(n=0, steps=500, warmup=20, train_n=40, train_steps=8000, autotune=5, kernel_timing=50):
     """Surrogate vs plain FP32 on the ellipsoid: the network is trained on the
     device from plain runs of a smaller ellipsoid (seeded pulse variants),
@@ -494,6 +528,9 @@ def main():
         # the same restatement on one core (a fifth of the steps: about 10 s
         # on the 1M-hex block)
         out["cpu_baseline_1core"] = cpu_baseline(w, max(1, args.cpu_steps // 5), 1)
+    if args.workload == "impact":
+        out["reference_benchmark"] = reference_benchmark_block()
+        note("reference benchmark done")
     if args.workload == "impact" and not args.no_surrogate:
         # the metric's second half: surrogate against plain on the 2.0M
         # ellipsoid (a failure there is reported, not fatal to the line)

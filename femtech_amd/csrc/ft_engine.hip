@@ -2038,6 +2038,10 @@ struct ft_engine {
   // graph_steps == 0 launches every step eagerly.
   int graph_steps = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // run loop's early stop: the control slots copied behind the launches
+  Ctl *h_ctl = nullptr;                // pinned, n_inst entries
+  int h_ctl_n = 0;
+  hipEvent_t done_ev = nullptr;
   // Persistent run loop (ft_set_persistent): barrier counter, 3 running
   // counters, blocks of the cooperative launch.
   bool persistent = false;
@@ -3038,6 +3042,8 @@ int ft_engine_destroy(ft_engine *g) {
       (void)hipEventDestroy(pr.second);
     }
   for (auto e : g->ev_pool) (void)hipEventDestroy(e);
+  if (g->done_ev) (void)hipEventDestroy(g->done_ev);
+  if (g->h_ctl) (void)hipHostFree(g->h_ctl);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
   return 0;
@@ -3437,6 +3443,50 @@ int capture_chunk(ft_engine *g) {
   return 0;
 }
 
+// Early stop of the launch loop: every kDoneCheck steps the control slots of
+// the next step are copied to pinned host memory behind the launches; a
+// later check that finds the copy complete and every instance done (end
+// time, step cap or no stable step) ends the loop, so a call asked for more
+// steps than the run has left does not keep launching kernels that exit at
+// once.  At most two check intervals of such launches are made.
+constexpr int kDoneCheck = 128;
+
+struct DoneWatch {
+  ft_engine *g;
+  bool pending = false;
+  int parity = 0;
+  int init() {
+    if (!g->done_ev && hipEventCreateWithFlags(&g->done_ev, hipEventDisableTiming) != hipSuccess)
+      return fail(FT_ERR_HIP, "event creation failed");
+    if (g->h_ctl_n < g->n_inst) {
+      if (g->h_ctl) (void)hipHostFree(g->h_ctl);
+      g->h_ctl = nullptr;
+      g->h_ctl_n = 0;
+      if (hipHostMalloc((void **)&g->h_ctl, g->n_inst * sizeof(Ctl)) != hipSuccess)
+        return fail(FT_ERR_ALLOC, "pinned control copy allocation failed");
+      g->h_ctl_n = g->n_inst;
+    }
+    return 0;
+  }
+  // true when an earlier copy shows every instance done; then enqueues the
+  // next copy
+  bool all_done() {
+    if (pending && hipEventQuery(g->done_ev) == hipSuccess) {
+      pending = false;
+      bool all = true;
+      for (int i = 0; i < g->n_inst && all; ++i) all = g->h_ctl[i].slot[parity].done != 0;
+      if (all) return true;
+    }
+    if (!pending &&
+        hipMemcpyAsync(g->h_ctl, g->ctl, g->n_inst * sizeof(Ctl), hipMemcpyDeviceToHost, g->stream) == hipSuccess &&
+        hipEventRecord(g->done_ev, g->stream) == hipSuccess) {
+      pending = true;
+      parity = g->parity;
+    }
+    return false;
+  }
+};
+
 int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   if (steps_done) *steps_done = 0;
@@ -3462,15 +3512,28 @@ int ft_run_steps(ft_engine *g, int max_steps, int *steps_done) {
     g->parity ^= max_steps & 1;
     k = max_steps;
   }
+  DoneWatch watch{g};
+  if (k < max_steps && (rc = watch.init())) return rc;
+  int since = 0;
+  bool stop = false;
   if (g->graph_steps > 0 && !g->timing && !g->surr_on) {
     for (; k + g->graph_steps <= max_steps; k += g->graph_steps) {
       if (!g->gexec[g->parity] && (rc = capture_chunk(g))) return rc;
       HIPCHK(hipGraphLaunch(g->gexec[g->parity], g->stream));
       g->parity ^= g->graph_steps & 1;
+      if ((since += g->graph_steps) >= kDoneCheck) {
+        since = 0;
+        if ((stop = watch.all_done())) break;
+      }
     }
   }
-  for (; k < max_steps; ++k)
+  for (; k < max_steps && !stop; ++k) {
     if ((rc = loop_body(g))) return rc;
+    if (++since >= kDoneCheck) {
+      since = 0;
+      stop = watch.all_done();
+    }
+  }
   if ((rc = read_slots(g, g->parity, s))) return rc;
   int done = 0;
   for (int i = 0; i < g->n_inst; ++i) done = std::max(done, s[i].step - before[i]);

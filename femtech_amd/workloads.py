@@ -5,7 +5,7 @@ part), dof_motion, motions, v0 and, where the laws need them, fibres.
 """
 import numpy as np
 
-from . import (MOTION_FIXED, MOTION_ROT_HALFSINE, mesh_block, mesh_ellipsoid, motion)
+from . import (MOTION_FIXED, MOTION_LINEAR, MOTION_ROT_HALFSINE, mesh_block, mesh_ellipsoid, motion)
 
 IMPACT_SEED = 20261202
 
@@ -34,6 +34,28 @@ def impact_block(n=100, slab=10, h=1e-3, jitter=0.15, seed=IMPACT_SEED):
     return dict(X=X, conn=conn, pid=np.zeros(ne, np.int32), material_id=np.array([1], np.int32),
                 props=props_row(1000.0, G, K - 2.0 * G / 3.0)[None, :], dof_motion=dm,
                 motions=motion(MOTION_FIXED)[None, :], v0=v0, fibres=None)
+
+
+def benchmark_cube(n=27, side=0.005, dmax=0.007, tmax=0.1):
+    """The reference's published benchmark problem (examples/Benchmarking-
+    Parallel: Benchmarking-Parallel.cpp:68-160 with 27elements.inp): an n^3
+    cube of side 5 mm, neo-Hookean rho 1040, mu = lambda = 100 Pa, symmetry
+    planes x = y = z = 0 and the face y = side pulled at dmax / tmax to
+    tmax = 0.1 s, dt = 0.8 x stable.  The mesh is generated (regular grid),
+    not read from the deck, whose coordinates agree with it to their nine
+    printed digits."""
+    X, conn = mesh_block(n, n, n, side / n)
+    nn, ne = X.size // 3, conn.size // 8
+    Xr = X.reshape(-1, 3)
+    tol = 1e-5
+    dm = -np.ones(3 * nn, np.int32)
+    for d in range(3):
+        dm[3 * np.nonzero(np.abs(Xr[:, d]) < tol)[0] + d] = 0
+    dm[3 * np.nonzero(np.abs(Xr[:, 1] - side) < tol)[0] + 1] = 1
+    motions = np.stack([motion(MOTION_FIXED), motion(MOTION_LINEAR, dmax / tmax)])
+    return dict(X=X, conn=conn, pid=np.zeros(ne, np.int32), material_id=np.array([1], np.int32),
+                props=props_row(1040.0, 100.0, 100.0)[None, :], dof_motion=dm, motions=motions, v0=None,
+                fibres=None, tmax=tmax, dt_factor=0.8, failure_dt=1e-11)
 
 
 def ellipsoid_head(n=126, semi_axes=(0.090, 0.075, 0.065), peak=6000.0, duration=0.010, seed=7):

@@ -247,6 +247,25 @@ def main():
     res["global_node_id"] = gid
     save("run_bench10", **inp, **res)
 
+    # the 27^3 benchmark mesh itself (the reference's published benchmark),
+    # kept lean, with its Abaqus report
+    ifile = os.path.join(REF, "examples", "Benchmarking-Parallel", "27elements.inp")
+    X, conn, pid, gid = ft.read_mesh(ifile)
+    c = cases.case_ex9_like(X, conn, pid, np.array([1], np.int32),
+                            cases.props_row(1040.0, 100.0, 100.0), tmax=0.1)
+    c["snaps"] = []
+    inp, res = golden_run("bench27", c, mesh_file=ifile)
+    if not (np.array_equal(res["ref_coords"], X) and np.array_equal(res["ref_conn"], conn)):
+        raise SystemExit("bench27: the reader's arrays differ from the reference's")
+    keep = ("ref_u", "ref_time", "ref_dt", "ref_energy", "ref_energy_file", "ref_steps")
+    out = {k: res[k] for k in keep}
+    out["abaqus_rpt"] = abaqus_rpt(os.path.join(REF, "examples", "Benchmarking-Parallel", "abaqus",
+                                                "27elements.rpt"))
+    inp.pop("snaps", None)
+    inp.pop("u0", None)
+    inp.pop("v0", None)
+    save("run_bench27", **inp, **out)
+
     # ex7: the reference's two-element column under its own loop, with the
     # Abaqus corner-node history (TensionNoDamping.dat: time, U1, U2, U3)
     ex7 = os.path.join(REF, "examples", "ex7")
