@@ -30,8 +30,8 @@
 //
 // Output: PREFIX.u.f64 (final displacement, 3 per node, FP64),
 // PREFIX.summary.txt (steps, time, dt, energies), the energy file
-// energy_<uid>.dat and, in compatibility mode, VTU/PVD files every
-// nSteps / plots steps.
+// energy_<uid>.dat and, in compatibility mode, VTU/PVD files and plot.dat
+// rows every nSteps / plots steps.
 #include "FemTech.h"
 #include "femtech_amd.h"
 
@@ -105,6 +105,25 @@ int corner_node(const Options &o) {
   return -1;
 }
 
+// plot.dat as the examples' CustomPlot writes it (ex9.cpp:250-279,
+// ex7.cpp:172-205): a header and zeros at Time = 0, then Time and the
+// displacement of the node at (L, L, L) at each plot step; ex9 prints
+// %11.5e, ex7 %11.3e.
+void custom_plot(const Options &o, bool first) {
+  const char *fmt = o.ex7 ? "%11.3e %11.3e  %11.3e  %11.3e\n" : "%11.5e %11.5e  %11.5e  %11.5e\n";
+  FILE *f = fopen("plot.dat", first ? "w" : "a");
+  if (!f) return;
+  if (first) {
+    fprintf(f, "# Results for Node ?\n");
+    fprintf(f, "# Time  DispX    DispY   DispZ\n");
+    fprintf(f, fmt, 0.0, 0.0, 0.0, 0.0);
+  } else {
+    const int n = corner_node(o);
+    if (n >= 0) fprintf(f, fmt, Time, displacements[3 * n], displacements[3 * n + 1], displacements[3 * n + 2]);
+  }
+  fclose(f);
+}
+
 void write_outputs(const Options &o, long steps, double energy[4]) {
   FILE *f = fopen((o.out + ".u.f64").c_str(), "wb");
   if (f) {
@@ -154,6 +173,7 @@ long run_ex7(const Options &o, const std::string &base) {
         plot.push_back(Time);
         plot.insert(plot.end(), displacements + 3 * corner, displacements + 3 * corner + 3);
       }
+      custom_plot(o, false);
     }
     ++step;
     ++taken;
@@ -204,12 +224,14 @@ long run_compatibility(const Options &o, const std::string &base, double energy[
         stepTime[plot] = Time;
         WritePVD(base.c_str(), plot);
       }
+      custom_plot(o, false);
     }
     ++step;
     ++taken;
     dt = ExplicitTimeStepReduction * StableTimeStep();
     MPI_Barrier(MPI_COMM_WORLD);
   }
+  custom_plot(o, false);  // the last step (ex9.cpp:179)
   (void)energy;
   return taken;
 }
@@ -319,6 +341,7 @@ int main(int argc, char **argv) {
   } else {
     WriteVTU(base.c_str(), 0);
     stepTime[0] = Time;
+    custom_plot(o, true);
     ShapeFunctions();
     AssembleLumpedMass();
     steps = o.ex7 ? run_ex7(o, base) : run_compatibility(o, base, energy);

@@ -38,6 +38,12 @@ def run_driver(tmp_path, name, extra):
     return z, u, summary
 
 
+def plot_rows(path):
+    lines = open(path).read().splitlines()
+    assert lines[:2] == ["# Results for Node ?", "# Time  DispX    DispY   DispZ"]
+    return np.array([[float(x) for x in line.split()] for line in lines[2:]])
+
+
 def energy_rows(path):
     return np.array([[float(x) for x in line.split()] for line in open(path)
                      if line.strip() and not line.startswith("#")])
@@ -64,6 +70,13 @@ def test_compatibility_mode_matches_reference(gpu_ready, tmp_path, name):
     vals = np.array(disp.text.split(), float)
     np.testing.assert_allclose(vals, u, rtol=1e-8, atol=1e-20)
     assert os.path.exists(tmp_path / "mesh.pvd")
+    # plot.dat as ex9's CustomPlot (%11.5e): zeros first, the final state last
+    rows = plot_rows(tmp_path / "plot.dat")
+    np.testing.assert_array_equal(rows[0], 0.0)
+    X = z["X"].reshape(-1, 3)
+    node = int(np.nonzero(np.all(np.abs(X - 0.005) < 1e-5, axis=1))[0][0])
+    np.testing.assert_allclose(rows[-1], np.concatenate([[float(summary["time"])], u[3 * node:3 * node + 3]]),
+                               rtol=1e-5, atol=1e-16)
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -118,6 +131,10 @@ def test_ex7_loop_matches_reference(gpu_ready, tmp_path):
     assert rows.shape == ef.shape
     np.testing.assert_allclose(rows[:, 0], ef[:, 0], rtol=1e-6)
     assert np.abs(rows[:, 1:4] - ef[:, 1:4]).max() <= 2e-6 * np.abs(ef[:, 1:4]).max()
+    # plot.dat as ex7's CustomPlot (%11.3e): the same history at that precision
+    rows = plot_rows(tmp_path / "plot.dat")
+    assert rows.shape == ref.shape
+    np.testing.assert_allclose(rows, ref, rtol=1e-3, atol=1e-15)
     ab = z["abaqus_corner"]
     ours = np.interp(ab[:, 0], corner[:, 0], corner[:, 1])
     theirs = np.interp(ab[:, 0], ref[:, 0], ref[:, 1])
