@@ -302,6 +302,10 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--layout", choices=["auto", "corner", "slots", "ell"], default="auto",
                     help="force-gather layout (auto: corner slots, else node slots, else the ELL)")
+    ap.add_argument("--snapshot-every", type=int, default=0,
+                    help="impact / ellipsoid: take the displacement to the host every N timed steps")
+    ap.add_argument("--snapshot-mode", choices=["async", "sync"], default="async",
+                    help="async: ft_snapshot_begin/end overlapping the next N steps; sync: ft_get_state between")
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="steps of the multi-threaded CPU sample (default: 50 impact, 10 ellipsoid)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -376,6 +380,23 @@ def main():
             if not any(e.instance(i)[3] == 0 for i in range(n_inst)) or done >= 1_000_000:
                 break
         args.steps = done
+    elif args.snapshot_every > 0:
+        # output every N steps: the displacement copied to the host while
+        # the next N steps compute (async), or read back between them (sync)
+        done, snaps = 0, 0
+        u_host = None
+        while done < args.steps:
+            k = min(args.snapshot_every, args.steps - done)
+            if args.snapshot_mode == "async":
+                e.snapshot_begin()
+                done += e.run_steps(k)
+                e.snapshot_end(copy=False)
+                u_host = e.snapshot_view()  # read in place (a writer would format it from here)
+            else:
+                u_host = np.empty(3 * nn)
+                ft._check(e._lib.ft_get_state(e._h, ft._dp(u_host), None, None, None))
+                done += e.run_steps(k)
+            snaps += 1
     else:
         done = e.run_steps(args.steps)
     t_sim, _, k = e.time()  # synchronises
@@ -477,6 +498,8 @@ def main():
                   "stop_cause": "an instance stops when its stable step collapses or its forces turn "
                                 "non-finite (element inversion); the ellipsoids' soft Ogden + Prony layer "
                                 "inverts under the rotational pulse in the FP64 oracle as well"}
+    if args.snapshot_every > 0 and not batch:
+        config["snapshots"] = {"every": args.snapshot_every, "mode": args.snapshot_mode, "count": snaps}
     out = {
         "metric": "element-steps/s on 1M-hex neo-Hookean; surrogate vs plain at <=1% stress err",
         "value": value,

@@ -58,6 +58,9 @@ _SIGS = {
     "ft_engine_info": [_VP, _I, _I, _I],
     "ft_engine_layout": [_VP, _I, _I],
     "ft_engine_set_layout": [_VP, ctypes.c_int],
+    "ft_snapshot_begin": [_VP],
+    "ft_snapshot_end": [_VP, _D, _D, _I],
+    "ft_snapshot_view": [_VP],
     "ft_set_boundary": [_VP, _I, ctypes.c_int, _D],
     "ft_set_state": [_VP, _D, _D],
     "ft_reset_history": [_VP],
@@ -120,7 +123,7 @@ _SIGS = {
     "ft_mlp_train": [_VP, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                      ctypes.c_ulonglong, _D],
 }
-_RET = {"ft_last_error": ctypes.c_char_p}
+_RET = {"ft_last_error": ctypes.c_char_p, "ft_snapshot_view": ctypes.c_void_p}
 
 
 class FemTechError(RuntimeError):
@@ -367,6 +370,29 @@ class Engine:
         done = ctypes.c_int(0)
         _check(self._lib.ft_run_steps(self._h, int(n), ctypes.byref(done)))
         return done.value
+
+    def snapshot_begin(self):
+        """Queue a copy of the current displacement; it reaches the host
+        while later run_steps calls compute."""
+        _check(self._lib.ft_snapshot_begin(self._h))
+
+    def snapshot_end(self, copy=True):
+        """(u, time, step) of the queued snapshot (waits for its copy); with
+        copy=False, u is None and snapshot_view() reads it in place."""
+        u = np.empty(3 * self.nn, np.float64) if copy else None
+        t, k = ctypes.c_double(0), ctypes.c_int(0)
+        _check(self._lib.ft_snapshot_end(self._h, _dp(u) if copy else None, ctypes.byref(t), ctypes.byref(k)))
+        return u, t.value, k.value
+
+    def snapshot_view(self):
+        """The last snapshot's displacement in pinned host memory (read-only,
+        valid until the next snapshot_begin), or None."""
+        p = self._lib.ft_snapshot_view(self._h)
+        if not p:
+            return None
+        a = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), shape=(3 * self.nn,))
+        a.flags.writeable = False
+        return a
 
     def time(self):
         t, dt, k = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)

@@ -2038,6 +2038,16 @@ struct ft_engine {
   // graph_steps == 0 launches every step eagerly.
   int graph_steps = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // asynchronous displacement snapshots (ft_snapshot_begin / ft_snapshot_end)
+  double *snap_dev = nullptr;   // 3 nn, node-major copy of u
+  Ctl *snap_ctl_dev = nullptr;  // the control slots with it
+  double *snap_host = nullptr;  // pinned
+  Ctl *snap_ctl_host = nullptr; // pinned
+  int snap_ctl_n = 0;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_taken = nullptr, snap_done = nullptr;
+  bool snap_pending = false;
+  int snap_parity = 0;
   // run loop's early stop: the control slots copied behind the launches
   Ctl *h_ctl = nullptr;                // pinned, n_inst entries
   int h_ctl_n = 0;
@@ -3043,6 +3053,14 @@ int ft_engine_destroy(ft_engine *g) {
     }
   for (auto e : g->ev_pool) (void)hipEventDestroy(e);
   if (g->done_ev) (void)hipEventDestroy(g->done_ev);
+  if (g->copy_stream) (void)hipStreamSynchronize(g->copy_stream);
+  for (hipEvent_t e : {g->snap_taken, g->snap_done})
+    if (e) (void)hipEventDestroy(e);
+  if (g->copy_stream) (void)hipStreamDestroy(g->copy_stream);
+  for (void *p : {(void *)g->snap_host, (void *)g->snap_ctl_host})
+    if (p) (void)hipHostFree(p);
+  for (void *p : {(void *)g->snap_dev, (void *)g->snap_ctl_dev})
+    if (p) (void)hipFree(p);
   if (g->h_ctl) (void)hipHostFree(g->h_ctl);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
@@ -3562,6 +3580,74 @@ int ft_get_time(ft_engine *g, double *time, double *dt, int *step) {
   if (rc) return rc;
   if (time) *time = s.t;
   if (dt) *dt = g->run_active ? s.dt_done : 0.0;
+  if (step) *step = s.step;
+  return 0;
+}
+
+// Component-major [3][nn] to node-major [nn][3].
+__global__ void k_to_node_major(int nn, const double *soa, double *aos) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  aos[3 * (size_t)n] = soa[n];
+  aos[3 * (size_t)n + 1] = soa[(size_t)nn + n];
+  aos[3 * (size_t)n + 2] = soa[2 * (size_t)nn + n];
+}
+
+// A node-major copy of the current displacement (and the control slots) is
+// made on the engine's stream behind the queued work, then moved to pinned
+// host memory on a second stream while later calls compute.
+int ft_snapshot_begin(ft_engine *g) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (g->snap_pending) return fail(FT_ERR_STATE, "a snapshot is pending: ft_snapshot_end first");
+  HIPCHK(hipSetDevice(g->device));
+  const size_t nb = 3 * (size_t)g->nn * sizeof(double);
+  if (!g->snap_dev) {
+    int rc = dalloc(&g->snap_dev, 3 * (size_t)g->nn);
+    if (rc) return rc;
+    if (hipHostMalloc((void **)&g->snap_host, nb) != hipSuccess) return fail(FT_ERR_ALLOC, "pinned snapshot allocation failed");
+    HIPCHK(hipStreamCreate(&g->copy_stream));
+    HIPCHK(hipEventCreateWithFlags(&g->snap_taken, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g->snap_done, hipEventDisableTiming));
+  }
+  if (g->snap_ctl_n < g->n_inst) {
+    if (g->snap_ctl_dev) (void)hipFree(g->snap_ctl_dev);
+    if (g->snap_ctl_host) (void)hipHostFree(g->snap_ctl_host);
+    g->snap_ctl_dev = nullptr;
+    g->snap_ctl_host = nullptr;
+    g->snap_ctl_n = 0;
+    int rc = dalloc(&g->snap_ctl_dev, (size_t)g->n_inst);
+    if (rc) return rc;
+    if (hipHostMalloc((void **)&g->snap_ctl_host, g->n_inst * sizeof(Ctl)) != hipSuccess)
+      return fail(FT_ERR_ALLOC, "pinned snapshot allocation failed");
+    g->snap_ctl_n = g->n_inst;
+  }
+  const size_t cb = g->n_inst * sizeof(Ctl);
+  hipLaunchKernelGGL(k_to_node_major, dim3((g->nn + 255) / 256), dim3(256), 0, g->stream, g->nn, g->u, g->snap_dev);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g->snap_ctl_dev, g->ctl, cb, hipMemcpyDeviceToDevice, g->stream));
+  HIPCHK(hipEventRecord(g->snap_taken, g->stream));
+  HIPCHK(hipStreamWaitEvent(g->copy_stream, g->snap_taken, 0));
+  HIPCHK(hipMemcpyAsync(g->snap_host, g->snap_dev, nb, hipMemcpyDeviceToHost, g->copy_stream));
+  HIPCHK(hipMemcpyAsync(g->snap_ctl_host, g->snap_ctl_dev, cb, hipMemcpyDeviceToHost, g->copy_stream));
+  HIPCHK(hipEventRecord(g->snap_done, g->copy_stream));
+  g->snap_parity = g->parity;
+  g->snap_pending = true;
+  return 0;
+}
+
+const double *ft_snapshot_view(ft_engine *g) {
+  if (!g || !g->snap_host || g->snap_pending) return nullptr;
+  return g->snap_host;
+}
+
+int ft_snapshot_end(ft_engine *g, double *u, double *time, int *step) {
+  if (!g) return fail(FT_ERR_ARG, "null engine");
+  if (!g->snap_pending) return fail(FT_ERR_STATE, "no snapshot pending: ft_snapshot_begin first");
+  HIPCHK(hipEventSynchronize(g->snap_done));
+  g->snap_pending = false;
+  if (u) memcpy(u, g->snap_host, 3 * (size_t)g->nn * sizeof(double));
+  const StepSlot &s = g->snap_ctl_host[0].slot[g->snap_parity];
+  if (time) *time = s.t;
   if (step) *step = s.step;
   return 0;
 }

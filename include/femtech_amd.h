@@ -166,6 +166,20 @@ int ft_get_time(ft_engine *eng, double *time, double *dt, int *step);
 /* u, v, a, fi (3 per node each; any may be NULL) at the end of the last
  * completed step (CheckEnergy-time values). */
 int ft_get_state(ft_engine *eng, double *u, double *v, double *a, double *fi);
+/* Output snapshots without stalling the loop (the displacements the
+ * examples write every nSteps / nPlotSteps steps, ex9.cpp:156-167 with
+ * WriteVTU.cpp:3-263 and the driver's CustomPlot): ft_snapshot_begin copies
+ * the current displacement on the device behind the queued steps and moves
+ * it to pinned host memory on a second stream while later ft_run_steps calls
+ * compute; ft_snapshot_end waits for that copy and gives u (node-major, 3 per
+ * node; nullable) with the time and step of instance 0.  One snapshot is in
+ * flight at a time (FT_ERR_STATE otherwise). */
+int ft_snapshot_begin(ft_engine *eng);
+int ft_snapshot_end(ft_engine *eng, double *u, double *time, int *step);
+/* The last completed snapshot's displacement in place (pinned host memory,
+ * node-major; valid until the next ft_snapshot_begin), for writers that read
+ * it without a copy; NULL before the first snapshot or while one is pending. */
+const double *ft_snapshot_view(ft_engine *eng);
 /* Energy balance (CheckEnergy.cpp:3-85): W_int, W_ext, KE, |KE+W_int-W_ext|
  * after the last completed step (of instance 0). */
 int ft_get_energy(ft_engine *eng, double out4[4]);

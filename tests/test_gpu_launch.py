@@ -181,3 +181,44 @@ def test_inverted_element_collapses_the_step(gpu_ready, precision):
         assert err.value.code == ft.FT_ERR_DT
         assert e.status() & ft.STATUS_DT_COLLAPSE
         assert e.status() & ft.STATUS_INVERTED
+
+
+@pytest.mark.parametrize("graph", [0, 8])
+def test_snapshot_overlaps_later_steps_and_equals_a_stopped_run(gpu_ready, graph):
+    """ft_snapshot_begin after 48 steps, 40 more steps, then ft_snapshot_end:
+    the snapshot is the displacement, time and step of step 48, bit for bit
+    as a run stopped there (with and without captured chunks); a second
+    begin while one is pending, or an end without a begin, is a call-order
+    error."""
+    from femtech_amd import workloads as W
+    w = W.impact_block(n=12, slab=2, seed=5)
+
+    def engine():
+        e = ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], ft.FT_FP32)
+        e.set_boundary(w["dof_motion"], w["motions"])
+        e.set_state(None, w["v0"])
+        e.lumped_mass()
+        e.run_init(1e9, 0.9, 1e-11)
+        e.set_graph(graph)
+        return e
+
+    with engine() as e:
+        with pytest.raises(ft.FemTechError) as ei:
+            e.snapshot_end()
+        assert ei.value.code == ft.FT_ERR_STATE
+        e.run_steps(48)
+        e.snapshot_begin()
+        with pytest.raises(ft.FemTechError) as ei:
+            e.snapshot_begin()
+        assert ei.value.code == ft.FT_ERR_STATE
+        e.run_steps(40)
+        us, ts, ks = e.snapshot_end()
+        np.testing.assert_array_equal(e.snapshot_view(), us)
+        later = e.state()[0]
+    with engine() as f:
+        f.run_steps(48)
+        u48 = f.state()[0]
+        t48, _, k48 = f.time()
+    assert ks == k48 == 48 and ts == t48
+    assert np.array_equal(us, u48)
+    assert not np.array_equal(later, u48)
