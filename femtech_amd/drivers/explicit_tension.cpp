@@ -16,7 +16,8 @@
 // conditions and the velocity/displacement updates run here, every library
 // call goes to the GPU.  --resident: the same problem with the boundary
 // conditions handed over as data and the loop run on the device through
-// ft_run_init / ft_run_steps.
+// ft_run_init / ft_run_steps; its plot.dat rows come from asynchronous
+// displacement snapshots (no VTU files).
 //
 // --loop ex7 (compatibility mode): the loop of ex7.cpp:56-125 instead of
 // ex9's: the initial force and accelerations come before any boundary
@@ -35,6 +36,7 @@
 #include "FemTech.h"
 #include "femtech_amd.h"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -256,22 +258,41 @@ long run_resident(const Options &o, double energy[4]) {
                                 : FT_FP64,
                             &eng);
   int done = 0, k = 0;
-  const int chunk = 1 << 20;
   long taken = 0;
   if (!rc) rc = ft_set_boundary(eng, dof_motion.data(), 2, motions);
   if (!rc) rc = ft_set_state(eng, nullptr, nullptr);
   if (!rc) rc = ft_assemble_lumped_mass(eng, nullptr);
   if (!rc) rc = ft_run_init(eng, o.tmax, ExplicitTimeStepReduction, FailureTimeStep);
+  // plot.dat rows every nSteps / plots steps, as the compatibility loop
+  // writes them: the displacement at each chunk boundary reaches the host
+  // while the next chunk computes (ft_snapshot_begin / ft_snapshot_end)
+  double t0 = 0.0, dt0 = 0.0;
+  if (!rc) rc = ft_get_time(eng, &t0, &dt0, &k);
+  const long n_steps = dt0 > 0.0 ? (long)(o.tmax / dt0) : 0;
+  const long chunk = o.plots <= 0 ? 1L << 20 : n_steps / o.plots > 0 ? n_steps / o.plots : 1;
+  if (!rc) custom_plot(o, true);
+  bool pending = false;
   while (!rc) {
-    const long left = o.max_steps < 0 ? chunk : o.max_steps - taken;
+    const long left = o.max_steps < 0 ? chunk : std::min(chunk, o.max_steps - taken);
     if (left <= 0) break;
-    rc = ft_run_steps(eng, (int)(left < chunk ? left : chunk), &done);
+    rc = ft_run_steps(eng, (int)left, &done);
     taken += done;
-    if (done == 0) break;
+    if (!rc && pending) {  // the previous boundary's rows, copied meanwhile
+      rc = ft_snapshot_end(eng, displacements, &Time, &k);
+      if (!rc) custom_plot(o, false);
+      pending = false;
+    }
+    if (rc || done == 0) break;
+    if (done == left && (rc = ft_snapshot_begin(eng)) == 0) pending = true;
+  }
+  if (!rc && pending) {
+    rc = ft_snapshot_end(eng, displacements, &Time, &k);
+    if (!rc) custom_plot(o, false);
   }
   if (!rc) rc = ft_get_state(eng, displacements, velocities, accelerations, fi);
   if (!rc) rc = ft_get_time(eng, &Time, &dt, &k);
   if (!rc) rc = ft_get_energy(eng, energy);
+  if (!rc) custom_plot(o, false);  // the last step (ex9.cpp:179)
   if (rc) {
     FILE_LOG_SINGLE(ERROR, "resident run: %s", ft_last_error());
     ft_engine_destroy(eng);

@@ -88,6 +88,28 @@ def test_resident_mode_matches_reference(gpu_ready, tmp_path, name):
     assert rel(E, z["ref_energy"][-1, :3]) < 1e-9
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_resident_mode_plot_rows_match_compatibility_mode(gpu_ready, tmp_path, name):
+    """Resident mode writes plot.dat from asynchronous displacement snapshots
+    at the same steps as the compatibility loop (every nSteps / plots steps
+    and the last step); both runs give the same rows at %11.5e."""
+    rows = {}
+    for mode, extra in (("compat", []), ("resident", ["--resident"])):
+        d = tmp_path / mode
+        d.mkdir()
+        z = golden("run_" + name)
+        write_case(str(d), z)
+        print("driver:", name, mode, flush=True)
+        res = subprocess.run([DRIVER, "mesh.inp", "--tmax", repr(CASES[name]), "--plots", "40"] + extra,
+                             cwd=str(d), capture_output=True, text=True, timeout=300,
+                             env=dict(os.environ, FEMTECH_UID="t"))
+        assert res.returncode == 0, (mode, res.stdout[-2000:], res.stderr[-2000:])
+        rows[mode] = plot_rows(d / "plot.dat")
+    assert rows["resident"].shape == rows["compat"].shape
+    assert len(rows["compat"]) >= 40
+    np.testing.assert_allclose(rows["resident"], rows["compat"], rtol=2e-5, atol=1e-20)
+
+
 def test_fp32_compatibility_mode_within_split_tolerance(gpu_ready, tmp_path):
     """FEMTECH_PRECISION=fp32: FP32 element arithmetic under the same driver."""
     z = golden("run_bench10")
