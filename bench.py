@@ -300,8 +300,8 @@ def main():
                          "cube rotation (same mesh and physics), so that no node is the same local "
                          "corner of one element only and the engine takes its general force gather")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--layout", choices=["auto", "corner", "slots", "ell"], default="auto",
-                    help="force-gather layout (auto: corner slots, else node slots, else the ELL)")
+    ap.add_argument("--layout", choices=["auto", "pairs", "corner", "slots", "ell"], default="auto",
+                    help="force-gather layout (auto: corner pairs, else node slots, else the ELL)")
     ap.add_argument("--snapshot-every", type=int, default=0,
                     help="impact / ellipsoid: take the displacement to the host every N timed steps")
     ap.add_argument("--snapshot-mode", choices=["async", "sync"], default="async",
@@ -347,7 +347,8 @@ def main():
     prec = ft.FT_FP32 if args.precision == "fp32" else ft.FT_FP64
     e = ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], prec)
     if args.layout != "auto":
-        e.set_layout({"corner": ft.LAYOUT_CORNER, "slots": ft.LAYOUT_NODE_SLOTS, "ell": ft.LAYOUT_ELL}[args.layout])
+        e.set_layout({"pairs": ft.LAYOUT_CORNER_PAIRS, "corner": ft.LAYOUT_CORNER, "slots": ft.LAYOUT_NODE_SLOTS,
+                      "ell": ft.LAYOUT_ELL}[args.layout])
     if batch:
         e.set_instances(w["node_off"], w["elem_off"])
         e.set_instance_end_times(w["end_times"])
@@ -455,7 +456,8 @@ def main():
     # v read/write 48 + mass 8 + BC flag 1 = 145 B, plus fi_prev read/write 48
     # for the FP64 energy balance.
     fixed_bytes = 260.0 * ne + (145.0 + 48.0) * nn
-    gather = {"layout": {1: "corner slots", 2: "node slots", 3: "ELL"}.get(layout[0], str(layout[0])),
+    gather = {"layout": {1: "corner slots", 2: "node slots", 3: "ELL",
+                         4: "corner pairs"}.get(layout[0], str(layout[0])),
               "ell_width": int(layout[1]),
               "element_node_order": "permuted by seeded cube rotations (seed %d)" % args.permute
               if args.permute is not None else "as generated"}

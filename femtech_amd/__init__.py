@@ -24,7 +24,7 @@ STATUS_DT_COLLAPSE = 4
 STATUS_DT_FP32 = 16
 STATUS_DT_BOUND = 32
 # force-gather layouts (ft_engine_layout / ft_engine_set_layout)
-LAYOUT_AUTO, LAYOUT_CORNER, LAYOUT_NODE_SLOTS, LAYOUT_ELL = 0, 1, 2, 3
+LAYOUT_AUTO, LAYOUT_CORNER, LAYOUT_NODE_SLOTS, LAYOUT_ELL, LAYOUT_CORNER_PAIRS = 0, 1, 2, 3, 4
 # error codes (include/femtech_amd.h): FemTech's own, then the engine's
 FT_ERR_MATERIAL = 1
 FT_ERR_INPUT = 3
@@ -297,17 +297,19 @@ class Engine:
         self.close()
 
     def layout(self):
-        """(layout, ell_width): LAYOUT_CORNER (per-node corner slots),
-        LAYOUT_NODE_SLOTS (per-node slots by element rank) or LAYOUT_ELL
-        (element-major forces gathered through the node-to-element table),
-        and that table's width."""
+        """(layout, ell_width): LAYOUT_CORNER_PAIRS (corner slots, face
+        neighbours along xi of one wavefront summing their shared corners in
+        registers), LAYOUT_CORNER (per-node corner slots), LAYOUT_NODE_SLOTS
+        (per-node slots by element rank) or LAYOUT_ELL (element-major forces
+        gathered through the node-to-element table), and that table's
+        width."""
         c, w = ctypes.c_int(0), ctypes.c_int(0)
         _check(self._lib.ft_engine_layout(self._h, ctypes.byref(c), ctypes.byref(w)))
         return c.value, w.value
 
     def set_layout(self, layout):
         """Choose the force-gather layout before run_init (LAYOUT_AUTO picks
-        the first of corner slots, node slots, ELL that applies)."""
+        the first of corner pairs, node slots, ELL that applies)."""
         _check(self._lib.ft_engine_set_layout(self._h, int(layout)))
 
     # --- set-up -----------------------------------------------------------

@@ -66,6 +66,11 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint64_t kInfBits = 0x7FF0000000000000ull;
 constexpr int kSkipBit = 1 << 30;
+// corner pairs (FT_LAYOUT_CORNER_PAIRS): the element adds its left lane's
+// xi+ corner forces to its own xi- corners / leaves its xi+ corners to its
+// right lane
+constexpr int kPairLeft = 1 << 28;
+constexpr int kPairRight = 1 << 27;
 constexpr int kPidMask = (1 << 20) - 1;
 constexpr int kMaxMotions = 64;  // distinct motions per instance (held in LDS by the nodal kernel)
 constexpr int kEaccStride = 8;  // doubles per nodal block in the energy accumulators
@@ -278,6 +283,18 @@ __host__ __device__ inline void motion_eval(const double *mo, const double *X, d
 // Element kernel
 // ---------------------------------------------------------------------------
 // Element of an array at a 32-bit byte offset from a (uniform) base.
+// The value of the lane to the left in the wavefront (DPP wave_shr:1; lane
+// 0 reads zero).
+__device__ __forceinline__ float lane_from_left(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double lane_from_left(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x138, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T &at_byte(T *base, unsigned off) {
   return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
@@ -325,6 +342,7 @@ struct ElemArgs {
   Ctl *ctl;              // run loop: per-instance control, slot `parity` gives dt_k and done
   int parity;
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
+  int pair;              // corner pairs (kPairLeft / kPairRight in einfo)
   double *cand_geom;       // FP32 run loop: kCandStride per element tile (candidate geometry)
   // per-Gauss-point fields (k_element_gp): 72 / 8 / 48 doubles per element in
   // the caller's element order (perm: internal -> caller)
@@ -507,8 +525,26 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
           dst[2] = outv[3 * k + 2];
         }
       } else {
+        bool right = false;  // corner pairs: the xi+ corners go to the right lane
+        if (a.pair) {        // uniform (corner slots)
+          // C3D8 corners on the face xi = -1 (0, 3, 4, 7) against the same
+          // corners of the left neighbour's face xi = +1 (1, 2, 5, 6); the
+          // lane to the left is active whenever this one is (tiles fill
+          // their lanes from the first)
+          const bool left = (info & kPairLeft) != 0;
+          right = (info & kPairRight) != 0;
+          constexpr int km[4] = {0, 3, 4, 7}, kp[4] = {1, 2, 5, 6};
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const R w = lane_from_left(outv[3 * kp[j] + d]);
+              if (left) outv[3 * km[j] + d] += w;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
+          if (right && ((k + 1) & 2)) continue;  // k = 1, 2, 5, 6
           R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);  // uniform
           const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
           at_byte(dst, o) = outv[3 * k];
@@ -749,7 +785,9 @@ __device__ inline void gather3(const int *ell, int nn, int width, const R *fe, i
   f[0] = f[1] = f[2] = 0.0;
   if (W < 0) {
     // Corner-slot layout fe[corner][node][xyz]: the element that has node n
-    // as its local corner k wrote slot k (zero where no element does).
+    // as its local corner k wrote slot k (zero where no element does; with
+    // corner pairs, zero also where the element's right partner wrote the
+    // pair's sum into its own slot).
     R v[8][3];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -1966,6 +2004,17 @@ struct ft_engine {
   std::vector<unsigned> h_eslot;    // the same in the caller's element order
   std::vector<int> h_ovf;           // host copy of ovf
   int layout = 0;                   // FT_LAYOUT_* in use
+  // Corner pairs (layout FT_LAYOUT_CORNER_PAIRS, set by build_pairs for the
+  // current tiles): element i and i - 1 of one wavefront share the face
+  // xi = +1 of i - 1; i adds i - 1's four corner forces there to its own and
+  // writes the sums, i - 1 writes only its other four.  plive: per node, bit
+  // k set when slot k is written (all xi- slots, and the xi+ slots of
+  // elements without a right partner); the other slots hold zeros, so the
+  // node sum reads all eight as with plain corner slots.
+  bool paired = false;
+  bool unpaired = false;  // set while ft_element_forces writes every corner
+  unsigned char *plive = nullptr;
+  long n_pairs = 0;
   std::vector<int> h_conn;  // caller's connectivity (element-force read-back)
   PartDev *parts = nullptr;
   std::vector<PartDev> h_parts;
@@ -1983,6 +2032,7 @@ struct ft_engine {
   int n_inst = 1;
   std::vector<int> inst_node, inst_elem;   // n_inst + 1 offsets each
   int4 *etiles = nullptr, *ntiles = nullptr;
+  std::vector<int4> h_etiles;   // host copy (launch order)
   double *cand_geom = nullptr;  // kCandStride per element tile (FP32 exact step size)
   double *gp_fields = nullptr;  // per-Gauss-point F, det F, PK2 (128 per element), on demand
   int *perm_dev = nullptr;      // internal -> caller element index, on demand
@@ -2208,6 +2258,7 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.ctl = in_loop ? g->ctl : nullptr;
   a.parity = g->parity;
   a.refine = (sizeof(R) == 4 && in_loop) ? 1 : 0;
+  a.pair = (g->paired && !g->unpaired) ? 1 : 0;
   a.cand_geom = g->cand_geom;
   a.gp_F = a.gp_detF = a.gp_pk2 = nullptr;
   a.perm = nullptr;
@@ -2585,6 +2636,103 @@ int injury_launch(ft_engine *g, const StepSlot *slot, double time, double dt) {
   return 0;
 }
 
+// Corner pairs: zero the corner slots no element writes (live bit clear).
+template <typename R>
+__global__ void k_zero_unwritten(int nn, const unsigned char *live, R *fe) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  const unsigned lv = live[n];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (!((lv >> k) & 1u)) {
+      R *dst = fe + 3 * ((size_t)k * nn + n);
+      dst[0] = dst[1] = dst[2] = R(0);
+    }
+}
+
+int zero_unwritten(ft_engine *g) {
+  if (!g->paired) return 0;
+  if (g->precision == FT_FP32)
+    hipLaunchKernelGGL(k_zero_unwritten<float>, dim3((g->nn + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream,
+                       g->nn, g->plive, reinterpret_cast<float *>(g->fe));
+  else
+    hipLaunchKernelGGL(k_zero_unwritten<double>, dim3((g->nn + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream,
+                       g->nn, g->plive, reinterpret_cast<double *>(g->fe));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Element passes outside the run loop (forces, per-element forces and
+// per-Gauss-point fields asked for by the caller) write all eight corners of
+// every element, as the plain corner layout does; the slots corner pairs
+// leave unwritten are zeroed again when the pass's results have been read.
+struct Unpaired {
+  ft_engine *g;
+  explicit Unpaired(ft_engine *e) : g(e) { g->unpaired = true; }
+  ~Unpaired() {
+    g->unpaired = false;
+    (void)zero_unwritten(g);
+  }
+};
+
+// einfo's pair bits from bits[e] (1: kPairLeft, 2: kPairRight).
+__global__ void k_pair_bits(int ne, const unsigned char *bits, int *einfo) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int b = bits[e];
+  einfo[e] = (einfo[e] & ~(kPairLeft | kPairRight)) | ((b & 1) ? kPairLeft : 0) | ((b & 2) ? kPairRight : 0);
+}
+
+// Corner pairs for the current layout and element tiles (FT_LAYOUT_CORNER_PAIRS;
+// every other layout clears them): element i pairs with i - 1 when both lie
+// in one wavefront of one tile and i's corners 0, 3, 4, 7 are i - 1's
+// corners 1, 2, 5, 6 (a face shared along xi: consecutive elements of a row
+// of a structured block).  Sets the pair bits in einfo and the per-node slot
+// mask the node sum reads.
+int build_pairs(ft_engine *g) {
+  const int ne = g->ne, nn = g->nn;
+  std::vector<unsigned char> bits(ne, 0), live(nn, 0xff);
+  long pairs = 0;
+  if (g->layout == FT_LAYOUT_CORNER_PAIRS && g->corner && !g->eslot) {
+    const int *conn = g->h_conn.data();
+    for (const int4 &t : g->h_etiles)
+      for (int i = t.x + 1; i < t.y; ++i) {
+        if ((i - t.x) % 64 == 0) continue;  // first lane of a wavefront
+        const int *a = conn + 8 * (size_t)g->perm[i - 1], *b = conn + 8 * (size_t)g->perm[i];
+        if (b[0] == a[1] && b[3] == a[2] && b[4] == a[5] && b[7] == a[6]) {
+          bits[i] |= 1;
+          bits[i - 1] |= 2;
+          ++pairs;
+        }
+      }
+    // the xi+ slots of elements with a right partner are never written
+    for (int i = 0; i < ne; ++i)
+      if (bits[i] & 2) {
+        const int *c = conn + 8 * (size_t)g->perm[i];
+        for (int k : {1, 2, 5, 6}) live[c[k]] &= (unsigned char)~(1u << k);
+      }
+  }
+  unsigned char *dbits = nullptr;
+  int rc = dalloc(&dbits, (size_t)ne);
+  if (rc) return rc;
+  if (hipMemcpy(dbits, bits.data(), ne, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dbits);
+    return fail(FT_ERR_HIP, "pair bit upload failed");
+  }
+  hipLaunchKernelGGL(k_pair_bits, dim3((ne + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream, ne, dbits, g->einfo);
+  const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(g->stream);
+  (void)hipFree(dbits);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(FT_ERR_HIP, "pair bits: %s", hipGetErrorString(e1 ? e1 : e2));
+  if (pairs > 0 && !g->plive && (rc = dalloc(&g->plive, (size_t)nn))) return rc;
+  if (pairs > 0) HIPCHK(hipMemcpy(g->plive, live.data(), nn, hipMemcpyHostToDevice));
+  g->paired = pairs > 0;
+  g->n_pairs = pairs;
+  // slots the new pairs leave unwritten may hold earlier forces
+  if ((rc = zero_unwritten(g))) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
 // Element tiles: at most sg.tile consecutive elements of one material
 // segment (internal order) and one instance.
 int build_element_tiles(ft_engine *g) {
@@ -2608,11 +2756,12 @@ int build_element_tiles(ft_engine *g) {
   g->etiles = nullptr;
   g->cand_geom = nullptr;
   g->n_etiles = (int)et.size();
+  g->h_etiles = et;
   int rc;
   if ((rc = dalloc(&g->etiles, et.size())) || (rc = dalloc(&g->cand_geom, kCandStride * et.size())))
     return rc;
   HIPCHK(hipMemcpy(g->etiles, et.data(), et.size() * sizeof(int4), hipMemcpyHostToDevice));
-  return 0;
+  return build_pairs(g);  // wavefront boundaries moved
 }
 
 // Tiles of consecutive entries that never span two instances: elements per
@@ -2811,12 +2960,14 @@ static int set_layout(ft_engine *g, int mode) {
     }
   }
   const bool slots_ok = g->width <= 16;  // ranks 8-15 in overflow slots
-  if (mode == FT_LAYOUT_AUTO) mode = corner_ok ? FT_LAYOUT_CORNER : slots_ok ? FT_LAYOUT_NODE_SLOTS : FT_LAYOUT_ELL;
-  if (mode == FT_LAYOUT_CORNER && !corner_ok)
+  if (mode == FT_LAYOUT_AUTO)
+    mode = corner_ok ? FT_LAYOUT_CORNER_PAIRS : slots_ok ? FT_LAYOUT_NODE_SLOTS : FT_LAYOUT_ELL;
+  if ((mode == FT_LAYOUT_CORNER || mode == FT_LAYOUT_CORNER_PAIRS) && !corner_ok)
     return fail(FT_ERR_ARG, "corner-slot layout: a node is the same local corner of two elements");
   if (mode == FT_LAYOUT_NODE_SLOTS && !slots_ok)
     return fail(FT_ERR_ARG, "node-slot layout: a node has %d elements (at most 16)", g->width);
-  if (mode != FT_LAYOUT_CORNER && mode != FT_LAYOUT_NODE_SLOTS && mode != FT_LAYOUT_ELL)
+  if (mode != FT_LAYOUT_CORNER && mode != FT_LAYOUT_CORNER_PAIRS && mode != FT_LAYOUT_NODE_SLOTS &&
+      mode != FT_LAYOUT_ELL)
     return fail(FT_ERR_ARG, "unknown layout %d", mode);
   // The new buffers are built first; the engine keeps its old layout if
   // any step fails.
@@ -2869,7 +3020,7 @@ static int set_layout(ft_engine *g, int mode) {
   g->h_ovf.swap(h_ovf);
   g->corner = corner;
   g->layout = mode;
-  return 0;
+  return build_pairs(g);
 }
 
 int ft_engine_set_layout(ft_engine *g, int mode) {
@@ -3041,7 +3192,7 @@ int ft_engine_destroy(ft_engine *g) {
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->eslot, g->ovf, g->fe_ovf, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
-                  g->etiles, g->cand_geom, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
+                  g->etiles, g->cand_geom, g->plive, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -3077,7 +3228,7 @@ int ft_engine_info(ft_engine *g, int *n_nodes, int *n_elems, int *precision) {
 
 int ft_engine_layout(ft_engine *g, int *layout, int *ell_width) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
-  if (layout) *layout = g->layout;
+  if (layout) *layout = g->layout == FT_LAYOUT_CORNER_PAIRS && !g->paired ? FT_LAYOUT_CORNER : g->layout;
   if (ell_width) *ell_width = g->width;
   return 0;
 }
@@ -3266,6 +3417,7 @@ int ft_element_forces(ft_engine *g, const double *u, double dt, int update_histo
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = upload_u(g, u);
   if (rc) return rc;
+  Unpaired unpaired_pass(g);
   if ((rc = element_pass(g, true, false, false, dt, nullptr, update_history))) return rc;
   if ((rc = check_status(g))) return rc;
   if (fe_out) {
@@ -3307,6 +3459,7 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
   int rc = upload_u(g, u);
   if (rc) return rc;
+  Unpaired unpaired_pass(g);
   if ((rc = element_pass(g, true, false, false, dt, nullptr, 1))) return rc;
   if ((rc = gather_forces(g))) return rc;
   if ((rc = check_status(g))) return rc;
@@ -3317,6 +3470,7 @@ int ft_get_force(ft_engine *g, const double *u, double dt, double *fi_out) {
 int ft_get_force_fields(ft_engine *g, const double *u, double dt, double *fi_out, double *F_out,
                         double *detF_out, double *pk2_out) {
   if (!g) return fail(FT_ERR_ARG, "null engine");
+  Unpaired unpaired_pass(g);
   const size_t ne = (size_t)g->ne;
   int rc;
   if (!g->gp_fields) {
@@ -4493,7 +4647,8 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
   // element kernel: connectivity 32 + einfo 4 + force write 24 es per element;
   // X (3 doubles) + u (3 doubles) per node; Prony history read + write and
   // fibre directions for the elements that carry them.
-  double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0);
+  // (corner pairs: the right partner's four shared corners are not written)
+  double elem = ne * (32.0 + 4.0 + 24.0 * es) + nn * (24.0 + 24.0) - (g->paired ? g->n_pairs * 12.0 * es : 0.0);
   if (g->eslot) elem += ne * 4.0;  // node-slot numbers
   for (const auto &sg : g->segs) {
     const double n = sg.end - sg.begin;

@@ -92,15 +92,25 @@ int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
  * element writes its corner forces to per-node slots numbered by its rank
  * among the node's elements (element order; ranks 8-15 of the nodes with
  * more than 8 elements in overflow slots); FT_LAYOUT_ELL: element-major
- * forces summed through the node-to-element table.  Creation picks the first
- * that applies (FT_LAYOUT_AUTO); ft_engine_set_layout chooses another that
- * applies, before ft_run_init (FT_ERR_ARG when it does not apply,
- * FT_ERR_STATE inside a run).  ell_width: the table's width (the largest node
- * valence, at least 8). */
+ * forces summed through the node-to-element table.  FT_LAYOUT_CORNER_PAIRS:
+ * corner slots where, in the run loop, two consecutive elements of a
+ * wavefront that share a face along xi (element i's local nodes 0, 3, 4, 7
+ * are element i - 1's nodes 1, 2, 5, 6: a row of a structured block) add
+ * the forces of that face's four corners in registers and element i writes
+ * the sums, so about half of the corner slots are written (the others
+ * hold zeros); it applies wherever FT_LAYOUT_CORNER does and reports
+ * FT_LAYOUT_CORNER when no element pairs.  Forces asked for outside the run
+ * loop (ft_get_force, ft_element_forces, ft_get_force_fields) are written
+ * without pairs, bitwise as with FT_LAYOUT_CORNER.  Creation picks the first that applies in the order
+ * CORNER_PAIRS, NODE_SLOTS, ELL (FT_LAYOUT_AUTO); ft_engine_set_layout
+ * chooses another that applies, before ft_run_init (FT_ERR_ARG when it does
+ * not apply, FT_ERR_STATE inside a run).  ell_width: the table's width (the
+ * largest node valence, at least 8). */
 #define FT_LAYOUT_AUTO 0
 #define FT_LAYOUT_CORNER 1
 #define FT_LAYOUT_NODE_SLOTS 2
 #define FT_LAYOUT_ELL 3
+#define FT_LAYOUT_CORNER_PAIRS 4
 int ft_engine_layout(ft_engine *eng, int *layout, int *ell_width);
 int ft_engine_set_layout(ft_engine *eng, int layout);
 
