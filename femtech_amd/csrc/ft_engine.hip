@@ -71,6 +71,16 @@ constexpr int kSkipBit = 1 << 30;
 // right lane
 constexpr int kPairLeft = 1 << 28;
 constexpr int kPairRight = 1 << 27;
+// corner pairs without a right partner: the xi+ corners go to the node's
+// free xi- slots (no element has the node at that corner: a boundary node)
+constexpr int kPlusToMinus = 1 << 26;
+// element tile flags: the tile's wavefronts own 63 elements each behind a
+// shadow lane (corner pairs, laws without history: a shadow of a Prony
+// element would read the history its owner is rewriting in the same
+// launch); the element before the tile belongs to its segment and instance
+// (its first wavefront's shadow lane computes it)
+constexpr int kTileShadows = 4;
+constexpr int kTileShadow = 2;
 constexpr int kPidMask = (1 << 20) - 1;
 constexpr int kMaxMotions = 64;  // distinct motions per instance (held in LDS by the nodal kernel)
 constexpr int kEaccStride = 8;  // doubles per nodal block in the energy accumulators
@@ -436,8 +446,19 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
   __shared__ double sh_min[kBlock / 64];
   if (L >= a.n_tiles) return;  // grid padding (uniform over the block)
   const int4 tile = a.tiles[L];
-  const int e = tile.x + threadIdx.x;
   const int end = tile.y;
+  // Corner pairs: each wavefront owns 63 consecutive elements (lanes 1-63)
+  // and its lane 0 recomputes the element before them (a shadow: it writes
+  // nothing, but hands its xi+ corner forces to lane 1), so every pair of
+  // face neighbours along xi meets in one wavefront.
+  const int lane = threadIdx.x & 63;
+  int e = tile.x + threadIdx.x;
+  bool owner = true, active = e < end;
+  if (a.pair && (tile.w & kTileShadows)) {  // uniform
+    e = tile.x + (int)(threadIdx.x >> 6) * 63 + lane - 1;
+    owner = lane != 0;
+    active = owner ? e < end : (e + 1 < end && ((threadIdx.x >> 6) != 0 || (tile.w & kTileShadow)));
+  }
   double dt_step = a.dt_fixed;
   double dt_guide = 1e300;  // no previous minimum: refine every element
   unsigned long long *dtmin = a.dtbits ? a.dtbits + tile.z : nullptr;
@@ -450,8 +471,8 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
   }
   double my_dt = 1e300;
   int flags = 0;
-  SurrTally tally = {{0, 0, 0, 0, 0, 0}};
-  if (e < end) {
+  SurrTally tally = {{0, 0, 0, 0, 0, 0}}, shadow_tally = tally;
+  if (active) {
     const int4 c0 = reinterpret_cast<const int4 *>(a.conn)[2 * e];
     const int4 c1 = reinterpret_cast<const int4 *>(a.conn)[2 * e + 1];
     const int nd[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
@@ -491,7 +512,8 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       const SurrIn sin{a.surr_y + (size_t)(e - a.surr_e0) * kSurrOut, a.surr_stride, a.surr_norm,
                        a.surr_audit != 0 && audited(e, a.surr_permille)};
       element_eval<R, FORCE, DT, MAT, HistSoA<R>, true>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
-                                                       a.update_hist != 0, outv, my_dt, flags, fib, &sin, &tally);
+                                                       a.update_hist != 0 && owner, outv, my_dt, flags, fib, &sin,
+                                                       owner ? &tally : &shadow_tally);
     } else if constexpr (GPOUT) {
       const size_t ce = (size_t)a.perm[e];
       const GpSink gs{a.gp_F + 72 * ce, a.gp_detF + 8 * ce, a.gp_pk2 + 48 * ce};
@@ -502,7 +524,7 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       // FP32 run loop: exact lengths only within kSkipMargin of the last minimum
       const double exact_below = sizeof(R) == 4 && a.refine ? dt_guide * kSkipMargin : 1e300;
       element_eval<R, FORCE, DT, MAT>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
-                                 a.update_hist != 0, outv, my_dt, flags, fib, nullptr, nullptr, nullptr,
+                                 a.update_hist != 0 && owner, outv, my_dt, flags, fib, nullptr, nullptr, nullptr,
                                  exact_below);
     }
     if (FORCE) {
@@ -525,14 +547,17 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
           dst[2] = outv[3 * k + 2];
         }
       } else {
-        bool right = false;  // corner pairs: the xi+ corners go to the right lane
-        if (a.pair) {        // uniform (corner slots)
+        // corner pairs: the xi+ corners go to the right lane, or to the
+        // node's free xi- slot
+        bool right = false, to_minus = false;
+        if (a.pair) {  // uniform (corner slots)
           // C3D8 corners on the face xi = -1 (0, 3, 4, 7) against the same
           // corners of the left neighbour's face xi = +1 (1, 2, 5, 6); the
-          // lane to the left is active whenever this one is (tiles fill
-          // their lanes from the first)
+          // lane to the left is active whenever this one has a left partner
+          // (a shadow lane 0, or an owner lane)
           const bool left = (info & kPairLeft) != 0;
           right = (info & kPairRight) != 0;
+          to_minus = (info & kPlusToMinus) != 0;
           constexpr int km[4] = {0, 3, 4, 7}, kp[4] = {1, 2, 5, 6};
 #pragma unroll
           for (int j = 0; j < 4; ++j)
@@ -542,14 +567,19 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
               if (left) outv[3 * km[j] + d] += w;
             }
         }
+        if (owner) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          if (right && ((k + 1) & 2)) continue;  // k = 1, 2, 5, 6
-          R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);  // uniform
-          const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
-          at_byte(dst, o) = outv[3 * k];
-          at_byte(dst, o + (unsigned)sizeof(R)) = outv[3 * k + 1];
-          at_byte(dst, o + 2u * (unsigned)sizeof(R)) = outv[3 * k + 2];
+          for (int k = 0; k < 8; ++k) {
+            const bool plus = ((k + 1) & 2) != 0;  // k = 1, 2, 5, 6
+            if (plus && right) continue;
+            // uniform slot bases (k, and for the free xi- slot k ^ 1)
+            R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);
+            if (plus && to_minus) dst = a.fe + 3 * (size_t)(k ^ 1) * a.nn;
+            const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
+            at_byte(dst, o) = outv[3 * k];
+            at_byte(dst, o + (unsigned)sizeof(R)) = outv[3 * k + 1];
+            at_byte(dst, o + 2u * (unsigned)sizeof(R)) = outv[3 * k + 2];
+          }
         }
       }
     }
@@ -578,7 +608,7 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       __shared__ double sh_c[7][3];
       if (threadIdx.x == 0) sh_arg = INT32_MAX;
       __syncthreads();
-      if (e < end && my_dt == bmin) atomicMin(&sh_arg, e);
+      if (active && my_dt == bmin) atomicMin(&sh_arg, e);
       __syncthreads();
       const int es = sh_arg;
       const double ce = a.parts[a.einfo[es] & kPidMask].ce;
@@ -783,7 +813,27 @@ template <typename R, int W>
 __device__ inline void gather3(const int *ell, int nn, int width, const R *fe, int n, double f[3],
                                const Overflow<R> &ov = Overflow<R>{nullptr, nullptr, 0}) {
   f[0] = f[1] = f[2] = 0.0;
-  if (W < 0) {
+  if (W == -2) {
+    // Corner pairs, every element paired with its right neighbour or
+    // writing its xi+ corners to free xi- slots: the four xi- slots hold
+    // the node's whole force.
+    R v[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = (0x7430 >> (4 * j)) & 15;  // 0, 3, 4, 7
+      const R *src = fe + 3 * (size_t)k * nn;  // uniform
+      const unsigned o = 3u * (unsigned)n;
+      v[j][0] = src[o];
+      v[j][1] = src[o + 1];
+      v[j][2] = src[o + 2];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      f[0] += (double)v[j][0];
+      f[1] += (double)v[j][1];
+      f[2] += (double)v[j][2];
+    }
+  } else if (W < 0) {
     // Corner-slot layout fe[corner][node][xyz]: the element that has node n
     // as its local corner k wrote slot k (zero where no element does; with
     // corner pairs, zero also where the element's right partner wrote the
@@ -2004,16 +2054,24 @@ struct ft_engine {
   std::vector<unsigned> h_eslot;    // the same in the caller's element order
   std::vector<int> h_ovf;           // host copy of ovf
   int layout = 0;                   // FT_LAYOUT_* in use
-  // Corner pairs (layout FT_LAYOUT_CORNER_PAIRS, set by build_pairs for the
-  // current tiles): element i and i - 1 of one wavefront share the face
-  // xi = +1 of i - 1; i adds i - 1's four corner forces there to its own and
-  // writes the sums, i - 1 writes only its other four.  plive: per node, bit
-  // k set when slot k is written (all xi- slots, and the xi+ slots of
-  // elements without a right partner); the other slots hold zeros, so the
-  // node sum reads all eight as with plain corner slots.
+  // Corner pairs (layout FT_LAYOUT_CORNER_PAIRS, set by build_pairs): when
+  // element i - 1 (same segment and instance) shares its face xi = +1 with
+  // element i, i adds i - 1's four corner forces there to its own and writes
+  // the sums, i - 1 writes only its other four (each wavefront's lane 0
+  // recomputes the element before the wavefront's own, so pairs meet in
+  // registers; segments of the Prony laws have no such shadow lanes and
+  // pair only within a wavefront).  An element without a right partner whose xi+ corners'
+  // matching xi- slots no element owns (a row end of a block) writes those
+  // corners there.  Complete pairing (every element one or the other): the
+  // node sum reads the four xi- slots only; otherwise all eight, the xi+
+  // slots the loop does not write holding zeros.  pzero, [2][nn]: per node
+  // the slots to zero before a pass outside the run loop (which writes every
+  // element's own eight corners; the free slots the loop writes) and after
+  // it (the xi+ slots the loop leaves unwritten).
   bool paired = false;
-  bool unpaired = false;  // set while ft_element_forces writes every corner
-  unsigned char *plive = nullptr;
+  bool pair_complete = false;  // every element paired or writing to free slots: the node sum reads 4 slots
+  bool unpaired = false;       // set while a pass outside the run loop writes every corner
+  unsigned char *pzero = nullptr;  // per node: slots to zero around passes outside the run loop
   long n_pairs = 0;
   std::vector<int> h_conn;  // caller's connectivity (element-force read-back)
   PartDev *parts = nullptr;
@@ -2501,7 +2559,9 @@ int launch_nodal_t(ft_engine *g, int mode, int t0, int t1) {
   const dim3 grid(grid_for_tiles(t1 - t0));
   if (g->rot_motions) {
     const size_t lds = nodal_lds_bytes<true>(g->mcap);
-    if (g->corner)
+    if (g->pair_complete)
+      hipLaunchKernelGGL((k_nodal<R, -2, true>), grid, dim3(kBlock), lds, g->lstream, a);
+    else if (g->corner)
       hipLaunchKernelGGL((k_nodal<R, -1, true>), grid, dim3(kBlock), lds, g->lstream, a);
     else if (g->width == 8)
       hipLaunchKernelGGL((k_nodal<R, 8, true>), grid, dim3(kBlock), lds, g->lstream, a);
@@ -2509,7 +2569,9 @@ int launch_nodal_t(ft_engine *g, int mode, int t0, int t1) {
       hipLaunchKernelGGL((k_nodal<R, 0, true>), grid, dim3(kBlock), lds, g->lstream, a);
   } else {
     const size_t lds = nodal_lds_bytes<false>(g->mcap);
-    if (g->corner)
+    if (g->pair_complete)
+      hipLaunchKernelGGL((k_nodal<R, -2, false>), grid, dim3(kBlock), lds, g->lstream, a);
+    else if (g->corner)
       hipLaunchKernelGGL((k_nodal<R, -1, false>), grid, dim3(kBlock), lds, g->lstream, a);
     else if (g->width == 8)
       hipLaunchKernelGGL((k_nodal<R, 8, false>), grid, dim3(kBlock), lds, g->lstream, a);
@@ -2588,9 +2650,12 @@ int launch_persistent(ft_engine *g, int steps) {
   bool all_nh = !g->segs.empty();
   for (const auto &sg : g->segs) all_nh = all_nh && sg.mat == 1;
   if (g->precision == FT_FP32) {
+    if (g->pair_complete)
+      return all_nh ? launch_persistent_k<float, 1, -2>(g, steps) : launch_persistent_k<float, -1, -2>(g, steps);
     if (g->corner) return all_nh ? launch_persistent_k<float, 1, -1>(g, steps) : launch_persistent_k<float, -1, -1>(g, steps);
     return launch_persistent_k<float, -1, 0>(g, steps);
   }
+  if (g->pair_complete) return launch_persistent_k<double, -1, -2>(g, steps);
   if (g->corner) return launch_persistent_k<double, -1, -1>(g, steps);
   return launch_persistent_k<double, -1, 0>(g, steps);
 }
@@ -2636,81 +2701,116 @@ int injury_launch(ft_engine *g, const StepSlot *slot, double time, double dt) {
   return 0;
 }
 
-// Corner pairs: zero the corner slots no element writes (live bit clear).
+// Corner pairs: zero the slots marked in mask[n].
 template <typename R>
-__global__ void k_zero_unwritten(int nn, const unsigned char *live, R *fe) {
+__global__ void k_zero_slots(int nn, const unsigned char *mask, R *fe) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
-  const unsigned lv = live[n];
+  const unsigned m = mask[n];
+  if (!m) return;
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    if (!((lv >> k) & 1u)) {
+    if ((m >> k) & 1u) {
       R *dst = fe + 3 * ((size_t)k * nn + n);
       dst[0] = dst[1] = dst[2] = R(0);
     }
 }
 
-int zero_unwritten(ft_engine *g) {
-  if (!g->paired) return 0;
+// which 0: the free slots the run loop writes; 1: the slots it leaves
+// unwritten.
+int zero_slots(ft_engine *g, int which) {
+  if (!g->paired || !g->pzero) return 0;
+  const unsigned char *mask = g->pzero + (size_t)which * g->nn;
   if (g->precision == FT_FP32)
-    hipLaunchKernelGGL(k_zero_unwritten<float>, dim3((g->nn + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream,
-                       g->nn, g->plive, reinterpret_cast<float *>(g->fe));
+    hipLaunchKernelGGL(k_zero_slots<float>, dim3((g->nn + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream,
+                       g->nn, mask, reinterpret_cast<float *>(g->fe));
   else
-    hipLaunchKernelGGL(k_zero_unwritten<double>, dim3((g->nn + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream,
-                       g->nn, g->plive, reinterpret_cast<double *>(g->fe));
+    hipLaunchKernelGGL(k_zero_slots<double>, dim3((g->nn + kBlock - 1) / kBlock), dim3(kBlock), 0, g->stream,
+                       g->nn, mask, reinterpret_cast<double *>(g->fe));
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 // Element passes outside the run loop (forces, per-element forces and
 // per-Gauss-point fields asked for by the caller) write all eight corners of
-// every element, as the plain corner layout does; the slots corner pairs
-// leave unwritten are zeroed again when the pass's results have been read.
+// every element into its own slots, as the plain corner layout does, and the
+// force read-back sums all eight slots: the free slots the run loop writes
+// are zeroed first, and the slots the run loop leaves unwritten are zeroed
+// again when the pass's results have been read.
 struct Unpaired {
   ft_engine *g;
-  explicit Unpaired(ft_engine *e) : g(e) { g->unpaired = true; }
+  explicit Unpaired(ft_engine *e) : g(e) {
+    g->unpaired = true;
+    (void)zero_slots(g, 0);
+  }
   ~Unpaired() {
     g->unpaired = false;
-    (void)zero_unwritten(g);
+    (void)zero_slots(g, 1);
   }
 };
 
-// einfo's pair bits from bits[e] (1: kPairLeft, 2: kPairRight).
+// einfo's pair bits from bits[e] (1: kPairLeft, 2: kPairRight, 4: kPlusToMinus).
 __global__ void k_pair_bits(int ne, const unsigned char *bits, int *einfo) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int b = bits[e];
-  einfo[e] = (einfo[e] & ~(kPairLeft | kPairRight)) | ((b & 1) ? kPairLeft : 0) | ((b & 2) ? kPairRight : 0);
+  einfo[e] = (einfo[e] & ~(kPairLeft | kPairRight | kPlusToMinus)) | ((b & 1) ? kPairLeft : 0) |
+             ((b & 2) ? kPairRight : 0) | ((b & 4) ? kPlusToMinus : 0);
 }
 
 // Corner pairs for the current layout and element tiles (FT_LAYOUT_CORNER_PAIRS;
 // every other layout clears them): element i pairs with i - 1 when both lie
-// in one wavefront of one tile and i's corners 0, 3, 4, 7 are i - 1's
-// corners 1, 2, 5, 6 (a face shared along xi: consecutive elements of a row
-// of a structured block).  Sets the pair bits in einfo and the per-node slot
-// mask the node sum reads.
+// in one material segment and instance and i's corners 0, 3, 4, 7 are
+// i - 1's corners 1, 2, 5, 6 (a face shared along xi: consecutive elements of
+// a row of a structured block).  Sets the pair bits in einfo, decides
+// whether pairing is complete and zeroes the slots the run loop leaves
+// unwritten.
 int build_pairs(ft_engine *g) {
   const int ne = g->ne, nn = g->nn;
-  std::vector<unsigned char> bits(ne, 0), live(nn, 0xff);
+  std::vector<unsigned char> bits(ne, 0), zero(2 * (size_t)nn, 0);
   long pairs = 0;
-  if (g->layout == FT_LAYOUT_CORNER_PAIRS && g->corner && !g->eslot) {
+  bool complete = false;
+  if (g->layout == FT_LAYOUT_CORNER_PAIRS && g->corner && !g->eslot && !g->h_etiles.empty()) {
     const int *conn = g->h_conn.data();
+    std::vector<int> inst_of(ne, 0);
+    for (int i = 0; i < g->n_inst; ++i)
+      for (int e = g->inst_elem[i]; e < g->inst_elem[i + 1]; ++e) inst_of[e] = i;
+    // with shadow lanes any two consecutive elements of a segment and
+    // instance meet in one wavefront; without (the Prony laws) only those
+    // after a wavefront's first lane
     for (const int4 &t : g->h_etiles)
-      for (int i = t.x + 1; i < t.y; ++i) {
-        if ((i - t.x) % 64 == 0) continue;  // first lane of a wavefront
+      for (int i = t.x; i < t.y; ++i) {
+        const bool meet = (t.w & kTileShadows) ? (i > t.x || (t.w & kTileShadow)) : (i - t.x) % 64 != 0;
+        if (!meet) continue;
         const int *a = conn + 8 * (size_t)g->perm[i - 1], *b = conn + 8 * (size_t)g->perm[i];
-        if (b[0] == a[1] && b[3] == a[2] && b[4] == a[5] && b[7] == a[6]) {
+        if (inst_of[g->perm[i - 1]] == inst_of[g->perm[i]] && b[0] == a[1] && b[3] == a[2] && b[4] == a[5] &&
+            b[7] == a[6]) {
           bits[i] |= 1;
           bits[i - 1] |= 2;
           ++pairs;
         }
       }
-    // the xi+ slots of elements with a right partner are never written
-    for (int i = 0; i < ne; ++i)
-      if (bits[i] & 2) {
-        const int *c = conn + 8 * (size_t)g->perm[i];
-        for (int k : {1, 2, 5, 6}) live[c[k]] &= (unsigned char)~(1u << k);
+    // slots some element owns (it has the node at that local corner)
+    std::vector<unsigned char> owned(nn, 0);
+    for (long i = 0; i < 8L * ne; ++i) owned[conn[i]] |= (unsigned char)(1u << (i % 8));
+    // An element without a right partner whose xi+ corners' matching xi-
+    // slots are free writes those corners there (the same choice in a batch
+    // as alone, so the instances' sums stay bitwise the same).  Slots to
+    // zero before a pass outside the loop: those free slots (written by the
+    // loop); after it: the xi+ slots the loop does not write.
+    complete = pairs > 0;
+    for (int i = 0; i < ne; ++i) {
+      const int *c = conn + 8 * (size_t)g->perm[i];
+      bool free4 = !(bits[i] & 2);
+      for (int k : {1, 2, 5, 6}) free4 = free4 && !((owned[c[k]] >> (k ^ 1)) & 1);
+      if (free4) {
+        bits[i] |= 4;
+        for (int k : {1, 2, 5, 6}) zero[c[k]] |= (unsigned char)(1u << (k ^ 1));
       }
+      if (bits[i] & 6)
+        for (int k : {1, 2, 5, 6}) zero[nn + (size_t)c[k]] |= (unsigned char)(1u << k);
+      complete = complete && (bits[i] & 6);
+    }
   }
   unsigned char *dbits = nullptr;
   int rc = dalloc(&dbits, (size_t)ne);
@@ -2723,12 +2823,13 @@ int build_pairs(ft_engine *g) {
   const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(g->stream);
   (void)hipFree(dbits);
   if (e1 != hipSuccess || e2 != hipSuccess) return fail(FT_ERR_HIP, "pair bits: %s", hipGetErrorString(e1 ? e1 : e2));
-  if (pairs > 0 && !g->plive && (rc = dalloc(&g->plive, (size_t)nn))) return rc;
-  if (pairs > 0) HIPCHK(hipMemcpy(g->plive, live.data(), nn, hipMemcpyHostToDevice));
+  if (pairs > 0 && !g->pzero && (rc = dalloc(&g->pzero, 2 * (size_t)nn))) return rc;
+  if (pairs > 0) HIPCHK(hipMemcpy(g->pzero, zero.data(), 2 * (size_t)nn, hipMemcpyHostToDevice));
   g->paired = pairs > 0;
+  g->pair_complete = complete;
   g->n_pairs = pairs;
-  // slots the new pairs leave unwritten may hold earlier forces
-  if ((rc = zero_unwritten(g))) return rc;
+  // slots the new pairing leaves unwritten may hold earlier forces
+  if ((rc = zero_slots(g, 0)) || (rc = zero_slots(g, 1))) return rc;
   HIPCHK(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -2739,14 +2840,28 @@ int build_element_tiles(ft_engine *g) {
   std::vector<int> inst_of_elem(g->ne);
   for (int i = 0; i < g->n_inst; ++i)
     for (int e = g->inst_elem[i]; e < g->inst_elem[i + 1]; ++e) inst_of_elem[e] = i;
+  // corner pairs: 63 elements per wavefront (lane 0 recomputes the element
+  // before them)
+  bool shadows = false;
+  if (g->layout == FT_LAYOUT_CORNER_PAIRS && g->corner && !g->eslot) {
+    const int *conn = g->h_conn.data();
+    for (const auto &sg : g->segs)
+      for (int i = sg.begin + 1; i < sg.end && !shadows; ++i) {
+        const int *a = conn + 8 * (size_t)g->perm[i - 1], *b = conn + 8 * (size_t)g->perm[i];
+        shadows = b[0] == a[1] && b[3] == a[2] && b[4] == a[5] && b[7] == a[6];
+      }
+  }
   std::vector<int4> et;
   for (auto &sg : g->segs) {
     sg.tile_begin = (int)et.size();
+    const bool seg_shadows = shadows && sg.mat != 5 && sg.mat != 7;
+    const int cap = seg_shadows ? sg.tile / 64 * 63 : sg.tile;
     for (int i = sg.begin; i < sg.end;) {
       const int inst = inst_of_elem[g->perm[i]];
       int j = i + 1;
-      while (j < sg.end && j - i < sg.tile && inst_of_elem[g->perm[j]] == inst) ++j;
-      et.push_back(make_int4(i, j, inst, 0));
+      while (j < sg.end && j - i < cap && inst_of_elem[g->perm[j]] == inst) ++j;
+      const bool shadow_ok = seg_shadows && i > sg.begin && inst_of_elem[g->perm[i - 1]] == inst;
+      et.push_back(make_int4(i, j, inst, (seg_shadows ? kTileShadows : 0) | (shadow_ok ? kTileShadow : 0)));
       i = j;
     }
     sg.tile_end = (int)et.size();
@@ -3020,7 +3135,8 @@ static int set_layout(ft_engine *g, int mode) {
   g->h_ovf.swap(h_ovf);
   g->corner = corner;
   g->layout = mode;
-  return build_pairs(g);
+  // corner pairs change the elements per tile
+  return g->h_etiles.empty() ? build_pairs(g) : build_element_tiles(g);
 }
 
 int ft_engine_set_layout(ft_engine *g, int mode) {
@@ -3192,7 +3308,7 @@ int ft_engine_destroy(ft_engine *g) {
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->eslot, g->ovf, g->fe_ovf, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
-                  g->etiles, g->cand_geom, g->plive, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
+                  g->etiles, g->cand_geom, g->pzero, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -4660,7 +4776,8 @@ int ft_traffic_model(ft_engine *g, double bytes[2]) {
   // the nodes with more than 8 elements) or 24 es per element + the ELL incidences 4 width per node; per
   // node mass 8, motion code 4, u and v_half read + write 2 x 48 (X only at
   // constrained nodes and fi only when a run stops: not counted).
-  double nodal = (g->corner ? nn * (24.0 * es) : ne * (24.0 * es) + nn * 4.0 * g->width) +
+  // (complete corner pairs: the four xi- slots)
+  double nodal = (g->corner ? nn * ((g->pair_complete ? 12.0 : 24.0) * es) : ne * (24.0 * es) + nn * 4.0 * g->width) +
                  (g->ovf ? nn * 4.0 + g->n_ovf * 3.0 * es * (g->width - 8) : 0.0) +
                  nn * (8.0 + 4.0 + 2.0 * 48.0);
   bytes[0] = elem;

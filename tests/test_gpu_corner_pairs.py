@@ -1,10 +1,14 @@
 """Corner pairs (FT_LAYOUT_CORNER_PAIRS): in the run loop, element i and
-element i - 1 of one wavefront that share a face along xi (i's local nodes
-0, 3, 4, 7 are i - 1's nodes 1, 2, 5, 6) add that face's corner forces in
-registers; element i writes the four sums into its own corner slots and
-element i - 1 writes only its other four (about half of the element-force
-writes); the slots no element writes hold zeros, so the node sum reads all
-eight as with plain corner slots.
+element i - 1 of one material segment that share a face along xi (i's local
+nodes 0, 3, 4, 7 are i - 1's nodes 1, 2, 5, 6) add that face's corner
+forces in registers (each wavefront's lane 0 recomputes the element before
+the wavefront's own 63); element i writes the four sums into its own corner
+slots and element i - 1 writes only its other four.  When every element
+without a right partner has its xi+ corners at nodes whose matching xi-
+slot is free (a structured block: the row ends), those corners go there and
+the node sum reads only the four xi- slots; otherwise (the ellipsoid, whose
+material segments interleave along xi) it reads all eight, and the slots no
+element writes hold zeros.
 
 The pair sum is one more floating-point addition before the node sum (the
 reference's scatter-add, GetForce_3D.cpp:39-44, adds the eight element
@@ -57,12 +61,22 @@ def test_structured_meshes_take_corner_pairs(gpu_ready):
         e.set_layout(ft.LAYOUT_CORNER)
         assert e.layout()[0] == ft.LAYOUT_CORNER
         plain = e.traffic_model()
-        # about half of the corner slots are not written (the node sum reads all eight)
-        assert pairs[0] < 0.85 * plain[0] and pairs[1] == plain[1]
+        # about half of the corner slots are written, and the node sum reads four
+        assert pairs[0] < 0.85 * plain[0] and pairs[1] < 0.8 * plain[1]
         e.set_layout(ft.LAYOUT_CORNER_PAIRS)
         assert e.layout()[0] == ft.LAYOUT_CORNER_PAIRS
     with engine_for(block_case(reverse=True), ft.FT_FP32) as e:
         assert e.layout()[0] == ft.LAYOUT_CORNER
+    # incomplete pairing: the node sum reads all eight slots
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+    w["fibres"] = w.get("fibres")
+    with engine_for(w, ft.FT_FP32) as e:
+        assert e.layout()[0] == ft.LAYOUT_CORNER_PAIRS
+        pairs = e.traffic_model()
+        e.set_layout(ft.LAYOUT_CORNER)
+        plain = e.traffic_model()
+        assert pairs[0] < plain[0] and pairs[1] == plain[1]
     # a mesh off the corner layout cannot take pairs
     from test_general_mesh import general_case
     with engine_for(general_case("impact"), ft.FT_FP64) as e:
@@ -131,10 +145,17 @@ def test_forces_outside_the_loop_are_unpaired(gpu_ready):
 
 
 @pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
-def test_pairs_graph_and_persistent(gpu_ready, precision):
+@pytest.mark.parametrize("kind", ["block", "ellipsoid"])
+def test_pairs_graph_and_persistent(gpu_ready, precision, kind):
     """Captured chunks equal the per-step launches bit for bit with corner
-    pairs; the persistent loop agrees to round-off."""
-    c = block_case(n=16, steps=120)
+    pairs, complete (block) or not (ellipsoid); the persistent loop agrees
+    to round-off."""
+    if kind == "block":
+        c = block_case(n=16, steps=120)
+    else:
+        from femtech_amd import workloads as W
+        w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+        c = dict(w, fibres=w.get("fibres"), tmax=1e9, dt_factor=0.9, failure_dt=1e-11, steps=120)
     chunks = [1, 16, 40, 63]
     plain = run_engine(c, precision, history=False, chunks=chunks)
     graph = run_engine(c, precision, history=False, graph=16, chunks=chunks)
@@ -142,3 +163,33 @@ def test_pairs_graph_and_persistent(gpu_ready, precision):
     for x, y in zip(plain["state"], graph["state"]):
         assert np.array_equal(x, y)
     assert rel(pers["state"][0], plain["state"][0]) < (1e-6 if precision == ft.FT_FP32 else 1e-12)
+
+
+@pytest.mark.parametrize("precision", [ft.FT_FP64, ft.FT_FP32])
+def test_pairs_incomplete_ellipsoid(gpu_ready, precision):
+    """The three-material ellipsoid (pairs only inside each material's runs
+    along xi, eight-slot node sums): corner pairs against plain corner
+    slots, FP64 to 1e-12 and FP32 to 1e-6, the Prony history included."""
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+    c = dict(w, fibres=w.get("fibres"), tmax=1e9, dt_factor=0.9, failure_dt=1e-11, steps=200)
+    a = run_engine(c, precision, history=False, layout=ft.LAYOUT_CORNER_PAIRS)
+    b = run_engine(c, precision, history=False, layout=ft.LAYOUT_CORNER)
+    assert a["done"] == b["done"] == c["steps"]
+    bar = 1e-12 if precision == ft.FT_FP64 else 1e-6
+    assert rel(a["state"][0], b["state"][0]) < bar
+
+
+def test_pairs_repeat_bitwise_with_prony_history(gpu_ready):
+    """Two runs of the ellipsoid (Ogden + Prony core, anisotropic HGO) with
+    corner pairs give the same bits: no lane reads an element's Prony
+    history while another rewrites it (the Prony segments pair only inside
+    a wavefront, without shadow lanes)."""
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+    c = dict(w, fibres=w.get("fibres"), tmax=1e9, dt_factor=0.9, failure_dt=1e-11, steps=150)
+    a = run_engine(c, ft.FT_FP32, history=False, chunks=[1, 2, 47, 100])
+    b = run_engine(c, ft.FT_FP32, history=False, chunks=[150])
+    for x, y in zip(a["state"], b["state"]):
+        assert np.array_equal(x, y)
+    assert a["time"] == b["time"]
