@@ -1120,7 +1120,8 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
                                              bool update_hist, R outv[24], double &my_dt,
                                              int &flags, const R fib[3] = nullptr,
                                              const SurrIn *sin = nullptr, SurrTally *tally = nullptr,
-                                             const GpSink *gps = nullptr, double dt_exact_below = 1e300) {
+                                             const GpSink *gps = nullptr, double dt_exact_below = 1e300,
+                                             bool check_finite = true) {
   if (DT && !skip_dt) {
     // StableTimeStep.cpp:11-30: dt_e = Le / ce on the deformed geometry.
     R dcf[7][3];
@@ -1339,10 +1340,15 @@ __host__ __device__ inline void element_eval(R cx[7][3], R cu[7][3], const PartD
 #pragma unroll
       for (int p = 0; p < 8; ++p) outv[3 * p2n(p) + d] = c[p];
     }
-    bool finite = true;
+    // (the run loop's node update checks the node sums instead: every
+    // element force reaches one, and a non-finite value keeps the sum
+    // non-finite)
+    if (check_finite) {
+      bool finite = true;
 #pragma unroll
-    for (int k = 0; k < 24; ++k) finite = finite && isfinite(outv[k]);
-    if (!finite) flags |= kFlagNonFinite;
+      for (int k = 0; k < 24; ++k) finite = finite && isfinite(outv[k]);
+      if (!finite) flags |= kFlagNonFinite;
+    }
   }
 }
 

@@ -525,7 +525,7 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       const double exact_below = sizeof(R) == 4 && a.refine ? dt_guide * kSkipMargin : 1e300;
       element_eval<R, FORCE, DT, MAT>(cx, cu, pp, (info & kSkipBit) != 0, dt_step, hist,
                                  a.update_hist != 0 && owner, outv, my_dt, flags, fib, nullptr, nullptr, nullptr,
-                                 exact_below);
+                                 exact_below, a.ctl == nullptr);
     }
     if (FORCE) {
       // Slot-major layouts: fe[slot][element][xyz] or fe[corner][node][xyz].
@@ -1093,6 +1093,9 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
       f[2] = a.fi[2 * (size_t)a.nn + n];
     } else {
       gather3<R, W>(a.ell, a.nn, a.width, a.fe, n, f, a.ov);
+      // non-finite element forces (the element kernels of the run loop
+      // leave this check to the node sums)
+      if (!(isfinite(f[0]) && isfinite(f[1]) && isfinite(f[2]))) atomicOr(a.status, FT_STATUS_NONFINITE);
     }
     const double m = a.mass[n];
     const int code = a.node_code[n];  // byte d: 0 free, else motion index + 1
