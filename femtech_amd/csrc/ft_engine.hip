@@ -74,6 +74,14 @@ constexpr int kPairRight = 1 << 27;
 // corner pairs without a right partner: the xi+ corners go to the node's
 // free xi- slots (no element has the node at that corner: a boundary node)
 constexpr int kPlusToMinus = 1 << 26;
+// corner pairs, an element with neither (a partner-less element whose
+// neighbour along xi lies in another material segment, or across a
+// wavefront boundary of a Prony segment): its xi+ corner j (1, 2, 5, 6 ->
+// j = 0..3) goes to the free xi- slot when bit kExtraFree + j is set, else
+// to its extras entry, which k_pair_extras adds to the owned xi- slot
+// before the node sum
+constexpr int kExtra = 1 << 25;
+constexpr int kExtraFree = 20;
 // element tile flags: the tile's wavefronts own 63 elements each behind a
 // shadow lane (corner pairs, laws without history: a shadow of a Prony
 // element would read the history its owner is rewriting in the same
@@ -353,6 +361,8 @@ struct ElemArgs {
   int parity;
   int refine;            // FP32 kernel in the run loop: FP64 dt of near-minimum elements
   int pair;              // corner pairs (kPairLeft / kPairRight in einfo)
+  R *xf;                 // corner pairs: extras values, 4 x 3 per kExtra element
+  const int *xidx;       // corner pairs: per element, its first extras entry (kExtra elements)
   double *cand_geom;       // FP32 run loop: kCandStride per element tile (candidate geometry)
   // per-Gauss-point fields (k_element_gp): 72 / 8 / 48 doubles per element in
   // the caller's element order (perm: internal -> caller)
@@ -549,7 +559,7 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
       } else {
         // corner pairs: the xi+ corners go to the right lane, or to the
         // node's free xi- slot
-        bool right = false, to_minus = false;
+        bool right = false, to_minus = false, extra = false;
         if (a.pair) {  // uniform (corner slots)
           // C3D8 corners on the face xi = -1 (0, 3, 4, 7) against the same
           // corners of the left neighbour's face xi = +1 (1, 2, 5, 6); the
@@ -558,6 +568,7 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
           const bool left = (info & kPairLeft) != 0;
           right = (info & kPairRight) != 0;
           to_minus = (info & kPlusToMinus) != 0;
+          extra = (info & kExtra) != 0;
           constexpr int km[4] = {0, 3, 4, 7}, kp[4] = {1, 2, 5, 6};
 #pragma unroll
           for (int j = 0; j < 4; ++j)
@@ -568,13 +579,22 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
             }
         }
         if (owner) {
+          const int xb = extra ? a.xidx[e] : 0;
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             const bool plus = ((k + 1) & 2) != 0;  // k = 1, 2, 5, 6
             if (plus && right) continue;
+            const int j = (k >> 1 & 2) | (k == 2 || k == 6);  // 1, 2, 5, 6 -> 0, 1, 2, 3
+            if (plus && extra && !((info >> (kExtraFree + j)) & 1)) {
+              R *x = a.xf + 3 * ((size_t)xb + j);
+              x[0] = outv[3 * k];
+              x[1] = outv[3 * k + 1];
+              x[2] = outv[3 * k + 2];
+              continue;
+            }
             // uniform slot bases (k, and for the free xi- slot k ^ 1)
             R *dst = a.fe + 3 * (size_t)k * (a.corner ? a.nn : a.ne);
-            if (plus && to_minus) dst = a.fe + 3 * (size_t)(k ^ 1) * a.nn;
+            if (plus && (to_minus || extra)) dst = a.fe + 3 * (size_t)(k ^ 1) * a.nn;
             const unsigned o = 3u * (unsigned)sizeof(R) * (unsigned)(a.corner ? nd[k] : e);
             at_byte(dst, o) = outv[3 * k];
             at_byte(dst, o + (unsigned)sizeof(R)) = outv[3 * k + 1];
@@ -1292,10 +1312,25 @@ __global__ void k_history(const double *eacc, int nblk, Ctl *ctl, int parity, do
 // ---------------------------------------------------------------------------
 constexpr int kStatusBarrier = 1 << 20;  // a barrier wait hit its bound
 
+// Corner pairs: the partner-less elements' extras added to the xi- slots
+// their nodes' owners wrote this step (each slot takes at most one extra:
+// the order is fixed).
+template <typename R>
+__device__ inline void pair_extras(int i, const int2 *xadd, const R *xf, R *fe) {
+  const int2 t = xadd[i];
+  R *dst = fe + t.x;
+  const R *src = xf + 3 * (size_t)t.y;
+  dst[0] += src[0];
+  dst[1] += src[1];
+  dst[2] += src[2];
+}
+
 template <typename R>
 struct PersistArgs {
   ElemArgs<R> ea;        // all element tiles
   NodalArgs<R> na;       // all node tiles, kStep
+  const int2 *xadd;      // corner-pair extras (n_xadd entries) added between the phases
+  int n_xadd;
   int steps;
   int parity0;
   int n_inst;
@@ -1399,6 +1434,11 @@ __global__ __launch_bounds__(kBlock, (persist_waves<R, MAT>())) void k_persisten
         element_tile_any<R>(ea, L);
     }
     grid_barrier(p.arrive, ++b, na.status, p.spins);
+    if (p.n_xadd > 0) {  // uniform
+      for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n_xadd; i += gridDim.x * blockDim.x)
+        pair_extras(i, p.xadd, ea.xf, ea.fe);
+      grid_barrier(p.arrive, ++b, na.status, p.spins);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) p.running[(k + 1) % 3] = 0;
     na.parity = parity;
     na.running = p.running + k % 3;
@@ -2072,10 +2112,16 @@ struct ft_engine {
   // element's own eight corners; the free slots the loop writes) and after
   // it (the xi+ slots the loop leaves unwritten).
   bool paired = false;
-  bool pair_complete = false;  // every element paired or writing to free slots: the node sum reads 4 slots
+  bool pair_complete = false;  // every xi+ corner force reaches a xi- slot: the node sum reads 4 slots
   bool unpaired = false;       // set while a pass outside the run loop writes every corner
   unsigned char *pzero = nullptr;  // per node: slots to zero around passes outside the run loop
   long n_pairs = 0;
+  // extras of the partner-less elements (kExtra): values [4 n][3], per
+  // element its first entry, and the (fe offset, entry) pairs k_pair_extras adds
+  void *xf = nullptr;
+  int *xidx = nullptr;
+  int2 *xadd = nullptr;
+  int n_xadd = 0;
   std::vector<int> h_conn;  // caller's connectivity (element-force read-back)
   PartDev *parts = nullptr;
   std::vector<PartDev> h_parts;
@@ -2320,6 +2366,8 @@ ElemArgs<R> elem_args(ft_engine *g, int t0, int t1, bool in_loop, double dt_fixe
   a.parity = g->parity;
   a.refine = (sizeof(R) == 4 && in_loop) ? 1 : 0;
   a.pair = (g->paired && !g->unpaired) ? 1 : 0;
+  a.xf = reinterpret_cast<R *>(g->xf);
+  a.xidx = g->xidx;
   a.cand_geom = g->cand_geom;
   a.gp_F = a.gp_detF = a.gp_pk2 = nullptr;
   a.perm = nullptr;
@@ -2615,6 +2663,8 @@ int launch_persistent_k(ft_engine *g, int steps) {
   p.na = nodal_args<R>(g, kStep, 0, g->n_ntiles);
   p.na.inline_motions = g->timed_motions ? 1 : 0;
   p.na.cand_geom = g->precision == FT_FP32 ? g->cand_geom : nullptr;
+  p.xadd = g->paired ? g->xadd : nullptr;
+  p.n_xadd = g->paired ? g->n_xadd : 0;
   p.steps = steps;
   p.parity0 = g->parity;
   p.n_inst = g->n_inst;
@@ -2752,13 +2802,35 @@ struct Unpaired {
   }
 };
 
-// einfo's pair bits from bits[e] (1: kPairLeft, 2: kPairRight, 4: kPlusToMinus).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_pair_extras(int n, const int2 *xadd, const R *xf, R *fe) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pair_extras(i, xadd, xf, fe);
+}
+
+int launch_pair_extras(ft_engine *g) {
+  if (!g->paired || g->n_xadd == 0) return 0;
+  Timed t(g, 2);
+  const dim3 grid((g->n_xadd + kBlock - 1) / kBlock);
+  if (g->precision == FT_FP32)
+    hipLaunchKernelGGL(k_pair_extras<float>, grid, dim3(kBlock), 0, g->lstream, g->n_xadd, g->xadd,
+                       reinterpret_cast<const float *>(g->xf), reinterpret_cast<float *>(g->fe));
+  else
+    hipLaunchKernelGGL(k_pair_extras<double>, grid, dim3(kBlock), 0, g->lstream, g->n_xadd, g->xadd,
+                       reinterpret_cast<const double *>(g->xf), reinterpret_cast<double *>(g->fe));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// einfo's pair bits from bits[e] (1: kPairLeft, 2: kPairRight, 4: kPlusToMinus,
+// 8: kExtra; bits 4-7: kExtraFree + 0..3).
 __global__ void k_pair_bits(int ne, const unsigned char *bits, int *einfo) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int b = bits[e];
-  einfo[e] = (einfo[e] & ~(kPairLeft | kPairRight | kPlusToMinus)) | ((b & 1) ? kPairLeft : 0) |
-             ((b & 2) ? kPairRight : 0) | ((b & 4) ? kPlusToMinus : 0);
+  einfo[e] = (einfo[e] & ~(kPairLeft | kPairRight | kPlusToMinus | kExtra | (15 << kExtraFree))) |
+             ((b & 1) ? kPairLeft : 0) | ((b & 2) ? kPairRight : 0) | ((b & 4) ? kPlusToMinus : 0) |
+             ((b & 8) ? kExtra : 0) | (((b >> 4) & 15) << kExtraFree);
 }
 
 // Corner pairs for the current layout and element tiles (FT_LAYOUT_CORNER_PAIRS;
@@ -2771,6 +2843,9 @@ __global__ void k_pair_bits(int ne, const unsigned char *bits, int *einfo) {
 int build_pairs(ft_engine *g) {
   const int ne = g->ne, nn = g->nn;
   std::vector<unsigned char> bits(ne, 0), zero(2 * (size_t)nn, 0);
+  std::vector<int> xidx(ne, 0);
+  std::vector<int2> xadd;
+  int n_extras = 0;
   long pairs = 0;
   bool complete = false;
   if (g->layout == FT_LAYOUT_CORNER_PAIRS && g->corner && !g->eslot && !g->h_etiles.empty()) {
@@ -2796,24 +2871,40 @@ int build_pairs(ft_engine *g) {
     // slots some element owns (it has the node at that local corner)
     std::vector<unsigned char> owned(nn, 0);
     for (long i = 0; i < 8L * ne; ++i) owned[conn[i]] |= (unsigned char)(1u << (i % 8));
-    // An element without a right partner whose xi+ corners' matching xi-
-    // slots are free writes those corners there (the same choice in a batch
-    // as alone, so the instances' sums stay bitwise the same).  Slots to
-    // zero before a pass outside the loop: those free slots (written by the
-    // loop); after it: the xi+ slots the loop does not write.
+    // An element without a right partner writes each xi+ corner to the
+    // matching xi- slot of its node when no element owns that slot, else to
+    // an extras entry added to it after the element pass (the same choices
+    // in a batch as alone, so the instances' sums stay bitwise the same).
+    // Every corner force then reaches a xi- slot.  Slots to zero before a
+    // pass outside the loop: the free slots the loop writes; after it: the
+    // xi+ slots the loop does not write.
     complete = pairs > 0;
-    for (int i = 0; i < ne; ++i) {
+    int n_extra = 0;
+    for (int i = 0; i < ne && complete; ++i) {
+      if (bits[i] & 2) continue;
       const int *c = conn + 8 * (size_t)g->perm[i];
-      bool free4 = !(bits[i] & 2);
-      for (int k : {1, 2, 5, 6}) free4 = free4 && !((owned[c[k]] >> (k ^ 1)) & 1);
-      if (free4) {
-        bits[i] |= 4;
-        for (int k : {1, 2, 5, 6}) zero[c[k]] |= (unsigned char)(1u << (k ^ 1));
+      int free_bits = 0;
+      for (int j = 0; j < 4; ++j) {
+        const int k = (0x6521 >> (4 * j)) & 15;  // 1, 2, 5, 6
+        if (!((owned[c[k]] >> (k ^ 1)) & 1)) {
+          free_bits |= 1 << j;
+          zero[c[k]] |= (unsigned char)(1u << (k ^ 1));
+        } else {
+          xadd.push_back(make_int2(3 * ((k ^ 1) * nn + c[k]), 4 * n_extra + j));
+        }
       }
-      if (bits[i] & 6)
-        for (int k : {1, 2, 5, 6}) zero[nn + (size_t)c[k]] |= (unsigned char)(1u << k);
-      complete = complete && (bits[i] & 6);
+      if (free_bits == 15) {
+        bits[i] |= 4;
+      } else {
+        bits[i] |= 8 | (free_bits << 4);
+        xidx[i] = 4 * n_extra++;
+      }
     }
+    for (int i = 0; i < ne && complete; ++i) {
+      const int *c = conn + 8 * (size_t)g->perm[i];
+      for (int k : {1, 2, 5, 6}) zero[nn + (size_t)c[k]] |= (unsigned char)(1u << k);
+    }
+    n_extras = n_extra;
   }
   unsigned char *dbits = nullptr;
   int rc = dalloc(&dbits, (size_t)ne);
@@ -2828,6 +2919,21 @@ int build_pairs(ft_engine *g) {
   if (e1 != hipSuccess || e2 != hipSuccess) return fail(FT_ERR_HIP, "pair bits: %s", hipGetErrorString(e1 ? e1 : e2));
   if (pairs > 0 && !g->pzero && (rc = dalloc(&g->pzero, 2 * (size_t)nn))) return rc;
   if (pairs > 0) HIPCHK(hipMemcpy(g->pzero, zero.data(), 2 * (size_t)nn, hipMemcpyHostToDevice));
+  for (void *p : {g->xf, (void *)g->xidx, (void *)g->xadd})
+    if (p) (void)hipFree(p);
+  g->xf = nullptr;
+  g->xidx = nullptr;
+  g->xadd = nullptr;
+  g->n_xadd = 0;
+  if (n_extras > 0) {
+    const size_t esz = g->precision == FT_FP32 ? 4 : 8;
+    if (hipMalloc(&g->xf, 12 * (size_t)n_extras * esz) != hipSuccess || hipMemset(g->xf, 0, 12 * (size_t)n_extras * esz) != hipSuccess ||
+        (rc = dalloc(&g->xidx, (size_t)ne)) || (rc = dalloc(&g->xadd, xadd.size())))
+      return rc ? rc : fail(FT_ERR_ALLOC, "corner-pair extras allocation failed");
+    HIPCHK(hipMemcpy(g->xidx, xidx.data(), ne * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g->xadd, xadd.data(), xadd.size() * sizeof(int2), hipMemcpyHostToDevice));
+    g->n_xadd = (int)xadd.size();
+  }
   g->paired = pairs > 0;
   g->pair_complete = complete;
   g->n_pairs = pairs;
@@ -3311,7 +3417,7 @@ int ft_engine_destroy(ft_engine *g) {
   void *ptrs[] = {g->X, g->conn, g->einfo, g->parts, g->u, g->vs, g->fi, g->tmp3, g->mass,
                   g->node_code, g->node_mask, g->motions, g->mstate, g->ctl, g->dtbits, g->status,
                   g->ell, g->fe, g->eslot, g->ovf, g->fe_ovf, g->hist, g->fib, g->eacc, g->hist_rows, g->aux, g->inj_flags, g->inj_perm,
-                  g->etiles, g->cand_geom, g->pzero, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
+                  g->etiles, g->cand_geom, g->pzero, g->xf, g->xidx, g->xadd, g->gp_fields, g->perm_dev, g->ntiles, g->mrange, g->motion_inst, g->inst_cap, g->inst_tmax,
                   g->inj_ps_old, g->inj_psxsr, g->inj_vol, g->inj_part, g->inj_scal, g->inj_vpart,
                   g->inj_vsum, g->inj_sel, g->inj_hist};
   for (void *p : ptrs)
@@ -3684,6 +3790,7 @@ int ft_run_init(ft_engine *g, double tmax, double dt_factor, double failure_dt) 
   // Initial GetForce + CalculateAccelerations (ex9.cpp:89-92); the Prony
   // update of this pass uses dt_1 like the reference's global dt.
   if ((rc = element_pass(g, true, false, true, 0.0, nullptr, 1))) return rc;
+  if ((rc = launch_pair_extras(g))) return rc;
   if ((rc = upload_caps(g, std::vector<int>(g->n_inst, 0)))) return rc;
   if ((rc = launch_nodal(g, kInitial))) return rc;  // stops: state of step 0
   if ((rc = check_status(g))) return rc;
@@ -3697,6 +3804,7 @@ int loop_body(ft_engine *g) {
   g->sample = (g->loop_iter++ % g->timing_every) == 0;
   int rc;
   if ((rc = element_pass(g, true, true, true, 0.0, nullptr, 1))) return rc;
+  if ((rc = launch_pair_extras(g))) return rc;
   // u still holds this step's displacement here (the nodal kernel below
   // advances it): the injury update of ex5.cpp:240 for t_k
   if (g->inj_on && (rc = injury_launch(g, &g->ctl->slot[g->parity], 0.0, 0.0))) return rc;

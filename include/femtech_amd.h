@@ -97,12 +97,12 @@ int ft_engine_info(ft_engine *eng, int *n_nodes, int *n_elems, int *precision);
  * wavefront that share a face along xi (element i's local nodes 0, 3, 4, 7
  * are element i - 1's nodes 1, 2, 5, 6: a row of a structured block) add
  * the forces of that face's four corners in registers and element i writes
- * the sums, so about half of the corner slots are written; when every
- * element without such a partner has its xi+ corners at nodes whose
- * matching xi- slot no element owns (the row ends of a block), those
- * corners go there and the node sum reads four slots instead of eight.  It
- * applies wherever FT_LAYOUT_CORNER does and reports FT_LAYOUT_CORNER when
- * no element pairs.  Forces asked for outside the run
+ * the sums, so about half of the corner slots are written; an element
+ * without such a partner writes each xi+ corner to the matching xi- slot of
+ * its node, directly when no element owns that slot (a row end of a block)
+ * or through a small list added after the element pass, so the node sum
+ * reads four slots instead of eight.  It applies wherever FT_LAYOUT_CORNER
+ * does and reports FT_LAYOUT_CORNER when no element pairs.  Forces asked for outside the run
  * loop (ft_get_force, ft_element_forces, ft_get_force_fields) are written
  * without pairs, bitwise as with FT_LAYOUT_CORNER.  Creation picks the first that applies in the order
  * CORNER_PAIRS, NODE_SLOTS, ELL (FT_LAYOUT_AUTO); ft_engine_set_layout

@@ -3,12 +3,12 @@ element i - 1 of one material segment that share a face along xi (i's local
 nodes 0, 3, 4, 7 are i - 1's nodes 1, 2, 5, 6) add that face's corner
 forces in registers (each wavefront's lane 0 recomputes the element before
 the wavefront's own 63); element i writes the four sums into its own corner
-slots and element i - 1 writes only its other four.  When every element
-without a right partner has its xi+ corners at nodes whose matching xi-
-slot is free (a structured block: the row ends), those corners go there and
-the node sum reads only the four xi- slots; otherwise (the ellipsoid, whose
-material segments interleave along xi) it reads all eight, and the slots no
-element writes hold zeros.
+slots and element i - 1 writes only its other four.  An element without a
+right partner writes each xi+ corner to the matching xi- slot of its node
+when no element owns that slot (a row end of a block), else to an extras
+entry that a small kernel adds to that slot after the element pass (the
+ellipsoid, whose material segments interleave along xi), so the node sum
+reads only the four xi- slots.
 
 The pair sum is one more floating-point addition before the node sum (the
 reference's scatter-add, GetForce_3D.cpp:39-44, adds the eight element
@@ -67,7 +67,8 @@ def test_structured_meshes_take_corner_pairs(gpu_ready):
         assert e.layout()[0] == ft.LAYOUT_CORNER_PAIRS
     with engine_for(block_case(reverse=True), ft.FT_FP32) as e:
         assert e.layout()[0] == ft.LAYOUT_CORNER
-    # incomplete pairing: the node sum reads all eight slots
+    # partner-less elements across material segments: extras, and still a
+    # four-slot node sum
     from femtech_amd import workloads as W
     w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
     w["fibres"] = w.get("fibres")
@@ -76,7 +77,7 @@ def test_structured_meshes_take_corner_pairs(gpu_ready):
         pairs = e.traffic_model()
         e.set_layout(ft.LAYOUT_CORNER)
         plain = e.traffic_model()
-        assert pairs[0] < plain[0] and pairs[1] == plain[1]
+        assert pairs[0] < plain[0] and pairs[1] < 0.8 * plain[1]
     # a mesh off the corner layout cannot take pairs
     from test_general_mesh import general_case
     with engine_for(general_case("impact"), ft.FT_FP64) as e:
@@ -148,8 +149,8 @@ def test_forces_outside_the_loop_are_unpaired(gpu_ready):
 @pytest.mark.parametrize("kind", ["block", "ellipsoid"])
 def test_pairs_graph_and_persistent(gpu_ready, precision, kind):
     """Captured chunks equal the per-step launches bit for bit with corner
-    pairs, complete (block) or not (ellipsoid); the persistent loop agrees
-    to round-off."""
+    pairs, without (block) or with extras (ellipsoid); the persistent loop
+    agrees to round-off."""
     if kind == "block":
         c = block_case(n=16, steps=120)
     else:
@@ -168,7 +169,7 @@ def test_pairs_graph_and_persistent(gpu_ready, precision, kind):
 @pytest.mark.parametrize("precision", [ft.FT_FP64, ft.FT_FP32])
 def test_pairs_incomplete_ellipsoid(gpu_ready, precision):
     """The three-material ellipsoid (pairs only inside each material's runs
-    along xi, eight-slot node sums): corner pairs against plain corner
+    along xi, the rest through extras): corner pairs against plain corner
     slots, FP64 to 1e-12 and FP32 to 1e-6, the Prony history included."""
     from femtech_amd import workloads as W
     w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
