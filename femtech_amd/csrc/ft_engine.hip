@@ -557,8 +557,8 @@ __device__ __forceinline__ void element_tile(const ElemArgs<R> &a, const int L) 
           dst[2] = outv[3 * k + 2];
         }
       } else {
-        // corner pairs: the xi+ corners go to the right lane, or to the
-        // node's free xi- slot
+        // corner pairs: the xi+ corners go to the right lane, to the node's
+        // free xi- slot, or to the element's extras entry
         bool right = false, to_minus = false, extra = false;
         if (a.pair) {  // uniform (corner slots)
           // C3D8 corners on the face xi = -1 (0, 3, 4, 7) against the same
