@@ -177,27 +177,47 @@ __device__ inline T block_min(T v, T *sh) {
   return r;
 }
 
-// Fixed-order block sums of N values with one barrier: each wavefront
-// reduces by the butterfly of block_sum, then the wavefront totals are added
-// in wavefront order (the same tree as N calls of block_sum).  The result is
-// valid in thread 0.
+// DPP move of both halves of a double (control CTRL, rows ROW_MASK): a lane
+// whose source lies outside its row, or whose row is masked off, reads zero.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the wavefront (every lane active) in a fixed tree, complete in
+// lane 63: prefix sums within each 16-lane row (row_shr 1, 2, 4, 8), row 0's
+// total added to row 1 and row 2's to row 3 (row_bcast:15), rows 0-1's
+// added to rows 2-3 (row_bcast:31).  Vector moves only, no LDS round trips.
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+  v += dpp_f64<0x111, 0xf>(v);
+  v += dpp_f64<0x112, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  return v;
+}
+
+// Fixed-order block sums of N <= 64 values with one barrier: each
+// wavefront's total (wave_sum_lane63), then the wavefront totals added in
+// wavefront order by lane i of the first wavefront, which returns the block
+// sum of value i (the other lanes return 0).
 template <int N>
-__device__ inline void block_sum_n(double (&v)[N], double (*sh)[N]) {
+__device__ inline double block_sum_spread(double (&v)[N], double (*sh)[N]) {
 #pragma unroll
-  for (int i = 0; i < N; ++i)
-    for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+  for (int i = 0; i < N; ++i) v[i] = wave_sum_lane63(v[i]);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0)
+  if (l == 63)
 #pragma unroll
     for (int i = 0; i < N; ++i) sh[w][i] = v[i];
   __syncthreads();
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double r = 0.0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k][i];
-      v[i] = r;
-    }
+  double r = 0.0;
+  if (threadIdx.x < N)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k][threadIdx.x];
+  return r;
 }
 
 // Fixed-order block sum (same tree every launch -> bitwise reproducible).
@@ -1182,21 +1202,22 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
     }
   }
   if (a.energy && (a.mode == kStep || go)) {  // uniform over the grid
+    // accumulators 0-4 of this block (W_int, W_ext, KE, and the next step's
+    // halves pInt, pExt), one per lane 0-4:
+    //   acc0 += (acc3 + W_int,k) / 2, acc1 += (acc4 + W_ext,k) / 2,
+    //   acc2 = KE_k / 2, acc3 = pInt, acc4 = pExt
     __shared__ double sh5[kBlock / 64][5];
     double t5[5] = {sWint, sWext, KE, pInt, pExt};
-    block_sum_n<5>(t5, sh5);
-    const double w = t5[0], x = t5[1], k = t5[2], pw = t5[3], px = t5[4];
-    if (threadIdx.x == 0) {
-      double *acc = a.eacc + kEaccStride * L;
-      if (a.mode == kStep) {
-        acc[0] += 0.5 * (acc[3] + w);
-        acc[1] += 0.5 * (acc[4] + x);
-        acc[2] = 0.5 * k;
-      }
-      if (go) {
-        acc[3] = pw;
-        acc[4] = px;
-      }
+    const double r = block_sum_spread<5>(t5, sh5);
+    double acc_old = 0.0;
+    if (threadIdx.x < 5) acc_old = a.eacc[kEaccStride * (size_t)L + threadIdx.x];
+    const double half = __shfl(acc_old, (int)(threadIdx.x + 3) & 63);  // lanes 0, 1: acc3, acc4
+    if (threadIdx.x < 5) {
+      const int i = threadIdx.x;
+      double *acc = a.eacc + kEaccStride * (size_t)L;
+      if (a.mode == kStep && i < 2) acc[i] = acc_old + 0.5 * (half + r);
+      if (a.mode == kStep && i == 2) acc[2] = 0.5 * r;
+      if (go && i >= 3) acc[i] = r;
     }
   }
   if ((tile.w & kTileFirst) && threadIdx.x == 0) {
