@@ -355,8 +355,11 @@ struct HistSoA {
 
 // A tile is the work of one block: a range of at most kBlock consecutive
 // elements (or nodes) of one instance.  int4 {begin, end, instance, flags};
-// node tiles carry kTileFirst on the first tile of their instance.
+// node tiles carry kTileFirst on the first tile of their instance and
+// kTileMotion when one of their nodes is constrained (only those tiles stage
+// the motion states).
 constexpr int kTileFirst = 1;
+constexpr int kTileMotion = 2;
 
 template <typename R>
 struct ElemArgs {
@@ -1056,6 +1059,13 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   const int inst = tile.z;
   Ctl *ic = a.ctl + inst;
   const StepSlot cur = ic->slot[a.parity];
+  // the instance's end time, step cap and motion range, read before the
+  // kernel's first store: uniform loads that nothing in the launch writes
+  // go through the scalar cache, together with the slot, instead of one
+  // vector round trip after another
+  const double tmax = a.tmax[inst];
+  const int cap = a.cap[inst];
+  const int mb = a.mrange[2 * inst], mc = a.mrange[2 * inst + 1];
   if (cur.done && a.mode != kResume) {
     // keep both ping-pong slots stopped so later launches stay no-ops
     if ((tile.w & kTileFirst) && threadIdx.x == 0) {
@@ -1088,7 +1098,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   // no stable step left: below the failure step (an inverted element's is
   // negative and counts as 0), or none finite at all (non-finite forces)
   const bool collapse = !(dtmin >= a.failure_dt) || !(dtmin < 1e308);
-  const bool go = (t_k < a.tmax[inst]) && (cur.step < a.cap[inst]) && !collapse;
+  const bool go = (t_k < tmax) && (cur.step < cap) && !collapse;
   // this instance's motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
   // and, when rotations are present, the parameters motion_apply reads (type,
   // axis, centre), in LDS sized for the run's largest motion count
@@ -1105,19 +1115,20 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
     else
       return ms_fixed[w][j];
   };
-  const int mb = a.mrange[2 * inst], mc = a.mrange[2 * inst + 1];
-  for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x) {
-    if (a.inline_motions)
-      motion_time(a.motions + (size_t)FT_MOTION_STRIDE * (mb + i % mc),
-                  motion_state_time(cur, a.factor, i / mc, dtmin), ms(i / mc, i % mc));
-    else
-      ms(i / mc, i % mc) = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
+  if (tile.w & kTileMotion) {  // uniform: a tile without constrained nodes reads no motion state
+    for (int i = threadIdx.x; i < 3 * mc; i += blockDim.x) {
+      if (a.inline_motions)
+        motion_time(a.motions + (size_t)FT_MOTION_STRIDE * (mb + i % mc),
+                    motion_state_time(cur, a.factor, i / mc, dtmin), ms(i / mc, i % mc));
+      else
+        ms(i / mc, i % mc) = a.mstate[(size_t)(i / mc) * a.n_motions + mb + i % mc];
+    }
+    if constexpr (ROT) {
+      for (int i = threadIdx.x; i < 7 * mc; i += blockDim.x)
+        mp[i / 7][i % 7] = a.motions[(size_t)FT_MOTION_STRIDE * (mb + i / 7) + i % 7];
+    }
+    __syncthreads();
   }
-  if constexpr (ROT) {
-    for (int i = threadIdx.x; i < 7 * mc; i += blockDim.x)
-      mp[i / 7][i % 7] = a.motions[(size_t)FT_MOTION_STRIDE * (mb + i / 7) + i % 7];
-  }
-  __syncthreads();
   const double *motions = a.motions + (size_t)FT_MOTION_STRIDE * mb;
   // Energy (CheckEnergy.cpp:15-52): the terms of step k pair each DOF's
   // displacement increment du_k with fi_{k-1} + fi_k.  The fi_{k-1} halves
@@ -1236,7 +1247,7 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
     } else {
       nx = cur;
       nx.done = 1;
-      if (collapse && cur.step < a.cap[inst] && t_k < a.tmax[inst]) {
+      if (collapse && cur.step < cap && t_k < tmax) {
         atomicOr(a.status, FT_STATUS_DT_COLLAPSE);
         nx.failed = 1;
       }
@@ -2161,6 +2172,8 @@ struct ft_engine {
   std::vector<int> inst_node, inst_elem;   // n_inst + 1 offsets each
   int4 *etiles = nullptr, *ntiles = nullptr;
   std::vector<int4> h_etiles;   // host copy (launch order)
+  std::vector<int4> h_ntiles;   // host copy
+  std::vector<int> h_code;      // host copy of node_code (empty until a boundary is set)
   double *cand_geom = nullptr;  // kCandStride per element tile (FP32 exact step size)
   double *gp_fields = nullptr;  // per-Gauss-point F, det F, PK2 (128 per element), on demand
   int *perm_dev = nullptr;      // internal -> caller element index, on demand
@@ -3009,6 +3022,20 @@ int build_element_tiles(ft_engine *g) {
   return build_pairs(g);  // wavefront boundaries moved
 }
 
+// kTileMotion on the node tiles that hold a constrained node (every tile
+// while no boundary is known), uploaded with the tiles.
+int mark_motion_tiles(ft_engine *g) {
+  const bool known = g->h_code.size() == (size_t)g->nn;
+  for (auto &t : g->h_ntiles) {
+    bool bc = !known;
+    for (int n = t.x; n < t.y && !bc; ++n) bc = g->h_code[n] != 0;
+    t.w = (t.w & ~kTileMotion) | (bc ? kTileMotion : 0);
+  }
+  if (!g->ntiles || g->h_ntiles.empty()) return 0;
+  HIPCHK(hipMemcpy(g->ntiles, g->h_ntiles.data(), g->h_ntiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return 0;
+}
+
 // Tiles of consecutive entries that never span two instances: elements per
 // material segment, nodes (kBlock per tile) in order.
 int build_tiles(ft_engine *g) {
@@ -3023,6 +3050,7 @@ int build_tiles(ft_engine *g) {
                              n == g->inst_node[i] ? kTileFirst : 0));
   }
   g->ntile_of_inst[g->n_inst] = (int)nt.size();
+  g->h_ntiles = nt;
   for (void *p : {(void *)g->ntiles, (void *)g->eacc})
     if (p) (void)hipFree(p);
   g->ntiles = nullptr;
@@ -3031,9 +3059,8 @@ int build_tiles(ft_engine *g) {
   if ((rc = dalloc(&g->ntiles, nt.size())) ||
       (rc = dalloc(&g->eacc, kEaccStride * (size_t)std::max(1, g->n_ntiles))))
     return rc;
-  HIPCHK(hipMemcpy(g->ntiles, nt.data(), nt.size() * sizeof(int4), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(g->eacc, 0, kEaccStride * (size_t)std::max(1, g->n_ntiles) * sizeof(double)));
-  return 0;
+  return mark_motion_tiles(g);
 }
 
 // Per-instance arrays of the control state (re)allocated for g->n_inst.
@@ -3116,6 +3143,8 @@ int rebuild_boundary(ft_engine *g) {
   HIPCHK(hipMemcpy(g->mrange, mr.data(), mr.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->node_code, code.data(), nn * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(g->node_mask, mask.data(), nn, hipMemcpyHostToDevice));
+  g->h_code = code;
+  if ((rc = mark_motion_tiles(g))) return rc;
   hipLaunchKernelGGL(k_elem_flags, dim3((g->ne + 255) / 256), dim3(256), 0, g->stream, g->ne, g->conn,
                      g->node_mask, g->einfo);
   HIPCHK(hipGetLastError());

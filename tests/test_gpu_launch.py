@@ -222,3 +222,29 @@ def test_snapshot_overlaps_later_steps_and_equals_a_stopped_run(gpu_ready, graph
     assert ks == k48 == 48 and ts == t48
     assert np.array_equal(us, u48)
     assert not np.array_equal(later, u48)
+
+
+@pytest.mark.parametrize("precision", [ft.FT_FP32, ft.FT_FP64])
+def test_boundary_replaced_after_creation(gpu_ready, precision):
+    """Only the node tiles that hold a constrained node stage the motion
+    states.  A boundary replaced after creation (first every DOF free, then
+    the ellipsoid's rotating surface; or set once) marks the tiles again: the
+    runs equal each other bit for bit."""
+    from femtech_amd import workloads as W
+    w = W.ellipsoid_head(n=12, peak=5e5, duration=2e-3)
+    outs = []
+    for first in (None, "free"):
+        with ft.Engine(w["X"], w["conn"], w["pid"], w["material_id"], w["props"], precision) as e:
+            if first == "free":
+                e.set_boundary(-np.ones_like(w["dof_motion"]), w["motions"])
+            e.set_boundary(w["dof_motion"], w["motions"])
+            e.set_fibres(w["fibres"])
+            e.lumped_mass()
+            e.run_init(1e9, 0.9, 1e-11)
+            e.run_steps(60)
+            assert e.status() == 0
+            outs.append((e.state()[0], e.time()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+    # the surface motion moved the nodes
+    assert np.abs(outs[0][0]).max() > 0
