@@ -2596,6 +2596,17 @@ int element_pass(ft_engine *g, bool force, bool dt, bool in_loop, double dt_fixe
   return element_pass_t<double>(g, force, dt, in_loop, dt_fixed, dtbits, update_hist);
 }
 
+// Motion states of timed motions: on a mesh of at most kInlineMotionTiles
+// node tiles each tile that holds a constrained node evaluates them itself
+// (no separate launch in the step: 27^3 cube, FP64, 445 steps 15.5 -> 14.9
+// ms); on larger meshes k_motion_states evaluates them once per step (on
+// the 2.0M-hex ellipsoid, whose every node tile holds surface nodes, the
+// in-tile evaluation cost the node update 10 us).
+constexpr int kInlineMotionTiles = 1024;
+inline bool inline_motion_states(const ft_engine *g) {
+  return g->timed_motions && g->n_ntiles <= kInlineMotionTiles;
+}
+
 // Nodal launches read the per-instance step caps and end times from
 // g->inst_cap / g->inst_tmax (set by the host before a run call).
 // Node tiles [t0, t1) (their energy accumulators at the same positions).
@@ -2629,7 +2640,7 @@ NodalArgs<R> nodal_args(ft_engine *g, int mode, int t0, int t1) {
   a.factor = g->factor;
   a.failure_dt = g->failure_dt;
   a.status = g->status;
-  a.inline_motions = 0;
+  a.inline_motions = inline_motion_states(g) ? 1 : 0;
   a.running = nullptr;
   a.mcap = g->mcap;
   a.cand_geom = g->precision == FT_FP32 ? g->cand_geom : nullptr;
@@ -2668,7 +2679,7 @@ int launch_nodal_t(ft_engine *g, int mode, int t0, int t1) {
 }
 
 int launch_nodal(ft_engine *g, int mode) {
-  if (g->timed_motions) {
+  if (g->timed_motions && !inline_motion_states(g)) {
     Timed t(g, 2);
     const int nb = std::max(1, std::min(64, (3 * g->n_motions + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_motion_states, dim3(nb), dim3(kBlock), 0, g->lstream, g->ctl, g->parity, g->factor,
