@@ -1105,8 +1105,9 @@ __device__ __forceinline__ void nodal_tile(const NodalArgs<R> &a, const int L) {
   // negative and counts as 0), or none finite at all (non-finite forces)
   const bool collapse = !(dtmin >= a.failure_dt) || !(dtmin < 1e308);
   const bool go = (t_k < tmax) && (cur.step < cap) && !collapse;
-  // this instance's motion states at t_{k-1}, t_k, t_{k+1} (k_motion_states)
-  // and, when rotations are present, the parameters motion_apply reads (type,
+  // this instance's motion states at t_{k-1}, t_k, t_{k+1} (from
+  // k_motion_states, or evaluated here with inline_motions) and, when
+  // rotations are present, the parameters motion_apply reads (type,
   // axis, centre), in LDS sized for the run's largest motion count
   // (runs without rotations keep the fixed-size table: its constant strides
   // cost no registers)
